@@ -1,0 +1,269 @@
+"""Benchmark of the capture-zone hot path on one MI355X.
+
+One step = one pass of the hot path over one batch of realizations of the
+single-site workload: fit + MVN draw of A..F for every realization, then
+backtrace + chronicle of every particle into the int32 count grid.  Inputs
+(realization parameters, site geometry, start points) are resident in HBM
+before the timed region; counts stay on the device.
+
+Workload (default): 25 wells, 40 piezometers, 10^4 realizations x 1000
+particles, 20-year backtrace, 400 x 400 nodes at 20 m, umbra 40 m, tol 1 m,
+maxstep 10 m.  Synthetic, seeded.
+
+The headline ``value`` is realizations/s of the FP32 tracer when its capture
+probabilities are within 0.01 (max over nodes) of the FP64-exact tracer on the
+same realizations; otherwise it is the FP64-exact rate.  Both rates, their
+particle-steps/s, the error, the dominant kernel's roofline fraction and a CPU
+baseline are in the JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks
+
+
+def step_flops(nwells):
+    """Canonical flops of one attempted Dormand-Prince step (6 confined
+    velocity evaluations of 12 + 10 Nw flops, plus 135 for the stages, error
+    estimate, step length and step control)."""
+    return 207 + 60 * nwells
+
+
+def kernel_flops(stats, nwells):
+    return (stats.attempted * step_flops(nwells) + 10 * stats.accepted
+            + 14 * stats.candidates)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nreal", type=int, default=10000)
+    ap.add_argument("--config", default="cfg1", choices=["cfg0", "cfg1"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="approximate CPU time budget of each CPU baseline")
+    return ap.parse_args()
+
+
+class DeviceRun:
+    """Inputs of one site resident on the GPU, and the launches of one step."""
+
+    def __init__(self, site, d, torch):
+        from onekapy_amd import _lib
+        from onekapy_amd.capturezone import start_points
+        self.torch = torch
+        self.lib = _lib
+        self.ctx = _lib.context(0)
+        self.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        dev = torch.device("cuda:0")
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+        self.site = site
+        self.R = len(d["K"])
+        self.K, self.n, self.b = t(d["K"]), t(d["n"]), t(d["b"])
+        self.q, self.z = t(d["q"]), t(d["z"])
+        self.coef = torch.zeros((self.R, 6), dtype=torch.float64, device=dev)
+        self.wells = t(site.wells_xy())
+        self.obs = t(site.obs_array())
+        starts = start_points(site.xt, site.yt, site.rt, site.npaths)
+        self.starts = t(starts)
+        pf = site.fixed_field()
+        self.grid = _lib.make_grid(pf.xmin, pf.ymin, pf.xmax, pf.ymax, pf.deltax, pf.deltay,
+                                   pf.nrows, pf.ncols)
+        self.counts = torch.zeros((pf.nrows, pf.ncols), dtype=torch.int32, device=dev)
+        self.status = torch.zeros(self.R, dtype=torch.int32, device=dev)
+        self.site_s = _lib.make_site(None, site.xt, site.yt, site.base, None,
+                                     wells_ptr=self.wells.data_ptr(), obs_ptr=self.obs.data_ptr())
+        self.site_s.nwells = self.wells.shape[0]
+        self.site_s.nobs = self.obs.shape[0]
+        torch.cuda.synchronize()
+
+    def step(self, precision):
+        """fit + draw, then trace + chronicle; returns the trace launch's stats."""
+        L, c = self.lib, self.ctx
+        self.counts.zero_()
+        c.check(c.lib.oka_fit_draw(c.h, self.site_s, self.R, self.K.data_ptr(), self.b.data_ptr(),
+                                   self.q.data_ptr(), self.z.data_ptr(), None, None,
+                                   self.coef.data_ptr(), self.status.data_ptr(),
+                                   L.PTR_DEVICE | L.ASYNC))
+        s = self.site
+        opts = L.make_trace_opts(s.npaths, s.duration, s.tol, s.maxstep, s.umbra,
+                                 self.starts.data_ptr(), precision, True)
+        stats = L.TraceStats_()
+        c.check(c.lib.oka_trace_chronicle(c.h, self.site_s, self.R, self.K.data_ptr(),
+                                          self.n.data_ptr(), self.b.data_ptr(), self.q.data_ptr(),
+                                          self.coef.data_ptr(), opts, self.grid, 0, None, 1,
+                                          self.counts.data_ptr(), None, None, None, stats,
+                                          L.PTR_DEVICE))
+        return stats
+
+
+def time_mode(run, precision, steps, warmup, torch):
+    for _ in range(warmup):
+        run.step(precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kms, st = [], None
+    for _ in range(steps):
+        st = run.step(precision)
+        kms.append(st.kernel_ms)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    counts = run.counts.cpu().numpy().copy()
+    return dict(seconds=dt, ms_per_step=1e3 * dt / steps, kernel_ms=float(np.mean(kms)),
+                stats=st, counts=counts)
+
+
+def cpu_baselines(site, d, coef, budget_s):
+    """Reference-speed Python restatement on 1 core, and the C oracle on all
+    cores, each on a bounded sample of the same workload."""
+    import oracle as O
+    from oracle import pyref
+    from onekapy_amd.capturezone import start_points
+    out = {}
+    # (1) numpy-scalar restatement, one process: the reference's own speed.
+    wxy = site.wells_xy()
+    m = pyref.PyModel(site.base, d["K"][0], d["n"][0], d["b"][0],
+                      [(wxy[i, 0], wxy[i, 1], 0.25, d["q"][0][i]) for i in range(len(wxy))],
+                      site.xt, site.yt, coef[0])
+    starts = start_points(site.xt, site.yt, site.rt, site.npaths)
+    from onekapy_amd.sites import fixed_bounds
+    pf = pyref.PyField(site.spacing, site.spacing, site.xt, site.yt)
+    pf.expand(*fixed_bounds(site.xt, site.yt, site.spacing, site.nrows, site.ncols))
+    t0 = time.perf_counter()
+    steps = 0
+    np_done = 0
+    for xs, ys in starts:
+        def feval(xy):
+            vx, vy = m.velocity_confined(xy[0], xy[1])
+            return np.array([-vx, -vy])
+        v, att, _acc, _st = pyref.backtrace(xs, ys, site.duration, site.tol, site.maxstep, feval)
+        pf.rasterize([a[0] for a in v], [a[1] for a in v], site.umbra)
+        steps += att
+        np_done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    frac = np_done / site.npaths
+    out["python"] = {"value": frac / el, "unit": "realizations/s", "cores": 1, "kind": "port",
+                     "particle_steps_per_s": steps / el,
+                     "sample": f"realization 0, first {np_done} of {site.npaths} particles "
+                               f"({steps} attempted steps, {el:.1f} s), numpy-scalar "
+                               f"restatement with the reference's loop structure"}
+    # (2) C oracle, FP64, OpenMP over realizations.
+    threads = O.max_threads()
+    g = O.make_grid(pf.xmin, pf.ymin, pf.xmax, pf.ymax, pf.dx, pf.dy, pf.nrows, pf.ncols)
+    opts = O.trace_opts(site.duration, site.tol, site.maxstep, True, O.MATH_GLIBC)
+    nr = max(threads, 2)
+    while True:
+        t0 = time.perf_counter()
+        _, st = O.chronicle_fixed(wxy, site.xt, site.yt, site.base, d["K"][:nr], d["n"][:nr],
+                                  d["b"][:nr], d["q"][:nr], coef[:nr], starts, site.umbra, opts,
+                                  g, nthreads=threads)
+        el = time.perf_counter() - t0
+        if el > 0.3 * budget_s or nr >= len(d["K"]):
+            break
+        nr = min(len(d["K"]), int(nr * max(2.0, 0.5 * budget_s / max(el, 1e-3))))
+    out["c_oracle"] = {"value": nr / el, "unit": "realizations/s", "cores": threads,
+                       "kind": "port", "particle_steps_per_s": st[0] / el,
+                       "sample": f"first {nr} realizations x {site.npaths} particles "
+                                 f"({int(st[0])} attempted steps, {el:.1f} s), C FP64 oracle, "
+                                 f"OpenMP"}
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(realizations shard across GPUs with one int32 sum-reduce of the count grid, "
+              f"not built here)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from onekapy_amd import sites
+    from onekapy_amd import _lib
+
+    site = sites.config1(args.nreal) if args.config == "cfg1" else sites.config0()
+    d = sites.draw(site, seed=1)
+    run = DeviceRun(site, d, torch)
+    nw = len(site.wells)
+    modes = {}
+    for name, prec in (("fp64", _lib.FP64_EXACT), ("fp32", _lib.FP32)):
+        print(f"bench: {name} {site.name}: {args.warmup} warmup + {args.steps} timed steps",
+              file=sys.stderr, flush=True)
+        modes[name] = time_mode(run, prec, args.steps, args.warmup, torch)
+        m = modes[name]
+        print(f"bench: {name} {m['ms_per_step']:.1f} ms/step, trace kernel {m['kernel_ms']:.1f} ms, "
+              f"{m['stats'].attempted} steps", file=sys.stderr, flush=True)
+    R = run.R
+    p64 = modes["fp64"]["counts"] / R
+    p32 = modes["fp32"]["counts"] / R
+    err_max = float(np.abs(p32 - p64).max())
+    err_l1 = float(np.abs(p32 - p64).sum())
+    head = "fp32" if err_max <= 0.01 else "fp64"
+    hm = modes[head]
+    value = R * args.steps / hm["seconds"]
+    steps_per_s = hm["stats"].attempted * args.steps / hm["seconds"]
+    flops = kernel_flops(hm["stats"], nw)
+    achieved = flops / (hm["kernel_ms"] * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[head]
+
+    cpu = None
+    if not args.no_cpu:
+        coef = run.coef.cpu().numpy()
+        print("bench: CPU baselines", file=sys.stderr, flush=True)
+        cpu = cpu_baselines(site, d, coef, args.cpu_seconds)
+
+    def mode_summary(name):
+        m = modes[name]
+        return {"realizations_per_s": R * args.steps / m["seconds"],
+                "particle_steps_per_s": m["stats"].attempted * args.steps / m["seconds"],
+                "ms_per_step": m["ms_per_step"], "trace_kernel_ms": m["kernel_ms"],
+                "attempted_steps": int(m["stats"].attempted),
+                "accepted_steps": int(m["stats"].accepted),
+                "visited_nodes": int(m["stats"].visited),
+                "escaped_traces": int(m["stats"].escaped),
+                "status_count": [int(v) for v in m["stats"].status_count],
+                "kernel_tflops": kernel_flops(m["stats"], nw) / (m["kernel_ms"] * 1e-3) / 1e12}
+
+    line = {
+        "metric": "realizations/sec and particle-steps/sec at max capture-prob error <=0.01 vs FP64",
+        "value": value,
+        "unit": "realizations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": hm["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "fp32" else "f64",
+        "data": "synthetic (seeded site and realization parameters)",
+        "config": {"workload": site.name, "realizations": R, "particles": site.npaths,
+                   "wells": nw, "observations": len(site.observations),
+                   "duration_d": site.duration, "grid": [site.nrows, site.ncols],
+                   "spacing_m": site.spacing, "umbra_m": site.umbra, "tol_m": site.tol,
+                   "maxstep_m": site.maxstep, "headline_mode": head},
+        "particle_steps_per_s": steps_per_s,
+        "error_vs_fp64": {"max": err_max, "l1": err_l1, "nodes": int(p64.size)},
+        "modes": {k: mode_summary(k) for k in modes},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": f"trace+chronicle ({head})",
+                     "flops_per_launch": flops},
+        "cpu_baseline": None if cpu is None else dict(cpu["python"], c_oracle=cpu["c_oracle"]),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

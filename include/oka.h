@@ -1,0 +1,222 @@
+/*
+ * oka.h -- C ABI of liboka.so, the MI355X (gfx950) implementation of OnekaPy's
+ * Monte Carlo capture-zone hot path.
+ *
+ * The reference has no FFI layer: its boundary is the Python API in
+ * oneka/model.py, oneka/capturezone.py, oneka/probabilityfield.py and
+ * oneka/stochastic.py.  Each entry point below replaces the reference function
+ * cited beside it; the Python package onekapy_amd binds them with ctypes and
+ * keeps the reference's Python signatures.
+ *
+ * Conventions
+ *   - Every function returns OKA_OK (0) or a negative OKA_ERR_* code; the text
+ *     of the last failure on a context is oka_last_error(ctx).
+ *   - Array arguments are host pointers (C-contiguous, row-major) unless the
+ *     call's flags include OKA_PTR_DEVICE, in which case every array argument
+ *     of that call is a device pointer on the context's device.
+ *   - Calls are synchronous unless flags include OKA_ASYNC (only honoured with
+ *     OKA_PTR_DEVICE): then the work is enqueued on the context's stream.
+ *   - A context is used by one host thread at a time.
+ */
+#ifndef OKA_H
+#define OKA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OKA_ABI_VERSION 1
+
+enum {
+    OKA_OK = 0,
+    OKA_ERR_ARG = -1,         /* invalid argument                                  */
+    OKA_ERR_RANGE = -2,       /* RangeError (model.py:51-53, probabilityfield.py:47-49) */
+    OKA_ERR_AQUIFER = -3,     /* AquiferError (model.py:56-58)                     */
+    OKA_ERR_LINALG = -4,      /* numpy.linalg.LinAlgError (model.py:598-604)        */
+    OKA_ERR_DEVICE = -5,      /* HIP runtime failure                               */
+    OKA_ERR_NOMEM = -6,       /* allocation failure                                */
+    OKA_ERR_CAPACITY = -7     /* an output buffer is too small (size reported)     */
+};
+
+/* Per-trace status.  The reference either finishes a trace, or truncates it
+ * when feval raises (capturezone.py:249-253).  STALLED and STEP_CAP are
+ * deliberate deviations: the reference loops forever when dt becomes NaN or 0
+ * (capturezone.py:221,247), and has no step cap. */
+enum {
+    OKA_TRACE_OK = 0,
+    OKA_TRACE_TRUNCATED = 1,
+    OKA_TRACE_STALLED = 2,
+    OKA_TRACE_STEP_CAP = 3
+};
+
+/* Arithmetic of the particle tracer. */
+enum {
+    OKA_FP64_EXACT = 0,  /* reference operation order, no contraction, correctly rounded pow/log */
+    OKA_FP32 = 1         /* FP32 tracing in target-relative coordinates, compensated positions */
+};
+
+/* What oka_model_eval computes at each point. */
+enum {
+    OKA_EVAL_POTENTIAL = 0,          /* model.py:207-237        -> 1 double  */
+    OKA_EVAL_POTENTIAL_WELLS = 1,    /* model.py:240-266        -> 1 double  */
+    OKA_EVAL_DISCHARGE = 2,          /* model.py:269-315        -> 2 doubles */
+    OKA_EVAL_HEAD = 3,               /* model.py:318-350        -> 1 double  */
+    OKA_EVAL_VELOCITY = 4,           /* model.py:353-389        -> 2 doubles */
+    OKA_EVAL_VELOCITY_CONFINED = 5   /* model.py:392-427        -> 2 doubles */
+};
+
+#define OKA_PTR_DEVICE 0x1u
+#define OKA_ASYNC 0x2u
+
+typedef struct oka_ctx oka_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+int oka_abi_version(void);
+int oka_ctx_create(oka_ctx **out, int device);
+void oka_ctx_destroy(oka_ctx *ctx);
+const char *oka_last_error(const oka_ctx *ctx);
+/* Use the caller's HIP stream (a hipStream_t), or NULL for the library's own. */
+int oka_ctx_set_stream(oka_ctx *ctx, void *hip_stream);
+void *oka_ctx_stream(oka_ctx *ctx);
+int oka_ctx_synchronize(oka_ctx *ctx);
+
+/* ---- one model: Model.compute_* (model.py:207-427) ----------------------- */
+typedef struct {
+    double base, conductivity, porosity, thickness;
+    double xo, yo;
+    double coef[6];
+    int32_t nwells;
+    int32_t reserved;
+    const double *wells;     /* nwells x 4: xw, yw, rw, qw (always host memory) */
+} oka_model;
+
+/* Evaluates `kind` at npts points xy (npts x 2).  out is npts x 1 or npts x 2.
+ * status (npts int32, may be NULL) receives 0 or OKA_ERR_AQUIFER per point;
+ * out holds NaN where status is nonzero.  FP64, reference operation order. */
+int oka_model_eval(oka_ctx *ctx, const oka_model *m, int32_t kind, int64_t npts,
+                   const double *xy, double *out, int32_t *status, uint32_t flags);
+
+/* ---- a site: geometry shared by every realization ------------------------ */
+typedef struct {
+    int32_t nwells;          /* wells, including the target well                */
+    int32_t nobs;            /* observations                                     */
+    const double *wells_xy;  /* nwells x 2 (host, or device with OKA_PTR_DEVICE) */
+    const double *obs;       /* nobs x 4: x, y, z_ev, z_std                      */
+    double xo, yo;           /* origin of the regional flow = the target well    */
+    double base;             /* aquifer base elevation                           */
+} oka_site;
+
+/* ---- fit + draw: Model.fit_regional_flow (model.py:430-606) and the MVN
+ *      draw of stochastic.py:239-241, for nreal realizations at once ----------
+ * For realization r: ev[r] (6), cov[r] (6x6) = inv(WA^T WA) from the weighted
+ * least squares of construct_fit with conductivity K[r], thickness b[r] and
+ * discharges q[r, :] (nreal x nwells); then coef[r] = ev[r] + L z[r] with L the
+ * lower Cholesky factor of cov[r] (z: nreal x 6 standard normals; NULL gives
+ * coef = ev, the deterministic path of deterministic.py:197-199).  ev, cov may
+ * be NULL.  status[r] (may be NULL) is 0, OKA_ERR_RANGE (an observation at or
+ * below the base) or OKA_ERR_LINALG; the call returns the first nonzero one.
+ * FP64: column-equilibrated normal equations, Cholesky, one refinement step. */
+int oka_fit_draw(oka_ctx *ctx, const oka_site *site, int32_t nreal, const double *K,
+                 const double *b, const double *q, const double *z, double *ev, double *cov,
+                 double *coef, int32_t *status, uint32_t flags);
+
+/* ---- trace options: compute_capturezone / compute_backtrace arguments ---- */
+typedef struct {
+    int32_t npaths;          /* particles per realization (capturezone.py:113)      */
+    int32_t precision;       /* OKA_FP64_EXACT or OKA_FP32                           */
+    int32_t confined;        /* 1: compute_velocity_confined, 0: compute_velocity    */
+    int32_t reserved;
+    double duration, tol, maxstep, umbra;
+    int64_t max_steps;       /* attempted-step cap per trace; <= 0: 1,000,000        */
+    const double *start_xy;  /* npaths x 2 start points (capturezone.py:114-115)     */
+} oka_trace_opts;
+
+/* A node lattice: node (i, j) is at (xmin + j*dx, ymin + i*dy); row i is y.
+ * xmax/ymax are the field's own attributes (probabilityfield.py:140-143,
+ * 212-213, 229-245), used for the "strictly inside" test. */
+typedef struct {
+    double xmin, ymin, xmax, ymax, dx, dy;
+    int32_t nrows, ncols;
+} oka_grid;
+
+/* The field's state at one moment of the expansion history: traces with global
+ * index >= first_trace (index r*npaths + p) and below the next epoch's
+ * first_trace were rasterized into a grid with this origin and size
+ * (probabilityfield.py:334-335).  Trace-time node (i, j) is final node
+ * (i + row_off, j + col_off). */
+typedef struct {
+    int64_t first_trace;
+    double xmin, ymin;
+    int32_t nrows, ncols;
+    int32_t row_off, col_off;
+} oka_epoch;
+
+typedef struct {
+    int64_t attempted;        /* attempted Dormand-Prince steps (all traces)         */
+    int64_t accepted;         /* accepted steps = segments                           */
+    int64_t status_count[4];  /* traces per OKA_TRACE_* status                       */
+    int64_t escaped;          /* traces whose vertex bbox is not strictly inside     */
+    int64_t visited;          /* (realization, node) marks added to counts           */
+    int64_t candidates;       /* nodes in the clipped candidate windows of segments  */
+    double kernel_ms;         /* device time of the tracing kernel(s)                */
+} oka_trace_stats;
+
+/* ---- trace + chronicle: compute_capturezone (capturezone.py:51-123) plus
+ *      rasterize/insert/register (probabilityfield.py:264-359) for nreal
+ *      realizations of one site ---------------------------------------------
+ * Realization r has conductivity K[r], porosity n[r], thickness b[r],
+ * discharges q[r, :] and regional coefficients coef[r, :] (A..F).  Each of its
+ * npaths traces is rasterized with the vector-to-raster range umbra; every node
+ * marked by any trace of r adds 1 to counts (nrows x ncols int32 of `grid`,
+ * accumulated).  With nepochs == 0 the grid does not grow and the clip window
+ * is the whole grid (a pre-expanded ProbabilityField); traces that would have
+ * grown it are counted in stats->escaped.  With nepochs > 0, epochs (sorted by
+ * first_trace, from oka_grid_history) reproduce the reference's expansion
+ * history exactly.  bbox (ntraces x 4: xmin, xmax, ymin, ymax), trace_status
+ * and trace_steps (attempted steps) are optional per-trace outputs, ntraces =
+ * nreal * npaths.  With rasterize == 0 only the traces and the optional
+ * outputs are computed. */
+int oka_trace_chronicle(oka_ctx *ctx, const oka_site *site, int32_t nreal, const double *K,
+                        const double *n, const double *b, const double *q,
+                        const double *coef, const oka_trace_opts *opts, const oka_grid *grid,
+                        int32_t nepochs, const oka_epoch *epochs, int32_t rasterize,
+                        int32_t *counts, double *bbox, int32_t *trace_status,
+                        int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags);
+
+/* ---- expansion history (probabilityfield.py:175-261 applied per trace in
+ *      order, capturezone.py:117-120) -- host computation ---------------------
+ * Given the field before the batch (init; init->nrows == 0 means an empty
+ * field) and each trace's bbox in order, computes the field after the batch
+ * (final) and the epochs.  epochs may be NULL to query *nepochs only; returns
+ * OKA_ERR_CAPACITY if max_epochs is too small. */
+int oka_grid_history(const oka_grid *init, int64_t ntraces, const double *bbox,
+                     oka_grid *final_grid, oka_epoch *epochs, int64_t max_epochs,
+                     int64_t *nepochs);
+
+/* ---- vertices of individual traces: compute_backtrace (capturezone.py:127-253)
+ * For each realization r and particle p, the accepted vertices (start point
+ * first) go to verts[(r*npaths + p) * cap * 2 ...]; nverts receives the full
+ * count (which may exceed cap: then only the first cap are stored). */
+int oka_trace_vertices(oka_ctx *ctx, const oka_site *site, int32_t nreal, const double *K,
+                       const double *n, const double *b, const double *q, const double *coef,
+                       const oka_trace_opts *opts, int64_t cap, double *verts,
+                       int64_t *nverts, int32_t *trace_status, int64_t *trace_steps,
+                       uint32_t flags);
+
+/* ---- one polyline into a registration grid: ProbabilityField.rasterize's
+ *      insert loop (probabilityfield.py:264-339) on a fixed grid -------------
+ * Marks rgrid (nrows x ncols uint8, nonzero = visited) for every node strictly
+ * within umbra of any segment of xy (nverts x 2).  No expansion is done. */
+int oka_rasterize(oka_ctx *ctx, const oka_grid *grid, int64_t nverts, const double *xy,
+                  double umbra, uint8_t *rgrid, uint32_t flags);
+
+/* ---- exact scalar helpers shared by host and device code (for tests) ----- */
+double oka_pow_fifth(double x);   /* correctly rounded pow(x, 0.2) (capturezone.py:247) */
+double oka_log(double x);         /* correctly rounded log(x) (model.py:264)            */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,31 @@
+"""onekapy_amd -- OnekaPy's Monte Carlo capture-zone hot path on AMD MI355X.
+
+The public API mirrors the reference's modules:
+  Model                          (oneka/model.py)
+  compute_capturezone,
+  compute_backtrace              (oneka/capturezone.py)
+  ProbabilityField               (oneka/probabilityfield.py)
+  create_stochastic_capturezone,
+  generate_random_variate,
+  compute_variate_mean,
+  isdistribution                 (oneka/stochastic.py)
+All evaluation, fitting, tracing and rasterization run in liboka.so
+(hand-written HIP for gfx950, C ABI in include/oka.h) through ctypes.
+"""
+from .model import Model, AquiferError, RangeError
+from .probabilityfield import ProbabilityField
+from .capturezone import (ModelVelocity, compute_backtrace, compute_capturezone, start_points,
+                          trace_vertices)
+from .stochastic import (DistributionError, Lognormal, Signed, compute_variate_mean,
+                         create_stochastic_capturezone, draw_realizations, fit_and_draw,
+                         generate_random_variate, isdistribution)
+from .engine import Realizations, chronicle_into_field
+from . import _lib
+
+__all__ = [
+    "Model", "AquiferError", "RangeError", "ProbabilityField", "ModelVelocity",
+    "compute_backtrace", "compute_capturezone", "start_points", "trace_vertices",
+    "DistributionError", "Lognormal", "Signed", "compute_variate_mean",
+    "create_stochastic_capturezone", "draw_realizations", "fit_and_draw",
+    "generate_random_variate", "isdistribution", "Realizations", "chronicle_into_field",
+]

@@ -1,0 +1,252 @@
+"""ctypes binding of liboka.so (declared in include/oka.h).
+
+The library is the product's only compute path: every class and function in
+this package that evaluates the model, fits, traces or rasterizes calls it.  If
+the library is missing, or no GPU is visible, calls raise ``OkaLibraryError``;
+nothing falls back to a CPU implementation.
+"""
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboka.so")
+
+OK = 0
+ERR_ARG, ERR_RANGE, ERR_AQUIFER, ERR_LINALG, ERR_DEVICE, ERR_NOMEM, ERR_CAPACITY = (
+    -1, -2, -3, -4, -5, -6, -7)
+
+TRACE_OK, TRACE_TRUNCATED, TRACE_STALLED, TRACE_STEP_CAP = 0, 1, 2, 3
+FP64_EXACT, FP32 = 0, 1
+(EVAL_POTENTIAL, EVAL_POTENTIAL_WELLS, EVAL_DISCHARGE, EVAL_HEAD, EVAL_VELOCITY,
+ EVAL_VELOCITY_CONFINED) = range(6)
+PTR_DEVICE = 0x1
+ASYNC = 0x2
+
+_d = C.c_double
+_i32 = C.c_int32
+_i64 = C.c_int64
+_u32 = C.c_uint32
+_vp = C.c_void_p
+
+# Every symbol include/oka.h declares (the CPU test checks the export table).
+EXPORTED = (
+    "oka_abi_version", "oka_ctx_create", "oka_ctx_destroy", "oka_last_error",
+    "oka_ctx_set_stream", "oka_ctx_stream", "oka_ctx_synchronize", "oka_model_eval",
+    "oka_fit_draw", "oka_trace_chronicle", "oka_grid_history", "oka_trace_vertices",
+    "oka_rasterize", "oka_pow_fifth", "oka_log",
+)
+
+
+class OkaLibraryError(RuntimeError):
+    """liboka.so is missing, failed to load, or no GPU is available."""
+
+
+class OkaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"liboka error {code}: {msg}")
+        self.code = code
+
+
+class Model_(C.Structure):
+    _fields_ = [("base", _d), ("conductivity", _d), ("porosity", _d), ("thickness", _d),
+                ("xo", _d), ("yo", _d), ("coef", _d * 6), ("nwells", _i32), ("reserved", _i32),
+                ("wells", _vp)]
+
+
+class Site_(C.Structure):
+    _fields_ = [("nwells", _i32), ("nobs", _i32), ("wells_xy", _vp), ("obs", _vp), ("xo", _d),
+                ("yo", _d), ("base", _d)]
+
+
+class TraceOpts_(C.Structure):
+    _fields_ = [("npaths", _i32), ("precision", _i32), ("confined", _i32), ("reserved", _i32),
+                ("duration", _d), ("tol", _d), ("maxstep", _d), ("umbra", _d),
+                ("max_steps", _i64), ("start_xy", _vp)]
+
+
+class Grid_(C.Structure):
+    _fields_ = [("xmin", _d), ("ymin", _d), ("xmax", _d), ("ymax", _d), ("dx", _d), ("dy", _d),
+                ("nrows", _i32), ("ncols", _i32)]
+
+
+class Epoch_(C.Structure):
+    _fields_ = [("first_trace", _i64), ("xmin", _d), ("ymin", _d), ("nrows", _i32),
+                ("ncols", _i32), ("row_off", _i32), ("col_off", _i32)]
+
+
+EPOCH_DTYPE = np.dtype([("first_trace", np.int64), ("xmin", np.float64), ("ymin", np.float64),
+                        ("nrows", np.int32), ("ncols", np.int32), ("row_off", np.int32),
+                        ("col_off", np.int32)])
+assert EPOCH_DTYPE.itemsize == C.sizeof(Epoch_)
+
+
+class TraceStats_(C.Structure):
+    _fields_ = [("attempted", _i64), ("accepted", _i64), ("status_count", _i64 * 4),
+                ("escaped", _i64), ("visited", _i64), ("candidates", _i64),
+                ("kernel_ms", _d)]
+
+
+_lib = None
+_lock = threading.Lock()
+_contexts = {}
+
+
+def load():
+    """Load liboka.so and declare its prototypes (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OkaLibraryError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (or make -C onekapy_amd/csrc)")
+    try:
+        L = C.CDLL(LIB_PATH)
+    except OSError as e:
+        raise OkaLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+    L.oka_abi_version.restype = C.c_int
+    L.oka_abi_version.argtypes = []
+    L.oka_ctx_create.restype = C.c_int
+    L.oka_ctx_create.argtypes = [C.POINTER(_vp), C.c_int]
+    L.oka_ctx_destroy.restype = None
+    L.oka_ctx_destroy.argtypes = [_vp]
+    L.oka_last_error.restype = C.c_char_p
+    L.oka_last_error.argtypes = [_vp]
+    L.oka_ctx_set_stream.restype = C.c_int
+    L.oka_ctx_set_stream.argtypes = [_vp, _vp]
+    L.oka_ctx_stream.restype = _vp
+    L.oka_ctx_stream.argtypes = [_vp]
+    L.oka_ctx_synchronize.restype = C.c_int
+    L.oka_ctx_synchronize.argtypes = [_vp]
+    L.oka_model_eval.restype = C.c_int
+    L.oka_model_eval.argtypes = [_vp, C.POINTER(Model_), _i32, _i64, _vp, _vp, _vp, _u32]
+    L.oka_fit_draw.restype = C.c_int
+    L.oka_fit_draw.argtypes = [_vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                               _vp, _u32]
+    L.oka_trace_chronicle.restype = C.c_int
+    L.oka_trace_chronicle.argtypes = [_vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp,
+                                      C.POINTER(TraceOpts_), C.POINTER(Grid_), _i32, _vp, _i32,
+                                      _vp, _vp, _vp, _vp, C.POINTER(TraceStats_), _u32]
+    L.oka_grid_history.restype = C.c_int
+    L.oka_grid_history.argtypes = [C.POINTER(Grid_), _i64, _vp, C.POINTER(Grid_), _vp, _i64,
+                                   C.POINTER(_i64)]
+    L.oka_trace_vertices.restype = C.c_int
+    L.oka_trace_vertices.argtypes = [_vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp,
+                                     C.POINTER(TraceOpts_), _i64, _vp, _vp, _vp, _vp, _u32]
+    L.oka_rasterize.restype = C.c_int
+    L.oka_rasterize.argtypes = [_vp, C.POINTER(Grid_), _i64, _vp, _d, _vp, _u32]
+    L.oka_pow_fifth.restype = _d
+    L.oka_pow_fifth.argtypes = [_d]
+    L.oka_log.restype = _d
+    L.oka_log.argtypes = [_d]
+    _lib = L
+    return L
+
+
+class Context:
+    """One liboka context (device, stream, staging buffers) per device."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = _vp()
+        rc = L.oka_ctx_create(C.byref(h), int(device))
+        if rc != OK:
+            raise OkaLibraryError(
+                f"oka_ctx_create(device={device}) failed with {rc}: no usable GPU for liboka")
+        self.h = h
+        self.device = device
+        self.lib = L
+
+    def check(self, rc):
+        if rc != OK:
+            msg = self.lib.oka_last_error(self.h)
+            raise OkaError(rc, msg.decode() if msg else "")
+        return rc
+
+    def set_stream(self, stream_ptr):
+        self.check(self.lib.oka_ctx_set_stream(self.h, stream_ptr))
+
+    def synchronize(self):
+        self.check(self.lib.oka_ctx_synchronize(self.h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.oka_ctx_destroy(self.h)
+        except Exception:
+            pass
+
+
+def context(device=0):
+    """The process-wide context for `device`, created on first use."""
+    with _lock:
+        ctx = _contexts.get(device)
+        if ctx is None:
+            ctx = Context(device)
+            _contexts[device] = ctx
+        return ctx
+
+
+def f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def ptr(a):
+    """Address of a numpy array's data (None for None)."""
+    if a is None:
+        return None
+    return a.ctypes.data
+
+
+def make_grid(xmin, ymin, xmax, ymax, dx, dy, nrows, ncols):
+    g = Grid_()
+    g.xmin, g.ymin, g.xmax, g.ymax = float(xmin), float(ymin), float(xmax), float(ymax)
+    g.dx, g.dy = float(dx), float(dy)
+    g.nrows, g.ncols = int(nrows), int(ncols)
+    return g
+
+
+def make_site(wells_xy, xo, yo, base=0.0, obs=None, wells_ptr=None, obs_ptr=None):
+    """Site_ from host arrays (kept alive by the caller) or raw device pointers."""
+    s = Site_()
+    s.nwells = int(len(wells_xy)) if wells_xy is not None else 0
+    s.nobs = int(len(obs)) if obs is not None else 0
+    s.wells_xy = wells_ptr if wells_ptr is not None else ptr(wells_xy)
+    s.obs = obs_ptr if obs_ptr is not None else ptr(obs)
+    s.xo, s.yo, s.base = float(xo), float(yo), float(base)
+    return s
+
+
+def make_trace_opts(npaths, duration, tol, maxstep, umbra, start_xy_ptr, precision=FP64_EXACT,
+                    confined=True, max_steps=0):
+    o = TraceOpts_()
+    o.npaths = int(npaths)
+    o.precision = int(precision)
+    o.confined = 1 if confined else 0
+    o.duration, o.tol, o.maxstep, o.umbra = float(duration), float(tol), float(maxstep), float(umbra)
+    o.max_steps = int(max_steps)
+    o.start_xy = start_xy_ptr
+    return o
+
+
+def grid_history(init_grid, bbox):
+    """oka_grid_history: (final Grid_, epochs structured array)."""
+    L = load()
+    bbox = f64(bbox).reshape(-1, 4)
+    final = Grid_()
+    n = _i64(0)
+    rc = L.oka_grid_history(C.byref(init_grid), bbox.shape[0], ptr(bbox), C.byref(final), None, 0,
+                            C.byref(n))
+    if rc != OK:
+        raise OkaError(rc, "oka_grid_history failed")
+    epochs = np.zeros(n.value, dtype=EPOCH_DTYPE)
+    rc = L.oka_grid_history(C.byref(init_grid), bbox.shape[0], ptr(bbox), C.byref(final),
+                            epochs.ctypes.data, n.value, C.byref(n))
+    if rc != OK:
+        raise OkaError(rc, "oka_grid_history failed")
+    return final, epochs

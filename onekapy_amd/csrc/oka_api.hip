@@ -1,0 +1,694 @@
+// oka_api.hip -- the extern "C" entry points of liboka.so (include/oka.h):
+// context and stream handling, staging of host arrays, launch configuration,
+// error reporting, and the host-side expansion-history scan.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "oka_internal.h"
+
+using namespace oka;
+
+namespace {
+
+// Device buffer slots owned by a context; grown on demand, freed at destroy.
+enum Slot {
+    kSlotWells, kSlotStart, kSlotK, kSlotN, kSlotB, kSlotQ, kSlotCoef, kSlotCounts,
+    kSlotBbox, kSlotTStatus, kSlotTSteps, kSlotStats, kSlotEpochs, kSlotBitmap,
+    kSlotVerts, kSlotNVerts, kSlotObs, kSlotLogs, kSlotZ, kSlotEv, kSlotCov, kSlotFStatus,
+    kSlotXY, kSlotOut, kSlotEStatus, kSlotRgrid, kSlotCount
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct oka_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int num_cu = 0;
+    size_t max_lds = 0;
+    std::string err;
+    DevBuf buf[kSlotCount];
+};
+
+namespace {
+
+int fail(oka_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        char tmp[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(tmp, sizeof(tmp), fmt, ap);
+        va_end(ap);
+        c->err = tmp;
+    }
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail((c), OKA_ERR_DEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                                  \
+    } while (0)
+
+int ensure(oka_ctx *c, Slot s, size_t bytes, void **out)
+{
+    DevBuf &b = c->buf[s];
+    if (bytes == 0)
+        bytes = 8;
+    if (b.cap < bytes) {
+        if (b.p) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipFree(b.p));
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        size_t want = bytes + bytes / 4;
+        HIPCHK(c, hipMalloc(&b.p, want));
+        b.cap = want;
+    }
+    *out = b.p;
+    return OKA_OK;
+}
+
+// Device view of an input array: the caller's device pointer, or a staged copy.
+int stage_in(oka_ctx *c, Slot s, const void *src, size_t bytes, uint32_t flags, const void **out)
+{
+    if (!src) {
+        *out = nullptr;
+        return OKA_OK;
+    }
+    if (flags & OKA_PTR_DEVICE) {
+        *out = src;
+        return OKA_OK;
+    }
+    void *d;
+    int rc = ensure(c, s, bytes, &d);
+    if (rc)
+        return rc;
+    if (bytes)
+        HIPCHK(c, hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, c->stream));
+    *out = d;
+    return OKA_OK;
+}
+
+// Device view of an output array (optionally preloaded with its host contents).
+int stage_out(oka_ctx *c, Slot s, void *dst, size_t bytes, uint32_t flags, bool preload,
+              void **out)
+{
+    if (!dst) {
+        *out = nullptr;
+        return OKA_OK;
+    }
+    if (flags & OKA_PTR_DEVICE) {
+        *out = dst;
+        return OKA_OK;
+    }
+    int rc = ensure(c, s, bytes, out);
+    if (rc)
+        return rc;
+    if (preload && bytes)
+        HIPCHK(c, hipMemcpyAsync(*out, dst, bytes, hipMemcpyHostToDevice, c->stream));
+    return OKA_OK;
+}
+
+int copy_back(oka_ctx *c, void *dst, const void *dev, size_t bytes, uint32_t flags)
+{
+    if (!dst || (flags & OKA_PTR_DEVICE) || bytes == 0)
+        return OKA_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    return OKA_OK;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+__global__ void k_bits_to_u8(const uint32_t *bits, uint8_t *out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && ((bits[i >> 5] >> (i & 31)) & 1u))
+        out[i] = 1;
+}
+
+// Zero-fill without hipMemsetAsync, so that the call can be graph-captured.
+int fill_u32(oka_ctx *c, void *p, int64_t nwords)
+{
+    if (nwords <= 0)
+        return OKA_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, c->stream,
+                       (uint32_t *)p, 0u, nwords);
+    HIPCHK(c, hipGetLastError());
+    return OKA_OK;
+}
+
+#define TRY(expr)                                                                             \
+    do {                                                                                      \
+        int rc_ = (expr);                                                                     \
+        if (rc_)                                                                              \
+            return rc_;                                                                       \
+    } while (0)
+
+int check_opts(oka_ctx *c, const oka_trace_opts *o)
+{
+    if (!o)
+        return fail(c, OKA_ERR_ARG, "trace options are NULL");
+    if (o->npaths < 0)
+        return fail(c, OKA_ERR_ARG, "npaths < 0");
+    if (o->npaths > 0 && !o->start_xy)
+        return fail(c, OKA_ERR_ARG, "start_xy is NULL");
+    if (o->precision != OKA_FP64_EXACT && o->precision != OKA_FP32)
+        return fail(c, OKA_ERR_ARG, "unknown precision %d", o->precision);
+    if (o->precision == OKA_FP32 && !o->confined)
+        return fail(c, OKA_ERR_ARG, "FP32 tracing supports the confined velocity only");
+    return OKA_OK;
+}
+
+constexpr int64_t kDefaultMaxSteps = 1000000;
+
+}  // namespace
+
+extern "C" {
+
+int oka_abi_version(void) { return OKA_ABI_VERSION; }
+
+int oka_ctx_create(oka_ctx **out, int device)
+{
+    if (!out)
+        return OKA_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return OKA_ERR_DEVICE;
+    if (device < 0 || device >= n)
+        return OKA_ERR_ARG;
+    oka_ctx *c = new oka_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return OKA_ERR_DEVICE;
+    }
+    c->stream = c->own;
+    c->num_cu = prop.multiProcessorCount;
+    c->max_lds = prop.sharedMemPerBlock;
+    *out = c;
+    return OKA_OK;
+}
+
+void oka_ctx_destroy(oka_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &b : c->buf)
+        if (b.p)
+            (void)hipFree(b.p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+}
+
+const char *oka_last_error(const oka_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int oka_ctx_set_stream(oka_ctx *c, void *s)
+{
+    if (!c)
+        return OKA_ERR_ARG;
+    c->stream = s ? (hipStream_t)s : c->own;
+    return OKA_OK;
+}
+
+void *oka_ctx_stream(oka_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int oka_ctx_synchronize(oka_ctx *c)
+{
+    if (!c)
+        return OKA_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OKA_OK;
+}
+
+double oka_pow_fifth(double x) { return host_pow_fifth(x); }
+double oka_log(double x) { return host_log(x); }
+
+int oka_model_eval(oka_ctx *c, const oka_model *m, int32_t kind, int64_t npts, const double *xy,
+                   double *out, int32_t *status, uint32_t flags)
+{
+    if (!c || !m)
+        return OKA_ERR_ARG;
+    if (kind < OKA_EVAL_POTENTIAL || kind > OKA_EVAL_VELOCITY_CONFINED)
+        return fail(c, OKA_ERR_ARG, "unknown evaluation kind %d", kind);
+    if (npts < 0 || m->nwells < 0 || (m->nwells > 0 && !m->wells))
+        return fail(c, OKA_ERR_ARG, "bad points or wells");
+    if (npts == 0)
+        return OKA_OK;
+    if (!xy || !out)
+        return fail(c, OKA_ERR_ARG, "xy/out are NULL");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int width = (kind == OKA_EVAL_DISCHARGE || kind == OKA_EVAL_VELOCITY ||
+                       kind == OKA_EVAL_VELOCITY_CONFINED) ? 2 : 1;
+    EvalArgs a{};
+    a.kind = kind;
+    a.nwells = m->nwells;
+    a.base = m->base;
+    a.K = m->conductivity;
+    a.n = m->porosity;
+    a.b = m->thickness;
+    a.xo = m->xo;
+    a.yo = m->yo;
+    for (int j = 0; j < 6; ++j)
+        a.coef[j] = m->coef[j];
+    a.npts = npts;
+    const void *d;
+    TRY(stage_in(c, kSlotWells, m->wells, sizeof(double) * 4 * m->nwells, 0, &d));
+    a.wells = (const double *)d;
+    TRY(stage_in(c, kSlotXY, xy, sizeof(double) * 2 * npts, flags, &d));
+    a.xy = (const double *)d;
+    void *o;
+    TRY(stage_out(c, kSlotOut, out, sizeof(double) * width * npts, flags, false, &o));
+    a.out = (double *)o;
+    TRY(stage_out(c, kSlotEStatus, status, sizeof(int32_t) * npts, flags, false, &o));
+    a.status = (int32_t *)o;
+    HIPCHK(c, launch_eval_fp64(a, c->stream));
+    TRY(copy_back(c, out, a.out, sizeof(double) * width * npts, flags));
+    TRY(copy_back(c, status, a.status, sizeof(int32_t) * npts, flags));
+    if (!(flags & OKA_ASYNC) || !(flags & OKA_PTR_DEVICE))
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OKA_OK;
+}
+
+int oka_fit_draw(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+                 const double *b, const double *q, const double *z, double *ev, double *cov,
+                 double *coef, int32_t *status, uint32_t flags)
+{
+    if (!c || !site)
+        return OKA_ERR_ARG;
+    if (nreal < 0 || site->nwells < 0 || site->nobs < 0)
+        return fail(c, OKA_ERR_ARG, "negative sizes");
+    if (nreal == 0)
+        return OKA_OK;
+    if (!K || !b || (site->nwells > 0 && (!q || !site->wells_xy)) || !site->obs)
+        return fail(c, OKA_ERR_ARG, "missing fit inputs");
+    if (site->nobs < 6)
+        return fail(c, OKA_ERR_LINALG, "fewer than 6 observations: the fit is underdetermined");
+    HIPCHK(c, hipSetDevice(c->device));
+    FitArgs a{};
+    a.nwells = site->nwells;
+    a.nobs = site->nobs;
+    a.nreal = nreal;
+    a.xo = site->xo;
+    a.yo = site->yo;
+    a.base = site->base;
+    const void *d;
+    TRY(stage_in(c, kSlotWells, site->wells_xy, sizeof(double) * 2 * site->nwells, flags, &d));
+    a.wells_xy = (const double *)d;
+    TRY(stage_in(c, kSlotObs, site->obs, sizeof(double) * 4 * site->nobs, flags, &d));
+    a.obs = (const double *)d;
+    TRY(stage_in(c, kSlotK, K, sizeof(double) * nreal, flags, &d));
+    a.K = (const double *)d;
+    TRY(stage_in(c, kSlotB, b, sizeof(double) * nreal, flags, &d));
+    a.b = (const double *)d;
+    TRY(stage_in(c, kSlotQ, q, sizeof(double) * nreal * site->nwells, flags, &d));
+    a.q = (const double *)d;
+    TRY(stage_in(c, kSlotZ, z, sizeof(double) * 6 * nreal, flags, &d));
+    a.z = (const double *)d;
+    void *o;
+    TRY(ensure(c, kSlotLogs, sizeof(double) * site->nobs * std::max(site->nwells, 1), &o));
+    a.logr2 = (const double *)o;
+    TRY(stage_out(c, kSlotEv, ev, sizeof(double) * 6 * nreal, flags, false, &o));
+    a.ev = (double *)o;
+    TRY(stage_out(c, kSlotCov, cov, sizeof(double) * 36 * nreal, flags, false, &o));
+    a.cov = (double *)o;
+    TRY(stage_out(c, kSlotCoef, coef, sizeof(double) * 6 * nreal, flags, false, &o));
+    a.coef = (double *)o;
+    // statuses are always produced, so that the call can report the first failure
+    int32_t *dst = status;
+    uint32_t sflags = flags;
+    std::vector<int32_t> host_status;
+    if (!status) {
+        host_status.resize(nreal);
+        dst = host_status.data();
+        sflags = 0;
+    }
+    TRY(stage_out(c, kSlotFStatus, dst, sizeof(int32_t) * nreal, sflags, false, &o));
+    a.status = (int32_t *)o;
+    HIPCHK(c, launch_fit_logs(a, (double *)a.logr2, c->stream));
+    HIPCHK(c, launch_fit_draw(a, c->stream));
+    TRY(copy_back(c, ev, a.ev, sizeof(double) * 6 * nreal, flags));
+    TRY(copy_back(c, cov, a.cov, sizeof(double) * 36 * nreal, flags));
+    TRY(copy_back(c, coef, a.coef, sizeof(double) * 6 * nreal, flags));
+    TRY(copy_back(c, dst, a.status, sizeof(int32_t) * nreal, sflags));
+    if ((flags & OKA_ASYNC) && (sflags & OKA_PTR_DEVICE))
+        return OKA_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int32_t *hs = dst;
+    std::vector<int32_t> tmp;
+    if (sflags & OKA_PTR_DEVICE) {
+        tmp.resize(nreal);
+        HIPCHK(c, hipMemcpy(tmp.data(), a.status, sizeof(int32_t) * nreal, hipMemcpyDeviceToHost));
+        hs = tmp.data();
+    }
+    for (int32_t r = 0; r < nreal; ++r) {
+        if (hs[r] == OKA_ERR_RANGE)
+            return fail(c, OKA_ERR_RANGE, "model.fit_coeficient: elevation < base (realization %d)", r);
+        if (hs[r] == OKA_ERR_LINALG)
+            return fail(c, OKA_ERR_LINALG,
+                        "normal matrix or covariance not positive definite (realization %d)", r);
+    }
+    return OKA_OK;
+}
+
+int oka_grid_history(const oka_grid *init, int64_t ntraces, const double *bbox,
+                     oka_grid *final_grid, oka_epoch *epochs, int64_t max_epochs,
+                     int64_t *nepochs)
+{
+    // probabilityfield.py:196-261 applied to each trace's bbox in order.
+    if (!init || !final_grid || !nepochs || (ntraces > 0 && !bbox) || ntraces < 0)
+        return OKA_ERR_ARG;
+    if (!(init->dx > 0.0) || !(init->dy > 0.0))
+        return OKA_ERR_RANGE;
+    struct St {
+        double xmin, xmax, ymin, ymax;
+        int64_t nrows, ncols, cl, rb;   // cl/rb: cumulative left/bottom growth
+    } s{init->xmin, init->xmax, init->ymin, init->ymax, init->nrows, init->ncols, 0, 0};
+    const double dx = init->dx, dy = init->dy;
+    std::vector<int64_t> first;
+    std::vector<St> states;
+    bool empty = init->nrows == 0 || init->ncols == 0;
+    if (!empty) {
+        first.push_back(0);
+        states.push_back(s);
+    }
+    for (int64_t k = 0; k < ntraces; ++k) {
+        const double bx0 = bbox[4 * k], bx1 = bbox[4 * k + 1];
+        const double by0 = bbox[4 * k + 2], by1 = bbox[4 * k + 3];
+        if (bx0 > bx1 || by0 > by1)
+            return OKA_ERR_RANGE;
+        bool changed = false;
+        if (empty) {
+            s.xmin = bx0 - dx;
+            s.ymin = by0 - dy;
+            const double nc = ceil((bx1 - s.xmin) / dx) + 2.0;
+            const double nr = ceil((by1 - s.ymin) / dy) + 2.0;
+            s.ncols = nc > 3.0 ? (int64_t)nc : 3;
+            s.nrows = nr > 3.0 ? (int64_t)nr : 3;
+            s.xmax = s.xmin + (double)(s.ncols - 1) * dx;
+            s.ymax = s.ymin + (double)(s.nrows - 1) * dy;
+            empty = false;
+            changed = true;
+        } else {
+            while (bx0 <= s.xmin) { s.xmin -= dx; s.ncols += 1; s.cl += 1; changed = true; }
+            while (bx1 >= s.xmax) { s.xmax += dx; s.ncols += 1; changed = true; }
+            while (by0 <= s.ymin) { s.ymin -= dy; s.nrows += 1; s.rb += 1; changed = true; }
+            while (by1 >= s.ymax) { s.ymax += dy; s.nrows += 1; changed = true; }
+        }
+        if (changed) {
+            if (!first.empty() && first.back() == k) {
+                states.back() = s;
+            } else {
+                first.push_back(k);
+                states.push_back(s);
+            }
+        }
+    }
+    if (s.nrows > INT32_MAX || s.ncols > INT32_MAX)
+        return OKA_ERR_RANGE;
+    final_grid->xmin = s.xmin;
+    final_grid->ymin = s.ymin;
+    final_grid->xmax = s.xmax;
+    final_grid->ymax = s.ymax;
+    final_grid->dx = dx;
+    final_grid->dy = dy;
+    final_grid->nrows = (int32_t)s.nrows;
+    final_grid->ncols = (int32_t)s.ncols;
+    *nepochs = (int64_t)first.size();
+    if (!epochs)
+        return OKA_OK;
+    if (max_epochs < (int64_t)first.size())
+        return OKA_ERR_CAPACITY;
+    for (size_t e = 0; e < first.size(); ++e) {
+        const St &t = states[e];
+        epochs[e].first_trace = first[e];
+        epochs[e].xmin = t.xmin;
+        epochs[e].ymin = t.ymin;
+        epochs[e].nrows = (int32_t)t.nrows;
+        epochs[e].ncols = (int32_t)t.ncols;
+        epochs[e].row_off = (int32_t)(s.rb - t.rb);
+        epochs[e].col_off = (int32_t)(s.cl - t.cl);
+    }
+    return OKA_OK;
+}
+
+static int trace_common(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+                        const double *n, const double *b, const double *q, const double *coef,
+                        const oka_trace_opts *o, uint32_t flags, TraceArgs &a)
+{
+    TRY(check_opts(c, o));
+    if (!site || nreal < 0 || site->nwells < 0)
+        return fail(c, OKA_ERR_ARG, "bad site or realization count");
+    if (nreal > 0 && (!K || !n || !b || !coef || (site->nwells > 0 && (!q || !site->wells_xy))))
+        return fail(c, OKA_ERR_ARG, "missing realization inputs");
+    HIPCHK(c, hipSetDevice(c->device));
+    a.nwells = site->nwells;
+    a.npaths = o->npaths;
+    a.xo = site->xo;
+    a.yo = site->yo;
+    a.nreal = nreal;
+    a.confined = o->confined ? 1 : 0;
+    a.duration = o->duration;
+    a.tol = o->tol;
+    a.maxstep = o->maxstep;
+    a.umbra = o->umbra;
+    a.max_steps = o->max_steps > 0 ? o->max_steps : kDefaultMaxSteps;
+    const void *d;
+    TRY(stage_in(c, kSlotWells, site->wells_xy, sizeof(double) * 2 * site->nwells, flags, &d));
+    a.wells_xy = (const double *)d;
+    TRY(stage_in(c, kSlotStart, o->start_xy, sizeof(double) * 2 * o->npaths, flags, &d));
+    a.start_xy = (const double *)d;
+    TRY(stage_in(c, kSlotK, K, sizeof(double) * nreal, flags, &d));
+    a.K = (const double *)d;
+    TRY(stage_in(c, kSlotN, n, sizeof(double) * nreal, flags, &d));
+    a.n = (const double *)d;
+    TRY(stage_in(c, kSlotB, b, sizeof(double) * nreal, flags, &d));
+    a.b = (const double *)d;
+    TRY(stage_in(c, kSlotQ, q, sizeof(double) * nreal * site->nwells, flags, &d));
+    a.q = (const double *)d;
+    TRY(stage_in(c, kSlotCoef, coef, sizeof(double) * 6 * nreal, flags, &d));
+    a.coef = (const double *)d;
+    return OKA_OK;
+}
+
+int oka_trace_chronicle(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+                        const double *n, const double *b, const double *q, const double *coef,
+                        const oka_trace_opts *o, const oka_grid *g, int32_t nepochs,
+                        const oka_epoch *epochs, int32_t rasterize, int32_t *counts,
+                        double *bbox, int32_t *trace_status, int64_t *trace_steps,
+                        oka_trace_stats *stats, uint32_t flags)
+{
+    if (!c)
+        return OKA_ERR_ARG;
+    if (!g)
+        return fail(c, OKA_ERR_ARG, "grid is NULL");
+    if (rasterize && (g->nrows <= 0 || g->ncols <= 0 || !counts))
+        return fail(c, OKA_ERR_ARG, "rasterization needs a non-empty grid and counts");
+    if (rasterize && (!(g->dx > 0.0) || !(g->dy > 0.0)))
+        return fail(c, OKA_ERR_RANGE, "grid spacing must be > 0");
+    if (nepochs < 0 || (nepochs > 0 && !epochs))
+        return fail(c, OKA_ERR_ARG, "bad epochs");
+    TraceArgs a{};
+    TRY(trace_common(c, site, nreal, K, n, b, q, coef, o, flags, a));
+    const int64_t ntr = (int64_t)nreal * o->npaths;
+    if (ntr == 0) {
+        if (stats)
+            memset(stats, 0, sizeof(*stats));
+        return OKA_OK;
+    }
+    a.gxmin = g->xmin;
+    a.gymin = g->ymin;
+    a.gxmax = g->xmax;
+    a.gymax = g->ymax;
+    a.gdx = g->dx;
+    a.gdy = g->dy;
+    a.gnrows = g->nrows;
+    a.gncols = g->ncols;
+    a.rasterize = rasterize ? 1 : 0;
+    const int64_t nnodes = (int64_t)g->nrows * g->ncols;
+    a.nwords = rasterize ? (int32_t)((nnodes + 31) / 32) : 0;
+    // epochs: the caller's, or the whole grid as one epoch
+    oka_epoch whole{0, g->xmin, g->ymin, g->nrows, g->ncols, 0, 0};
+    const void *d;
+    if (nepochs == 0) {
+        TRY(stage_in(c, kSlotEpochs, &whole, sizeof(whole), 0, &d));
+        a.nepochs = 1;
+    } else {
+        TRY(stage_in(c, kSlotEpochs, epochs, sizeof(oka_epoch) * nepochs, flags, &d));
+        a.nepochs = nepochs;
+    }
+    a.epochs = (const oka_epoch *)d;
+    void *o2;
+    TRY(stage_out(c, kSlotCounts, rasterize ? counts : nullptr, sizeof(int32_t) * nnodes, flags,
+                  true, &o2));
+    a.counts = (int32_t *)o2;
+    TRY(stage_out(c, kSlotBbox, bbox, sizeof(double) * 4 * ntr, flags, false, &o2));
+    a.bbox = (double *)o2;
+    TRY(stage_out(c, kSlotTStatus, trace_status, sizeof(int32_t) * ntr, flags, false, &o2));
+    a.trace_status = (int32_t *)o2;
+    TRY(stage_out(c, kSlotTSteps, trace_steps, sizeof(int64_t) * ntr, flags, false, &o2));
+    a.trace_steps = (int64_t *)o2;
+    TRY(ensure(c, kSlotStats, sizeof(unsigned long long) * kStatCount, &o2));
+    a.stats = (unsigned long long *)o2;
+    TRY(fill_u32(c, a.stats, 2 * kStatCount));  // 64-bit slots
+
+    // launch shape: one realization per 256-lane workgroup, grid-stride
+    const bool lds_bitmap = (size_t)a.nwords * 4 <= (size_t)kLdsBitmapMaxWords * 4 &&
+                            trace_fp64_lds_bytes(a.nwells, a.nwords, true) <= c->max_lds;
+    int blocks = nreal;
+    if (rasterize && !lds_bitmap) {
+        // per-block bitmaps in HBM: bound the resident set
+        const int cap_blocks = std::max(1, c->num_cu * 8);
+        blocks = std::min(nreal, cap_blocks);
+        TRY(ensure(c, kSlotBitmap, sizeof(uint32_t) * (size_t)a.nwords * blocks, &o2));
+        a.gbitmap = (uint32_t *)o2;
+    }
+    if (trace_fp64_lds_bytes(a.nwells, 0, false) > c->max_lds)
+        return fail(c, OKA_ERR_ARG, "too many wells for LDS (%d)", a.nwells);
+    const bool timed = stats != nullptr;
+    if (timed)
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (o->precision == OKA_FP64_EXACT)
+        HIPCHK(c, launch_trace_fp64(a, blocks, lds_bitmap, c->stream));
+    else
+        HIPCHK(c, launch_trace_fp32(a, blocks, lds_bitmap, c->stream));
+    if (timed)
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    TRY(copy_back(c, rasterize ? counts : nullptr, a.counts, sizeof(int32_t) * nnodes, flags));
+    TRY(copy_back(c, bbox, a.bbox, sizeof(double) * 4 * ntr, flags));
+    TRY(copy_back(c, trace_status, a.trace_status, sizeof(int32_t) * ntr, flags));
+    TRY(copy_back(c, trace_steps, a.trace_steps, sizeof(int64_t) * ntr, flags));
+    if ((flags & OKA_ASYNC) && (flags & OKA_PTR_DEVICE) && !stats)
+        return OKA_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (stats) {
+        unsigned long long h[kStatCount];
+        HIPCHK(c, hipMemcpy(h, a.stats, sizeof(h), hipMemcpyDeviceToHost));
+        stats->attempted = (int64_t)h[kStatAttempted];
+        stats->accepted = (int64_t)h[kStatAccepted];
+        for (int j = 0; j < 4; ++j)
+            stats->status_count[j] = (int64_t)h[kStatStatus0 + j];
+        stats->escaped = (int64_t)h[kStatEscaped];
+        stats->visited = (int64_t)h[kStatVisited];
+        stats->candidates = (int64_t)h[kStatCandidates];
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats->kernel_ms = ms;
+    }
+    return OKA_OK;
+}
+
+int oka_trace_vertices(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+                       const double *n, const double *b, const double *q, const double *coef,
+                       const oka_trace_opts *o, int64_t cap, double *verts, int64_t *nverts,
+                       int32_t *trace_status, int64_t *trace_steps, uint32_t flags)
+{
+    if (!c)
+        return OKA_ERR_ARG;
+    if (cap < 0 || (cap > 0 && !verts) || !nverts)
+        return fail(c, OKA_ERR_ARG, "bad vertex buffers");
+    TraceArgs a{};
+    TRY(trace_common(c, site, nreal, K, n, b, q, coef, o, flags, a));
+    if (o->precision != OKA_FP64_EXACT)
+        return fail(c, OKA_ERR_ARG, "vertices are produced by the FP64-exact tracer only");
+    const int64_t ntr = (int64_t)nreal * o->npaths;
+    if (ntr == 0)
+        return OKA_OK;
+    a.gxmin = -INFINITY;
+    a.gymin = -INFINITY;
+    a.gxmax = INFINITY;
+    a.gymax = INFINITY;
+    a.nepochs = 1;
+    a.vcap = cap;
+    void *o2;
+    TRY(stage_out(c, kSlotVerts, verts, sizeof(double) * 2 * cap * ntr, flags, false, &o2));
+    a.verts = (double *)o2;
+    TRY(stage_out(c, kSlotNVerts, nverts, sizeof(int64_t) * ntr, flags, false, &o2));
+    a.nverts = (int64_t *)o2;
+    TRY(stage_out(c, kSlotTStatus, trace_status, sizeof(int32_t) * ntr, flags, false, &o2));
+    a.trace_status = (int32_t *)o2;
+    TRY(stage_out(c, kSlotTSteps, trace_steps, sizeof(int64_t) * ntr, flags, false, &o2));
+    a.trace_steps = (int64_t *)o2;
+    HIPCHK(c, launch_vertices_fp64(a, c->stream));
+    TRY(copy_back(c, verts, a.verts, sizeof(double) * 2 * cap * ntr, flags));
+    TRY(copy_back(c, nverts, a.nverts, sizeof(int64_t) * ntr, flags));
+    TRY(copy_back(c, trace_status, a.trace_status, sizeof(int32_t) * ntr, flags));
+    TRY(copy_back(c, trace_steps, a.trace_steps, sizeof(int64_t) * ntr, flags));
+    if (!(flags & OKA_ASYNC) || !(flags & OKA_PTR_DEVICE))
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OKA_OK;
+}
+
+int oka_rasterize(oka_ctx *c, const oka_grid *g, int64_t nverts, const double *xy, double umbra,
+                  uint8_t *rgrid, uint32_t flags)
+{
+    if (!c || !g)
+        return OKA_ERR_ARG;
+    if (g->nrows <= 0 || g->ncols <= 0 || !rgrid)
+        return fail(c, OKA_ERR_ARG, "empty grid");
+    if (nverts < 2)
+        return OKA_OK;
+    if (!xy)
+        return fail(c, OKA_ERR_ARG, "xy is NULL");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t nnodes = (int64_t)g->nrows * g->ncols;
+    const int64_t nwords = (nnodes + 31) / 32;
+    RasterArgs a{};
+    a.xmin = g->xmin;
+    a.ymin = g->ymin;
+    a.dx = g->dx;
+    a.dy = g->dy;
+    a.umbra = umbra;
+    a.nrows = g->nrows;
+    a.ncols = g->ncols;
+    a.nseg = nverts - 1;
+    const void *d;
+    TRY(stage_in(c, kSlotXY, xy, sizeof(double) * 2 * nverts, flags, &d));
+    a.xy = (const double *)d;
+    void *bm;
+    TRY(ensure(c, kSlotBitmap, sizeof(uint32_t) * nwords, &bm));
+    a.bitmap = (uint32_t *)bm;
+    TRY(fill_u32(c, bm, nwords));
+    HIPCHK(c, launch_raster_fp64(a, c->stream));
+    void *rg;
+    TRY(stage_out(c, kSlotRgrid, rgrid, (size_t)nnodes, flags, true, &rg));
+    hipLaunchKernelGGL(k_bits_to_u8, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0,
+                       c->stream, (const uint32_t *)bm, (uint8_t *)rg, nnodes);
+    HIPCHK(c, hipGetLastError());
+    TRY(copy_back(c, rgrid, rg, (size_t)nnodes, flags));
+    if (!(flags & OKA_ASYNC) || !(flags & OKA_PTR_DEVICE))
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OKA_OK;
+}
+
+}  // extern "C"

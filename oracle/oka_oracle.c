@@ -1,0 +1,754 @@
+/*
+ * oka_oracle.c -- CPU FP64 restatement of OnekaPy's capture-zone hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  The product is the HIP library in
+ * onekapy_amd/; nothing in the product links or calls this file.
+ *
+ * Built with -O2 -ffp-contract=off so that every a*b+c below is two rounded
+ * operations, exactly as numpy's scalar arithmetic in the reference.  Each
+ * function cites the reference lines it restates (paths relative to the
+ * OnekaPy tree).
+ */
+#include "oka_oracle.h"
+
+#include <math.h>
+#include <quadmath.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* model.py:264 and model.py:312-313 use these literals, not 1/(4 pi), 1/(2 pi). */
+#define OR_C4PI 0.07957747154594767
+#define OR_C2PI 0.15915494309189535
+
+/* capturezone.py:199 */
+#define OR_EPS 2.220446049250313e-16
+
+/* Dormand-Prince 5(4) tableau, capturezone.py:202-209.  Each entry is the
+ * correctly rounded quotient, as Python evaluates e.g. 3/40. */
+static const double A21 = 1.0 / 5.0;
+static const double A31 = 3.0 / 40.0, A32 = 9.0 / 40.0;
+static const double A41 = 44.0 / 45.0, A42 = -56.0 / 15.0, A43 = 32.0 / 9.0;
+static const double A51 = 19372.0 / 6561.0, A52 = -25360.0 / 2187.0, A53 = 64448.0 / 6561.0,
+                    A54 = -212.0 / 729.0;
+static const double A61 = 9017.0 / 3168.0, A62 = -355.0 / 33.0, A63 = 46732.0 / 5247.0,
+                    A64 = 49.0 / 176.0, A65 = -5103.0 / 18656.0;
+static const double B1 = 35.0 / 384.0, B3 = 500.0 / 1113.0, B4 = 125.0 / 192.0,
+                    B5 = -2187.0 / 6784.0, B6 = 11.0 / 84.0;
+static const double E1 = 71.0 / 57600.0, E2 = -1.0 / 40.0, E3 = -71.0 / 16695.0,
+                    E4 = 71.0 / 1920.0, E5 = -17253.0 / 339200.0, E6 = 22.0 / 525.0;
+
+/* ------------------------------------------------------------------------ */
+/* math-mode primitives                                                      */
+
+double or_pow02(double x, int32_t mathmode)
+{
+    /* (tol/(est+EPS))**(1/5): numpy float64 ** float -> C pow(x, 0.2). */
+    if (mathmode == OR_MATH_CR)
+        return (double)powq((__float128)x, (__float128)0.2);
+    return pow(x, 0.2);
+}
+
+double or_log(double x, int32_t mathmode)
+{
+    if (mathmode == OR_MATH_CR)
+        return (double)logq((__float128)x);
+    return log(x);
+}
+
+/* Python float ** 2 calls C pow(x, 2.0); its correctly rounded value is x*x. */
+static double or_sq(double x, int32_t mathmode)
+{
+    if (mathmode == OR_MATH_CR)
+        return x * x;
+    return pow(x, 2.0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* model.py                                                                  */
+
+double or_potential_wells(const or_model *m, double x, double y, int32_t mathmode)
+{
+    /* model.py:258-266 */
+    double potential = 0.0;
+    for (int32_t w = 0; w < m->nwells; ++w) {
+        const double *wl = m->wells + 4 * w;
+        double dx = x - wl[0];
+        double dy = y - wl[1];
+        double r2 = dx * dx + dy * dy;
+        potential += (wl[3] * or_log(r2, mathmode)) * OR_C4PI;
+    }
+    return potential;
+}
+
+double or_potential(const or_model *m, double x, double y, int32_t mathmode)
+{
+    /* model.py:226-237 */
+    double dx = x - m->xo;
+    double dy = y - m->yo;
+    const double *c = m->coef;
+    double potential = c[0] * or_sq(dx, mathmode) + c[1] * or_sq(dy, mathmode);
+    potential = potential + (c[2] * dx) * dy;
+    potential = potential + c[3] * dx;
+    potential = potential + c[4] * dy;
+    potential = potential + c[5];
+    potential += or_potential_wells(m, x, y, mathmode);
+    return potential;
+}
+
+void or_discharge(const or_model *m, double x, double y, double *out)
+{
+    /* model.py:299-315 */
+    double dx = x - m->xo;
+    double dy = y - m->yo;
+    const double *c = m->coef;
+    double Qx = -(((2.0 * c[0]) * dx + c[2] * dy) + c[3]);
+    double Qy = -(((2.0 * c[1]) * dy + c[2] * dx) + c[4]);
+    for (int32_t w = 0; w < m->nwells; ++w) {
+        const double *wl = m->wells + 4 * w;
+        double ddx = x - wl[0];
+        double ddy = y - wl[1];
+        double r2 = ddx * ddx + ddy * ddy;
+        double nq = -wl[3];
+        Qx += ((nq * ddx) / r2) * OR_C2PI;
+        Qy += ((nq * ddy) / r2) * OR_C2PI;
+    }
+    out[0] = Qx;
+    out[1] = Qy;
+}
+
+int32_t or_head(const or_model *m, double x, double y, int32_t mathmode, double *head)
+{
+    /* model.py:341-350 */
+    double potential = or_potential(m, x, y, mathmode);
+    double K = m->conductivity, b = m->thickness;
+    double half_kb2 = (0.5 * K) * or_sq(b, mathmode);
+    if (potential <= 0.0)
+        return -1;
+    if (potential < half_kb2)
+        *head = sqrt((2.0 * potential) / K);
+    else
+        *head = (potential + half_kb2) / (K * b);
+    return 0;
+}
+
+int32_t or_velocity(const or_model *m, double x, double y, int32_t mathmode, double *out)
+{
+    /* model.py:377-389 */
+    double Q[2], head;
+    or_discharge(m, x, y, Q);
+    if (or_head(m, x, y, mathmode, &head) != 0)
+        return -1;
+    if (head <= 0.0)
+        return -1;
+    if (head > m->thickness) {
+        out[0] = Q[0] / (m->thickness * m->porosity);
+        out[1] = Q[1] / (m->thickness * m->porosity);
+    } else {
+        out[0] = Q[0] / (head * m->porosity);
+        out[1] = Q[1] / (head * m->porosity);
+    }
+    return 0;
+}
+
+void or_velocity_confined(const or_model *m, double x, double y, double *out)
+{
+    /* model.py:423-427 */
+    double Q[2];
+    or_discharge(m, x, y, Q);
+    out[0] = Q[0] / (m->thickness * m->porosity);
+    out[1] = Q[1] / (m->thickness * m->porosity);
+}
+
+/* stochastic.py:253-260: feval(xy) = -V(xy).  Returns 0 or -1 (exception). */
+static int feval(const or_model *m, const or_trace_opts *o, double x, double y, double *k)
+{
+    double V[2];
+    if (o->confined) {
+        or_velocity_confined(m, x, y, V);
+    } else if (or_velocity(m, x, y, o->mathmode, V) != 0) {
+        return -1;
+    }
+    k[0] = -V[0];
+    k[1] = -V[1];
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* capturezone.py:127-253 (compute_backtrace)                                */
+
+static void push_vertex(double *verts, int64_t cap, int64_t *n, double x, double y)
+{
+    if (verts && *n < cap) {
+        verts[2 * *n] = x;
+        verts[2 * *n + 1] = y;
+    }
+    *n += 1;
+}
+
+int32_t or_backtrace(const or_model *m, double xs, double ys, const or_trace_opts *o,
+                     double *verts, int64_t cap, or_trace_result *res)
+{
+    const double duration = o->duration, tol = o->tol, maxstep = o->maxstep;
+    double sg = duration > 0.0 ? 1.0 : (duration < 0.0 ? -1.0 : (duration == 0.0 ? 0.0 : NAN));
+    double t = 0.0;
+    double dt = 0.1 * sg;                                     /* capturezone.py:212-213 */
+    int64_t nv = 0, attempted = 0, accepted = 0;
+    int32_t status = OR_TRACE_OK;
+    double x = xs, y = ys;
+    double k1[2], k2[2], k3[2], k4[2], k5[2], k6[2], k7[2];
+
+    push_vertex(verts, cap, &nv, xs, ys);                     /* capturezone.py:215 */
+    if (feval(m, o, x, y, k1) != 0) {                         /* capturezone.py:219 */
+        status = OR_TRACE_TRUNCATED;
+        goto done;
+    }
+    while (fabs(t) < fabs(duration)) {                        /* capturezone.py:221 */
+        if (o->max_steps > 0 && attempted >= o->max_steps) {
+            status = OR_TRACE_STEP_CAP;
+            break;
+        }
+        if (fabs(t + dt) > fabs(duration))                    /* capturezone.py:223-224 */
+            dt = duration - t;
+        attempted += 1;
+
+        /* capturezone.py:227-231 */
+        if (feval(m, o, x + dt * (A21 * k1[0]), y + dt * (A21 * k1[1]), k2)) goto trunc;
+        if (feval(m, o, x + dt * (A31 * k1[0] + A32 * k2[0]),
+                  y + dt * (A31 * k1[1] + A32 * k2[1]), k3)) goto trunc;
+        if (feval(m, o, x + dt * ((A41 * k1[0] + A42 * k2[0]) + A43 * k3[0]),
+                  y + dt * ((A41 * k1[1] + A42 * k2[1]) + A43 * k3[1]), k4)) goto trunc;
+        if (feval(m, o, x + dt * (((A51 * k1[0] + A52 * k2[0]) + A53 * k3[0]) + A54 * k4[0]),
+                  y + dt * (((A51 * k1[1] + A52 * k2[1]) + A53 * k3[1]) + A54 * k4[1]), k5))
+            goto trunc;
+        if (feval(m, o,
+                  x + dt * ((((A61 * k1[0] + A62 * k2[0]) + A63 * k3[0]) + A64 * k4[0]) + A65 * k5[0]),
+                  y + dt * ((((A61 * k1[1] + A62 * k2[1]) + A63 * k3[1]) + A64 * k4[1]) + A65 * k5[1]),
+                  k6))
+            goto trunc;
+
+        /* capturezone.py:233 (a7[1] == 0 is not in the sum) */
+        double xt = x + dt * ((((B1 * k1[0] + B3 * k3[0]) + B4 * k4[0]) + B5 * k5[0]) + B6 * k6[0]);
+        double yt = y + dt * ((((B1 * k1[1] + B3 * k3[1]) + B4 * k4[1]) + B5 * k5[1]) + B6 * k6[1]);
+
+        /* capturezone.py:236: k2 is re-bound to feval(xyt); called k7 here. */
+        if (feval(m, o, xt, yt, k7)) goto trunc;
+
+        /* capturezone.py:237-239 */
+        double ex = dt * (((((E1 * k1[0] + E2 * k7[0]) + E3 * k3[0]) + E4 * k4[0]) + E5 * k5[0]) + E6 * k6[0]);
+        double ey = dt * (((((E1 * k1[1] + E2 * k7[1]) + E3 * k3[1]) + E4 * k4[1]) + E5 * k5[1]) + E6 * k6[1]);
+        double ax = fabs(ex), ay = fabs(ey);
+        double est = (isnan(ax) || isnan(ay)) ? NAN : (ay > ax ? ay : ax);
+        double d0 = xt - x, d1 = yt - y;
+        /* np.linalg.norm of a 2-vector = sqrt(ddot) and OpenBLAS ddot contracts. */
+        double ds = sqrt(fma(d1, d1, d0 * d0));
+
+        if (est < tol && ds < maxstep) {                      /* capturezone.py:241-245 */
+            t = t + dt;
+            k1[0] = k7[0];
+            k1[1] = k7[1];
+            x = xt;
+            y = yt;
+            accepted += 1;
+            push_vertex(verts, cap, &nv, x, y);
+        }
+
+        /* capturezone.py:247: Python min() keeps the first of equal/unordered. */
+        double c = or_pow02(tol / (est + OR_EPS), o->mathmode);
+        double c2 = maxstep / (ds + OR_EPS);
+        if (c2 < c) c = c2;
+        if (10.0 < c) c = 10.0;
+        dt = (0.9 * c) * dt;
+        if (isnan(dt) || dt == 0.0) {   /* the reference would never leave the loop */
+            status = OR_TRACE_STALLED;
+            break;
+        }
+    }
+    goto done;
+trunc:
+    status = OR_TRACE_TRUNCATED;                              /* capturezone.py:249-253 */
+done:
+    if (res) {
+        res->attempted = attempted;
+        res->accepted = accepted;
+        res->nverts = nv;
+        res->t_end = t;
+        res->status = status;
+    }
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* probabilityfield.py                                                       */
+
+int32_t or_field_init(or_field *f, double dx, double dy, double xo, double yo)
+{
+    /* probabilityfield.py:125-151 */
+    memset(f, 0, sizeof(*f));
+    if (dx <= 0.0 || dy <= 0.0)
+        return -1;
+    f->deltax = dx;
+    f->deltay = dy;
+    if (isnan(xo) || isnan(yo)) {
+        f->nrows = 0;
+        f->ncols = 0;
+        return 0;
+    }
+    f->xmin = xo - dx;
+    f->xmax = xo + dx;
+    f->ymin = yo - dy;
+    f->ymax = yo + dy;
+    f->nrows = 3;
+    f->ncols = 3;
+    f->pgrid = (double *)calloc(9, sizeof(double));
+    f->rgrid = (uint8_t *)calloc(9, 1);
+    f->total_weight = 0.0;
+    return 0;
+}
+
+void or_field_free(or_field *f)
+{
+    free(f->pgrid);
+    free(f->rgrid);
+    f->pgrid = NULL;
+    f->rgrid = NULL;
+}
+
+int32_t or_field_expand(or_field *f, double xmin, double xmax, double ymin, double ymax)
+{
+    /* probabilityfield.py:196-261 */
+    if (xmin > xmax || ymin > ymax)
+        return -1;
+    if (f->ncols == 0 || f->nrows == 0) {
+        f->xmin = xmin - f->deltax;
+        f->ymin = ymin - f->deltay;
+        double nc = ceil((xmax - f->xmin) / f->deltax) + 2.0;
+        double nr = ceil((ymax - f->ymin) / f->deltay) + 2.0;
+        f->ncols = nc > 3.0 ? (int64_t)nc : 3;
+        f->nrows = nr > 3.0 ? (int64_t)nr : 3;
+        f->xmax = f->xmin + (double)(f->ncols - 1) * f->deltax;
+        f->ymax = f->ymin + (double)(f->nrows - 1) * f->deltay;
+        free(f->pgrid);
+        free(f->rgrid);
+        f->pgrid = (double *)calloc((size_t)(f->nrows * f->ncols), sizeof(double));
+        f->rgrid = (uint8_t *)calloc((size_t)(f->nrows * f->ncols), 1);
+        f->total_weight = 0.0;
+        return 0;
+    }
+    int64_t nrows = f->nrows, ncols = f->ncols, rshift = 0, cshift = 0;
+    while (xmin <= f->xmin) { f->xmin -= f->deltax; ncols += 1; cshift += 1; }
+    while (xmax >= f->xmax) { f->xmax += f->deltax; ncols += 1; }
+    while (ymin <= f->ymin) { f->ymin -= f->deltay; nrows += 1; rshift += 1; }
+    while (ymax >= f->ymax) { f->ymax += f->deltay; nrows += 1; }
+    if (nrows != f->nrows || ncols != f->ncols) {
+        double *pg = (double *)calloc((size_t)(nrows * ncols), sizeof(double));
+        uint8_t *rg = (uint8_t *)calloc((size_t)(nrows * ncols), 1);
+        for (int64_t i = 0; i < f->nrows; ++i) {
+            memcpy(pg + (i + rshift) * ncols + cshift, f->pgrid + i * f->ncols,
+                   (size_t)f->ncols * sizeof(double));
+            memcpy(rg + (i + rshift) * ncols + cshift, f->rgrid + i * f->ncols, (size_t)f->ncols);
+        }
+        free(f->pgrid);
+        free(f->rgrid);
+        f->pgrid = pg;
+        f->rgrid = rg;
+        f->nrows = nrows;
+        f->ncols = ncols;
+    }
+    return 0;
+}
+
+double or_distancesquared(double ax, double ay, double bx, double by, double cx, double cy)
+{
+    /* probabilityfield.py:407-427 */
+    double bax = bx - ax, bay = by - ay;
+    double cax = cx - ax, cay = cy - ay;
+    double perpdot = bax * cay - bay * cax;
+    double dot = bax * cax + bay * cay;
+    double length2 = bax * bax + bay * bay;
+    double alpha2 = (perpdot * perpdot) / length2;
+    double beta2 = (dot * dot) / length2;
+    if (dot < 0.0)
+        return alpha2 + beta2;
+    if (beta2 > length2)
+        return ((alpha2 + beta2) - 2.0 * dot) + length2;
+    return alpha2;
+}
+
+/* Candidate index range of probabilityfield.py:298-301 along one axis:
+ * [max(0, floor((lo - umbra - org)/d)), min(n, floor((hi + umbra - org)/d) + 1)). */
+static void cand_range(double lo, double hi, double umbra, double org, double d, int64_t n,
+                       int64_t *first, int64_t *last)
+{
+    double a = floor(((lo - umbra) - org) / d);
+    double b = floor(((hi + umbra) - org) / d) + 1.0;
+    if (a < 0.0) a = 0.0;
+    if (b > (double)n) b = (double)n;
+    *first = (int64_t)a;
+    *last = b > a ? (int64_t)b : *first;
+}
+
+/* probabilityfield.py:292-310 on an explicit geometry; marks into rgrid. */
+static void insert_geom(uint8_t *rgrid, double xmin, double ymin, double dx, double dy,
+                        int64_t nrows, int64_t ncols, double ax, double ay, double bx,
+                        double by, double umbra)
+{
+    double u2 = umbra * umbra;
+    double lox = bx < ax ? bx : ax, hix = bx > ax ? bx : ax;
+    double loy = by < ay ? by : ay, hiy = by > ay ? by : ay;
+    int64_t left, right, bottom, top;
+    cand_range(lox, hix, umbra, xmin, dx, ncols, &left, &right);
+    cand_range(loy, hiy, umbra, ymin, dy, nrows, &bottom, &top);
+    for (int64_t j = left; j < right; ++j) {
+        for (int64_t i = bottom; i < top; ++i) {
+            uint8_t *cell = rgrid + i * ncols + j;
+            if (!*cell) {
+                double cx = xmin + (double)j * dx;
+                double cy = ymin + (double)i * dy;
+                if (or_distancesquared(ax, ay, bx, by, cx, cy) < u2)
+                    *cell = 1;
+            }
+        }
+    }
+}
+
+void or_field_insert(or_field *f, double ax, double ay, double bx, double by, double umbra)
+{
+    insert_geom(f->rgrid, f->xmin, f->ymin, f->deltax, f->deltay, f->nrows, f->ncols, ax, ay, bx,
+                by, umbra);
+}
+
+int32_t or_field_rasterize(or_field *f, const double *xy, int64_t n, double umbra)
+{
+    /* probabilityfield.py:334-339 */
+    if (n <= 0)
+        return -1;   /* min() of an empty list raises in the reference */
+    double xmn = xy[0], xmx = xy[0], ymn = xy[1], ymx = xy[1];
+    for (int64_t k = 1; k < n; ++k) {
+        double x = xy[2 * k], y = xy[2 * k + 1];
+        if (x < xmn) xmn = x;
+        if (x > xmx) xmx = x;
+        if (y < ymn) ymn = y;
+        if (y > ymx) ymx = y;
+    }
+    if (or_field_expand(f, xmn, xmx, ymn, ymx) != 0)
+        return -1;
+    for (int64_t k = 0; k + 1 < n; ++k)
+        or_field_insert(f, xy[2 * k], xy[2 * k + 1], xy[2 * k + 2], xy[2 * k + 3], umbra);
+    return 0;
+}
+
+void or_field_register(or_field *f, double weight)
+{
+    /* probabilityfield.py:357-359 */
+    f->total_weight += weight;
+    int64_t nn = f->nrows * f->ncols;
+    for (int64_t k = 0; k < nn; ++k) {
+        if (f->rgrid[k])
+            f->pgrid[k] += weight;
+        f->rgrid[k] = 0;
+    }
+}
+
+void or_field_reset(or_field *f)
+{
+    /* probabilityfield.py:376 */
+    memset(f->rgrid, 0, (size_t)(f->nrows * f->ncols));
+}
+
+/* ------------------------------------------------------------------------ */
+/* capturezone.py:51-123 (compute_capturezone)                               */
+
+int32_t or_capturezone(const or_model *m, int32_t npaths, const double *start_xy, or_field *f,
+                       double umbra, double weight, const or_trace_opts *o, int64_t *stats)
+{
+    int64_t cap = 1024;
+    double *verts = (double *)malloc((size_t)cap * 2 * sizeof(double));
+    for (int32_t p = 0; p < npaths; ++p) {
+        or_trace_result r;
+        for (;;) {
+            or_backtrace(m, start_xy[2 * p], start_xy[2 * p + 1], o, verts, cap, &r);
+            if (r.nverts <= cap)
+                break;
+            cap = r.nverts;
+            free(verts);
+            verts = (double *)malloc((size_t)cap * 2 * sizeof(double));
+        }
+        if (stats) {
+            stats[0] += r.attempted;
+            stats[1] += r.accepted;
+            stats[2 + r.status] += 1;
+        }
+        if (or_field_rasterize(f, verts, r.nverts, umbra) != 0) {
+            free(verts);
+            return -1;
+        }
+    }
+    or_field_register(f, weight);                             /* capturezone.py:123 */
+    free(verts);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* model.py:430-606 (fit_regional_flow), normal-equation restatement          */
+
+static int chol6(const double *A, double *L)
+{
+    /* Lower Cholesky factor of the symmetric 6x6 A (row-major, lower part read). */
+    memset(L, 0, 36 * sizeof(double));
+    for (int j = 0; j < 6; ++j) {
+        double d = A[j * 6 + j];
+        for (int k = 0; k < j; ++k)
+            d = d - L[j * 6 + k] * L[j * 6 + k];
+        if (!(d > 0.0))
+            return -2;
+        double ljj = sqrt(d);
+        L[j * 6 + j] = ljj;
+        for (int i = j + 1; i < 6; ++i) {
+            double v = A[i * 6 + j];
+            for (int k = 0; k < j; ++k)
+                v = v - L[i * 6 + k] * L[j * 6 + k];
+            L[i * 6 + j] = v / ljj;
+        }
+    }
+    return 0;
+}
+
+static void chol6_solve(const double *L, const double *s, const double *g, double *out)
+{
+    /* out = S (L L^T)^{-1} S g for diagonal S = diag(s). */
+    double w[6], y[6];
+    for (int j = 0; j < 6; ++j) {
+        double v = s[j] * g[j];
+        for (int k = 0; k < j; ++k)
+            v = v - L[j * 6 + k] * w[k];
+        w[j] = v / L[j * 6 + j];
+    }
+    for (int j = 5; j >= 0; --j) {
+        double v = w[j];
+        for (int k = j + 1; k < 6; ++k)
+            v = v - L[k * 6 + j] * y[k];
+        y[j] = v / L[j * 6 + j];
+    }
+    for (int j = 0; j < 6; ++j)
+        out[j] = s[j] * y[j];
+}
+
+/* One weighted row of model.py:545-565.  Returns 0 or -1 (RangeError). */
+static int fit_row(const or_model *m, const double *ob, double xo, double yo, int32_t mathmode,
+                   double *a, double *wb)
+{
+    double x = ob[0], y = ob[1], z_ev = ob[2], z_std = ob[3];
+    double K = m->conductivity, b = m->thickness;
+    double head = z_ev - m->base;
+    double pot_ev, pot_std;
+    if (head >= b) {
+        pot_ev = (K * b) * (head - 0.5 * b);
+        pot_std = (K * b) * z_std;
+    } else if (head > 0.0) {
+        pot_ev = (0.5 * K) * (or_sq(head, mathmode) + or_sq(z_std, mathmode));
+        pot_std = (K * head) * z_std;
+    } else {
+        return -1;
+    }
+    double dx = x - xo, dy = y - yo;
+    a[0] = or_sq(dx, mathmode) / pot_std;
+    a[1] = or_sq(dy, mathmode) / pot_std;
+    a[2] = (dx * dy) / pot_std;
+    a[3] = dx / pot_std;
+    a[4] = dy / pot_std;
+    a[5] = 1.0 / pot_std;
+    *wb = (pot_ev - or_potential_wells(m, x, y, mathmode)) / pot_std;
+    return 0;
+}
+
+int32_t or_fit(const or_model *m, int32_t nobs, const double *obs, double xo, double yo,
+               int32_t mathmode, double *ev, double *cov)
+{
+    double N[36] = {0}, rhs[6] = {0};
+    double *rows = (double *)malloc((size_t)(nobs > 0 ? nobs : 1) * 7 * sizeof(double));
+    for (int32_t i = 0; i < nobs; ++i) {
+        double *a = rows + 7 * i;
+        if (fit_row(m, obs + 4 * i, xo, yo, mathmode, a, a + 6) != 0) {
+            free(rows);
+            return -1;
+        }
+        for (int j = 0; j < 6; ++j) {
+            for (int k = 0; k <= j; ++k)
+                N[j * 6 + k] = N[j * 6 + k] + a[j] * a[k];
+            rhs[j] = rhs[j] + a[j] * a[6];
+        }
+    }
+    double s[6], Ns[36], L[36];
+    for (int j = 0; j < 6; ++j)
+        s[j] = 1.0 / sqrt(N[j * 6 + j]);
+    for (int j = 0; j < 6; ++j)
+        for (int k = 0; k <= j; ++k)
+            Ns[j * 6 + k] = (s[j] * N[j * 6 + k]) * s[k];
+    if (chol6(Ns, L) != 0) {
+        free(rows);
+        return -2;
+    }
+    chol6_solve(L, s, rhs, ev);
+    /* One step of iterative refinement on the residual of the weighted rows. */
+    double g[6] = {0}, d[6];
+    for (int32_t i = 0; i < nobs; ++i) {
+        const double *a = rows + 7 * i;
+        double r = a[6];
+        for (int j = 0; j < 6; ++j)
+            r = r - a[j] * ev[j];
+        for (int j = 0; j < 6; ++j)
+            g[j] = g[j] + a[j] * r;
+    }
+    chol6_solve(L, s, g, d);
+    for (int j = 0; j < 6; ++j)
+        ev[j] = ev[j] + d[j];
+    /* cov = inv(WA^T WA) = S (L L^T)^{-1} S  (model.py:604) */
+    double Li[36] = {0};
+    for (int j = 0; j < 6; ++j) {
+        Li[j * 6 + j] = 1.0 / L[j * 6 + j];
+        for (int i = j + 1; i < 6; ++i) {
+            double v = 0.0;
+            for (int k = j; k < i; ++k)
+                v = v - L[i * 6 + k] * Li[k * 6 + j];
+            Li[i * 6 + j] = v / L[i * 6 + i];
+        }
+    }
+    for (int j = 0; j < 6; ++j) {
+        for (int k = 0; k <= j; ++k) {
+            double v = 0.0;
+            for (int i = j; i < 6; ++i)
+                v = v + Li[i * 6 + j] * Li[i * 6 + k];
+            double c = (s[j] * v) * s[k];
+            cov[j * 6 + k] = c;
+            cov[k * 6 + j] = c;
+        }
+    }
+    free(rows);
+    return 0;
+}
+
+int32_t or_mvn(const double *ev, const double *cov, const double *z, double *coef)
+{
+    double L[36];
+    if (chol6(cov, L) != 0)
+        return -2;
+    for (int j = 0; j < 6; ++j) {
+        double acc = 0.0;
+        for (int k = 0; k <= j; ++k)
+            acc = acc + L[j * 6 + k] * z[k];
+        coef[j] = ev[j] + acc;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* fixed-grid batch (stochastic.py:220-265 with a pre-expanded field)        */
+
+int32_t or_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+int32_t or_chronicle_fixed(int32_t nwells, const double *wells_xy, double xo, double yo,
+                           double base, int32_t nreal, const double *K, const double *n,
+                           const double *b, const double *q, const double *coef,
+                           int32_t npaths, const double *start_xy, double umbra,
+                           const or_trace_opts *o, const or_grid *g, int32_t *counts,
+                           int64_t *stats, int32_t nthreads)
+{
+    const int64_t nn = g->nrows * g->ncols;
+    int32_t err = 0;
+    for (int k = 0; k < 8; ++k)
+        stats[k] = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0)
+        nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        int64_t lst[8] = {0};
+        uint8_t *rg = (uint8_t *)calloc((size_t)nn, 1);
+        double *wl = (double *)malloc((size_t)(nwells > 0 ? nwells : 1) * 4 * sizeof(double));
+        int64_t cap = 1024;
+        double *verts = (double *)malloc((size_t)cap * 2 * sizeof(double));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int32_t r = 0; r < nreal; ++r) {
+            or_model m;
+            m.base = base;
+            m.conductivity = K[r];
+            m.porosity = n[r];
+            m.thickness = b[r];
+            m.xo = xo;
+            m.yo = yo;
+            for (int j = 0; j < 6; ++j)
+                m.coef[j] = coef[6 * r + j];
+            for (int32_t w = 0; w < nwells; ++w) {
+                wl[4 * w] = wells_xy[2 * w];
+                wl[4 * w + 1] = wells_xy[2 * w + 1];
+                wl[4 * w + 2] = 0.0;
+                wl[4 * w + 3] = q[(int64_t)r * nwells + w];
+            }
+            m.nwells = nwells;
+            m.wells = wl;
+            for (int32_t p = 0; p < npaths; ++p) {
+                or_trace_result tr;
+                for (;;) {
+                    or_backtrace(&m, start_xy[2 * p], start_xy[2 * p + 1], o, verts, cap, &tr);
+                    if (tr.nverts <= cap)
+                        break;
+                    cap = tr.nverts;
+                    free(verts);
+                    verts = (double *)malloc((size_t)cap * 2 * sizeof(double));
+                }
+                lst[0] += tr.attempted;
+                lst[1] += tr.accepted;
+                lst[2 + tr.status] += 1;
+                double xmn = verts[0], xmx = verts[0], ymn = verts[1], ymx = verts[1];
+                for (int64_t v = 1; v < tr.nverts; ++v) {
+                    double x = verts[2 * v], y = verts[2 * v + 1];
+                    if (x < xmn) xmn = x;
+                    if (x > xmx) xmx = x;
+                    if (y < ymn) ymn = y;
+                    if (y > ymx) ymx = y;
+                }
+                /* probabilityfield.py:229-245: no growth iff strictly inside. */
+                if (!(xmn > g->xmin && xmx < g->xmax && ymn > g->ymin && ymx < g->ymax))
+                    lst[6] += 1;
+                for (int64_t v = 0; v + 1 < tr.nverts; ++v)
+                    insert_geom(rg, g->xmin, g->ymin, g->dx, g->dy, g->nrows, g->ncols,
+                                verts[2 * v], verts[2 * v + 1], verts[2 * v + 2],
+                                verts[2 * v + 3], umbra);
+            }
+            for (int64_t c = 0; c < nn; ++c) {
+                if (rg[c]) {
+                    rg[c] = 0;
+                    lst[7] += 1;
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+                    counts[c] += 1;
+                }
+            }
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            for (int k = 0; k < 8; ++k)
+                stats[k] += lst[k];
+        }
+        free(rg);
+        free(wl);
+        free(verts);
+    }
+    return err;
+}

@@ -1,0 +1,188 @@
+"""GPU parity: liboka (HIP, gfx950) against the oracle on identical inputs.
+
+Bar: bit-exact for the FP64-exact path (fit, draw, vertices, counts, step
+counts); the FP32 tracer is held to max |p32 - p64| <= 0.01.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from onekapy_amd import (Model, ProbabilityField, Realizations, chronicle_into_field,
+                         compute_backtrace, compute_capturezone, create_stochastic_capturezone,
+                         fit_and_draw, sites, start_points, trace_vertices, _lib)
+from helpers import golden, oracle_chronicle, oracle_fit_draw, oracle_grid, small_site
+
+pytestmark = pytest.mark.gpu
+
+
+def _kat_model():
+    k = golden("reference_kats.json")["model"]
+    return Model(k["base"], k["conductivity"], k["porosity"], k["thickness"],
+                 [tuple(w) for w in k["wells"]], k["xo"], k["yo"], np.array(k["coef"]))
+
+
+def test_model_kats_on_gpu(gpu_ctx):
+    g = golden("reference_kats.json")
+    m = _kat_model()
+    assert np.isclose(m.compute_potential(*g["potential"]["xy"]), g["potential"]["value"], rtol=1e-6)
+    assert np.isclose(m.compute_head(*g["head"]["xy"]), g["head"]["value"], rtol=1e-6)
+    assert np.allclose(m.compute_discharge(*g["discharge"]["xy"]), g["discharge"]["value"],
+                       rtol=1e-6)
+    for case in g["velocity"]["cases"]:
+        assert np.allclose(m.compute_velocity(*case["xy"]), case["value"], rtol=1e-6)
+
+
+def test_model_eval_bit_exact_vs_oracle(gpu_ctx):
+    m = _kat_model()
+    om = O.OracleModel(m.base, m.conductivity, m.porosity, m.thickness, m.wells, m.xo, m.yo,
+                       m.coef)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-500, 800, 300)
+    y = rng.uniform(-500, 800, 300)
+    qx, qy = m.compute_discharge(x, y)
+    vx, vy = m.compute_velocity_confined(x, y)
+    pot = m.compute_potential(x, y)
+    for i in range(len(x)):
+        assert (qx[i], qy[i]) == tuple(om.discharge(x[i], y[i]))
+        assert (vx[i], vy[i]) == tuple(om.velocity_confined(x[i], y[i]))
+        assert pot[i] == om.potential(x[i], y[i], O.MATH_CR)
+
+
+def test_fit_kat_and_library_on_gpu(gpu_ctx):
+    g = golden("reference_kats.json")["fit"]
+    m = _kat_model()
+    ev, cov = m.fit_regional_flow([tuple(o) for o in g["obs"]], *g["origin"])
+    assert np.allclose(ev.ravel(), g["ev"], rtol=g["rtol"])
+    assert np.allclose(cov, g["cov"], rtol=g["rtol"])
+    # the reference's own library route: lstsq + inv on the weighted rows
+    WA, Wb = m.construct_fit([tuple(o) for o in g["obs"]], *g["origin"])
+    ev_np = np.linalg.lstsq(WA, Wb, rcond=-1)[0].ravel()
+    cov_np = np.linalg.inv(WA.T @ WA)
+    assert np.allclose(ev.ravel(), ev_np, rtol=1e-9, atol=0)
+    assert np.allclose(cov, cov_np, rtol=1e-8, atol=0)
+
+
+def test_fit_draw_bit_exact_vs_oracle(gpu_ctx):
+    s, d = small_site(nreal=64)
+    ev, cov, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
+                                 d["q"], d["z"])
+    oev, ocov, ocoef = oracle_fit_draw(s, d)
+    assert np.array_equal(ev, oev)
+    assert np.array_equal(cov, ocov)
+    assert np.array_equal(coef, ocoef)
+
+
+def _site_real(s, d, coef):
+    return Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+
+
+def test_vertices_bit_exact_vs_oracle(gpu_ctx):
+    s, d = small_site(nreal=4, npaths=24)
+    _, _, coef = oracle_fit_draw(s, d)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    verts, nverts, status = trace_vertices(s.wells_xy(), s.xt, s.yt, _site_real(s, d, coef),
+                                           starts, s.duration, s.tol, s.maxstep)
+    wxy = s.wells_xy()
+    for r in range(4):
+        om = O.OracleModel(s.base, d["K"][r], d["n"][r], d["b"][r],
+                           np.c_[wxy, np.full(len(wxy), 0.25), d["q"][r]], s.xt, s.yt, coef[r])
+        for p in range(s.npaths):
+            ov, res = om.backtrace(starts[p, 0], starts[p, 1], s.duration, s.tol, s.maxstep)
+            k = r * s.npaths + p
+            assert nverts[k] == res.nverts
+            assert status[k] == res.status == O.TRACE_OK
+            assert np.array_equal(verts[k, :nverts[k]], ov)
+
+
+def test_chronicle_fixed_grid_cfg0_bit_exact(gpu_ctx):
+    s = sites.config0()
+    d = sites.draw(s, seed=11)
+    _, _, coef = oracle_fit_draw(s, d)
+    pf = s.fixed_field()
+    real = _site_real(s, d, coef)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    res = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration, s.tol,
+                               s.maxstep, s.umbra)
+    ocounts, ost = oracle_chronicle(s, d, coef, pf)
+    assert res.passes == 1 and ost[6] == 0
+    assert np.array_equal(res.counts, ocounts)
+    assert res.stats.attempted == ost[0]
+    assert res.stats.accepted == ost[1]
+    assert res.stats.visited == ost[7]
+    assert ocounts.max() > 0
+
+
+def test_stochastic_history_matches_sequential_oracle(gpu_ctx):
+    """create_stochastic_capturezone from a fresh 3x3 field (two-pass history)
+    equals the reference's sequential rasterize/register loop."""
+    s, d = small_site(nreal=6, npaths=20)
+    pf = create_stochastic_capturezone(0, s.npaths, s.duration, s.nrealizations, s.base,
+                                       s.c_dist, s.p_dist, s.t_dist, s.wells, s.observations,
+                                       s.spacing, s.umbra, True, s.tol, s.maxstep, seed=5)
+    from onekapy_amd.stochastic import draw_realizations
+    d = draw_realizations(s.wells, s.c_dist, s.p_dist, s.t_dist, s.nrealizations,
+                          np.random.default_rng(5))
+    _, _, coef = oracle_fit_draw(s, d)
+    of = O.OracleField(s.spacing, s.spacing, s.xt, s.yt)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    opts = O.trace_opts(s.duration, s.tol, s.maxstep, True, O.MATH_CR)
+    wxy = s.wells_xy()
+    for r in range(s.nrealizations):
+        om = O.OracleModel(s.base, d["K"][r], d["n"][r], d["b"][r],
+                           np.c_[wxy, np.full(len(wxy), 0.25), d["q"][r]], s.xt, s.yt, coef[r])
+        of.capturezone(om, starts, s.umbra, 1.0, opts)
+    assert (pf.nrows, pf.ncols) == of.shape
+    assert (pf.xmin, pf.xmax, pf.ymin, pf.ymax) == (of.s.xmin, of.s.xmax, of.s.ymin, of.s.ymax)
+    assert np.array_equal(pf.pgrid, of.pgrid)
+    assert pf.total_weight == of.s.total_weight == float(s.nrealizations)
+
+
+def test_compute_capturezone_dropin(gpu_ctx):
+    """compute_capturezone into an existing field, twice, equals the oracle's
+    sequential reference loop, including growth between calls."""
+    s, d = small_site(nreal=2, npaths=16)
+    _, _, coef = oracle_fit_draw(s, d)
+    pf = ProbabilityField(s.spacing, s.spacing, s.xt, s.yt)
+    of = O.OracleField(s.spacing, s.spacing, s.xt, s.yt)
+    opts = O.trace_opts(s.duration, s.tol, s.maxstep, True, O.MATH_CR)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    wxy = s.wells_xy()
+    for r, w in zip(range(2), (1.0, 0.5)):
+        wells = [(wxy[i, 0], wxy[i, 1], 0.25, d["q"][r][i]) for i in range(len(wxy))]
+        m = Model(s.base, d["K"][r], d["n"][r], d["b"][r], wells, s.xt, s.yt, coef[r])
+        compute_capturezone(s.xt, s.yt, s.rt, s.npaths, s.duration, pf, s.umbra, w, s.tol,
+                            s.maxstep, m.velocity_function(True))
+        om = O.OracleModel(s.base, d["K"][r], d["n"][r], d["b"][r], wells, s.xt, s.yt, coef[r])
+        of.capturezone(om, starts, s.umbra, w, opts)
+        assert (pf.nrows, pf.ncols) == of.shape
+        assert np.array_equal(pf.pgrid, of.pgrid)
+
+
+def test_compute_backtrace_dropin(gpu_ctx):
+    m = _kat_model()
+    m.coef = np.array([1e-4, 2e-4, 0.0, 1e-2, -1e-2, 500.0])
+    fe = m.velocity_function(True)
+    verts = compute_backtrace(150.0, 150.0, 100.0, 1.0, 10.0, fe)
+    om = O.OracleModel(m.base, m.conductivity, m.porosity, m.thickness, m.wells, m.xo, m.yo,
+                       m.coef)
+    ov, res = om.backtrace(150.0, 150.0, 100.0, 1.0, 10.0)
+    assert len(verts) == res.nverts
+    assert np.array_equal(np.array(verts), ov)
+
+
+def test_fp32_error_cfg0(gpu_ctx):
+    s = sites.config0()
+    d = sites.draw(s, seed=11)
+    _, _, coef = oracle_fit_draw(s, d)
+    real = _site_real(s, d, coef)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    out = {}
+    for prec in (_lib.FP64_EXACT, _lib.FP32):
+        pf = s.fixed_field()
+        out[prec] = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
+                                         s.tol, s.maxstep, s.umbra, precision=prec).counts
+    p64 = out[_lib.FP64_EXACT] / s.nrealizations
+    p32 = out[_lib.FP32] / s.nrealizations
+    assert np.abs(p32 - p64).max() <= 0.01
