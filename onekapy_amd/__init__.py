@@ -19,6 +19,7 @@ from .capturezone import (ModelVelocity, compute_backtrace, compute_capturezone,
 from .stochastic import (DistributionError, Lognormal, Signed, compute_variate_mean,
                          create_stochastic_capturezone, draw_realizations, fit_and_draw,
                          generate_random_variate, isdistribution)
+from .deterministic import create_deterministic_capturezone
 from .engine import Realizations, chronicle_into_field
 from . import _lib
 
@@ -26,6 +27,7 @@ __all__ = [
     "Model", "AquiferError", "RangeError", "ProbabilityField", "ModelVelocity",
     "compute_backtrace", "compute_capturezone", "start_points", "trace_vertices",
     "DistributionError", "Lognormal", "Signed", "compute_variate_mean",
-    "create_stochastic_capturezone", "draw_realizations", "fit_and_draw",
+    "create_stochastic_capturezone", "create_deterministic_capturezone", "draw_realizations",
+    "fit_and_draw",
     "generate_random_variate", "isdistribution", "Realizations", "chronicle_into_field",
 ]

@@ -39,7 +39,9 @@ class Realizations:
         R = self.K.shape[0]
         self.n = _lib.f64(n).reshape(R)
         self.b = _lib.f64(b).reshape(R)
-        self.q = _lib.f64(q).reshape(R, -1)
+        q = _lib.f64(q)
+        nw = q.shape[-1] if q.ndim == 2 else (q.size // R if R else 0)
+        self.q = q.reshape(R, nw)
         self.coef = _lib.f64(coef).reshape(R, 6)
 
     def __len__(self):

@@ -261,7 +261,8 @@ int32_t or_backtrace(const or_model *m, double xs, double ys, const or_trace_opt
         if (c2 < c) c = c2;
         if (10.0 < c) c = 10.0;
         dt = (0.9 * c) * dt;
-        if (isnan(dt) || dt == 0.0) {   /* the reference would never leave the loop */
+        /* the reference would never leave the loop */
+        if (fabs(t) < fabs(duration) && (isnan(dt) || dt == 0.0)) {
             status = OR_TRACE_STALLED;
             break;
         }

@@ -140,7 +140,7 @@ def backtrace(xs, ys, duration, tol, maxstep, feval, max_steps=0):
                 verts.append(xy)
                 accepted += 1
             dt = 0.9 * min((tol / (est + eps)) ** (1 / 5), maxstep / (ds + eps), 10) * dt
-            if not (dt != 0) or dt != dt:
+            if abs(t) < abs(duration) and (not (dt != 0) or dt != dt):
                 status = 2
                 break
     except ArithmeticError:

@@ -1,0 +1,115 @@
+"""GPU edge cases and size-independent properties, each against the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from onekapy_amd import (Realizations, chronicle_into_field, sites, start_points, trace_vertices,
+                         _lib)
+from onekapy_amd.sites import fixed_field
+from helpers import oracle_chronicle, oracle_fit_draw, oracle_grid, small_site
+
+pytestmark = pytest.mark.gpu
+
+
+def _one_model(wells, K=10.0, n=0.25, b=20.0, coef=(0, 0, 0, 0, 0, 0)):
+    w = np.array(wells, dtype=float)
+    real = Realizations([K], [n], [b], w[:, 3].reshape(1, -1), np.array(coef, float))
+    om = O.OracleModel(0.0, K, n, b, w, 0.0, 0.0, coef)
+    return np.ascontiguousarray(w[:, :2]), real, om
+
+
+@pytest.mark.parametrize("case", ["on_well", "step_cap", "zero_duration", "forward",
+                                  "unconfined_dry", "unconfined_ok"])
+def test_trace_status_cases_match_oracle(gpu_ctx, case):
+    wells = [(0.0, 0.0, 0.25, 500.0), (30.0, 0.0, 0.25, 200.0), (-40.0, 25.0, 0.25, -150.0)]
+    coef = (1e-5, -2e-5, 0.0, 1e-2, -5e-3, 2000.0)
+    duration, confined, cap, start = 1000.0, True, 0, (1.25, 0.0)
+    if case == "on_well":
+        start = (30.0, 0.0)
+    elif case == "step_cap":
+        cap = 7
+    elif case == "zero_duration":
+        duration = 0.0
+    elif case == "forward":
+        duration = -500.0
+    elif case == "unconfined_dry":
+        coef, confined = (0, 0, 0, 0, 0, -1e9), False
+    elif case == "unconfined_ok":
+        confined = False
+    wxy, real, om = _one_model(wells, coef=coef)
+    verts, nverts, status = trace_vertices(wxy, 0.0, 0.0, real, np.array([start]), duration, 1.0,
+                                           10.0, confined, cap)
+    ov, res = om.backtrace(start[0], start[1], duration, 1.0, 10.0, confined=confined,
+                           max_steps=cap)
+    assert status[0] == res.status
+    assert nverts[0] == res.nverts
+    assert np.array_equal(verts[0, :nverts[0]], ov)
+
+
+def test_empty_batches_are_noops(gpu_ctx):
+    s, d = small_site(nreal=2, npaths=4)
+    _, _, coef = oracle_fit_draw(s, d)
+    pf = s.fixed_field()
+    real = Realizations(d["K"][:0], d["n"][:0], d["b"][:0], d["q"][:0], coef[:0])
+    res = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, start_points(s.xt, s.yt, s.rt, 4),
+                               s.duration, s.tol, s.maxstep, s.umbra)
+    assert res.counts.sum() == 0
+    real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+    res = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, np.zeros((0, 2)), s.duration,
+                               s.tol, s.maxstep, s.umbra)
+    assert res.counts.sum() == 0
+
+
+def test_hbm_bitmap_path_large_grid(gpu_ctx):
+    """A 1200 x 1100 lattice (41250 bitmap words) exceeds the LDS budget: the
+    per-block bitmaps live in HBM; results are unchanged."""
+    s, d = small_site(nreal=5, npaths=40)
+    _, _, coef = oracle_fit_draw(s, d)
+    pf = fixed_field(s.xt, s.yt, 5.0, 1100, 1200)
+    real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+    res = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real,
+                               start_points(s.xt, s.yt, s.rt, s.npaths), s.duration, s.tol,
+                               s.maxstep, 10.0)
+    s.umbra = 10.0
+    ocounts, ost = oracle_chronicle(s, d, coef, pf)
+    assert ost[6] == 0 and res.passes == 1
+    assert np.array_equal(res.counts, ocounts)
+
+
+def test_config1_subset_bit_exact_and_invariants(gpu_ctx):
+    s = sites.config1(8)
+    d = sites.draw(s, seed=2)
+    _, _, coef = oracle_fit_draw(s, d)
+    pf = s.fixed_field()
+    real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    res = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration, s.tol,
+                               s.maxstep, s.umbra)
+    ocounts, ost = oracle_chronicle(s, d, coef, pf)
+    assert np.array_equal(res.counts, ocounts)
+    assert (res.stats.attempted, res.stats.accepted, res.stats.visited) == (ost[0], ost[1], ost[7])
+    # invariants: a node counts at most once per realization; the nodes within
+    # umbra of the start ring are visited by every realization
+    assert res.counts.max() <= s.nrealizations
+    assert res.counts.sum() == res.stats.visited
+    jt = int(round((s.xt - pf.xmin) / pf.deltax))
+    it = int(round((s.yt - pf.ymin) / pf.deltay))
+    assert res.counts[it, jt] == s.nrealizations
+
+
+def test_fp32_error_config1_subset(gpu_ctx):
+    s = sites.config1(200)
+    d = sites.draw(s, seed=3)
+    from onekapy_amd import fit_and_draw
+    _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
+                              d["q"], d["z"])
+    real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    p = {}
+    for prec in (_lib.FP64_EXACT, _lib.FP32):
+        pf = s.fixed_field()
+        p[prec] = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
+                                       s.tol, s.maxstep, s.umbra, precision=prec).counts / 200.0
+    err = np.abs(p[_lib.FP32] - p[_lib.FP64_EXACT])
+    assert err.max() <= 0.01
+    assert err.sum() <= 0.01 * (p[_lib.FP64_EXACT] > 0).sum()
