@@ -52,7 +52,40 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU time budget of each CPU baseline")
+    ap.add_argument("--modes", default="fp64,fp32",
+                    help="tracers to run (the error gate needs both)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (to scale HBM counters)")
     return ap.parse_args()
+
+
+def copy_calibration(torch):
+    """A plain copy of 2^28 float32 (1 GiB read + 1 GiB write) for scaling the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30}))
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trace_traffic_cfg1.json")
+
+
+def measured_traffic(kernel_prefix, nreal):
+    """HBM bytes per launch of the named kernel at the default workload, from
+    the committed rocprofv3 counter summary (None when absent or not
+    applicable to this batch size)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    k = t.get("kernels", {}).get(kernel_prefix)
+    if not k or t.get("realizations") != nreal:
+        return None
+    return k["bytes_per_launch"]
 
 
 class DeviceRun:
@@ -189,6 +222,9 @@ def main():
               f"not built here)", file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     from onekapy_amd import sites
     from onekapy_amd import _lib
 
@@ -197,7 +233,10 @@ def main():
     run = DeviceRun(site, d, torch)
     nw = len(site.wells)
     modes = {}
+    wanted = [m.strip() for m in args.modes.split(",") if m.strip()]
     for name, prec in (("fp64", _lib.FP64_EXACT), ("fp32", _lib.FP32)):
+        if name not in wanted:
+            continue
         print(f"bench: {name} {site.name}: {args.warmup} warmup + {args.steps} timed steps",
               file=sys.stderr, flush=True)
         modes[name] = time_mode(run, prec, args.steps, args.warmup, torch)
@@ -205,11 +244,15 @@ def main():
         print(f"bench: {name} {m['ms_per_step']:.1f} ms/step, trace kernel {m['kernel_ms']:.1f} ms, "
               f"{m['stats'].attempted} steps", file=sys.stderr, flush=True)
     R = run.R
-    p64 = modes["fp64"]["counts"] / R
-    p32 = modes["fp32"]["counts"] / R
-    err_max = float(np.abs(p32 - p64).max())
-    err_l1 = float(np.abs(p32 - p64).sum())
-    head = "fp32" if err_max <= 0.01 else "fp64"
+    if "fp64" in modes and "fp32" in modes:
+        p64 = modes["fp64"]["counts"] / R
+        p32 = modes["fp32"]["counts"] / R
+        err_max = float(np.abs(p32 - p64).max())
+        err_l1 = float(np.abs(p32 - p64).sum())
+        head = "fp32" if err_max <= 0.01 else "fp64"
+    else:
+        err_max = err_l1 = None
+        head = next(iter(modes))
     hm = modes[head]
     value = R * args.steps / hm["seconds"]
     steps_per_s = hm["stats"].attempted * args.steps / hm["seconds"]
@@ -254,10 +297,11 @@ def main():
                    "spacing_m": site.spacing, "umbra_m": site.umbra, "tol_m": site.tol,
                    "maxstep_m": site.maxstep, "headline_mode": head},
         "particle_steps_per_s": steps_per_s,
-        "error_vs_fp64": {"max": err_max, "l1": err_l1, "nodes": int(p64.size)},
+        "error_vs_fp64": {"max": err_max, "l1": err_l1, "nodes": int(site.nrows * site.ncols)},
         "modes": {k: mode_summary(k) for k in modes},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak,
+                     "traffic": measured_traffic(f"k_trace_{head}", R),
                      "kernel": f"trace+chronicle ({head})",
                      "flops_per_launch": flops},
         "cpu_baseline": None if cpu is None else dict(cpu["python"], c_oracle=cpu["c_oracle"]),
