@@ -56,7 +56,63 @@ def parse():
                     help="tracers to run (the error gate needs both)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (to scale HBM counters)")
+    ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
+    ap.add_argument("--only-cv", action="store_true", help="run only the control-variate leg")
+    ap.add_argument("--cv-traced", type=int, default=20000)
+    ap.add_argument("--cv-surrogate", type=int, default=1000000)
+    ap.add_argument("--cv-judge", type=int, default=1000000,
+                    help="realizations of the FP64 plain run that judges the estimates")
+    ap.add_argument("--cv-steps", type=int, default=3000)
     return ap.parse_args()
+
+
+def control_variate_leg(site, args, fp32_rate):
+    """Learned control variate on the same site: N traced realizations (FP32)
+    + M surrogate evaluations, cross-fitted over 2 folds, judged against an
+    FP64 plain run.  Effective realizations/s = the plain-run size with the
+    same maximum (over window cells) standard error, divided by the
+    pipeline's wall time (draws, fits, pilot, tracing, training, surrogate)."""
+    from onekapy_amd import _lib
+    from onekapy_amd.stochastic import draw_realizations, fit_and_draw
+    from onekapy_amd.surrogate import cell_probabilities, control_variate, trace_counts
+    print(f"bench: control variate: {args.cv_traced} traced + {args.cv_surrogate} surrogate",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    res = control_variate(site, args.cv_traced, args.cv_surrogate, folds=2,
+                          train_steps=args.cv_steps, seed=123, precision=_lib.FP32)
+    wall = time.perf_counter() - t0
+    print(f"bench: control variate {wall:.2f} s {res.times}", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(321)
+    d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, args.cv_judge, rng)
+    _, _, coef = fit_and_draw(site.wells_xy(), site.xt, site.yt, site.base, site.observations,
+                              d["K"], d["b"], d["q"], d["z"])
+    print(f"bench: FP64 judge run of {args.cv_judge} realizations", file=sys.stderr, flush=True)
+    tj = time.perf_counter()
+    counts, _ = trace_counts(site, d, coef, _lib.FP64_EXACT)
+    tj = time.perf_counter() - tj
+    p_ref = cell_probabilities(counts, args.cv_judge, res.window)
+    err_cv = np.abs(res.p_cv - p_ref)
+    err_plain = np.abs(res.p_plain - p_ref)
+    n_eff = args.cv_traced * res.var_plain.max() / res.var_cv.max()
+    eff_rate = n_eff / wall
+    return {
+        "traced": args.cv_traced, "surrogate_evals": args.cv_surrogate, "folds": 2,
+        "train_steps_per_fold": args.cv_steps, "network": "34->512->512->4096 GELU, BF16 MFMA",
+        "window": {"row0": res.window.row0, "col0": res.window.col0, "cell_nodes": res.window.cell,
+                   "cells": [res.window.nrows, res.window.ncols]},
+        "judge": {"realizations": args.cv_judge, "tracer": "fp64", "seconds": tj},
+        "max_error_cv": float(err_cv.max()), "l1_error_cv": float(err_cv.sum()),
+        "max_error_plain_same_traced": float(err_plain.max()),
+        "l1_error_plain_same_traced": float(err_plain.sum()),
+        "max_std_cv": float(np.sqrt(res.var_cv.max())),
+        "max_std_plain_same_traced": float(np.sqrt(res.var_plain.max())),
+        "variance_ratio_at_max": float(res.var_plain.max() / res.var_cv.max()),
+        "variance_ratio_sum": float(res.var_plain.sum() / res.var_cv.sum()),
+        "effective_realizations": float(n_eff),
+        "wall_seconds": wall, "phase_seconds": res.times,
+        "effective_realizations_per_s": float(eff_rate),
+        "vs_fp32_plain_rate": float(eff_rate / fp32_rate) if fp32_rate else None,
+    }
 
 
 def copy_calibration(torch):
@@ -229,6 +285,10 @@ def main():
     from onekapy_amd import _lib
 
     site = sites.config1(args.nreal) if args.config == "cfg1" else sites.config0()
+    if args.only_cv:
+        cv = control_variate_leg(site, args, None)
+        print(json.dumps({"control_variate": cv}))
+        return
     d = sites.draw(site, seed=1)
     run = DeviceRun(site, d, torch)
     nw = len(site.wells)
@@ -265,6 +325,10 @@ def main():
         coef = run.coef.cpu().numpy()
         print("bench: CPU baselines", file=sys.stderr, flush=True)
         cpu = cpu_baselines(site, d, coef, args.cpu_seconds)
+
+    cv = None
+    if not args.no_cv and "fp32" in modes:
+        cv = control_variate_leg(site, args, R * args.steps / modes["fp32"]["seconds"])
 
     def mode_summary(name):
         m = modes[name]
@@ -305,6 +369,7 @@ def main():
                      "kernel": f"trace+chronicle ({head})",
                      "flops_per_launch": flops},
         "cpu_baseline": None if cpu is None else dict(cpu["python"], c_oracle=cpu["c_oracle"]),
+        "control_variate": cv,
     }
     print(json.dumps(line))
 

@@ -185,6 +185,27 @@ int oka_trace_chronicle(oka_ctx *ctx, const oka_site *site, int32_t nreal, const
                         int32_t *counts, double *bbox, int32_t *trace_status,
                         int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags);
 
+/* ---- per-realization marks pooled over a window: the training targets and
+ *      the correction term of the learned control variate --------------------
+ * No reference counterpart: the reference registers each realization into
+ * pgrid (probabilityfield.py:342-359) and keeps nothing per realization.
+ * The window is nrows x ncols coarse cells of cell x cell fine nodes, whose
+ * fine node (0, 0) is fine node (row0, col0) of `grid` (cells may lie partly
+ * outside the grid).  For realization r, marks[r * nrows * ncols + c] is the
+ * number of fine nodes of cell c that the realization visited (capped at 255).
+ * Traces run on the fixed grid as in oka_trace_chronicle with nepochs == 0;
+ * counts (may be NULL) accumulates as there. */
+typedef struct {
+    int32_t row0, col0;      /* fine-grid index of the window's first node     */
+    int32_t cell;            /* fine nodes per cell side (1..15)               */
+    int32_t nrows, ncols;    /* cells                                          */
+} oka_window;
+
+int oka_trace_marks(oka_ctx *ctx, const oka_site *site, int32_t nreal, const double *K,
+                    const double *n, const double *b, const double *q, const double *coef,
+                    const oka_trace_opts *opts, const oka_grid *grid, const oka_window *win,
+                    uint8_t *marks, int32_t *counts, oka_trace_stats *stats, uint32_t flags);
+
 /* ---- expansion history (probabilityfield.py:175-261 applied per trace in
  *      order, capturezone.py:117-120) -- host computation ---------------------
  * Given the field before the batch (init; init->nrows == 0 means an empty
@@ -211,6 +232,49 @@ int oka_trace_vertices(oka_ctx *ctx, const oka_site *site, int32_t nreal, const 
  * within umbra of any segment of xy (nverts x 2).  No expansion is done. */
 int oka_rasterize(oka_ctx *ctx, const oka_grid *grid, int64_t nverts, const double *xy,
                   double umbra, uint8_t *rgrid, uint32_t flags);
+
+/* ---- learned control variate: the surrogate MLP --------------------------
+ * No reference counterpart (the reference traces every realization).  A
+ * 3-layer perceptron n_in -> n_hidden -> n_hidden -> n_out with GELU after the
+ * hidden layers maps a realization's standardized parameters to per-cell
+ * capture logits.  Weights are FP32 masters with BF16 copies; every matrix
+ * product runs in hand-written BF16 MFMA kernels with FP32 accumulation.  The
+ * network lives in device memory owned by the library.
+ *
+ * Parameter vector (FP32, this order): W1 [n_hidden x n_in], b1 [n_hidden],
+ * W2 [n_hidden x n_hidden], b2, W3 [n_out x n_hidden], b3 [n_out].
+ * Inputs x are FP32 [n x n_in] row-major; targets are window marks (uint8,
+ * oka_trace_marks) times target_scale, i.e. fractions in [0, 1].
+ * Requirements: n_in <= 64, n_hidden % 64 == 0, n_out % 64 == 0.  All array
+ * arguments follow the call's flags (OKA_PTR_DEVICE or host). */
+typedef struct oka_mlp oka_mlp;
+
+int oka_mlp_create(oka_ctx *ctx, int32_t n_in, int32_t n_hidden, int32_t n_out,
+                   int32_t max_batch, oka_mlp **out);
+void oka_mlp_destroy(oka_mlp *mlp);
+int64_t oka_mlp_param_count(const oka_mlp *mlp);
+int oka_mlp_set_params(oka_mlp *mlp, const float *params, uint32_t flags);
+int oka_mlp_get_params(oka_mlp *mlp, float *params, uint32_t flags);
+/* Gradient of the mean binary cross-entropy (logits vs targets, averaged over
+ * batch x n_out) with respect to every parameter, for one batch. */
+int oka_mlp_grad(oka_mlp *mlp, int32_t batch, const float *x, const uint8_t *marks,
+                 float target_scale, float *grad, float *loss, uint32_t flags);
+/* nsteps Adam steps; step s uses samples order[s*batch .. (s+1)*batch).
+ * losses (nsteps, may be NULL) receives each step's mean loss. */
+int oka_mlp_train(oka_mlp *mlp, int64_t nsamples, const float *x, const uint8_t *marks,
+                  float target_scale, const int32_t *order, int32_t batch, int32_t nsteps,
+                  float lr, float beta1, float beta2, float eps, float *losses, uint32_t flags);
+/* probs [n x n_out] = sigmoid(logits) (FP32). */
+int oka_mlp_predict(oka_mlp *mlp, int64_t n, const float *x, float *probs, uint32_t flags);
+/* Per output c, over the n samples, with s = sigmoid(logit_c) and, when marks
+ * is not NULL, d = marks * target_scale - s (FP64 accumulators, added to):
+ *   sums[c] += sum s,  sums_sq[c] += sum s^2,  diffs[c] += sum d,
+ *   diffs_sq[c] += sum d^2
+ * -- the terms of the control-variate estimator and of its variance.  Only
+ * sums is required; the others may be NULL. */
+int oka_mlp_accumulate(oka_mlp *mlp, int64_t n, const float *x, const uint8_t *marks,
+                       float target_scale, double *sums, double *sums_sq, double *diffs,
+                       double *diffs_sq, uint32_t flags);
 
 /* ---- exact scalar helpers shared by host and device code (for tests) ----- */
 double oka_pow_fifth(double x);   /* correctly rounded pow(x, 0.2) (capturezone.py:247) */

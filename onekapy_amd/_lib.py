@@ -36,7 +36,9 @@ EXPORTED = (
     "oka_abi_version", "oka_ctx_create", "oka_ctx_destroy", "oka_last_error",
     "oka_ctx_set_stream", "oka_ctx_stream", "oka_ctx_synchronize", "oka_model_eval",
     "oka_fit_draw", "oka_trace_chronicle", "oka_grid_history", "oka_trace_vertices",
-    "oka_rasterize", "oka_pow_fifth", "oka_log",
+    "oka_rasterize", "oka_pow_fifth", "oka_log", "oka_trace_marks", "oka_mlp_create",
+    "oka_mlp_destroy", "oka_mlp_param_count", "oka_mlp_set_params", "oka_mlp_get_params",
+    "oka_mlp_grad", "oka_mlp_train", "oka_mlp_predict", "oka_mlp_accumulate",
 )
 
 
@@ -81,6 +83,10 @@ EPOCH_DTYPE = np.dtype([("first_trace", np.int64), ("xmin", np.float64), ("ymin"
                         ("nrows", np.int32), ("ncols", np.int32), ("row_off", np.int32),
                         ("col_off", np.int32)])
 assert EPOCH_DTYPE.itemsize == C.sizeof(Epoch_)
+
+
+class Window_(C.Structure):
+    _fields_ = [("row0", _i32), ("col0", _i32), ("cell", _i32), ("nrows", _i32), ("ncols", _i32)]
 
 
 class TraceStats_(C.Structure):
@@ -137,6 +143,29 @@ def load():
                                      C.POINTER(TraceOpts_), _i64, _vp, _vp, _vp, _vp, _u32]
     L.oka_rasterize.restype = C.c_int
     L.oka_rasterize.argtypes = [_vp, C.POINTER(Grid_), _i64, _vp, _d, _vp, _u32]
+    L.oka_trace_marks.restype = C.c_int
+    L.oka_trace_marks.argtypes = [_vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp,
+                                  C.POINTER(TraceOpts_), C.POINTER(Grid_), C.POINTER(Window_),
+                                  _vp, _vp, C.POINTER(TraceStats_), _u32]
+    L.oka_mlp_create.restype = C.c_int
+    L.oka_mlp_create.argtypes = [_vp, _i32, _i32, _i32, _i32, C.POINTER(_vp)]
+    L.oka_mlp_destroy.restype = None
+    L.oka_mlp_destroy.argtypes = [_vp]
+    L.oka_mlp_param_count.restype = _i64
+    L.oka_mlp_param_count.argtypes = [_vp]
+    L.oka_mlp_set_params.restype = C.c_int
+    L.oka_mlp_set_params.argtypes = [_vp, _vp, _u32]
+    L.oka_mlp_get_params.restype = C.c_int
+    L.oka_mlp_get_params.argtypes = [_vp, _vp, _u32]
+    L.oka_mlp_grad.restype = C.c_int
+    L.oka_mlp_grad.argtypes = [_vp, _i32, _vp, _vp, C.c_float, _vp, C.POINTER(C.c_float), _u32]
+    L.oka_mlp_train.restype = C.c_int
+    L.oka_mlp_train.argtypes = [_vp, _i64, _vp, _vp, C.c_float, _vp, _i32, _i32, C.c_float,
+                                C.c_float, C.c_float, C.c_float, _vp, _u32]
+    L.oka_mlp_predict.restype = C.c_int
+    L.oka_mlp_predict.argtypes = [_vp, _i64, _vp, _vp, _u32]
+    L.oka_mlp_accumulate.restype = C.c_int
+    L.oka_mlp_accumulate.argtypes = [_vp, _i64, _vp, _vp, C.c_float, _vp, _vp, _vp, _vp, _u32]
     L.oka_pow_fifth.restype = _d
     L.oka_pow_fifth.argtypes = [_d]
     L.oka_log.restype = _d

@@ -10,39 +10,12 @@
 #include <string>
 #include <vector>
 
+#include "oka_ctx.h"
 #include "oka_internal.h"
 
 using namespace oka;
 
-namespace {
-
-// Device buffer slots owned by a context; grown on demand, freed at destroy.
-enum Slot {
-    kSlotWells, kSlotStart, kSlotK, kSlotN, kSlotB, kSlotQ, kSlotCoef, kSlotCounts,
-    kSlotBbox, kSlotTStatus, kSlotTSteps, kSlotStats, kSlotEpochs, kSlotBitmap,
-    kSlotVerts, kSlotNVerts, kSlotObs, kSlotLogs, kSlotZ, kSlotEv, kSlotCov, kSlotFStatus,
-    kSlotXY, kSlotOut, kSlotEStatus, kSlotRgrid, kSlotCount
-};
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-};
-
-}  // namespace
-
-struct oka_ctx {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int num_cu = 0;
-    size_t max_lds = 0;
-    std::string err;
-    DevBuf buf[kSlotCount];
-};
-
-namespace {
+namespace oka {
 
 int fail(oka_ctx *c, int code, const char *fmt, ...)
 {
@@ -56,14 +29,6 @@ int fail(oka_ctx *c, int code, const char *fmt, ...)
     }
     return code;
 }
-
-#define HIPCHK(c, expr)                                                                       \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return fail((c), OKA_ERR_DEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
-                        __FILE__, __LINE__);                                                  \
-    } while (0)
 
 int ensure(oka_ctx *c, Slot s, size_t bytes, void **out)
 {
@@ -159,12 +124,10 @@ int fill_u32(oka_ctx *c, void *p, int64_t nwords)
     return OKA_OK;
 }
 
-#define TRY(expr)                                                                             \
-    do {                                                                                      \
-        int rc_ = (expr);                                                                     \
-        if (rc_)                                                                              \
-            return rc_;                                                                       \
-    } while (0)
+
+}  // namespace oka
+
+namespace {
 
 int check_opts(oka_ctx *c, const oka_trace_opts *o)
 {
@@ -501,19 +464,23 @@ static int trace_common(oka_ctx *c, const oka_site *site, int32_t nreal, const d
     return OKA_OK;
 }
 
-int oka_trace_chronicle(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
-                        const double *n, const double *b, const double *q, const double *coef,
-                        const oka_trace_opts *o, const oka_grid *g, int32_t nepochs,
-                        const oka_epoch *epochs, int32_t rasterize, int32_t *counts,
-                        double *bbox, int32_t *trace_status, int64_t *trace_steps,
-                        oka_trace_stats *stats, uint32_t flags)
+static int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+                          const double *n, const double *b, const double *q, const double *coef,
+                          const oka_trace_opts *o, const oka_grid *g, int32_t nepochs,
+                          const oka_epoch *epochs, int32_t rasterize, int32_t *counts,
+                          double *bbox, int32_t *trace_status, int64_t *trace_steps,
+                          oka_trace_stats *stats, uint32_t flags, const oka_window *win,
+                          uint8_t *marks)
 {
     if (!c)
         return OKA_ERR_ARG;
     if (!g)
         return fail(c, OKA_ERR_ARG, "grid is NULL");
-    if (rasterize && (g->nrows <= 0 || g->ncols <= 0 || !counts))
-        return fail(c, OKA_ERR_ARG, "rasterization needs a non-empty grid and counts");
+    if (rasterize && (g->nrows <= 0 || g->ncols <= 0 || (!counts && !marks)))
+        return fail(c, OKA_ERR_ARG, "rasterization needs a non-empty grid and an output");
+    if (marks && (!win || win->cell < 1 || win->cell > 15 || win->nrows <= 0 ||
+                  win->ncols <= 0 || (int64_t)win->nrows * win->ncols > 16384))
+        return fail(c, OKA_ERR_ARG, "bad window (cell 1..15, at most 16384 cells)");
     if (rasterize && (!(g->dx > 0.0) || !(g->dy > 0.0)))
         return fail(c, OKA_ERR_RANGE, "grid spacing must be > 0");
     if (nepochs < 0 || (nepochs > 0 && !epochs))
@@ -561,10 +528,22 @@ int oka_trace_chronicle(oka_ctx *c, const oka_site *site, int32_t nreal, const d
     TRY(ensure(c, kSlotStats, sizeof(unsigned long long) * kStatCount, &o2));
     a.stats = (unsigned long long *)o2;
     TRY(fill_u32(c, a.stats, 2 * kStatCount));  // 64-bit slots
+    size_t win_lds = 0;
+    if (marks) {
+        a.w_row0 = win->row0;
+        a.w_col0 = win->col0;
+        a.w_cell = win->cell;
+        a.w_nr = win->nrows;
+        a.w_nc = win->ncols;
+        win_lds = (size_t)win->nrows * win->ncols * 4;
+        TRY(stage_out(c, kSlotMarks, marks, (size_t)nreal * win->nrows * win->ncols, flags, false,
+                      &o2));
+        a.wmarks = (uint8_t *)o2;
+    }
 
     // launch shape: one realization per 256-lane workgroup, grid-stride
     const bool lds_bitmap = (size_t)a.nwords * 4 <= (size_t)kLdsBitmapMaxWords * 4 &&
-                            trace_fp64_lds_bytes(a.nwells, a.nwords, true) <= c->max_lds;
+                            trace_fp64_lds_bytes(a.nwells, a.nwords, true) + win_lds <= c->max_lds;
     int blocks = nreal;
     if (rasterize && !lds_bitmap) {
         // per-block bitmaps in HBM: bound the resident set
@@ -573,7 +552,7 @@ int oka_trace_chronicle(oka_ctx *c, const oka_site *site, int32_t nreal, const d
         TRY(ensure(c, kSlotBitmap, sizeof(uint32_t) * (size_t)a.nwords * blocks, &o2));
         a.gbitmap = (uint32_t *)o2;
     }
-    if (trace_fp64_lds_bytes(a.nwells, 0, false) > c->max_lds)
+    if (trace_fp64_lds_bytes(a.nwells, 0, false) + win_lds > c->max_lds)
         return fail(c, OKA_ERR_ARG, "too many wells for LDS (%d)", a.nwells);
     const bool timed = stats != nullptr;
     if (timed)
@@ -588,6 +567,8 @@ int oka_trace_chronicle(oka_ctx *c, const oka_site *site, int32_t nreal, const d
     TRY(copy_back(c, bbox, a.bbox, sizeof(double) * 4 * ntr, flags));
     TRY(copy_back(c, trace_status, a.trace_status, sizeof(int32_t) * ntr, flags));
     TRY(copy_back(c, trace_steps, a.trace_steps, sizeof(int64_t) * ntr, flags));
+    if (marks)
+        TRY(copy_back(c, marks, a.wmarks, (size_t)nreal * win->nrows * win->ncols, flags));
     if ((flags & OKA_ASYNC) && (flags & OKA_PTR_DEVICE) && !stats)
         return OKA_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -606,6 +587,31 @@ int oka_trace_chronicle(oka_ctx *c, const oka_site *site, int32_t nreal, const d
         stats->kernel_ms = ms;
     }
     return OKA_OK;
+}
+
+int oka_trace_chronicle(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+                        const double *n, const double *b, const double *q, const double *coef,
+                        const oka_trace_opts *o, const oka_grid *g, int32_t nepochs,
+                        const oka_epoch *epochs, int32_t rasterize, int32_t *counts,
+                        double *bbox, int32_t *trace_status, int64_t *trace_steps,
+                        oka_trace_stats *stats, uint32_t flags)
+{
+    if (c && rasterize && !counts)
+        return fail(c, OKA_ERR_ARG, "rasterization needs counts");
+    return chronicle_impl(c, site, nreal, K, n, b, q, coef, o, g, nepochs, epochs, rasterize,
+                          counts, bbox, trace_status, trace_steps, stats, flags, nullptr,
+                          nullptr);
+}
+
+int oka_trace_marks(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+                    const double *n, const double *b, const double *q, const double *coef,
+                    const oka_trace_opts *o, const oka_grid *g, const oka_window *win,
+                    uint8_t *marks, int32_t *counts, oka_trace_stats *stats, uint32_t flags)
+{
+    if (c && !marks)
+        return fail(c, OKA_ERR_ARG, "marks is NULL");
+    return chronicle_impl(c, site, nreal, K, n, b, q, coef, o, g, 0, nullptr, 1, counts,
+                          nullptr, nullptr, nullptr, stats, flags, win, marks);
 }
 
 int oka_trace_vertices(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,

@@ -1,0 +1,67 @@
+// oka_ctx.h -- the context object and host-side staging helpers shared by
+// the entry-point translation units of liboka.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "oka_internal.h"
+
+namespace oka {
+
+// Device buffer slots owned by a context; grown on demand, freed at destroy.
+enum Slot {
+    kSlotWells, kSlotStart, kSlotK, kSlotN, kSlotB, kSlotQ, kSlotCoef, kSlotCounts,
+    kSlotBbox, kSlotTStatus, kSlotTSteps, kSlotStats, kSlotEpochs, kSlotBitmap,
+    kSlotVerts, kSlotNVerts, kSlotObs, kSlotLogs, kSlotZ, kSlotEv, kSlotCov, kSlotFStatus,
+    kSlotXY, kSlotOut, kSlotEStatus, kSlotRgrid, kSlotMarks, kSlotMlpX, kSlotMlpMarks,
+    kSlotMlpOrder, kSlotMlpOut, kSlotMlpOut2, kSlotMlpLoss, kSlotCount
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace oka
+
+struct oka_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int num_cu = 0;
+    size_t max_lds = 0;
+    std::string err;
+    oka::DevBuf buf[oka::kSlotCount];
+};
+
+namespace oka {
+
+int fail(oka_ctx *c, int code, const char *fmt, ...);
+int ensure(oka_ctx *c, Slot s, size_t bytes, void **out);
+int stage_in(oka_ctx *c, Slot s, const void *src, size_t bytes, uint32_t flags, const void **out);
+int stage_out(oka_ctx *c, Slot s, void *dst, size_t bytes, uint32_t flags, bool preload,
+              void **out);
+int copy_back(oka_ctx *c, void *dst, const void *dev, size_t bytes, uint32_t flags);
+int fill_u32(oka_ctx *c, void *p, int64_t nwords);
+
+}  // namespace oka
+
+#define HIPCHK(c, expr)                                                                       \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return ::oka::fail((c), OKA_ERR_DEVICE, "%s: %s (%s:%d)", #expr,                  \
+                               hipGetErrorString(e_), __FILE__, __LINE__);                    \
+    } while (0)
+
+#define TRY(expr)                                                                             \
+    do {                                                                                      \
+        int rc_ = (expr);                                                                     \
+        if (rc_)                                                                              \
+            return rc_;                                                                       \
+    } while (0)

@@ -256,6 +256,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
     const float gx0 = (float)(a.gxmin - a.xo), gx1 = (float)(a.gxmax - a.xo);
     const float gy0 = (float)(a.gymin - a.yo), gy1 = (float)(a.gymax - a.yo);
     uint32_t *bm = kLdsBitmap ? s_bm : a.gbitmap + (int64_t)blockIdx.x * a.nwords;
+    unsigned int *s_win = nullptr;
+    if (a.rasterize && a.wmarks) {
+        s_win = s_bm + ((kLdsBitmap && a.rasterize) ? a.nwords : 0);
+        for (int c = threadIdx.x; c < a.w_nr * a.w_nc; c += blockDim.x)
+            s_win[c] = 0u;
+    }
 
     for (int r = blockIdx.x; r < a.nreal; r += gridDim.x) {
         if (tid < kStatCount)
@@ -347,20 +353,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
             if (n_st[j])
                 atomicAdd(&s_stat[kStatStatus0 + j], n_st[j]);
         __syncthreads();
-        if (a.rasterize) {
-            unsigned long long vis = 0;
-            for (int w = tid; w < a.nwords; w += blockDim.x) {
-                uint32_t bits = bm[w];
-                vis += (unsigned long long)__popc(bits);
-                while (bits) {
-                    const int bit = __ffs(bits) - 1;
-                    bits &= bits - 1u;
-                    atomicAdd(&a.counts[w * 32 + bit], 1);
-                }
-            }
-            atomicAdd(&s_stat[kStatVisited], vis);
-        }
+        if (a.rasterize)
+            atomicAdd(&s_stat[kStatVisited], flush_bitmap(a, bm, s_win));
         __syncthreads();
+        if (s_win) {
+            store_window(a, r, s_win);
+            __syncthreads();
+        }
         if (tid < kStatCount && a.stats)
             atomicAdd(&a.stats[tid], s_stat[tid]);
         __syncthreads();
@@ -377,7 +376,8 @@ size_t trace_fp32_lds_bytes(int nwells, int nwords, bool lds_bitmap)
 hipError_t launch_trace_fp32(const TraceArgs &a, int grid_blocks, bool lds_bitmap, hipStream_t s)
 {
     const bool use_lds = lds_bitmap || !a.rasterize;
-    const size_t lds = trace_fp32_lds_bytes(a.nwells, a.nwords, use_lds && a.rasterize);
+    const size_t lds = trace_fp32_lds_bytes(a.nwells, a.nwords, use_lds && a.rasterize) +
+                       (a.wmarks ? (size_t)a.w_nr * a.w_nc * 4 : 0);
     dim3 grid(grid_blocks), block(kTraceBlock);
     if (use_lds)
         hipLaunchKernelGGL(k_trace_fp32<true>, grid, block, lds, s, a);

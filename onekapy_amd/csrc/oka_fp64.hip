@@ -343,10 +343,17 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp64(TraceArgs a)
     const double u2 = umbra * umbra;
     const bool raster = !kVerts && a.rasterize;
     uint32_t *bm = nullptr;
-    if (kMode == kModeLdsBitmap)
-        bm = reinterpret_cast<uint32_t *>(smem + 3 * a.nwells);
-    else if (kMode == kModeHbmBitmap)
+    uint32_t *lds_tail = reinterpret_cast<uint32_t *>(smem + 3 * a.nwells);
+    if (kMode == kModeLdsBitmap) {
+        bm = lds_tail;
+        lds_tail += a.rasterize ? a.nwords : 0;
+    } else if (kMode == kModeHbmBitmap) {
         bm = a.gbitmap + (int64_t)blockIdx.x * a.nwords;
+    }
+    unsigned int *s_win = (raster && a.wmarks) ? lds_tail : nullptr;
+    if (s_win)
+        for (int c = threadIdx.x; c < a.w_nr * a.w_nc; c += blockDim.x)
+            s_win[c] = 0u;
 
     for (int r = blockIdx.x; r < a.nreal; r += gridDim.x) {
         if (tid < kStatCount)
@@ -436,20 +443,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp64(TraceArgs a)
                 atomicAdd(&s_stat[kStatStatus0 + j], n_st[j]);
         __syncthreads();
 
-        if (raster) {
-            unsigned long long vis = 0;
-            for (int w = tid; w < a.nwords; w += blockDim.x) {
-                uint32_t bits = bm[w];
-                vis += (unsigned long long)__popc(bits);
-                while (bits) {
-                    const int bit = __ffs(bits) - 1;
-                    bits &= bits - 1u;
-                    atomicAdd(&a.counts[w * 32 + bit], 1);
-                }
-            }
-            atomicAdd(&s_stat[kStatVisited], vis);
-        }
+        if (raster)
+            atomicAdd(&s_stat[kStatVisited], flush_bitmap(a, bm, s_win));
         __syncthreads();
+        if (s_win) {
+            store_window(a, r, s_win);
+            __syncthreads();
+        }
         if (tid < kStatCount && a.stats)
             atomicAdd(&a.stats[tid], s_stat[tid]);
         __syncthreads();
@@ -565,7 +565,8 @@ size_t trace_fp64_lds_bytes(int nwells, int nwords, bool lds_bitmap)
 
 hipError_t launch_trace_fp64(const TraceArgs &a, int grid_blocks, bool lds_bitmap, hipStream_t s)
 {
-    const size_t lds = trace_fp64_lds_bytes(a.nwells, a.nwords, lds_bitmap && a.rasterize);
+    const size_t lds = trace_fp64_lds_bytes(a.nwells, a.nwords, lds_bitmap && a.rasterize) +
+                       (a.wmarks ? (size_t)a.w_nr * a.w_nc * 4 : 0);
     if (lds_bitmap || !a.rasterize)
         return launch_mode<kModeLdsBitmap>(a, grid_blocks, lds, s);
     return launch_mode<kModeHbmBitmap>(a, grid_blocks, lds, s);

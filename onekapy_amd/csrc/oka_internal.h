@@ -57,7 +57,54 @@ struct TraceArgs {
     int64_t vcap;
     double *verts;              // ntraces x vcap x 2
     int64_t *nverts;
+    // per-realization window marks (oka_trace_marks); wmarks == nullptr: off
+    int32_t w_row0, w_col0, w_cell, w_nr, w_nc;
+    uint8_t *wmarks;            // nreal x (w_nr * w_nc)
 };
+
+// Flush of one realization's visited-node bitmap (probabilityfield.py:357-359):
+// one atomic add per set bit into the count grid, and, when a window is
+// requested, the per-cell number of visited fine nodes into LDS counters.
+// Called by every lane of the workgroup; returns this lane's share of the
+// visited count.
+__device__ inline unsigned long long flush_bitmap(const TraceArgs &a, const uint32_t *bm,
+                                                  unsigned int *s_win)
+{
+    unsigned long long vis = 0;
+    for (int w = threadIdx.x; w < a.nwords; w += blockDim.x) {
+        uint32_t bits = bm[w];
+        vis += (unsigned long long)__popc(bits);
+        while (bits) {
+            const int bit = __ffs(bits) - 1;
+            bits &= bits - 1u;
+            const int idx = w * 32 + bit;
+            if (a.counts)
+                atomicAdd(&a.counts[idx], 1);
+            if (s_win) {
+                const int i = idx / a.gncols, j = idx - i * a.gncols;
+                const int ci = i - a.w_row0, cj = j - a.w_col0;
+                if (ci >= 0 && cj >= 0) {
+                    const int r = ci / a.w_cell, c = cj / a.w_cell;
+                    if (r < a.w_nr && c < a.w_nc)
+                        atomicAdd(&s_win[r * a.w_nc + c], 1u);
+                }
+            }
+        }
+    }
+    return vis;
+}
+
+// Writes the window counters of realization r (capped at 255) and clears them.
+__device__ inline void store_window(const TraceArgs &a, int r, unsigned int *s_win)
+{
+    const int n = a.w_nr * a.w_nc;
+    uint8_t *dst = a.wmarks + (int64_t)r * n;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        const unsigned int v = s_win[c];
+        dst[c] = (uint8_t)(v < 255u ? v : 255u);
+        s_win[c] = 0u;
+    }
+}
 
 struct EvalArgs {
     int32_t kind;

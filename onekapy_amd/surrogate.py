@@ -1,0 +1,319 @@
+"""Learned control variate for the capture probability.
+
+No reference counterpart: the reference estimates each node's capture
+probability as the fraction of traced realizations whose traces come within
+umbra of it (stochastic.py:220-265, probabilityfield.py:342-359).  Here a
+surrogate network s(theta) predicts, from a realization's parameters theta
+(discharges, conductivity, porosity, thickness, A..F), the fraction of each
+window cell's nodes that the realization visits, and serves as a control
+variate:
+
+    p_hat(c) = mean over M surrogate-only draws of s_c
+             + mean over N traced realizations of (y_c - s_c)
+
+which is unbiased for the cell capture probability p(c) = E[y_c] whatever the
+surrogate, and whose variance Var(y - s)/N + Var(s)/M is far below the plain
+Var(y)/N when s is a good predictor.  The N traced realizations are used both
+to train and to correct by K-fold cross-fitting (each fold is corrected by a
+network that never saw it), so no traced realization is spent on training
+alone.  The plain tracer remains the fallback and the judge of the error.
+
+The network (n_in -> 512 -> 512 -> 4096, GELU, BF16 weights, FP32
+accumulation) is trained and evaluated by liboka's hand-written BF16 MFMA
+kernels (oka_mlp_*).
+"""
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from .capturezone import start_points
+from .stochastic import draw_realizations, fit_and_draw
+
+
+class SurrogateMLP:
+    """n_in -> n_hidden -> n_hidden -> n_out perceptron on the GPU (oka_mlp_*)."""
+
+    def __init__(self, n_in, n_hidden=512, n_out=4096, max_batch=8192, seed=0, device=0):
+        self.ctx = _lib.context(device)
+        self.n_in, self.n_hidden, self.n_out = int(n_in), int(n_hidden), int(n_out)
+        self.max_batch = int(max_batch)
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.oka_mlp_create(self.ctx.h, self.n_in, self.n_hidden,
+                                                   self.n_out, self.max_batch, C.byref(h)))
+        self.h = h
+        self.n_params = int(self.ctx.lib.oka_mlp_param_count(h))
+        self.set_params(self.initial_params(seed))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.ctx.lib.oka_mlp_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def shapes(self):
+        I, H, O = self.n_in, self.n_hidden, self.n_out
+        return [("W1", (H, I)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (O, H)),
+                ("b3", (O,))]
+
+    def split(self, flat):
+        """The flat parameter vector as a dict of arrays (views)."""
+        out, k = {}, 0
+        for name, shp in self.shapes():
+            n = int(np.prod(shp))
+            out[name] = flat[k:k + n].reshape(shp)
+            k += n
+        return out
+
+    def initial_params(self, seed):
+        """Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) for weights and biases."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        fan = {"W1": self.n_in, "b1": self.n_in, "W2": self.n_hidden, "b2": self.n_hidden,
+               "W3": self.n_hidden, "b3": self.n_hidden}
+        for name, shp in self.shapes():
+            bound = 1.0 / math.sqrt(fan[name])
+            parts.append(rng.uniform(-bound, bound, shp).astype(np.float32).ravel())
+        return np.concatenate(parts)
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        assert flat.size == self.n_params
+        self.ctx.check(self.ctx.lib.oka_mlp_set_params(self.h, flat.ctypes.data, 0))
+
+    def get_params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.oka_mlp_get_params(self.h, out.ctypes.data, 0))
+        return out
+
+    def grad(self, x, marks, target_scale):
+        """(gradient, loss) of the mean BCE on one batch."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        marks = np.ascontiguousarray(marks, dtype=np.uint8)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_float(0.0)
+        self.ctx.check(self.ctx.lib.oka_mlp_grad(self.h, x.shape[0], x.ctypes.data,
+                                                 marks.ctypes.data, float(target_scale),
+                                                 g.ctypes.data, C.byref(loss), 0))
+        return g, loss.value
+
+    def train(self, x, marks, target_scale, epochs, batch=512, lr=1e-3, beta1=0.9, beta2=0.999,
+              eps=1e-8, seed=0):
+        """Adam over shuffled minibatches; returns the per-step losses."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        marks = np.ascontiguousarray(marks, dtype=np.uint8)
+        n = x.shape[0]
+        per_epoch = n // batch
+        if per_epoch == 0:
+            raise ValueError("fewer samples than one batch")
+        rng = np.random.default_rng(seed)
+        losses = []
+        for _ in range(int(epochs)):
+            order = rng.permutation(n)[:per_epoch * batch].astype(np.int32)
+            out = np.zeros(per_epoch, dtype=np.float32)
+            self.ctx.check(self.ctx.lib.oka_mlp_train(
+                self.h, n, x.ctypes.data, marks.ctypes.data, float(target_scale),
+                order.ctypes.data, batch, per_epoch, lr, beta1, beta2, eps, out.ctypes.data, 0))
+            losses.append(out)
+        return np.concatenate(losses)
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        probs = np.zeros((x.shape[0], self.n_out), dtype=np.float32)
+        self.ctx.check(self.ctx.lib.oka_mlp_predict(self.h, x.shape[0], x.ctypes.data,
+                                                    probs.ctypes.data, 0))
+        return probs
+
+    def accumulate(self, x, marks=None, target_scale=1.0):
+        """Sums over samples of s, s^2 and (with marks) y - s, (y - s)^2 per output."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        acc = {k: np.zeros(self.n_out) for k in ("s", "s2", "d", "d2")}
+        mp = None
+        if marks is not None:
+            marks = np.ascontiguousarray(marks, dtype=np.uint8)
+            mp = marks.ctypes.data
+        self.ctx.check(self.ctx.lib.oka_mlp_accumulate(
+            self.h, x.shape[0], x.ctypes.data, mp, float(target_scale), acc["s"].ctypes.data,
+            acc["s2"].ctypes.data, acc["d"].ctypes.data if mp else None,
+            acc["d2"].ctypes.data if mp else None, 0))
+        return acc
+
+
+def features(d, coef):
+    """Per-realization parameters: discharges, K, n, b, A..F (nwells + 9 columns)."""
+    return np.concatenate([d["q"], d["K"][:, None], d["n"][:, None], d["b"][:, None],
+                           np.asarray(coef)], axis=1)
+
+
+class Standardizer:
+    def __init__(self, x):
+        self.mean = x.mean(0)
+        self.std = x.std(0)
+        self.std[self.std == 0] = 1.0
+
+    def __call__(self, x):
+        return ((x - self.mean) / self.std).astype(np.float32)
+
+
+def _site_struct(site):
+    wxy = site.wells_xy()
+    return wxy, _lib.make_site(wxy, site.xt, site.yt)
+
+
+def trace_counts(site, d, coef, precision=_lib.FP32):
+    """Fine-grid counts of a batch of realizations on the site's fixed grid."""
+    pf = site.fixed_field()
+    g = pf.grid()
+    wxy, s = _site_struct(site)
+    starts = start_points(site.xt, site.yt, site.rt, site.npaths)
+    o = _lib.make_trace_opts(site.npaths, site.duration, site.tol, site.maxstep, site.umbra,
+                             _lib.ptr(starts), precision, site.confined)
+    counts = np.zeros((pf.nrows, pf.ncols), np.int32)
+    st = _lib.TraceStats_()
+    K, n, b, q = (np.ascontiguousarray(d[k]) for k in ("K", "n", "b", "q"))
+    coef = np.ascontiguousarray(coef)
+    ctx = _lib.context()
+    ctx.check(ctx.lib.oka_trace_chronicle(ctx.h, s, len(K), _lib.ptr(K), _lib.ptr(n), _lib.ptr(b),
+                                          _lib.ptr(q), _lib.ptr(coef), o, g, 0, None, 1,
+                                          _lib.ptr(counts), None, None, None, st, 0))
+    return counts, st
+
+
+def choose_window(counts, cells=64, margin=1.1):
+    """A cells x cells window of square cells covering the visited nodes."""
+    nz = np.argwhere(counts > 0)
+    if nz.size == 0:
+        r = counts.shape[0] // 2
+        c = counts.shape[1] // 2
+        return _window(r - cells // 2, c - cells // 2, 1, cells)
+    r0, r1 = nz[:, 0].min(), nz[:, 0].max()
+    c0, c1 = nz[:, 1].min(), nz[:, 1].max()
+    ext = max(r1 - r0 + 1, c1 - c0 + 1)
+    cell = max(1, min(15, int(math.ceil(ext * margin / cells))))
+    rc, cc = (r0 + r1) // 2, (c0 + c1) // 2
+    return _window(int(rc - cells // 2 * cell), int(cc - cells // 2 * cell), cell, cells)
+
+
+def _window(row0, col0, cell, cells):
+    w = _lib.Window_()
+    w.row0, w.col0, w.cell, w.nrows, w.ncols = row0, col0, cell, cells, cells
+    return w
+
+
+def trace_marks(site, d, coef, window, precision=_lib.FP32):
+    """Per-realization visited-node counts per window cell (uint8 [R, cells])."""
+    pf = site.fixed_field()
+    g = pf.grid()
+    wxy, s = _site_struct(site)
+    starts = start_points(site.xt, site.yt, site.rt, site.npaths)
+    o = _lib.make_trace_opts(site.npaths, site.duration, site.tol, site.maxstep, site.umbra,
+                             _lib.ptr(starts), precision, site.confined)
+    R = len(d["K"])
+    marks = np.zeros((R, window.nrows * window.ncols), np.uint8)
+    st = _lib.TraceStats_()
+    K, n, b, q = (np.ascontiguousarray(d[k]) for k in ("K", "n", "b", "q"))
+    coef = np.ascontiguousarray(coef)
+    ctx = _lib.context()
+    ctx.check(ctx.lib.oka_trace_marks(ctx.h, s, R, _lib.ptr(K), _lib.ptr(n), _lib.ptr(b),
+                                      _lib.ptr(q), _lib.ptr(coef), o, g, window, _lib.ptr(marks),
+                                      None, st, 0))
+    return marks, st
+
+
+def cell_probabilities(counts, nreal, window):
+    """Fine-grid counts -> the window cells' capture probability (mean over each
+    cell's nodes of count / nreal; nodes outside the grid count as 0)."""
+    c, nr, nc = window.cell, window.nrows, window.ncols
+    out = np.zeros((nr * c, nc * c))
+    r0, c0 = window.row0, window.col0
+    gr0, gc0 = max(r0, 0), max(c0, 0)
+    gr1 = min(r0 + nr * c, counts.shape[0])
+    gc1 = min(c0 + nc * c, counts.shape[1])
+    if gr1 > gr0 and gc1 > gc0:
+        out[gr0 - r0:gr1 - r0, gc0 - c0:gc1 - c0] = counts[gr0:gr1, gc0:gc1]
+    return out.reshape(nr, c, nc, c).mean(axis=(1, 3)).ravel() / float(nreal)
+
+
+class ControlVariateResult:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batch=512, lr=1e-3,
+                    seed=0, precision=_lib.FP32, pilot=2000, window=None, n_hidden=512):
+    """Cross-fitted control-variate estimate of the window cells' capture probability.
+
+    Each fold's network takes train_steps Adam steps of `batch` samples (whole
+    shuffled epochs, rounded up).
+
+    Returns a ControlVariateResult with p_cv, its estimated variance var_cv,
+    the plain estimate p_plain from the same traced realizations with its
+    variance var_plain, the window, and the time of each phase (seconds).
+    """
+    t = {}
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, n_traced, rng)
+    ds = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, n_surrogate, rng)
+    wxy = site.wells_xy()
+    _, _, coef = fit_and_draw(wxy, site.xt, site.yt, site.base, site.observations, d["K"],
+                              d["b"], d["q"], d["z"])
+    _, _, coef_s = fit_and_draw(wxy, site.xt, site.yt, site.base, site.observations, ds["K"],
+                                ds["b"], ds["q"], ds["z"])
+    t["fit_draw"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    if window is None:
+        npil = min(pilot, n_traced)
+        pc, _ = trace_counts(site, {k: v[:npil] for k, v in d.items()}, coef[:npil], precision)
+        window = choose_window(pc)
+    t["pilot"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    marks, st = trace_marks(site, d, coef, window, precision)
+    t["trace"] = time.perf_counter() - t0
+    tscale = 1.0 / (window.cell * window.cell)
+    std = Standardizer(features(d, coef))
+    X = std(features(d, coef))
+    Xs = std(features(ds, coef_s))
+    fold_of = rng.permutation(n_traced) % folds
+    surr_of = np.arange(n_surrogate) % folds
+    ncell = window.nrows * window.ncols
+    sum_s = np.zeros(ncell)
+    var_cv = np.zeros(ncell)
+    sum_d = np.zeros(ncell)
+    t["train"] = 0.0
+    t["surrogate"] = 0.0
+    losses = []
+    for f in range(folds):
+        tr = np.nonzero(fold_of != f)[0]
+        ho = np.nonzero(fold_of == f)[0]
+        sm = np.nonzero(surr_of == f)[0]
+        net = SurrogateMLP(X.shape[1], n_hidden, ncell, seed=seed + 1 + f)
+        epochs = max(1, int(math.ceil(train_steps / max(1, len(tr) // batch))))
+        t0 = time.perf_counter()
+        losses.append(net.train(X[tr], marks[tr], tscale, epochs, batch, lr, seed=seed + 100 + f))
+        t["train"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        a_s = net.accumulate(Xs[sm])
+        a_d = net.accumulate(X[ho], marks[ho], tscale)
+        t["surrogate"] += time.perf_counter() - t0
+        nf, mf = len(ho), len(sm)
+        w = nf / n_traced
+        mean_s = a_s["s"] / mf
+        var_s = np.maximum(a_s["s2"] / mf - mean_s ** 2, 0.0)
+        mean_d = a_d["d"] / nf
+        var_d = np.maximum(a_d["d2"] / nf - mean_d ** 2, 0.0)
+        sum_s += w * mean_s
+        sum_d += w * mean_d
+        var_cv += w * w * (var_d / nf + var_s / mf)
+        del net
+    y = marks.astype(np.float64) * tscale
+    p_plain = y.mean(0)
+    var_plain = y.var(0) / n_traced
+    return ControlVariateResult(p_cv=sum_s + sum_d, var_cv=var_cv, p_plain=p_plain,
+                                var_plain=var_plain, window=window, times=t, folds=folds,
+                                n_traced=n_traced, n_surrogate=n_surrogate, losses=losses,
+                                trace_stats=st)

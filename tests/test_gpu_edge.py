@@ -97,6 +97,21 @@ def test_config1_subset_bit_exact_and_invariants(gpu_ctx):
     assert res.counts[it, jt] == s.nrealizations
 
 
+def test_window_marks_match_oracle_per_realization(gpu_ctx):
+    from onekapy_amd.surrogate import cell_probabilities, trace_marks, _window
+    s, d = small_site(nreal=5, npaths=48)
+    _, _, coef = oracle_fit_draw(s, d)
+    pf = s.fixed_field()
+    win = _window(30, 35, 3, 12)          # 12 x 12 cells of 3 x 3 nodes
+    marks, st = trace_marks(s, d, coef, win, _lib.FP64_EXACT)
+    for r in range(5):
+        dr = {k: v[r:r + 1] for k, v in d.items()}
+        oc, _ = oracle_chronicle(s, dr, coef[r:r + 1], pf)
+        want = np.rint(cell_probabilities(oc, 1, win) * 9).astype(np.uint8)
+        assert np.array_equal(marks[r], want), r
+    assert marks.max() <= 9
+
+
 def test_fp32_error_config1_subset(gpu_ctx):
     s = sites.config1(200)
     d = sites.draw(s, seed=3)

@@ -1,0 +1,77 @@
+"""The surrogate network's BF16 MFMA kernels against the CPU oracle, and the
+learned control variate against the plain estimator."""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+from onekapy_amd import sites, _lib
+from onekapy_amd.surrogate import (SurrogateMLP, cell_probabilities, choose_window,
+                                   control_variate, trace_counts)
+from onekapy_amd.stochastic import draw_realizations, fit_and_draw
+
+pytestmark = pytest.mark.gpu
+
+
+def _net_and_batch(nin=34, nh=128, nout=256, B=128, seed=0):
+    net = SurrogateMLP(nin, nh, nout, max_batch=256, seed=seed)
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1, (B, nin)).astype(np.float32)
+    marks = rng.integers(0, 26, (B, nout)).astype(np.uint8)
+    return net, x, marks
+
+
+def test_forward_matches_oracle(gpu_ctx):
+    net, x, marks = _net_and_batch()
+    probs = net.predict(x)
+    p = {k: v.astype(np.float64) for k, v in net.split(net.get_params()).items()}
+    _, _, ref = mlp_ref.loss_and_grad(p, x, marks / 25.0, bf16=True)
+    assert np.abs(probs - ref).max() < 2e-3
+
+
+def test_gradient_matches_oracle(gpu_ctx):
+    net, x, marks = _net_and_batch(seed=3)
+    g, loss = net.grad(x, marks, 1.0 / 25.0)
+    p = {k: v.astype(np.float64) for k, v in net.split(net.get_params()).items()}
+    ref_loss, ref, _ = mlp_ref.loss_and_grad(p, x, marks / 25.0, bf16=True)
+    assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss) + 1e-6
+    gs = net.split(g)
+    for name in ("W1", "b1", "W2", "b2", "W3", "b3"):
+        a, b = gs[name].astype(np.float64).ravel(), ref[name].ravel()
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert rel < 2e-2, (name, rel)
+        assert a @ b / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.9995, name
+
+
+def test_training_fits_a_smooth_target(gpu_ctx):
+    rng = np.random.default_rng(5)
+    nin, nout = 34, 256
+    x = rng.normal(0, 1, (8192, nin)).astype(np.float32)
+    w = rng.normal(0, 1, (nin, nout))
+    y = 1.0 / (1.0 + np.exp(-(x @ w) / np.sqrt(nin) * 3.0))
+    marks = np.rint(y * 25.0).astype(np.uint8)
+    net = SurrogateMLP(nin, 256, nout, max_batch=512, seed=1)
+    losses = net.train(x, marks, 1.0 / 25.0, epochs=8, batch=256, lr=2e-3, seed=2)
+    assert losses[-20:].mean() < 0.6 * losses[:5].mean()
+    pred = net.predict(x[:512])
+    assert np.abs(pred - marks[:512] / 25.0).mean() < 0.08
+
+
+def test_control_variate_is_unbiased_and_reduces_variance(gpu_ctx):
+    s = sites.config0()
+    res = control_variate(s, n_traced=16384, n_surrogate=131072, folds=2, train_steps=1500,
+                          batch=512, lr=1e-3, seed=7, n_hidden=256)
+    # the judge: a large plain run (FP32 tracer, same window)
+    rng = np.random.default_rng(99)
+    R = 40000
+    d = draw_realizations(s.wells, s.c_dist, s.p_dist, s.t_dist, R, rng)
+    _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
+                              d["q"], d["z"])
+    counts, _ = trace_counts(s, d, coef, _lib.FP32)
+    p_ref = cell_probabilities(counts, R, res.window)
+    var_ref = p_ref * (1 - p_ref) / R
+    z = (res.p_cv - p_ref) / np.sqrt(res.var_cv + var_ref + 1e-12)
+    live = res.var_cv + var_ref > 1e-9
+    assert np.abs(z[live]).max() < 6.0
+    assert np.sqrt(np.mean(z[live] ** 2)) < 1.5
+    gain = res.var_plain.sum() / res.var_cv.sum()
+    assert gain > 3.0, gain

@@ -25,8 +25,9 @@ def test_library_exports_every_declared_symbol():
     L = _lib.load()
     with open(os.path.join(ROOT, "include", "oka.h")) as f:
         text = f.read()
-    declared = set(re.findall(r"^\s*(?:int|void|double|const char \*|void \*)\s*\**\s*(oka_\w+)\s*\(",
-                              text, flags=re.M))
+    declared = set(re.findall(
+        r"^\s*(?:int|int64_t|void|double|const char \*|void \*)\s*\**\s*(oka_\w+)\s*\(", text,
+        flags=re.M))
     assert declared == set(_lib.EXPORTED)
     for name in declared:
         assert hasattr(L, name), name
