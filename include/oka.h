@@ -276,6 +276,40 @@ int oka_mlp_accumulate(oka_mlp *mlp, int64_t n, const float *x, const uint8_t *m
                        float target_scale, double *sums, double *sums_sq, double *diffs,
                        double *diffs_sq, uint32_t flags);
 
+/* ---- realization draws on the device --------------------------------------
+ * The distribution vocabulary of generate_random_variate (stochastic.py:274-306:
+ * constant, uniform (lo, hi), triangular (lo, mode, hi)) plus lognormal
+ * (mu, sigma of the underlying normal), each optionally sign-flipped with
+ * probability p_negative (injection wells).  Draws come from Philox4x32-10
+ * (Salmon et al. 2011) keyed by seed, with counter (realization, slot), so any
+ * realization's draws are reproducible on their own. */
+enum { OKA_DIST_CONST = 0, OKA_DIST_UNIFORM = 1, OKA_DIST_TRIANGULAR = 2, OKA_DIST_LOGNORMAL = 3 };
+
+typedef struct {
+    int32_t kind;
+    int32_t reserved;
+    double a, b, c;          /* const: a; uniform: a, b; triangular: a, b (mode), c; lognormal: a (mu), b (sigma) */
+    double p_negative;
+} oka_dist;
+
+/* Realizations first .. first + n - 1 of the stream `seed`: q (n x nwells),
+ * K, por (porosity), thick (n) and z (n x 6 standard normals).  dists holds
+ * nwells discharge distributions then conductivity, porosity, thickness. */
+int oka_draw_realizations(oka_ctx *ctx, int32_t nwells, const oka_dist *dists, int64_t n,
+                          uint64_t seed, int64_t first, double *q, double *K, double *por,
+                          double *thick, double *z, uint32_t flags);
+
+/* The surrogate term of the control variate for n fresh realizations of the
+ * stream `seed` (starting at realization `first`), computed entirely on the
+ * device in chunks: draws, fit + MVN draw (as oka_fit_draw), standardized
+ * features (q, K, porosity, thickness, A..F) -> (f - mean) / std, network, and
+ * per output c: sums[c] += sum of s_c, sums_sq[c] += sum of s_c^2.  site and
+ * dists are host memory; feat_mean / feat_std (n_in doubles) too; sums and
+ * sums_sq follow flags. */
+int oka_cv_surrogate_sums(oka_ctx *ctx, oka_mlp *mlp, const oka_site *site, const oka_dist *dists,
+                          int64_t n, uint64_t seed, int64_t first, const double *feat_mean,
+                          const double *feat_std, double *sums, double *sums_sq, uint32_t flags);
+
 /* ---- exact scalar helpers shared by host and device code (for tests) ----- */
 double oka_pow_fifth(double x);   /* correctly rounded pow(x, 0.2) (capturezone.py:247) */
 double oka_log(double x);         /* correctly rounded log(x) (model.py:264)            */

@@ -39,7 +39,15 @@ EXPORTED = (
     "oka_rasterize", "oka_pow_fifth", "oka_log", "oka_trace_marks", "oka_mlp_create",
     "oka_mlp_destroy", "oka_mlp_param_count", "oka_mlp_set_params", "oka_mlp_get_params",
     "oka_mlp_grad", "oka_mlp_train", "oka_mlp_predict", "oka_mlp_accumulate",
+    "oka_draw_realizations", "oka_cv_surrogate_sums",
 )
+
+DIST_CONST, DIST_UNIFORM, DIST_TRIANGULAR, DIST_LOGNORMAL = 0, 1, 2, 3
+
+
+class Dist_(C.Structure):
+    _fields_ = [("kind", _i32), ("reserved", _i32), ("a", _d), ("b", _d), ("c", _d),
+                ("p_negative", _d)]
 
 
 class OkaLibraryError(RuntimeError):
@@ -166,6 +174,12 @@ def load():
     L.oka_mlp_predict.argtypes = [_vp, _i64, _vp, _vp, _u32]
     L.oka_mlp_accumulate.restype = C.c_int
     L.oka_mlp_accumulate.argtypes = [_vp, _i64, _vp, _vp, C.c_float, _vp, _vp, _vp, _vp, _u32]
+    L.oka_draw_realizations.restype = C.c_int
+    L.oka_draw_realizations.argtypes = [_vp, _i32, _vp, _i64, C.c_uint64, _i64, _vp, _vp, _vp,
+                                        _vp, _vp, _u32]
+    L.oka_cv_surrogate_sums.restype = C.c_int
+    L.oka_cv_surrogate_sums.argtypes = [_vp, _vp, C.POINTER(Site_), _vp, _i64, C.c_uint64, _i64,
+                                        _vp, _vp, _vp, _vp, _u32]
     L.oka_pow_fifth.restype = _d
     L.oka_pow_fifth.argtypes = [_d]
     L.oka_log.restype = _d

@@ -18,7 +18,8 @@ enum Slot {
     kSlotBbox, kSlotTStatus, kSlotTSteps, kSlotStats, kSlotEpochs, kSlotBitmap,
     kSlotVerts, kSlotNVerts, kSlotObs, kSlotLogs, kSlotZ, kSlotEv, kSlotCov, kSlotFStatus,
     kSlotXY, kSlotOut, kSlotEStatus, kSlotRgrid, kSlotMarks, kSlotMlpX, kSlotMlpMarks,
-    kSlotMlpOrder, kSlotMlpOut, kSlotMlpOut2, kSlotMlpLoss, kSlotCount
+    kSlotMlpOrder, kSlotMlpOut, kSlotMlpOut2, kSlotMlpLoss, kSlotCvDists, kSlotCvSite,
+    kSlotCvDraw, kSlotCount
 };
 
 struct DevBuf {

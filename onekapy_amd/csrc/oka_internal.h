@@ -137,6 +137,27 @@ struct FitArgs {
     int32_t *status;
 };
 
+struct DrawArgs {
+    int32_t nwells;
+    const oka_dist *dists;      // device: nwells + 3
+    int64_t n, first;
+    uint64_t seed;
+    double *q, *K, *por, *thick, *z;
+};
+
+struct FeatureArgs {
+    int32_t nwells;
+    int64_t n;
+    const double *q, *K, *por, *thick, *coef;
+    const double *mean, *std;   // device: nwells + 9
+    float *x;                   // n x (nwells + 9)
+};
+
+// oka_rng.hip
+hipError_t launch_draw(const DrawArgs &a, hipStream_t s);
+hipError_t launch_features(const FeatureArgs &a, hipStream_t s);
+uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, int word);
+
 // oka_fp64.hip
 hipError_t launch_trace_fp64(const TraceArgs &a, int grid_blocks, bool lds_bitmap,
                              hipStream_t s);

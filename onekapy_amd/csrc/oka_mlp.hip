@@ -66,15 +66,18 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
             acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // split-K (kEpiAtomicF32): this block sums K in [kbeg, kend)
+    const int kbeg = kEpi == kEpiAtomicF32 ? (int)blockIdx.z * g.ksplit : 0;
+    const int kend = kEpi == kEpiAtomicF32 ? kbeg + g.ksplit : g.K;
     const uint16_t *pa = g.A + (int64_t)(m0 + lr) * g.lda + lc;
     const uint16_t *pb = g.B + (int64_t)(n0 + lr) * g.ldb + lc;
-    uint4 ra = *reinterpret_cast<const uint4 *>(pa);
-    uint4 rb = *reinterpret_cast<const uint4 *>(pb);
-    for (int k0 = 0; k0 < g.K; k0 += kKStep) {
+    uint4 ra = *reinterpret_cast<const uint4 *>(pa + kbeg);
+    uint4 rb = *reinterpret_cast<const uint4 *>(pb + kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
         *reinterpret_cast<uint4 *>(&sA[lr][lc]) = ra;
         *reinterpret_cast<uint4 *>(&sB[lr][lc]) = rb;
         __syncthreads();
-        if (k0 + kKStep < g.K) {   // prefetch the next K slice while computing
+        if (k0 + kKStep < kend) {   // prefetch the next K slice while computing
             ra = *reinterpret_cast<const uint4 *>(pa + k0 + kKStep);
             rb = *reinterpret_cast<const uint4 *>(pb + k0 + kKStep);
         }
@@ -94,6 +97,8 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
 
     // epilogue: lane holds C[m = mb + 4*(lane/16) + e][n = nb + lane%16], e = 0..3
     double part_s = 0.0, part_d = 0.0, part_l = 0.0;
+    const int64_t step = g.dstate ? g.dstate[0] : 0;
+    const int64_t row_base = g.dstate ? step * g.step_rows : g.row0;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const int n = n0 + wn + j * 16 + (lane & 15);
@@ -120,9 +125,11 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
                     g.Ch[(int64_t)m * g.N + n] = hv[e];
                 } else if (kEpi == kEpiStoreF32) {
                     g.Cf[(int64_t)m * g.ldc + n] = v;
+                } else if (kEpi == kEpiAtomicF32) {
+                    atomicAdd(&g.Cf[(int64_t)m * g.ldc + n], v);
                 } else if (kEpi == kEpiLossGrad) {
                     const float z = v + bias;
-                    const int64_t row = g.rowidx ? g.rowidx[g.row0 + m] : g.row0 + m;
+                    const int64_t row = g.rowidx ? g.rowidx[row_base + m] : row_base + m;
                     const float y = (float)g.marks[row * g.N + n] * g.tscale;
                     hv[e] = f2bf((sigmoid(z) - y) * g.gscale);
                     g.Ch[(int64_t)m * g.N + n] = hv[e];
@@ -178,7 +185,7 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
         for (int off = 32; off > 0; off >>= 1)
             part_l += __shfl_xor(part_l, off);
         if (lane == 0)
-            atomicAdd(g.loss, part_l);
+            atomicAdd(g.loss + step, part_l);
     }
     (void)part_s;
     (void)part_d;
@@ -188,12 +195,15 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
 // (BF16 [M, kin], zero padded) and X0^T ([kin, M]); rows >= mvalid are zero.
 __global__ void __launch_bounds__(256) k_prep_input(const float *x, int n_in, int kin,
                                                     const int32_t *rowidx, int64_t row0, int M,
-                                                    int mvalid, uint16_t *x0, uint16_t *x0t)
+                                                    int mvalid, uint16_t *x0, uint16_t *x0t,
+                                                    const int64_t *dstate)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)M * kin)
         return;
     const int m = (int)(t / kin), k = (int)(t % kin);
+    if (dstate)
+        row0 = dstate[0] * M;
     float v = 0.f;
     if (m < mvalid && k < n_in) {
         const int64_t row = rowidx ? rowidx[row0 + m] : row0 + m;
@@ -225,11 +235,16 @@ __global__ void __launch_bounds__(256) k_bias_grad(const uint16_t *gt, int M, fl
 
 __global__ void __launch_bounds__(256) k_adam(float *p, const float *grad, float *m, float *v,
                                               int64_t n, float lr, float b1, float b2, float eps,
-                                              float c1, float c2)
+                                              int64_t t, const int64_t *dstate)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    if (dstate)
+        t = dstate[1] + 1;
+    // bias corrections 1/(1 - beta^t)
+    const float c1 = (float)(1.0 / (1.0 - exp((double)t * log((double)b1))));
+    const float c2 = (float)(1.0 / (1.0 - exp((double)t * log((double)b2))));
     const float gi = grad[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -238,17 +253,67 @@ __global__ void __launch_bounds__(256) k_adam(float *p, const float *grad, float
     p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 }
 
-// FP32 master W [rows, cols] -> BF16 W and W^T.
+// FP32 master W [rows, cols] -> BF16 W and W^T, one 32 x 32 tile per
+// workgroup through LDS so that both stores are coalesced.
 __global__ void __launch_bounds__(256) k_refresh_bf16(const float *w, int rows, int cols,
                                                       uint16_t *wh, uint16_t *wt)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)rows * cols)
-        return;
-    const int r = (int)(t / cols), c = (int)(t % cols);
-    const uint16_t h = f2bf(w[t]);
-    wh[t] = h;
-    wt[(int64_t)c * rows + r] = h;
+    __shared__ uint16_t tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) {
+            const uint16_t h = f2bf(w[(int64_t)r * cols + c]);
+            wh[(int64_t)r * cols + c] = h;
+            tile[k][tx] = h;
+        }
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols)
+            wt[(int64_t)c * rows + r] = tile[tx][k];
+    }
+}
+
+// G = acc * gelu'(Z) as BF16, row-major and transposed (after a split-K product).
+__global__ void __launch_bounds__(256) k_gelu_bwd(const float *acc, const float *z, int M, int N,
+                                                  uint16_t *g, uint16_t *gt)
+{
+    __shared__ uint16_t tile[32][33];
+    const int n0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int m = m0 + k, n = n0 + tx;
+        if (m < M && n < N) {
+            const int64_t i = (int64_t)m * N + n;
+            const uint16_t h = f2bf(acc[i] * gelu_grad(z[i]));
+            g[i] = h;
+            tile[k][tx] = h;
+        }
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int n = n0 + k, m = m0 + tx;
+        if (m < M && n < N)
+            gt[(int64_t)n * M + m] = tile[tx][k];
+    }
+}
+
+__global__ void k_step_advance(int64_t *dstate)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        dstate[0] += 1;
+        dstate[1] += 1;
+    }
+}
+
+__global__ void k_fill_f32(float *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0.f;
 }
 
 __global__ void k_zero_f64(double *p, int64_t n)
@@ -262,9 +327,9 @@ __global__ void k_zero_f64(double *p, int64_t n)
 
 using namespace mlp;
 
-hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s)
+hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s, int splits)
 {
-    dim3 grid(g.N / kTile, g.M / kTile), block(256);
+    dim3 grid(g.N / kTile, g.M / kTile, epi == kEpiAtomicF32 ? splits : 1), block(256);
     switch (epi) {
     case kEpiGeluFwd: hipLaunchKernelGGL(k_gemm_nt<kEpiGeluFwd>, grid, block, 0, s, g); break;
     case kEpiLogits: hipLaunchKernelGGL(k_gemm_nt<kEpiLogits>, grid, block, 0, s, g); break;
@@ -273,6 +338,7 @@ hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s)
     case kEpiLossGrad: hipLaunchKernelGGL(k_gemm_nt<kEpiLossGrad>, grid, block, 0, s, g); break;
     case kEpiAccum: hipLaunchKernelGGL(k_gemm_nt<kEpiAccum>, grid, block, 0, s, g); break;
     case kEpiProbs: hipLaunchKernelGGL(k_gemm_nt<kEpiProbs>, grid, block, 0, s, g); break;
+    case kEpiAtomicF32: hipLaunchKernelGGL(k_gemm_nt<kEpiAtomicF32>, grid, block, 0, s, g); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -280,11 +346,11 @@ hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s)
 
 hipError_t launch_prep_input(const float *x, int n_in, int kin, const int32_t *rowidx,
                              int64_t row0, int M, int mvalid, uint16_t *x0, uint16_t *x0t,
-                             hipStream_t s)
+                             hipStream_t s, const int64_t *dstate)
 {
     const int64_t n = (int64_t)M * kin;
     hipLaunchKernelGGL(k_prep_input, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n_in,
-                       kin, rowidx, row0, M, mvalid, x0, x0t);
+                       kin, rowidx, row0, M, mvalid, x0, x0t, dstate);
     return hipGetLastError();
 }
 
@@ -295,19 +361,41 @@ hipError_t launch_bias_grad(const uint16_t *gt, int N, int M, float *db, hipStre
 }
 
 hipError_t launch_adam(float *p, const float *grad, float *m, float *v, int64_t n, float lr,
-                       float b1, float b2, float eps, float c1, float c2, hipStream_t s)
+                       float b1, float b2, float eps, int64_t t, const int64_t *dstate,
+                       hipStream_t s)
 {
     hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, grad, m, v,
-                       n, lr, b1, b2, eps, c1, c2);
+                       n, lr, b1, b2, eps, t, dstate);
     return hipGetLastError();
 }
 
 hipError_t launch_refresh_bf16(const float *w, int rows, int cols, uint16_t *wh, uint16_t *wt,
                                hipStream_t s)
 {
-    const int64_t n = (int64_t)rows * cols;
-    hipLaunchKernelGGL(k_refresh_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, w,
-                       rows, cols, wh, wt);
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+    hipLaunchKernelGGL(k_refresh_bf16, grid, dim3(256), 0, s, w, rows, cols, wh, wt);
+    return hipGetLastError();
+}
+
+hipError_t launch_gelu_bwd(const float *acc, const float *z, int M, int N, uint16_t *g,
+                           uint16_t *gt, hipStream_t s)
+{
+    dim3 grid((N + 31) / 32, (M + 31) / 32);
+    hipLaunchKernelGGL(k_gelu_bwd, grid, dim3(256), 0, s, acc, z, M, N, g, gt);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_advance(int64_t *dstate, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, dstate);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float *p, int64_t n, hipStream_t s)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

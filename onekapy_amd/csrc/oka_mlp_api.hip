@@ -24,6 +24,8 @@ struct oka_mlp {
              *h2t = nullptr, *g3 = nullptr, *g3t = nullptr, *g2 = nullptr, *g2t = nullptr,
              *g1 = nullptr, *g1t = nullptr;
     float *z1 = nullptr, *z2 = nullptr;
+    float *acc = nullptr;        // split-K partial sums [max_batch x n_hidden]
+    int64_t *dstate = nullptr;   // graph replay: [0] step within the call, [1] Adam step
     int64_t step = 0;
     std::vector<void *> allocs;
 };
@@ -62,14 +64,43 @@ GemmArgs gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int Mr, in
     return g;
 }
 
-// Forward through the two hidden layers for rows of x (rowidx/row0) into the
-// workspace; Bp rows (a multiple of 64), the first `valid` of them real.
-int hidden_forward(oka_mlp *M, int Bp, int valid, const float *x, const int32_t *rowidx,
-                   int64_t row0)
+// G = (A B^T) * gelu'(Z) in BF16 (+ transposed).  Products whose output tiles
+// would not fill the GPU are split over K into FP32 partial sums first.
+int gelu_bwd_product(oka_mlp *M, const uint16_t *A, int lda, const uint16_t *B, int ldb, int Mr,
+                     int N, int K, const float *Z, uint16_t *G, uint16_t *Gt)
 {
     oka_ctx *c = M->ctx;
     hipStream_t s = c->stream;
-    HIPCHK(c, launch_prep_input(x, M->nin, M->kin, rowidx, row0, Bp, valid, M->x0, M->x0t, s));
+    const int tiles = (Mr / 64) * (N / 64);
+    int splits = 1;
+    while (tiles * splits < 2 * c->num_cu && K / (splits * 2) >= 128 && K % (splits * 2 * 32) == 0)
+        splits *= 2;
+    GemmArgs g = gemm(A, lda, B, ldb, Mr, N, K);
+    if (splits == 1) {
+        g.Zsaved = Z;
+        g.Ch = G;
+        g.Cht = Gt;
+        HIPCHK(c, launch_gemm(kEpiGeluBwd, g, s));
+        return OKA_OK;
+    }
+    HIPCHK(c, launch_fill_f32(M->acc, (int64_t)Mr * N, s));
+    g.Cf = M->acc;
+    g.ldc = N;
+    g.ksplit = K / splits;
+    HIPCHK(c, launch_gemm(kEpiAtomicF32, g, s, splits));
+    HIPCHK(c, launch_gelu_bwd(M->acc, Z, Mr, N, G, Gt, s));
+    return OKA_OK;
+}
+
+// Forward through the two hidden layers for rows of x (rowidx/row0) into the
+// workspace; Bp rows (a multiple of 64), the first `valid` of them real.
+int hidden_forward(oka_mlp *M, int Bp, int valid, const float *x, const int32_t *rowidx,
+                   int64_t row0, const int64_t *dstate = nullptr)
+{
+    oka_ctx *c = M->ctx;
+    hipStream_t s = c->stream;
+    HIPCHK(c, launch_prep_input(x, M->nin, M->kin, rowidx, row0, Bp, valid, M->x0, M->x0t, s,
+                                dstate));
     GemmArgs g = gemm(M->x0, M->kin, M->w1h, M->kin, Bp, M->nh, M->kin);
     g.bias = M->p + M->ob1;
     g.Cf = M->z1;
@@ -89,16 +120,19 @@ int hidden_forward(oka_mlp *M, int Bp, int valid, const float *x, const int32_t 
 
 // Loss and full gradient (into M->grad) for one batch of B rows.
 int forward_backward(oka_mlp *M, int B, const float *x, const uint8_t *marks, float tscale,
-                     const int32_t *rowidx, int64_t row0, double *loss)
+                     const int32_t *rowidx, int64_t row0, double *loss,
+                     const int64_t *dstate = nullptr)
 {
     oka_ctx *c = M->ctx;
     hipStream_t s = c->stream;
-    TRY(hidden_forward(M, B, B, x, rowidx, row0));
+    TRY(hidden_forward(M, B, B, x, rowidx, row0, dstate));
     GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, B, M->nout, M->nh);
     g.bias = M->p + M->ob3;
     g.marks = marks;
     g.rowidx = rowidx;
     g.row0 = row0;
+    g.dstate = dstate;
+    g.step_rows = B;
     g.tscale = tscale;
     g.gscale = 1.0f / ((float)B * (float)M->nout);
     g.Ch = M->g3;
@@ -111,22 +145,15 @@ int forward_backward(oka_mlp *M, int B, const float *x, const uint8_t *marks, fl
     g.ldc = M->nh;
     HIPCHK(c, launch_gemm(kEpiStoreF32, g, s));
     HIPCHK(c, launch_bias_grad(M->g3t, M->nout, B, M->grad + M->ob3, s));
-    g = gemm(M->g3, M->nout, M->w3t, M->nout, B, M->nh, M->nout);
-    g.Zsaved = M->z2;
-    g.Ch = M->g2;
-    g.Cht = M->g2t;
-    HIPCHK(c, launch_gemm(kEpiGeluBwd, g, s));
+    TRY(gelu_bwd_product(M, M->g3, M->nout, M->w3t, M->nout, B, M->nh, M->nout, M->z2, M->g2,
+                         M->g2t));
     // layer 2
     g = gemm(M->g2t, B, M->h1t, B, M->nh, M->nh, B);
     g.Cf = M->grad + M->oW2;
     g.ldc = M->nh;
     HIPCHK(c, launch_gemm(kEpiStoreF32, g, s));
     HIPCHK(c, launch_bias_grad(M->g2t, M->nh, B, M->grad + M->ob2, s));
-    g = gemm(M->g2, M->nh, M->w2t, M->nh, B, M->nh, M->nh);
-    g.Zsaved = M->z1;
-    g.Ch = M->g1;
-    g.Cht = M->g1t;
-    HIPCHK(c, launch_gemm(kEpiGeluBwd, g, s));
+    TRY(gelu_bwd_product(M, M->g2, M->nh, M->w2t, M->nh, B, M->nh, M->nh, M->z1, M->g1, M->g1t));
     // layer 1
     g = gemm(M->g1t, B, M->x0t, B, M->nh, M->kin, B);
     g.Cf = M->grad + M->oW1;
@@ -231,6 +258,8 @@ int oka_mlp_create(oka_ctx *c, int32_t n_in, int32_t n_hidden, int32_t n_out, in
     rc = rc ? rc : alloc_t(M, B * M->nh, &M->g1t);
     rc = rc ? rc : alloc_t(M, B * M->nh, &M->z1);
     rc = rc ? rc : alloc_t(M, B * M->nh, &M->z2);
+    rc = rc ? rc : alloc_t(M, B * M->nh, &M->acc);
+    rc = rc ? rc : alloc_t(M, 2, &M->dstate);
     if (rc == OKA_OK) {
         // zero state (padding columns of W1 must stay exactly zero)
         for (float *q : {M->p, M->grad, M->m, M->v})
@@ -332,15 +361,43 @@ int oka_mlp_train(oka_mlp *M, int64_t nsamples, const float *x, const uint8_t *m
     void *dl;
     TRY(ensure(c, kSlotMlpLoss, sizeof(double) * std::max(nsteps, 1), &dl));
     HIPCHK(c, launch_zero_f64((double *)dl, nsteps, c->stream));
-    for (int32_t st = 0; st < nsteps; ++st) {
-        TRY(forward_backward(M, batch, (const float *)dx, (const uint8_t *)dm, target_scale,
-                             (const int32_t *)dord, (int64_t)st * batch, (double *)dl + st));
-        M->step += 1;
-        const float c1 = (float)(1.0 / (1.0 - pow((double)beta1, (double)M->step)));
-        const float c2 = (float)(1.0 / (1.0 - pow((double)beta2, (double)M->step)));
-        HIPCHK(c, launch_adam(M->p, M->grad, M->m, M->v, M->np, lr, beta1, beta2, eps, c1, c2,
-                              c->stream));
-        TRY(refresh_bf16(M));
+    // One training step (forward, backward, Adam, BF16 refresh) is captured as a
+    // HIP graph and replayed nsteps times; the step-dependent values (batch
+    // rows, loss slot, Adam step) come from the device counters M->dstate.
+    const int64_t st0[2] = {0, M->step};
+    HIPCHK(c, hipMemcpyAsync(M->dstate, st0, sizeof(st0), hipMemcpyHostToDevice, c->stream));
+    if (nsteps > 0) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = forward_backward(M, batch, (const float *)dx, (const uint8_t *)dm, target_scale,
+                                  (const int32_t *)dord, 0, (double *)dl, M->dstate);
+        if (rc == OKA_OK && launch_adam(M->p, M->grad, M->m, M->v, M->np, lr, beta1, beta2, eps,
+                                        0, M->dstate, c->stream) != hipSuccess)
+            rc = fail(c, OKA_ERR_DEVICE, "adam launch failed");
+        if (rc == OKA_OK)
+            rc = refresh_bf16(M);
+        if (rc == OKA_OK && launch_step_advance(M->dstate, c->stream) != hipSuccess)
+            rc = fail(c, OKA_ERR_DEVICE, "step launch failed");
+        hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        if (rc != OKA_OK) {
+            if (graph)
+                (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIPCHK(c, ec);
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            HIPCHK(c, ei);
+        }
+        hipError_t el = hipSuccess;
+        for (int32_t st = 0; st < nsteps && el == hipSuccess; ++st)
+            el = hipGraphLaunch(exec, c->stream);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(c, el);
+        M->step += nsteps;
     }
     if (losses && nsteps > 0) {
         std::vector<double> h(nsteps);
@@ -435,6 +492,187 @@ int oka_mlp_accumulate(oka_mlp *M, int64_t n, const float *x, const uint8_t *mar
         return OKA_OK;
     double *acc[4] = {sums, sums_sq, diffs, diffs_sq};
     return infer(M, n, x, marks, target_scale, nullptr, acc, flags);
+}
+
+}  // extern "C"
+
+// ---- realization draws and the device-side surrogate term -----------------
+
+namespace {
+
+int check_dists(oka_ctx *c, int32_t nwells, const oka_dist *dists)
+{
+    if (nwells < 0 || !dists)
+        return fail(c, OKA_ERR_ARG, "bad distributions");
+    for (int32_t i = 0; i < nwells + 3; ++i) {
+        const oka_dist &d = dists[i];
+        if (d.kind < OKA_DIST_CONST || d.kind > OKA_DIST_LOGNORMAL)
+            return fail(c, OKA_ERR_ARG, "unknown distribution kind %d", d.kind);
+        if (d.kind == OKA_DIST_TRIANGULAR && !(d.a <= d.b && d.b <= d.c && d.a < d.c))
+            return fail(c, OKA_ERR_ARG, "triangular needs lo <= mode <= hi, lo < hi");
+        if (!(d.p_negative >= 0.0 && d.p_negative <= 1.0))
+            return fail(c, OKA_ERR_ARG, "p_negative must be in [0, 1]");
+    }
+    return OKA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oka_draw_realizations(oka_ctx *c, int32_t nwells, const oka_dist *dists, int64_t n,
+                          uint64_t seed, int64_t first, double *q, double *K, double *por,
+                          double *thick, double *z, uint32_t flags)
+{
+    if (!c)
+        return OKA_ERR_ARG;
+    TRY(check_dists(c, nwells, dists));
+    if (n < 0 || first < 0)
+        return fail(c, OKA_ERR_ARG, "negative count or offset");
+    if (n == 0)
+        return OKA_OK;
+    if ((nwells > 0 && !q) || !K || !por || !thick || !z)
+        return fail(c, OKA_ERR_ARG, "missing outputs");
+    HIPCHK(c, hipSetDevice(c->device));
+    DrawArgs a{};
+    a.nwells = nwells;
+    a.n = n;
+    a.first = first;
+    a.seed = seed;
+    const void *dd;
+    TRY(stage_in(c, kSlotCvDists, dists, sizeof(oka_dist) * (nwells + 3), 0, &dd));
+    a.dists = (const oka_dist *)dd;
+    void *o;
+    TRY(stage_out(c, kSlotQ, q, sizeof(double) * n * nwells, flags, false, &o));
+    a.q = (double *)o;
+    TRY(stage_out(c, kSlotK, K, sizeof(double) * n, flags, false, &o));
+    a.K = (double *)o;
+    TRY(stage_out(c, kSlotN, por, sizeof(double) * n, flags, false, &o));
+    a.por = (double *)o;
+    TRY(stage_out(c, kSlotB, thick, sizeof(double) * n, flags, false, &o));
+    a.thick = (double *)o;
+    TRY(stage_out(c, kSlotZ, z, sizeof(double) * n * 6, flags, false, &o));
+    a.z = (double *)o;
+    HIPCHK(c, launch_draw(a, c->stream));
+    TRY(copy_back(c, q, a.q, sizeof(double) * n * nwells, flags));
+    TRY(copy_back(c, K, a.K, sizeof(double) * n, flags));
+    TRY(copy_back(c, por, a.por, sizeof(double) * n, flags));
+    TRY(copy_back(c, thick, a.thick, sizeof(double) * n, flags));
+    TRY(copy_back(c, z, a.z, sizeof(double) * n * 6, flags));
+    return sync_unless_async(c, flags);
+}
+
+int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist *dists,
+                          int64_t n, uint64_t seed, int64_t first, const double *feat_mean,
+                          const double *feat_std, double *sums, double *sums_sq, uint32_t flags)
+{
+    if (!c || !M || !site || !feat_mean || !feat_std || !sums)
+        return OKA_ERR_ARG;
+    TRY(check_dists(c, site->nwells, dists));
+    const int nw = site->nwells, nobs = site->nobs, nin = nw + 9;
+    if (M->nin != nin)
+        return fail(c, OKA_ERR_ARG, "network has %d inputs, the site needs %d", M->nin, nin);
+    if (nobs < 6 || !site->obs || (nw > 0 && !site->wells_xy))
+        return fail(c, OKA_ERR_ARG, "site needs wells and at least 6 observations");
+    if (n < 0 || first < 0)
+        return fail(c, OKA_ERR_ARG, "negative count or offset");
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    // site, distributions, standardization (host memory)
+    const size_t site_doubles = (size_t)2 * nw + 4 * nobs + (size_t)nobs * nw + 2 * nin;
+    void *sb;
+    TRY(ensure(c, kSlotCvSite, sizeof(double) * site_doubles, &sb));
+    double *d_wells = (double *)sb, *d_obs = d_wells + 2 * nw, *d_logr2 = d_obs + 4 * nobs;
+    double *d_mean = d_logr2 + (size_t)nobs * nw, *d_std = d_mean + nin;
+    HIPCHK(c, hipMemcpyAsync(d_wells, site->wells_xy, sizeof(double) * 2 * nw,
+                             hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(d_obs, site->obs, sizeof(double) * 4 * nobs, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(d_mean, feat_mean, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(d_std, feat_std, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+    const void *dd;
+    TRY(stage_in(c, kSlotCvDists, dists, sizeof(oka_dist) * (nw + 3), 0, &dd));
+    // per-chunk scratch
+    const int64_t chunk = M->max_batch;
+    const size_t per = (size_t)nw + 3 + 6 + 6;   // q, K, por, thick, z, coef (doubles)
+    void *wb;
+    TRY(ensure(c, kSlotCvDraw, (sizeof(double) * per + sizeof(float) * nin + 4) * chunk + 256,
+               &wb));
+    double *q = (double *)wb, *K = q + chunk * nw, *por = K + chunk, *thick = por + chunk;
+    double *z = thick + chunk, *coef = z + chunk * 6;
+    float *x = (float *)(coef + chunk * 6);
+    int32_t *status = (int32_t *)(x + chunk * nin);
+    // accumulators
+    void *acc_base = nullptr;
+    double *d_sums = sums, *d_sq = sums_sq;
+    if (!(flags & OKA_PTR_DEVICE)) {
+        TRY(ensure(c, kSlotMlpOut2, sizeof(double) * 2 * M->nout, &acc_base));
+        d_sums = (double *)acc_base;
+        d_sq = sums_sq ? d_sums + M->nout : nullptr;
+        HIPCHK(c, hipMemcpyAsync(d_sums, sums, sizeof(double) * M->nout, hipMemcpyHostToDevice, s));
+        if (sums_sq)
+            HIPCHK(c, hipMemcpyAsync(d_sq, sums_sq, sizeof(double) * M->nout,
+                                     hipMemcpyHostToDevice, s));
+    }
+    FitArgs fa{};
+    fa.nwells = nw;
+    fa.nobs = nobs;
+    fa.wells_xy = d_wells;
+    fa.obs = d_obs;
+    fa.xo = site->xo;
+    fa.yo = site->yo;
+    fa.base = site->base;
+    fa.logr2 = d_logr2;
+    fa.K = K;
+    fa.b = thick;
+    fa.q = q;
+    fa.z = z;
+    fa.coef = coef;
+    fa.status = status;
+    HIPCHK(c, launch_fit_logs(fa, d_logr2, s));
+    for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+        const int valid = (int)std::min<int64_t>(chunk, n - r0);
+        DrawArgs da{};
+        da.nwells = nw;
+        da.dists = (const oka_dist *)dd;
+        da.n = valid;
+        da.first = first + r0;
+        da.seed = seed;
+        da.q = q;
+        da.K = K;
+        da.por = por;
+        da.thick = thick;
+        da.z = z;
+        HIPCHK(c, launch_draw(da, s));
+        fa.nreal = valid;
+        HIPCHK(c, launch_fit_draw(fa, s));
+        FeatureArgs fe{};
+        fe.nwells = nw;
+        fe.n = valid;
+        fe.q = q;
+        fe.K = K;
+        fe.por = por;
+        fe.thick = thick;
+        fe.coef = coef;
+        fe.mean = d_mean;
+        fe.std = d_std;
+        fe.x = x;
+        HIPCHK(c, launch_features(fe, s));
+        const int Bp = (valid + 63) / 64 * 64;
+        TRY(hidden_forward(M, Bp, valid, x, nullptr, 0));
+        GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
+        g.bias = M->p + M->ob3;
+        g.mvalid = valid;
+        g.sums = d_sums;
+        g.sums_sq = d_sq;
+        HIPCHK(c, launch_gemm(kEpiAccum, g, s));
+    }
+    if (!(flags & OKA_PTR_DEVICE)) {
+        HIPCHK(c, hipMemcpyAsync(sums, d_sums, sizeof(double) * M->nout, hipMemcpyDeviceToHost, s));
+        if (sums_sq)
+            HIPCHK(c, hipMemcpyAsync(sums_sq, d_sq, sizeof(double) * M->nout,
+                                     hipMemcpyDeviceToHost, s));
+    }
+    return sync_unless_async(c, flags);
 }
 
 }  // extern "C"

@@ -1,0 +1,152 @@
+// oka_rng.hip -- realization draws on the device and the surrogate's features.
+//
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC 2011) is a counter-based generator: the 128-bit output for
+// counter (realization, slot, sub) and key (seed) needs no state, so every
+// realization's draws are reproducible on their own and the launch shape does
+// not matter.
+#include <math.h>
+
+#include "oka_internal.h"
+
+namespace oka {
+namespace {
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+    return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
+__host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1)
+{
+    for (int i = 0; i < 10; ++i) {
+        const uint32_t hi0 = mulhi32(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = mulhi32(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Two uniforms in (0, 1) with 53 random bits each.
+__device__ inline void uniforms(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, double &u1,
+                                double &u2)
+{
+    const U4 o = philox4x32_10(U4{(uint32_t)r, (uint32_t)((uint64_t)r >> 32), slot, sub},
+                               (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t a = ((uint64_t)o.x << 21) ^ (uint64_t)(o.y >> 11);
+    const uint64_t b = ((uint64_t)o.z << 21) ^ (uint64_t)(o.w >> 11);
+    u1 = ((double)(a & ((1ull << 53) - 1)) + 0.5) * 0x1p-53;
+    u2 = ((double)(b & ((1ull << 53) - 1)) + 0.5) * 0x1p-53;
+}
+
+__device__ inline double sample(const oka_dist &d, uint64_t seed, int64_t r, uint32_t slot)
+{
+    double u1, u2;
+    uniforms(seed, r, slot, 0, u1, u2);
+    double v, us = u2;
+    switch (d.kind) {
+    case OKA_DIST_UNIFORM:
+        v = d.a + (d.b - d.a) * u1;
+        break;
+    case OKA_DIST_TRIANGULAR: {
+        // inverse CDF of triangular(lo = a, mode = b, hi = c)
+        const double fc = (d.b - d.a) / (d.c - d.a);
+        v = u1 < fc ? d.a + sqrt(u1 * (d.c - d.a) * (d.b - d.a))
+                    : d.c - sqrt((1.0 - u1) * (d.c - d.a) * (d.c - d.b));
+        break;
+    }
+    case OKA_DIST_LOGNORMAL: {
+        const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        v = exp(d.a + d.b * g);
+        double w;
+        uniforms(seed, r, slot, 1, us, w);
+        break;
+    }
+    default:
+        v = d.a;
+        break;
+    }
+    if (d.p_negative > 0.0 && us < d.p_negative)
+        v = -v;
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_draw(DrawArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n)
+        return;
+    const int64_t r = a.first + i;
+    const int nw = a.nwells;
+    for (int w = 0; w < nw; ++w)
+        a.q[i * nw + w] = sample(a.dists[w], a.seed, r, (uint32_t)w);
+    a.K[i] = sample(a.dists[nw], a.seed, r, (uint32_t)nw);
+    a.por[i] = sample(a.dists[nw + 1], a.seed, r, (uint32_t)nw + 1);
+    a.thick[i] = sample(a.dists[nw + 2], a.seed, r, (uint32_t)nw + 2);
+    for (int j = 0; j < 3; ++j) {
+        double u1, u2;
+        uniforms(a.seed, r, (uint32_t)(nw + 3 + j), 0, u1, u2);
+        const double rad = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincos(6.283185307179586 * u2, &sn, &cs);
+        a.z[i * 6 + 2 * j] = rad * cs;
+        a.z[i * 6 + 2 * j + 1] = rad * sn;
+    }
+}
+
+// x[i] = (f - mean) / std with f = (q_1..q_nw, K, porosity, thickness, A..F).
+__global__ void __launch_bounds__(256) k_features(FeatureArgs a)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nin = a.nwells + 9;
+    if (t >= a.n * nin)
+        return;
+    const int64_t i = t / nin;
+    const int j = (int)(t % nin);
+    double f;
+    if (j < a.nwells)
+        f = a.q[i * a.nwells + j];
+    else if (j == a.nwells)
+        f = a.K[i];
+    else if (j == a.nwells + 1)
+        f = a.por[i];
+    else if (j == a.nwells + 2)
+        f = a.thick[i];
+    else
+        f = a.coef[i * 6 + (j - a.nwells - 3)];
+    a.x[t] = (float)((f - a.mean[j]) / a.std[j]);
+}
+
+}  // namespace
+
+hipError_t launch_draw(const DrawArgs &a, hipStream_t s)
+{
+    if (a.n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_draw, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_features(const FeatureArgs &a, hipStream_t s)
+{
+    const int64_t n = a.n * (a.nwells + 9);
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, int word)
+{
+    const U4 o = philox4x32_10(U4{(uint32_t)r, (uint32_t)((uint64_t)r >> 32), slot, sub},
+                               (uint32_t)seed, (uint32_t)(seed >> 32));
+    return word == 0 ? o.x : word == 1 ? o.y : word == 2 ? o.z : o.w;
+}
+
+}  // namespace oka

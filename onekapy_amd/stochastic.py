@@ -19,6 +19,7 @@ The reference draws from numpy's unseeded global RandomState and an unseeded
 default_rng per realization; here one seeded numpy Generator makes a run
 reproducible.
 """
+import ctypes as C
 import logging
 
 import numpy as np
@@ -121,6 +122,49 @@ def isdistribution(arg, lb, ub):
     return False
 
 
+def oka_dist(arg):
+    """The library's description (oka_dist) of one distribution of the vocabulary."""
+    d = _lib.Dist_()
+    if isinstance(arg, Signed):
+        d = oka_dist(arg.dist)
+        d.p_negative = arg.p
+        return d
+    if isinstance(arg, Lognormal):
+        d.kind, d.a, d.b = _lib.DIST_LOGNORMAL, arg.mu, arg.sigma
+    elif type(arg) is not tuple:
+        d.kind, d.a = _lib.DIST_CONST, float(arg)
+    elif len(arg) == 2:
+        d.kind, d.a, d.b = _lib.DIST_UNIFORM, float(arg[0]), float(arg[1])
+    elif len(arg) == 3:
+        d.kind, d.a, d.b, d.c = _lib.DIST_TRIANGULAR, float(arg[0]), float(arg[1]), float(arg[2])
+    else:
+        raise DistributionError('<arg> must be a scalar, pair, or triple.')
+    return d
+
+
+def oka_dists(stochastic_wells, c_dist, p_dist, t_dist):
+    """Array of oka_dist: each well's discharge, then K, porosity, thickness."""
+    items = [oka_dist(w[3]) for w in stochastic_wells] + [oka_dist(c_dist), oka_dist(p_dist),
+                                                          oka_dist(t_dist)]
+    return (_lib.Dist_ * len(items))(*items)
+
+
+def draw_realizations_device(stochastic_wells, c_dist, p_dist, t_dist, nrealizations, seed,
+                             first=0):
+    """The same draws as draw_realizations, from the library's Philox4x32-10
+    stream `seed` (realizations first .. first + n - 1)."""
+    R, nw = int(nrealizations), len(stochastic_wells)
+    out = dict(q=np.zeros((R, nw)), K=np.zeros(R), n=np.zeros(R), b=np.zeros(R),
+               z=np.zeros((R, 6)))
+    dists = oka_dists(stochastic_wells, c_dist, p_dist, t_dist)
+    ctx = _lib.context()
+    ctx.check(ctx.lib.oka_draw_realizations(
+        ctx.h, nw, C.addressof(dists), R, int(seed) & (2 ** 64 - 1), int(first),
+        _lib.ptr(out["q"]), _lib.ptr(out["K"]), _lib.ptr(out["n"]), _lib.ptr(out["b"]),
+        _lib.ptr(out["z"]), 0))
+    return out
+
+
 def draw_realizations(stochastic_wells, c_dist, p_dist, t_dist, nrealizations, rng):
     """Host draws of every realization's random inputs (stochastic.py:222-241).
 
@@ -140,10 +184,11 @@ def draw_realizations(stochastic_wells, c_dist, p_dist, t_dist, nrealizations, r
     return dict(q=q, K=K, n=n, b=b, z=z)
 
 
-def fit_and_draw(wells_xy, xo, yo, base, observations, K, b, q, z):
+def fit_and_draw(wells_xy, xo, yo, base, observations, K, b, q, z, moments=True):
     """GPU fit of every realization and its MVN draw (model.py:430-606, stochastic.py:239-241).
 
     Returns (ev [R, 6], cov [R, 6, 6], coef [R, 6]).  z may be None (coef = ev).
+    With moments=False only coef is computed and ev, cov are None.
     """
     from .model import RangeError
     obs = _lib.f64([[float(v) for v in o[:4]] for o in observations]).reshape(-1, 4)
@@ -152,8 +197,8 @@ def fit_and_draw(wells_xy, xo, yo, base, observations, K, b, q, z):
     R = K.shape[0]
     q = _lib.f64(q).reshape(R, wells_xy.shape[0])
     zz = None if z is None else _lib.f64(z).reshape(R, 6)
-    ev = np.zeros((R, 6))
-    cov = np.zeros((R, 6, 6))
+    ev = np.zeros((R, 6)) if moments else None
+    cov = np.zeros((R, 6, 6)) if moments else None
     coef = np.zeros((R, 6))
     status = np.zeros(R, dtype=np.int32)
     ctx = _lib.context()

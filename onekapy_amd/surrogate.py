@@ -142,6 +142,25 @@ class SurrogateMLP:
             acc["d2"].ctypes.data if mp else None, 0))
         return acc
 
+    def surrogate_sums(self, site, n, seed, first, std):
+        """Sums of s and s^2 over n fresh realizations drawn, fitted and
+        featurized on the device (oka_cv_surrogate_sums)."""
+        from .stochastic import oka_dists
+        acc = {"s": np.zeros(self.n_out), "s2": np.zeros(self.n_out)}
+        wxy = site.wells_xy()
+        obs = site.obs_array()
+        s = _lib.make_site(wxy, site.xt, site.yt, site.base, obs)
+        dists = oka_dists(site.wells, site.c_dist, site.p_dist, site.t_dist)
+        mean = np.ascontiguousarray(std.mean, dtype=np.float64)
+        sd = np.ascontiguousarray(std.std, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.oka_cv_surrogate_sums(
+            self.ctx.h, self.h, s, C.addressof(dists), int(n), int(seed) & (2 ** 64 - 1),
+            int(first), mean.ctypes.data, sd.ctypes.data, acc["s"].ctypes.data,
+            acc["s2"].ctypes.data, 0))
+        if not np.all(np.isfinite(acc["s"])):
+            raise ArithmeticError("the fit failed for some surrogate draws")
+        return acc
+
 
 def features(d, coef):
     """Per-realization parameters: discharges, K, n, b, A..F (nwells + 9 columns)."""
@@ -258,12 +277,9 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, n_traced, rng)
-    ds = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, n_surrogate, rng)
     wxy = site.wells_xy()
     _, _, coef = fit_and_draw(wxy, site.xt, site.yt, site.base, site.observations, d["K"],
-                              d["b"], d["q"], d["z"])
-    _, _, coef_s = fit_and_draw(wxy, site.xt, site.yt, site.base, site.observations, ds["K"],
-                                ds["b"], ds["q"], ds["z"])
+                              d["b"], d["q"], d["z"], moments=False)
     t["fit_draw"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     if window is None:
@@ -277,9 +293,9 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     tscale = 1.0 / (window.cell * window.cell)
     std = Standardizer(features(d, coef))
     X = std(features(d, coef))
-    Xs = std(features(ds, coef_s))
     fold_of = rng.permutation(n_traced) % folds
-    surr_of = np.arange(n_surrogate) % folds
+    surr_seed = int(rng.integers(1, 2 ** 63))
+    surr_split = np.linspace(0, n_surrogate, folds + 1).astype(np.int64)
     ncell = window.nrows * window.ncols
     sum_s = np.zeros(ncell)
     var_cv = np.zeros(ncell)
@@ -290,17 +306,17 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     for f in range(folds):
         tr = np.nonzero(fold_of != f)[0]
         ho = np.nonzero(fold_of == f)[0]
-        sm = np.nonzero(surr_of == f)[0]
         net = SurrogateMLP(X.shape[1], n_hidden, ncell, seed=seed + 1 + f)
         epochs = max(1, int(math.ceil(train_steps / max(1, len(tr) // batch))))
         t0 = time.perf_counter()
         losses.append(net.train(X[tr], marks[tr], tscale, epochs, batch, lr, seed=seed + 100 + f))
         t["train"] += time.perf_counter() - t0
         t0 = time.perf_counter()
-        a_s = net.accumulate(Xs[sm])
+        mf = int(surr_split[f + 1] - surr_split[f])
+        a_s = net.surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std)
         a_d = net.accumulate(X[ho], marks[ho], tscale)
         t["surrogate"] += time.perf_counter() - t0
-        nf, mf = len(ho), len(sm)
+        nf = len(ho)
         w = nf / n_traced
         mean_s = a_s["s"] / mf
         var_s = np.maximum(a_s["s2"] / mf - mean_s ** 2, 0.0)

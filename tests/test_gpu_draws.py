@@ -1,0 +1,49 @@
+"""Device draws (Philox4x32-10) against the Python checker and the target
+distributions."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import philox
+from onekapy_amd import Lognormal, Signed
+from onekapy_amd.stochastic import draw_realizations_device
+
+pytestmark = pytest.mark.gpu
+
+
+def test_uniform_draws_are_the_philox_stream(gpu_ctx):
+    wells = [(0.0, 0.0, 0.25, (0.0, 1.0)), (10.0, 0.0, 0.25, 3.0)]
+    seed = 0x1234_5678_9ABC_DEF1
+    d = draw_realizations_device(wells, (2.0, 5.0), 0.25, (20.0, 30.0), 64, seed, first=1000)
+    for i in range(64):
+        r = 1000 + i
+        assert d["q"][i, 0] == philox.uniforms(seed, r, 0)[0]
+        assert d["q"][i, 1] == 3.0
+        assert d["K"][i] == 2.0 + 3.0 * philox.uniforms(seed, r, 2)[0]
+        assert d["n"][i] == 0.25
+        assert d["b"][i] == 20.0 + 10.0 * philox.uniforms(seed, r, 4)[0]
+    # realizations are reproducible one by one, whatever the batch they are in
+    d2 = draw_realizations_device(wells, (2.0, 5.0), 0.25, (20.0, 30.0), 8, seed, first=1010)
+    assert np.array_equal(d2["K"], d["K"][10:18]) and np.array_equal(d2["z"], d["z"][10:18])
+
+
+def test_distribution_moments(gpu_ctx):
+    n = 400000
+    wells = [(0.0, 0.0, 0.25, (300.0, 500.0, 700.0)),
+             (1.0, 0.0, 0.25, Signed(Lognormal(math.log(800.0), 0.6), 0.1))]
+    d = draw_realizations_device(wells, Lognormal(math.log(10.0), 0.5), (0.2, 0.3), (20.0, 30.0),
+                                 n, 7)
+    tri = d["q"][:, 0]
+    assert tri.min() >= 300.0 and tri.max() <= 700.0
+    assert abs(tri.mean() - 500.0) < 0.5 and abs(tri.var() - (4e4 * 3) / 18 * 1.0) < 200.0
+    sg = d["q"][:, 1]
+    assert abs((sg < 0).mean() - 0.1) < 0.003
+    la = np.log(np.abs(sg))
+    assert abs(la.mean() - math.log(800.0)) < 0.005 and abs(la.std() - 0.6) < 0.005
+    lk = np.log(d["K"])
+    assert abs(lk.mean() - math.log(10.0)) < 0.005 and abs(lk.std() - 0.5) < 0.005
+    assert abs(d["n"].mean() - 0.25) < 0.001 and abs(d["b"].var() - 100.0 / 12.0) < 0.1
+    z = d["z"]
+    assert np.abs(z.mean(0)).max() < 0.01 and np.abs(z.std(0) - 1.0).max() < 0.01
+    assert np.abs(np.corrcoef(z.T) - np.eye(6)).max() < 0.01
