@@ -58,11 +58,13 @@ def parse():
                     help="only run a plain 1 GiB device-to-device copy (to scale HBM counters)")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--only-cv", action="store_true", help="run only the control-variate leg")
-    ap.add_argument("--cv-traced", type=int, default=20000)
-    ap.add_argument("--cv-surrogate", type=int, default=1000000)
+    ap.add_argument("--cv-traced", type=int, default=40000)
+    ap.add_argument("--cv-surrogate", type=int, default=10000000)
     ap.add_argument("--cv-judge", type=int, default=1000000,
                     help="realizations of the FP64 plain run that judges the estimates")
     ap.add_argument("--cv-steps", type=int, default=3000)
+    ap.add_argument("--cv-batch", type=int, default=1024)
+    ap.add_argument("--cv-lr", type=float, default=1.5e-3)
     return ap.parse_args()
 
 
@@ -79,7 +81,8 @@ def control_variate_leg(site, args, fp32_rate):
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     res = control_variate(site, args.cv_traced, args.cv_surrogate, folds=2,
-                          train_steps=args.cv_steps, seed=123, precision=_lib.FP32)
+                          train_steps=args.cv_steps, batch=args.cv_batch, lr=args.cv_lr, seed=123,
+                          precision=_lib.FP32)
     wall = time.perf_counter() - t0
     print(f"bench: control variate {wall:.2f} s {res.times}", file=sys.stderr, flush=True)
     rng = np.random.default_rng(321)
@@ -97,7 +100,8 @@ def control_variate_leg(site, args, fp32_rate):
     eff_rate = n_eff / wall
     return {
         "traced": args.cv_traced, "surrogate_evals": args.cv_surrogate, "folds": 2,
-        "train_steps_per_fold": args.cv_steps, "network": "34->512->512->4096 GELU, BF16 MFMA",
+        "train_steps_per_fold": args.cv_steps, "batch": args.cv_batch, "lr": args.cv_lr,
+        "network": "34->512->512->4096 GELU, BF16 MFMA",
         "window": {"row0": res.window.row0, "col0": res.window.col0, "cell_nodes": res.window.cell,
                    "cells": [res.window.nrows, res.window.ncols]},
         "judge": {"realizations": args.cv_judge, "tracer": "fp64", "seconds": tj},

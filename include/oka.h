@@ -299,6 +299,14 @@ int oka_draw_realizations(oka_ctx *ctx, int32_t nwells, const oka_dist *dists, i
                           uint64_t seed, int64_t first, double *q, double *K, double *por,
                           double *thick, double *z, uint32_t flags);
 
+/* Exact moments of window marks [n x ncell] (uint8): per cell c,
+ * sums[c] += sum over rows of marks[r, c] and, when sums_sq is set,
+ * sums_sq[c] += sum of marks[r, c]^2 -- the plain Monte Carlo estimate of the
+ * cells' capture probability and its variance.  marks, sums and sums_sq
+ * follow flags. */
+int oka_mark_moments(oka_ctx *ctx, int64_t n, int32_t ncell, const uint8_t *marks, uint64_t *sums,
+                     uint64_t *sums_sq, uint32_t flags);
+
 /* The surrogate term of the control variate for n fresh realizations of the
  * stream `seed` (starting at realization `first`), computed entirely on the
  * device in chunks: draws, fit + MVN draw (as oka_fit_draw), standardized

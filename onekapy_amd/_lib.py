@@ -39,7 +39,7 @@ EXPORTED = (
     "oka_rasterize", "oka_pow_fifth", "oka_log", "oka_trace_marks", "oka_mlp_create",
     "oka_mlp_destroy", "oka_mlp_param_count", "oka_mlp_set_params", "oka_mlp_get_params",
     "oka_mlp_grad", "oka_mlp_train", "oka_mlp_predict", "oka_mlp_accumulate",
-    "oka_draw_realizations", "oka_cv_surrogate_sums",
+    "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_mark_moments",
 )
 
 DIST_CONST, DIST_UNIFORM, DIST_TRIANGULAR, DIST_LOGNORMAL = 0, 1, 2, 3
@@ -180,6 +180,8 @@ def load():
     L.oka_cv_surrogate_sums.restype = C.c_int
     L.oka_cv_surrogate_sums.argtypes = [_vp, _vp, C.POINTER(Site_), _vp, _i64, C.c_uint64, _i64,
                                         _vp, _vp, _vp, _vp, _u32]
+    L.oka_mark_moments.restype = C.c_int
+    L.oka_mark_moments.argtypes = [_vp, _i64, _i32, _vp, _vp, _vp, _u32]
     L.oka_pow_fifth.restype = _d
     L.oka_pow_fifth.argtypes = [_d]
     L.oka_log.restype = _d

@@ -156,6 +156,9 @@ struct FeatureArgs {
 // oka_rng.hip
 hipError_t launch_draw(const DrawArgs &a, hipStream_t s);
 hipError_t launch_features(const FeatureArgs &a, hipStream_t s);
+// sums[c] += sum over rows of marks[r, c], sums_sq[c] += sum of marks[r, c]^2
+hipError_t launch_mark_moments(const uint8_t *marks, int64_t n, int32_t ncell, uint64_t *sums,
+                               uint64_t *sums_sq, hipStream_t s);
 uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, int word);
 
 // oka_fp64.hip

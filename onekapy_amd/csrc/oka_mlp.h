@@ -1,4 +1,4 @@
-// oka_mlp.h -- argument block and launchers of the surrogate-network kernels.
+// oka_mlp.h -- argument blocks and launchers of the surrogate-network kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,14 +7,15 @@
 namespace oka {
 
 enum MlpEpi {
+    kEpiNone = -1,     // no second product in a grouped launch
     kEpiGeluFwd = 0,   // Z = acc + b (FP32), H = gelu(Z) (BF16, + transposed)
     kEpiLogits = 1,    // logits = acc + b (FP32)
     kEpiGeluBwd = 2,   // G = acc * gelu'(Zsaved) (BF16, + transposed)
-    kEpiStoreF32 = 3,  // C = acc (FP32)
+    kEpiStoreF32 = 3,  // C = acc (FP32); with dbias, dbias[m] = sum over K of A[m, :]
     kEpiLossGrad = 4,  // G = (sigmoid(acc + b) - y) * gscale (BF16, + transposed); loss
     kEpiAccum = 5,     // sums[n] += s, sums_sq += s^2, diffs += y - s, diffs_sq += (y - s)^2
     kEpiProbs = 6,     // C = sigmoid(acc + b) (FP32)
-    kEpiAtomicF32 = 7  // C += acc (FP32 atomics; split-K partial sums, grid.z = K splits)
+    kEpiPartialF32 = 7 // C + z * M * ldc = acc over K range z of `splits` (split-K partial sums)
 };
 
 struct GemmArgs {
@@ -22,36 +23,64 @@ struct GemmArgs {
     const uint16_t *B;       // [N, K] BF16, row stride ldb
     int M, N, K, lda, ldb;
     int mvalid;              // rows >= mvalid are padding (kEpiAccum, kEpiProbs)
+    int splits;              // K ranges (kEpiPartialF32), else 1
     const float *bias;       // [N]
     float *Cf;               // FP32 output, row stride ldc
     int ldc;
     uint16_t *Ch;            // BF16 output [M, N]
     uint16_t *Cht;           // BF16 transposed output [N, M]
     const float *Zsaved;     // pre-activation [M, N] for kEpiGeluBwd
-    const uint8_t *marks;    // targets: marks[row * N + n] * tscale
-    const int32_t *rowidx;   // target/input row of tile row m: rowidx[row0 + m], or row0 + m
+    float *dbias;            // kEpiStoreF32: row sums of A (the bias gradient of a dW product)
+    const uint8_t *marks;    // targets * (1 / tscale): kEpiLossGrad [M, N] (the batch's rows),
+                             // kEpiAccum marks[(row0 + m) * N + n]
     int64_t row0;
     float tscale, gscale;
-    double *sums, *sums_sq, *diffs, *diffs_sq, *loss;
-    // graph-replayed training: when dstate is set, the tile rows start at
-    // dstate[0] * step_rows and the loss goes to loss[dstate[0]]
+    double *sums, *sums_sq, *diffs, *diffs_sq;
+    double *loss;            // kEpiLossGrad: summed BCE (nullptr: not computed)
+    // graph-replayed training: when dstate is set the loss goes to loss[dstate[0]]
     const int64_t *dstate;
-    int step_rows;
-    int ksplit;              // kEpiAtomicF32: K range per grid.z slice
 };
 
-hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s, int splits = 1);
+// One FP32 master segment of the parameter vector updated by Adam: a weight
+// matrix [rows, cols] with its BF16 copies (wh row-major, wt transposed), or a
+// bias vector laid out as [len / 32, 32] with wh = wt = nullptr.
+struct AdamSeg {
+    int64_t off;
+    int rows, cols;
+    uint16_t *wh, *wt;
+};
+
+struct AdamArgs {
+    float *p;
+    const float *grad;
+    float *m, *v;
+    float lr, b1, b2, eps;
+    // Adam step t = tbase + dstate[1] when dstate is set (the first kernel of
+    // a captured training step advances dstate[1]), else tbase; the update
+    // also advances dstate[0], the batch counter that the step's other kernels
+    // have finished reading.
+    int64_t tbase;
+    int64_t *dstate;
+    AdamSeg seg[6];
+    int nseg;
+};
+
+constexpr int kMaxSplitSlices = 8;   // split-K partial slices summed by launch_gelu_bwd
+
+__host__ __device__ inline int gemm_tiles(const GemmArgs &g) { return (g.N / 64) * (g.M / 64) * g.splits; }
+
+// One launch of up to two independent products (the second with e2 != kEpiNone).
+hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s);
+hipError_t launch_gemm2(int e1, const GemmArgs &g1, int e2, const GemmArgs &g2, hipStream_t s);
+// The batch's inputs (and, with marks set, its target rows into ym [M, nout]).
 hipError_t launch_prep_input(const float *x, int n_in, int kin, const int32_t *rowidx,
                              int64_t row0, int M, int mvalid, uint16_t *x0, uint16_t *x0t,
-                             hipStream_t s, const int64_t *dstate = nullptr);
-hipError_t launch_bias_grad(const uint16_t *gt, int N, int M, float *db, hipStream_t s);
-// Adam; the step number t is dstate[1] + 1 when dstate is set, else `t`.
-hipError_t launch_adam(float *p, const float *grad, float *m, float *v, int64_t n, float lr,
-                       float b1, float b2, float eps, int64_t t, const int64_t *dstate,
-                       hipStream_t s);
-hipError_t launch_gelu_bwd(const float *acc, const float *z, int M, int N, uint16_t *g,
+                             const uint8_t *marks, int nout, uint8_t *ym, hipStream_t s,
+                             int64_t *dstate = nullptr);
+hipError_t launch_adam(const AdamArgs &a, hipStream_t s);
+// G = (sum of the `splits` partial slices [M, N] of acc) * gelu'(Z) as BF16 (+ transposed).
+hipError_t launch_gelu_bwd(const float *acc, int splits, const float *z, int M, int N, uint16_t *g,
                            uint16_t *gt, hipStream_t s);
-hipError_t launch_step_advance(int64_t *dstate, hipStream_t s);
 hipError_t launch_fill_f32(float *p, int64_t n, hipStream_t s);
 hipError_t launch_refresh_bf16(const float *w, int rows, int cols, uint16_t *wh, uint16_t *wt,
                                hipStream_t s);

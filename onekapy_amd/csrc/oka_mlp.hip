@@ -9,6 +9,9 @@
 //   forward  Z = X W^T + b                    A = X,    B = W
 //   dX       G W   = G (W^T)^T                A = G,    B = W^T copy
 //   dW       G^T X = (G^T) (X^T)^T            A = G^T,  B = X^T copy
+// A dW product also forms the bias gradient (the row sums of G^T) from the
+// A slices its first column of tiles loads, and one launch may carry two
+// independent products (dW of a layer beside dX of the same layer).
 #include <math.h>
 #include <stdint.h>
 
@@ -32,6 +35,17 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// sum of the 8 BF16 values packed in a 16-byte word
+__device__ __forceinline__ float sum8(uint4 w)
+{
+    const uint32_t u[4] = {w.x, w.y, w.z, w.w};
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        s += __uint_as_float(u[i] << 16) + __uint_as_float(u[i] & 0xffff0000u);
+    return s;
+}
+
 __device__ __forceinline__ float gelu(float z)
 {
     return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
@@ -46,59 +60,149 @@ __device__ __forceinline__ float gelu_grad(float z)
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
-constexpr int kTile = 64;   // block tile M and N
-constexpr int kKStep = 32;  // K per MFMA
-constexpr int kPad = 8;     // LDS row padding (bf16 elements)
+constexpr int kTile = 64;       // block tile M and N
+constexpr int kBK = 64;         // K per LDS stage (two MFMA K steps of 32)
+constexpr int kLd = kBK + 8;    // LDS row stride in BF16 elements (16-byte pad)
 
+struct GemmSmem {
+    uint16_t a[2][kTile][kLd];  // double-buffered A and B slices
+    uint16_t b[2][kTile][kLd];
+    double red[4];
+    float col[4][kTile];        // kEpiAccum: the workgroup's column sums
+};
+
+__device__ __forceinline__ uint4 ld16(const uint16_t *p) { return *reinterpret_cast<const uint4 *>(p); }
+__device__ __forceinline__ void st16(uint16_t *p, uint4 v) { *reinterpret_cast<uint4 *>(p) = v; }
+
+struct Stage {
+    uint4 a0, a1, b0, b1;   // rows lr and lr + 32 of the A and B slices
+};
+
+// One 64 x 64 tile of C (tile index = (z * M/64 + row tile) * N/64 + column
+// tile).  256 lanes: wave w computes the 32 x 32 quadrant (w / 2, w % 2);
+// lane t loads 16 bytes of rows t/8 and t/8 + 32 of each operand per K stage
+// of 64.  Global loads run two stages ahead of the MFMAs (two register sets)
+// and LDS is double buffered, with one barrier per stage.
 template <int kEpi>
-__global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
+__device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem &sm)
 {
-    __shared__ __align__(16) uint16_t sA[kTile][kKStep + kPad];
-    __shared__ __align__(16) uint16_t sB[kTile][kKStep + kPad];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+    const int gx = g.N / kTile, gxy = gx * (g.M / kTile);
+    const int bz = tile / gxy;
+    const int rem = tile - bz * gxy;
+    const int m0 = (rem / gx) * kTile, n0 = (rem - (rem / gx) * gx) * kTile;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int lr = tid >> 2, lc = (tid & 3) * 8;   // tile loader: row, column chunk
+    const int lr = tid >> 3, lc = (tid & 7) * 8;
+    const int kspan = g.K / g.splits;
+    const int nst = kspan / kBK;
+    const int64_t a32 = 32 * (int64_t)g.lda, b32 = 32 * (int64_t)g.ldb;
+    const uint16_t *pa = g.A + (int64_t)(m0 + lr) * g.lda + lc + (int64_t)bz * kspan;
+    const uint16_t *pb = g.B + (int64_t)(n0 + lr) * g.ldb + lc + (int64_t)bz * kspan;
+    const bool rowsum = kEpi == kEpiStoreF32 && g.dbias != nullptr && n0 == 0;
+    float rs0 = 0.f, rs1 = 0.f;
+
+    // loss-gradient targets: fetched before the K loop (16 bytes per lane)
+    uint8_t ymk[2][2][4];
+    const int64_t step = g.dstate ? g.dstate[0] : 0;
+    if constexpr (kEpi == kEpiLossGrad) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    ymk[j][i][e] = g.marks[(int64_t)(m0 + wm + i * 16 + (lane >> 4) * 4 + e) * g.N +
+                                           n0 + wn + j * 16 + (lane & 15)];
+    }
+
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
             acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // split-K (kEpiAtomicF32): this block sums K in [kbeg, kend)
-    const int kbeg = kEpi == kEpiAtomicF32 ? (int)blockIdx.z * g.ksplit : 0;
-    const int kend = kEpi == kEpiAtomicF32 ? kbeg + g.ksplit : g.K;
-    const uint16_t *pa = g.A + (int64_t)(m0 + lr) * g.lda + lc;
-    const uint16_t *pb = g.B + (int64_t)(n0 + lr) * g.ldb + lc;
-    uint4 ra = *reinterpret_cast<const uint4 *>(pa + kbeg);
-    uint4 rb = *reinterpret_cast<const uint4 *>(pb + kbeg);
-    for (int k0 = kbeg; k0 < kend; k0 += kKStep) {
-        *reinterpret_cast<uint4 *>(&sA[lr][lc]) = ra;
-        *reinterpret_cast<uint4 *>(&sB[lr][lc]) = rb;
-        __syncthreads();
-        if (k0 + kKStep < kend) {   // prefetch the next K slice while computing
-            ra = *reinterpret_cast<const uint4 *>(pa + k0 + kKStep);
-            rb = *reinterpret_cast<const uint4 *>(pb + k0 + kKStep);
+
+    auto load = [&](Stage &r, int st) {
+        const int64_t o = (int64_t)st * kBK;
+        r.a0 = ld16(pa + o);
+        r.a1 = ld16(pa + a32 + o);
+        r.b0 = ld16(pb + o);
+        r.b1 = ld16(pb + b32 + o);
+    };
+    auto put = [&](const Stage &r, int b) {
+        st16(&sm.a[b][lr][lc], r.a0);
+        st16(&sm.a[b][lr + 32][lc], r.a1);
+        st16(&sm.b[b][lr][lc], r.b0);
+        st16(&sm.b[b][lr + 32][lc], r.b1);
+        if (rowsum) {
+            rs0 += sum8(r.a0);
+            rs1 += sum8(r.a1);
         }
-        bf16x8 a[2], b[2];
+    };
+    auto mma = [&](int b) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            a[i] = *reinterpret_cast<const bf16x8 *>(&sA[wm + i * 16 + (lane & 15)][(lane >> 4) * 8]);
-            b[i] = *reinterpret_cast<const bf16x8 *>(&sB[wn + i * 16 + (lane & 15)][(lane >> 4) * 8]);
+        for (int ks = 0; ks < kBK; ks += 32) {
+            bf16x8 a[2], bb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a[i] = *reinterpret_cast<const bf16x8 *>(
+                    &sm.a[b][wm + i * 16 + (lane & 15)][ks + (lane >> 4) * 8]);
+                bb[i] = *reinterpret_cast<const bf16x8 *>(
+                    &sm.b[b][wn + i * 16 + (lane & 15)][ks + (lane >> 4) * 8]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
         }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    };
+    // stage s is computed from LDS buffer s % 2; register set s % 2 carries
+    // stage s + 1 (then s + 3) while stage s is computed
+    Stage r0, r1;
+    load(r0, 0);
+    if (nst > 1)
+        load(r1, 1);
+    put(r0, 0);
+    if (nst > 2)
+        load(r0, 2);
+    __syncthreads();
+    for (int s = 0; s < nst; s += 2) {
+        mma(0);
+        if (s + 1 < nst)
+            put(r1, 1);   // buffer 1 was last read before the previous barrier
+        if (s + 3 < nst)
+            load(r1, s + 3);
         __syncthreads();
+        if (s + 1 < nst) {
+            mma(1);
+            if (s + 2 < nst)
+                put(r0, 0);
+            if (s + 4 < nst)
+                load(r0, s + 4);
+            __syncthreads();
+        }
+    }
+    if (rowsum) {
+        // the 8 lanes loading one row are consecutive
+#pragma unroll
+        for (int off = 1; off < 8; off <<= 1) {
+            rs0 += __shfl_xor(rs0, off);
+            rs1 += __shfl_xor(rs1, off);
+        }
+        if ((tid & 7) == 0) {
+            g.dbias[m0 + lr] = rs0;
+            g.dbias[m0 + lr + 32] = rs1;
+        }
     }
 
     // epilogue: lane holds C[m = mb + 4*(lane/16) + e][n = nb + lane%16], e = 0..3
-    double part_s = 0.0, part_d = 0.0, part_l = 0.0;
-    const int64_t step = g.dstate ? g.dstate[0] : 0;
-    const int64_t row_base = g.dstate ? step * g.step_rows : g.row0;
+    const bool want_loss = kEpi == kEpiLossGrad && g.loss != nullptr;
+    double part_l = 0.0;
+    float *cf = g.Cf;
+    if (kEpi == kEpiPartialF32)
+        cf += (int64_t)bz * g.M * g.ldc;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const int n = n0 + wn + j * 16 + (lane & 15);
@@ -115,30 +219,29 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
                 const float v = acc[i][j][e];
                 if (kEpi == kEpiGeluFwd) {
                     const float z = v + bias;
-                    g.Cf[(int64_t)m * g.ldc + n] = z;
+                    if (cf)   // the pre-activation is kept for the backward pass only
+                        cf[(int64_t)m * g.ldc + n] = z;
                     hv[e] = f2bf(gelu(z));
                     g.Ch[(int64_t)m * g.N + n] = hv[e];
                 } else if (kEpi == kEpiLogits) {
-                    g.Cf[(int64_t)m * g.ldc + n] = v + bias;
+                    cf[(int64_t)m * g.ldc + n] = v + bias;
                 } else if (kEpi == kEpiGeluBwd) {
                     hv[e] = f2bf(v * gelu_grad(g.Zsaved[(int64_t)m * g.N + n]));
                     g.Ch[(int64_t)m * g.N + n] = hv[e];
-                } else if (kEpi == kEpiStoreF32) {
-                    g.Cf[(int64_t)m * g.ldc + n] = v;
-                } else if (kEpi == kEpiAtomicF32) {
-                    atomicAdd(&g.Cf[(int64_t)m * g.ldc + n], v);
+                } else if (kEpi == kEpiStoreF32 || kEpi == kEpiPartialF32) {
+                    cf[(int64_t)m * g.ldc + n] = v;
                 } else if (kEpi == kEpiLossGrad) {
                     const float z = v + bias;
-                    const int64_t row = g.rowidx ? g.rowidx[row_base + m] : row_base + m;
-                    const float y = (float)g.marks[row * g.N + n] * g.tscale;
+                    const float y = (float)ymk[j][i][e] * g.tscale;
                     hv[e] = f2bf((sigmoid(z) - y) * g.gscale);
                     g.Ch[(int64_t)m * g.N + n] = hv[e];
-                    part_l += (double)(fmaxf(z, 0.f) - z * y + log1pf(__expf(-fabsf(z))));
+                    if (want_loss)
+                        part_l += (double)(fmaxf(z, 0.f) - z * y + log1pf(__expf(-fabsf(z))));
                 } else if (kEpi == kEpiAccum || kEpi == kEpiProbs) {
                     if (m < g.mvalid) {
                         const float s = sigmoid(v + bias);
                         if (kEpi == kEpiProbs) {
-                            g.Cf[(int64_t)m * g.ldc + n] = s;
+                            cf[(int64_t)m * g.ldc + n] = s;
                         } else {
                             col_s += s;
                             col_s2 += s * s;
@@ -152,7 +255,7 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
                     }
                 }
             }
-            if (kEpi == kEpiGeluFwd || kEpi == kEpiGeluBwd || kEpi == kEpiLossGrad) {
+            if ((kEpi == kEpiGeluFwd || kEpi == kEpiGeluBwd || kEpi == kEpiLossGrad) && g.Cht) {
                 // transposed copy: 4 consecutive rows of column n are contiguous in C^T
                 uint2 packed;
                 packed.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
@@ -161,7 +264,8 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
             }
         }
         if (kEpi == kEpiAccum) {
-            // lanes l, l^16, l^32, l^48 hold the same column
+            // lanes l, l^16, l^32, l^48 hold the same column; two waves share
+            // each column, so the workgroup's sums meet in LDS
             col_s += __shfl_xor(col_s, 16);
             col_s += __shfl_xor(col_s, 32);
             col_s2 += __shfl_xor(col_s2, 16);
@@ -171,86 +275,164 @@ __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g)
             col_d2 += __shfl_xor(col_d2, 16);
             col_d2 += __shfl_xor(col_d2, 32);
             if (lane < 16) {
-                atomicAdd(&g.sums[n], (double)col_s);
-                if (g.sums_sq)
-                    atomicAdd(&g.sums_sq[n], (double)col_s2);
-                if (g.marks && g.diffs)
-                    atomicAdd(&g.diffs[n], (double)col_d);
-                if (g.marks && g.diffs_sq)
-                    atomicAdd(&g.diffs_sq[n], (double)col_d2);
+                const int cl = n - n0;
+                atomicAdd(&sm.col[0][cl], col_s);
+                atomicAdd(&sm.col[1][cl], col_s2);
+                atomicAdd(&sm.col[2][cl], col_d);
+                atomicAdd(&sm.col[3][cl], col_d2);
             }
         }
     }
-    if (kEpi == kEpiLossGrad) {
+    if (want_loss) {   // one atomic per workgroup
         for (int off = 32; off > 0; off >>= 1)
             part_l += __shfl_xor(part_l, off);
         if (lane == 0)
-            atomicAdd(g.loss + step, part_l);
+            sm.red[wave] = part_l;
+        __syncthreads();
+        if (tid == 0)
+            atomicAdd(g.loss + step, sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3]);
     }
-    (void)part_s;
-    (void)part_d;
+}
+
+// Tiles [0, tiles(g1)) belong to g1, the rest to g2.
+template <int kEpi1, int kEpi2>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g1, GemmArgs g2)
+{
+    __shared__ GemmSmem sm;
+    if constexpr (kEpi2 == kEpiNone) {
+        gemm_tile<kEpi1>(g1, blockIdx.x, sm);
+    } else {
+        const int n1 = gemm_tiles(g1);
+        if ((int)blockIdx.x < n1)
+            gemm_tile<kEpi1>(g1, blockIdx.x, sm);
+        else
+            gemm_tile<kEpi2>(g2, blockIdx.x - n1, sm);
+    }
+}
+
+// kEpiAccum: workgroup b sums the tiles of column tile b % (N/64) in row
+// tiles [rt * (b / (N/64)), + rt) in LDS, then adds them to the FP64 sums.
+__global__ void __launch_bounds__(256) k_gemm_accum(GemmArgs g, int rt)
+{
+    __shared__ GemmSmem sm;
+    const int gx = g.N / kTile, gy = g.M / kTile;
+    const int ct = blockIdx.x % gx, rg = blockIdx.x / gx;
+    for (int i = threadIdx.x; i < 4 * kTile; i += blockDim.x)
+        sm.col[i / kTile][i % kTile] = 0.f;
+    __syncthreads();
+    const int end = min(gy, (rg + 1) * rt);
+    for (int r = rg * rt; r < end; ++r)
+        gemm_tile<kEpiAccum>(g, r * gx + ct, sm);
+    __syncthreads();
+    if (threadIdx.x < kTile) {
+        const int n = ct * kTile + threadIdx.x;
+        atomicAdd(&g.sums[n], (double)sm.col[0][threadIdx.x]);
+        if (g.sums_sq)
+            atomicAdd(&g.sums_sq[n], (double)sm.col[1][threadIdx.x]);
+        if (g.marks && g.diffs)
+            atomicAdd(&g.diffs[n], (double)sm.col[2][threadIdx.x]);
+        if (g.marks && g.diffs_sq)
+            atomicAdd(&g.diffs_sq[n], (double)sm.col[3][threadIdx.x]);
+    }
 }
 
 // x (FP32, row stride n_in) of rows rowidx[row0 + m] (or row0 + m) -> X0
 // (BF16 [M, kin], zero padded) and X0^T ([kin, M]); rows >= mvalid are zero.
+// With marks set, the same rows of marks [*, nout] are gathered into ym
+// [M, nout] (16 bytes per lane).  In a captured training step (dstate set)
+// row0 = dstate[0] * M, and this kernel, the step's first, advances the Adam
+// step counter dstate[1].
 __global__ void __launch_bounds__(256) k_prep_input(const float *x, int n_in, int kin,
                                                     const int32_t *rowidx, int64_t row0, int M,
                                                     int mvalid, uint16_t *x0, uint16_t *x0t,
-                                                    const int64_t *dstate)
+                                                    const uint8_t *marks, int nout, uint8_t *ym,
+                                                    int64_t *dstate)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)M * kin)
-        return;
-    const int m = (int)(t / kin), k = (int)(t % kin);
-    if (dstate)
+    if (dstate) {
         row0 = dstate[0] * M;
-    float v = 0.f;
-    if (m < mvalid && k < n_in) {
-        const int64_t row = rowidx ? rowidx[row0 + m] : row0 + m;
-        v = x[row * n_in + k];
+        if (t == 0)
+            dstate[1] += 1;
     }
-    const uint16_t h = f2bf(v);
-    x0[(int64_t)m * kin + k] = h;
-    x0t[(int64_t)k * M + m] = h;
+    if (t < (int64_t)M * kin) {
+        const int m = (int)(t / kin), k = (int)(t % kin);
+        float v = 0.f;
+        if (m < mvalid && k < n_in) {
+            const int64_t row = rowidx ? rowidx[row0 + m] : row0 + m;
+            v = x[row * n_in + k];
+        }
+        const uint16_t h = f2bf(v);
+        x0[(int64_t)m * kin + k] = h;
+        x0t[(int64_t)k * M + m] = h;
+    }
+    const int chunks = nout / 16;
+    if (marks && t < (int64_t)M * chunks) {
+        const int m = (int)(t / chunks), q = (int)(t % chunks);
+        uint4 w = make_uint4(0u, 0u, 0u, 0u);
+        if (m < mvalid) {
+            const int64_t row = rowidx ? rowidx[row0 + m] : row0 + m;
+            w = *reinterpret_cast<const uint4 *>(marks + row * nout + q * 16);
+        }
+        *reinterpret_cast<uint4 *>(ym + (int64_t)m * nout + q * 16) = w;
+    }
 }
 
-// db[n] = sum over m of G^T[n, m] (one workgroup per n).
-__global__ void __launch_bounds__(256) k_bias_grad(const uint16_t *gt, int M, float *db)
+// Adam over one 32 x 32 tile of one segment per workgroup; weight segments
+// also refresh their BF16 copies (the transposed one through LDS so that both
+// stores are coalesced).
+__global__ void __launch_bounds__(256) k_adam(AdamArgs a)
 {
-    __shared__ float s[256];
-    const int n = blockIdx.x;
-    float acc = 0.f;
-    for (int m = threadIdx.x; m < M; m += blockDim.x)
-        acc += bf2f(gt[(int64_t)n * M + m]);
-    s[threadIdx.x] = acc;
+    __shared__ uint16_t tile[32][33];
+    __shared__ float corr[2];
+    int b = blockIdx.x, s = 0, ct = 1;
+    for (; s < a.nseg; ++s) {
+        ct = (a.seg[s].cols + 31) / 32;
+        const int tiles = ct * ((a.seg[s].rows + 31) / 32);
+        if (b < tiles)
+            break;
+        b -= tiles;
+    }
+    if (s >= a.nseg)
+        return;   // the grid is sized to the segments; never taken
+    const AdamSeg sg = a.seg[s];
+    const int r0 = (b / ct) * 32, c0 = (b % ct) * 32;
+    if (threadIdx.x == 0) {
+        const int64_t t = a.dstate ? a.tbase + a.dstate[1] : a.tbase;
+        // bias corrections 1/(1 - beta^t)
+        corr[0] = (float)(1.0 / (1.0 - exp((double)t * log((double)a.b1))));
+        corr[1] = (float)(1.0 / (1.0 - exp((double)t * log((double)a.b2))));
+        if (a.dstate && blockIdx.x == 0)
+            a.dstate[0] += 1;   // no kernel of this launch reads dstate[0]
+    }
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            s[threadIdx.x] += s[threadIdx.x + w];
-        __syncthreads();
+    const float c1 = corr[0], c2 = corr[1];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 lanes
+    for (int k = ty; k < 32; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < sg.rows && c < sg.cols) {
+            const int64_t i = sg.off + (int64_t)r * sg.cols + c;
+            const float gi = a.grad[i];
+            const float mi = a.b1 * a.m[i] + (1.f - a.b1) * gi;
+            const float vi = a.b2 * a.v[i] + (1.f - a.b2) * gi * gi;
+            a.m[i] = mi;
+            a.v[i] = vi;
+            const float pi = a.p[i] - a.lr * (mi * c1) / (sqrtf(vi * c2) + a.eps);
+            a.p[i] = pi;
+            if (sg.wh) {
+                const uint16_t h = f2bf(pi);
+                sg.wh[(int64_t)r * sg.cols + c] = h;
+                tile[k][tx] = h;
+            }
+        }
     }
-    if (threadIdx.x == 0)
-        db[n] = s[0];
-}
-
-__global__ void __launch_bounds__(256) k_adam(float *p, const float *grad, float *m, float *v,
-                                              int64_t n, float lr, float b1, float b2, float eps,
-                                              int64_t t, const int64_t *dstate)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    if (dstate)
-        t = dstate[1] + 1;
-    // bias corrections 1/(1 - beta^t)
-    const float c1 = (float)(1.0 / (1.0 - exp((double)t * log((double)b1))));
-    const float c2 = (float)(1.0 / (1.0 - exp((double)t * log((double)b2))));
-    const float gi = grad[i];
-    const float mi = b1 * m[i] + (1.f - b1) * gi;
-    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+    if (sg.wh) {
+        __syncthreads();
+        for (int k = ty; k < 32; k += 8) {
+            const int c = c0 + k, r = r0 + tx;
+            if (r < sg.rows && c < sg.cols)
+                sg.wt[(int64_t)c * sg.rows + r] = tile[tx][k];
+        }
+    }
 }
 
 // FP32 master W [rows, cols] -> BF16 W and W^T, one 32 x 32 tile per
@@ -277,19 +459,39 @@ __global__ void __launch_bounds__(256) k_refresh_bf16(const float *w, int rows, 
     }
 }
 
-// G = acc * gelu'(Z) as BF16, row-major and transposed (after a split-K product).
-__global__ void __launch_bounds__(256) k_gelu_bwd(const float *acc, const float *z, int M, int N,
-                                                  uint16_t *g, uint16_t *gt)
+// G = (sum over the split-K slices of acc) * gelu'(Z) as BF16, row-major and
+// transposed; the slices are summed in order, so the result is deterministic.
+__global__ void __launch_bounds__(256) k_gelu_bwd(const float *acc, int splits, const float *z,
+                                                  int M, int N, uint16_t *g, uint16_t *gt)
 {
     __shared__ uint16_t tile[32][33];
     const int n0 = blockIdx.x * 32, m0 = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int k = ty; k < 32; k += 8) {
+    const int64_t slice = (int64_t)M * N;
+    // every load of the lane (4 rows x up to kMaxSplitSlices slices) is issued
+    // before the first sum
+    float part[4][kMaxSplitSlices], zz[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int m = m0 + ty + 8 * r, n = n0 + tx;
+        const bool ok = m < M && n < N;
+        const int64_t i = (int64_t)m * N + n;
+        zz[r] = ok ? z[i] : 0.f;
+#pragma unroll
+        for (int p = 0; p < kMaxSplitSlices; ++p)
+            part[r][p] = ok && p < splits ? acc[i + p * slice] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = ty + 8 * r;
         const int m = m0 + k, n = n0 + tx;
+        float s = part[r][0];
+#pragma unroll
+        for (int p = 1; p < kMaxSplitSlices; ++p)
+            s += part[r][p];
         if (m < M && n < N) {
-            const int64_t i = (int64_t)m * N + n;
-            const uint16_t h = f2bf(acc[i] * gelu_grad(z[i]));
-            g[i] = h;
+            const uint16_t h = f2bf(s * gelu_grad(zz[r]));
+            g[(int64_t)m * N + n] = h;
             tile[k][tx] = h;
         }
     }
@@ -298,14 +500,6 @@ __global__ void __launch_bounds__(256) k_gelu_bwd(const float *acc, const float 
         const int n = n0 + k, m = m0 + tx;
         if (m < M && n < N)
             gt[(int64_t)n * M + m] = tile[tx][k];
-    }
-}
-
-__global__ void k_step_advance(int64_t *dstate)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        dstate[0] += 1;
-        dstate[1] += 1;
     }
 }
 
@@ -327,45 +521,72 @@ __global__ void k_zero_f64(double *p, int64_t n)
 
 using namespace mlp;
 
-hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s, int splits)
+hipError_t launch_gemm2(int e1, const GemmArgs &g1, int e2, const GemmArgs &g2, hipStream_t s)
 {
-    dim3 grid(g.N / kTile, g.M / kTile, epi == kEpiAtomicF32 ? splits : 1), block(256);
-    switch (epi) {
-    case kEpiGeluFwd: hipLaunchKernelGGL(k_gemm_nt<kEpiGeluFwd>, grid, block, 0, s, g); break;
-    case kEpiLogits: hipLaunchKernelGGL(k_gemm_nt<kEpiLogits>, grid, block, 0, s, g); break;
-    case kEpiGeluBwd: hipLaunchKernelGGL(k_gemm_nt<kEpiGeluBwd>, grid, block, 0, s, g); break;
-    case kEpiStoreF32: hipLaunchKernelGGL(k_gemm_nt<kEpiStoreF32>, grid, block, 0, s, g); break;
-    case kEpiLossGrad: hipLaunchKernelGGL(k_gemm_nt<kEpiLossGrad>, grid, block, 0, s, g); break;
-    case kEpiAccum: hipLaunchKernelGGL(k_gemm_nt<kEpiAccum>, grid, block, 0, s, g); break;
-    case kEpiProbs: hipLaunchKernelGGL(k_gemm_nt<kEpiProbs>, grid, block, 0, s, g); break;
-    case kEpiAtomicF32: hipLaunchKernelGGL(k_gemm_nt<kEpiAtomicF32>, grid, block, 0, s, g); break;
-    default: return hipErrorInvalidValue;
+    if (g1.M % kTile || g1.N % kTile || g1.splits < 1 || g1.K % (g1.splits * kBK))
+        return hipErrorInvalidValue;
+    if (e2 != kEpiNone && (g2.M % kTile || g2.N % kTile || g2.splits < 1 ||
+                           g2.K % (g2.splits * kBK)))
+        return hipErrorInvalidValue;
+    if (e1 == kEpiAccum && e2 == kEpiNone) {
+        constexpr int kRowTiles = 8;
+        const int gy = g1.M / kTile;
+        const dim3 grid((g1.N / kTile) * ((gy + kRowTiles - 1) / kRowTiles));
+        hipLaunchKernelGGL(k_gemm_accum, grid, dim3(256), 0, s, g1, kRowTiles);
+        return hipGetLastError();
     }
-    return hipGetLastError();
+    const int n = gemm_tiles(g1) + (e2 == kEpiNone ? 0 : gemm_tiles(g2));
+    const dim3 grid(n), block(256);
+#define OKA_GEMM_CASE(E1, E2)                                                                 \
+    if (e1 == (E1) && e2 == (E2)) {                                                           \
+        hipLaunchKernelGGL((k_gemm<(E1), (E2)>), grid, block, 0, s, g1, g2);                  \
+        return hipGetLastError();                                                             \
+    }
+    OKA_GEMM_CASE(kEpiGeluFwd, kEpiNone)
+    OKA_GEMM_CASE(kEpiLogits, kEpiNone)
+    OKA_GEMM_CASE(kEpiGeluBwd, kEpiNone)
+    OKA_GEMM_CASE(kEpiStoreF32, kEpiNone)
+    OKA_GEMM_CASE(kEpiLossGrad, kEpiNone)
+    OKA_GEMM_CASE(kEpiProbs, kEpiNone)
+    OKA_GEMM_CASE(kEpiPartialF32, kEpiNone)
+    OKA_GEMM_CASE(kEpiStoreF32, kEpiGeluBwd)
+    OKA_GEMM_CASE(kEpiStoreF32, kEpiPartialF32)
+#undef OKA_GEMM_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s)
+{
+    GemmArgs none{};
+    return launch_gemm2(epi, g, kEpiNone, none, s);
 }
 
 hipError_t launch_prep_input(const float *x, int n_in, int kin, const int32_t *rowidx,
                              int64_t row0, int M, int mvalid, uint16_t *x0, uint16_t *x0t,
-                             hipStream_t s, const int64_t *dstate)
+                             const uint8_t *marks, int nout, uint8_t *ym, hipStream_t s,
+                             int64_t *dstate)
 {
-    const int64_t n = (int64_t)M * kin;
+    if (marks && (nout % 16 || !ym))
+        return hipErrorInvalidValue;
+    int64_t n = (int64_t)M * kin;
+    if (marks)
+        n = n > (int64_t)M * (nout / 16) ? n : (int64_t)M * (nout / 16);
     hipLaunchKernelGGL(k_prep_input, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n_in,
-                       kin, rowidx, row0, M, mvalid, x0, x0t, dstate);
+                       kin, rowidx, row0, M, mvalid, x0, x0t, marks, nout, ym, dstate);
     return hipGetLastError();
 }
 
-hipError_t launch_bias_grad(const uint16_t *gt, int N, int M, float *db, hipStream_t s)
+hipError_t launch_adam(const AdamArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bias_grad, dim3(N), dim3(256), 0, s, gt, M, db);
-    return hipGetLastError();
-}
-
-hipError_t launch_adam(float *p, const float *grad, float *m, float *v, int64_t n, float lr,
-                       float b1, float b2, float eps, int64_t t, const int64_t *dstate,
-                       hipStream_t s)
-{
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, grad, m, v,
-                       n, lr, b1, b2, eps, t, dstate);
+    int blocks = 0;
+    for (int i = 0; i < a.nseg; ++i) {
+        if (a.seg[i].rows < 1 || a.seg[i].cols < 1)
+            return hipErrorInvalidValue;
+        blocks += ((a.seg[i].cols + 31) / 32) * ((a.seg[i].rows + 31) / 32);
+    }
+    if (blocks == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_adam, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -377,17 +598,13 @@ hipError_t launch_refresh_bf16(const float *w, int rows, int cols, uint16_t *wh,
     return hipGetLastError();
 }
 
-hipError_t launch_gelu_bwd(const float *acc, const float *z, int M, int N, uint16_t *g,
-                           uint16_t *gt, hipStream_t s)
+hipError_t launch_gelu_bwd(const float *acc, int splits, const float *z, int M, int N,
+                           uint16_t *g, uint16_t *gt, hipStream_t s)
 {
+    if (splits < 1 || splits > kMaxSplitSlices)
+        return hipErrorInvalidValue;
     dim3 grid((N + 31) / 32, (M + 31) / 32);
-    hipLaunchKernelGGL(k_gelu_bwd, grid, dim3(256), 0, s, acc, z, M, N, g, gt);
-    return hipGetLastError();
-}
-
-hipError_t launch_step_advance(int64_t *dstate, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, dstate);
+    hipLaunchKernelGGL(k_gelu_bwd, grid, dim3(256), 0, s, acc, splits, z, M, N, g, gt);
     return hipGetLastError();
 }
 

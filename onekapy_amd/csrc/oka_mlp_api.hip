@@ -24,8 +24,10 @@ struct oka_mlp {
              *h2t = nullptr, *g3 = nullptr, *g3t = nullptr, *g2 = nullptr, *g2t = nullptr,
              *g1 = nullptr, *g1t = nullptr;
     float *z1 = nullptr, *z2 = nullptr;
-    float *acc = nullptr;        // split-K partial sums [max_batch x n_hidden]
-    int64_t *dstate = nullptr;   // graph replay: [0] step within the call, [1] Adam step
+    uint8_t *ym = nullptr;       // the batch's target rows [max_batch x n_out]
+    float *acc = nullptr;        // split-K partial slices, acc_cap floats
+    size_t acc_cap = 0;
+    int64_t *dstate = nullptr;   // graph replay: [0] step within the call, [1] steps begun
     int64_t step = 0;
     std::vector<void *> allocs;
 };
@@ -61,105 +63,159 @@ GemmArgs gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int Mr, in
     g.N = N;
     g.K = K;
     g.mvalid = Mr;
+    g.splits = 1;
     return g;
 }
 
-// G = (A B^T) * gelu'(Z) in BF16 (+ transposed).  Products whose output tiles
-// would not fill the GPU are split over K into FP32 partial sums first.
-int gelu_bwd_product(oka_mlp *M, const uint16_t *A, int lda, const uint16_t *B, int ldb, int Mr,
-                     int N, int K, const float *Z, uint16_t *G, uint16_t *Gt)
+// The product G = (A B^T) * gelu'(Z) (BF16, + transposed) as one GEMM launch
+// argument block (*epi its epilogue).  A product whose output tiles would not
+// fill the GPU is split over K into FP32 partial slices of M->acc instead; the
+// return value is the number of slices, and when it exceeds 1 the caller
+// follows the product with launch_gelu_bwd.
+int gelu_bwd_plan(oka_mlp *M, const uint16_t *A, int lda, const uint16_t *B, int ldb, int Mr,
+                  int N, int K, const float *Z, uint16_t *G, uint16_t *Gt, GemmArgs *out, int *epi)
 {
-    oka_ctx *c = M->ctx;
-    hipStream_t s = c->stream;
     const int tiles = (Mr / 64) * (N / 64);
     int splits = 1;
-    while (tiles * splits < 2 * c->num_cu && K / (splits * 2) >= 128 && K % (splits * 2 * 32) == 0)
+    while (splits < kMaxSplitSlices && tiles * splits < 2 * M->ctx->num_cu &&
+           K % (splits * 2 * 64) == 0 && K / (splits * 2) >= 128 &&
+           (size_t)Mr * N * splits * 2 <= M->acc_cap)
         splits *= 2;
     GemmArgs g = gemm(A, lda, B, ldb, Mr, N, K);
     if (splits == 1) {
         g.Zsaved = Z;
         g.Ch = G;
         g.Cht = Gt;
-        HIPCHK(c, launch_gemm(kEpiGeluBwd, g, s));
-        return OKA_OK;
+        *epi = kEpiGeluBwd;
+    } else {
+        g.Cf = M->acc;
+        g.ldc = N;
+        g.splits = splits;
+        *epi = kEpiPartialF32;
     }
-    HIPCHK(c, launch_fill_f32(M->acc, (int64_t)Mr * N, s));
-    g.Cf = M->acc;
-    g.ldc = N;
-    g.ksplit = K / splits;
-    HIPCHK(c, launch_gemm(kEpiAtomicF32, g, s, splits));
-    HIPCHK(c, launch_gelu_bwd(M->acc, Z, Mr, N, G, Gt, s));
-    return OKA_OK;
+    *out = g;
+    return splits;
 }
 
 // Forward through the two hidden layers for rows of x (rowidx/row0) into the
 // workspace; Bp rows (a multiple of 64), the first `valid` of them real.
 int hidden_forward(oka_mlp *M, int Bp, int valid, const float *x, const int32_t *rowidx,
-                   int64_t row0, const int64_t *dstate = nullptr)
+                   int64_t row0, bool train, int64_t *dstate = nullptr,
+                   const uint8_t *marks = nullptr)
 {
     oka_ctx *c = M->ctx;
     hipStream_t s = c->stream;
-    HIPCHK(c, launch_prep_input(x, M->nin, M->kin, rowidx, row0, Bp, valid, M->x0, M->x0t, s,
-                                dstate));
+    HIPCHK(c, launch_prep_input(x, M->nin, M->kin, rowidx, row0, Bp, valid, M->x0, M->x0t, marks,
+                                M->nout, M->ym, s, dstate));
+    // inference keeps neither the FP32 pre-activations nor the transposed copies
     GemmArgs g = gemm(M->x0, M->kin, M->w1h, M->kin, Bp, M->nh, M->kin);
     g.bias = M->p + M->ob1;
-    g.Cf = M->z1;
+    g.Cf = train ? M->z1 : nullptr;
     g.ldc = M->nh;
     g.Ch = M->h1;
-    g.Cht = M->h1t;
+    g.Cht = train ? M->h1t : nullptr;
     HIPCHK(c, launch_gemm(kEpiGeluFwd, g, s));
     g = gemm(M->h1, M->nh, M->w2h, M->nh, Bp, M->nh, M->nh);
     g.bias = M->p + M->ob2;
-    g.Cf = M->z2;
+    g.Cf = train ? M->z2 : nullptr;
     g.ldc = M->nh;
     g.Ch = M->h2;
-    g.Cht = M->h2t;
+    g.Cht = train ? M->h2t : nullptr;
     HIPCHK(c, launch_gemm(kEpiGeluFwd, g, s));
     return OKA_OK;
 }
 
-// Loss and full gradient (into M->grad) for one batch of B rows.
+// Loss (when `loss` is set) and full gradient (into M->grad) for one batch of
+// B rows: 7 launches, each dW product sharing its launch with the layer's dX
+// product and forming its bias gradient.
 int forward_backward(oka_mlp *M, int B, const float *x, const uint8_t *marks, float tscale,
                      const int32_t *rowidx, int64_t row0, double *loss,
-                     const int64_t *dstate = nullptr)
+                     int64_t *dstate = nullptr)
 {
     oka_ctx *c = M->ctx;
     hipStream_t s = c->stream;
-    TRY(hidden_forward(M, B, B, x, rowidx, row0, dstate));
+    TRY(hidden_forward(M, B, B, x, rowidx, row0, true, dstate, marks));
     GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, B, M->nout, M->nh);
     g.bias = M->p + M->ob3;
-    g.marks = marks;
-    g.rowidx = rowidx;
-    g.row0 = row0;
+    g.marks = M->ym;
     g.dstate = dstate;
-    g.step_rows = B;
     g.tscale = tscale;
     g.gscale = 1.0f / ((float)B * (float)M->nout);
     g.Ch = M->g3;
     g.Cht = M->g3t;
     g.loss = loss;
     HIPCHK(c, launch_gemm(kEpiLossGrad, g, s));
-    // layer 3: dW3 = G3^T H2, db3, G2 = (G3 W3) * gelu'(Z2)
-    g = gemm(M->g3t, B, M->h2t, B, M->nout, M->nh, B);
-    g.Cf = M->grad + M->oW3;
-    g.ldc = M->nh;
-    HIPCHK(c, launch_gemm(kEpiStoreF32, g, s));
-    HIPCHK(c, launch_bias_grad(M->g3t, M->nout, B, M->grad + M->ob3, s));
-    TRY(gelu_bwd_product(M, M->g3, M->nout, M->w3t, M->nout, B, M->nh, M->nout, M->z2, M->g2,
-                         M->g2t));
-    // layer 2
-    g = gemm(M->g2t, B, M->h1t, B, M->nh, M->nh, B);
-    g.Cf = M->grad + M->oW2;
-    g.ldc = M->nh;
-    HIPCHK(c, launch_gemm(kEpiStoreF32, g, s));
-    HIPCHK(c, launch_bias_grad(M->g2t, M->nh, B, M->grad + M->ob2, s));
-    TRY(gelu_bwd_product(M, M->g2, M->nh, M->w2t, M->nh, B, M->nh, M->nh, M->z1, M->g1, M->g1t));
-    // layer 1
-    g = gemm(M->g1t, B, M->x0t, B, M->nh, M->kin, B);
-    g.Cf = M->grad + M->oW1;
-    g.ldc = M->kin;
-    HIPCHK(c, launch_gemm(kEpiStoreF32, g, s));
-    HIPCHK(c, launch_bias_grad(M->g1t, M->nh, B, M->grad + M->ob1, s));
+    struct Layer {
+        const uint16_t *gt, *xt;     // G^T [n_out_l, B], X^T [n_in_l, B]
+        int nout_l, nin_l;
+        int64_t oW, ob;
+        const uint16_t *g, *wt;      // G [B, n_out_l], W^T [n_in_l, n_out_l] (dX product)
+        const float *z;              // pre-activation of the layer input
+        uint16_t *gin, *gint;        // dX * gelu'(z): G of the layer below
+    };
+    const Layer layers[3] = {
+        {M->g3t, M->h2t, M->nout, M->nh, M->oW3, M->ob3, M->g3, M->w3t, M->z2, M->g2, M->g2t},
+        {M->g2t, M->h1t, M->nh, M->nh, M->oW2, M->ob2, M->g2, M->w2t, M->z1, M->g1, M->g1t},
+        {M->g1t, M->x0t, M->nh, M->kin, M->oW1, M->ob1, nullptr, nullptr, nullptr, nullptr,
+         nullptr}};
+    for (const Layer &L : layers) {
+        // dW = G^T X and db = row sums of G^T
+        GemmArgs dw = gemm(L.gt, B, L.xt, B, L.nout_l, L.nin_l, B);
+        dw.Cf = M->grad + L.oW;
+        dw.ldc = L.nin_l;
+        dw.dbias = M->grad + L.ob;
+        if (!L.g) {
+            HIPCHK(c, launch_gemm(kEpiStoreF32, dw, s));
+            break;
+        }
+        GemmArgs dx;
+        int epi;
+        const int splits = gelu_bwd_plan(M, L.g, L.nout_l, L.wt, L.nout_l, B, L.nin_l, L.nout_l,
+                                         L.z, L.gin, L.gint, &dx, &epi);
+        HIPCHK(c, launch_gemm2(kEpiStoreF32, dw, epi, dx, s));
+        if (splits > 1)
+            HIPCHK(c, launch_gelu_bwd(M->acc, splits, L.z, B, L.nin_l, L.gin, L.gint, s));
+    }
+    return OKA_OK;
+}
+
+// One captured training step's inputs (device pointers).
+struct TrainStep {
+    int batch;
+    const float *x;
+    const uint8_t *marks;
+    float tscale;
+    const int32_t *order;
+    double *loss;
+    const AdamArgs *adam;
+};
+
+constexpr int kStepsPerGraph = 16;
+
+// Capture `steps` consecutive training steps on the context stream as one
+// executable graph.
+int capture_steps(oka_mlp *M, const TrainStep &t, int steps, hipGraphExec_t *out)
+{
+    oka_ctx *c = M->ctx;
+    *out = nullptr;
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = OKA_OK;
+    for (int k = 0; k < steps && rc == OKA_OK; ++k) {
+        rc = forward_backward(M, t.batch, t.x, t.marks, t.tscale, t.order, 0, t.loss, M->dstate);
+        if (rc == OKA_OK && launch_adam(*t.adam, c->stream) != hipSuccess)
+            rc = fail(c, OKA_ERR_DEVICE, "adam launch failed");
+    }
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (rc != OKA_OK) {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    HIPCHK(c, ec);
+    const hipError_t ei = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(c, ei);
     return OKA_OK;
 }
 
@@ -258,7 +314,9 @@ int oka_mlp_create(oka_ctx *c, int32_t n_in, int32_t n_hidden, int32_t n_out, in
     rc = rc ? rc : alloc_t(M, B * M->nh, &M->g1t);
     rc = rc ? rc : alloc_t(M, B * M->nh, &M->z1);
     rc = rc ? rc : alloc_t(M, B * M->nh, &M->z2);
-    rc = rc ? rc : alloc_t(M, B * M->nh, &M->acc);
+    rc = rc ? rc : alloc_t(M, B * M->nout, &M->ym);
+    M->acc_cap = std::max(B * M->nh, (size_t)4 * c->num_cu * 64 * 64);
+    rc = rc ? rc : alloc_t(M, M->acc_cap, &M->acc);
     rc = rc ? rc : alloc_t(M, 2, &M->dstate);
     if (rc == OKA_OK) {
         // zero state (padding columns of W1 must stay exactly zero)
@@ -329,9 +387,11 @@ int oka_mlp_grad(oka_mlp *M, int32_t batch, const float *x, const uint8_t *marks
     const void *dx, *dm;
     TRY(stage_in(c, kSlotMlpX, x, sizeof(float) * batch * M->nin, flags, &dx));
     TRY(stage_in(c, kSlotMlpMarks, marks, (size_t)batch * M->nout, flags, &dm));
-    void *dl;
-    TRY(ensure(c, kSlotMlpLoss, sizeof(double), &dl));
-    HIPCHK(c, launch_zero_f64((double *)dl, 1, c->stream));
+    void *dl = nullptr;
+    if (loss) {
+        TRY(ensure(c, kSlotMlpLoss, sizeof(double), &dl));
+        HIPCHK(c, launch_zero_f64((double *)dl, 1, c->stream));
+    }
     TRY(forward_backward(M, batch, (const float *)dx, (const uint8_t *)dm, target_scale, nullptr, 0,
                          (double *)dl));
     TRY(copy_params(M, grad, false, M->grad, kind_from_dev(flags)));
@@ -358,44 +418,58 @@ int oka_mlp_train(oka_mlp *M, int64_t nsamples, const float *x, const uint8_t *m
     TRY(stage_in(c, kSlotMlpX, x, sizeof(float) * nsamples * M->nin, flags, &dx));
     TRY(stage_in(c, kSlotMlpMarks, marks, (size_t)nsamples * M->nout, flags, &dm));
     TRY(stage_in(c, kSlotMlpOrder, order, sizeof(int32_t) * (size_t)nsteps * batch, flags, &dord));
-    void *dl;
-    TRY(ensure(c, kSlotMlpLoss, sizeof(double) * std::max(nsteps, 1), &dl));
-    HIPCHK(c, launch_zero_f64((double *)dl, nsteps, c->stream));
-    // One training step (forward, backward, Adam, BF16 refresh) is captured as a
-    // HIP graph and replayed nsteps times; the step-dependent values (batch
-    // rows, loss slot, Adam step) come from the device counters M->dstate.
-    const int64_t st0[2] = {0, M->step};
+    void *dl = nullptr;
+    if (losses && nsteps > 0) {
+        TRY(ensure(c, kSlotMlpLoss, sizeof(double) * nsteps, &dl));
+        HIPCHK(c, launch_zero_f64((double *)dl, nsteps, c->stream));
+    }
+    // Training steps (forward, backward, Adam with the BF16 refresh: 10
+    // launches each) are captured kStepsPerGraph at a time as a HIP graph and
+    // replayed; the step-dependent values (batch rows, loss slot, Adam step)
+    // come from the device counters M->dstate, which each step's first and
+    // last kernels advance.
+    const int64_t st0[2] = {0, 0};
     HIPCHK(c, hipMemcpyAsync(M->dstate, st0, sizeof(st0), hipMemcpyHostToDevice, c->stream));
     if (nsteps > 0) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = forward_backward(M, batch, (const float *)dx, (const uint8_t *)dm, target_scale,
-                                  (const int32_t *)dord, 0, (double *)dl, M->dstate);
-        if (rc == OKA_OK && launch_adam(M->p, M->grad, M->m, M->v, M->np, lr, beta1, beta2, eps,
-                                        0, M->dstate, c->stream) != hipSuccess)
-            rc = fail(c, OKA_ERR_DEVICE, "adam launch failed");
-        if (rc == OKA_OK)
-            rc = refresh_bf16(M);
-        if (rc == OKA_OK && launch_step_advance(M->dstate, c->stream) != hipSuccess)
-            rc = fail(c, OKA_ERR_DEVICE, "step launch failed");
-        hipError_t ec = hipStreamEndCapture(c->stream, &graph);
-        if (rc != OKA_OK) {
-            if (graph)
-                (void)hipGraphDestroy(graph);
-            return rc;
-        }
-        HIPCHK(c, ec);
-        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ei != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            HIPCHK(c, ei);
+        AdamArgs a{};
+        a.p = M->p;
+        a.grad = M->grad;
+        a.m = M->m;
+        a.v = M->v;
+        a.lr = lr;
+        a.b1 = beta1;
+        a.b2 = beta2;
+        a.eps = eps;
+        a.tbase = M->step;
+        a.dstate = M->dstate;
+        const AdamSeg segs[6] = {
+            {M->oW1, M->nh, M->kin, M->w1h, M->w1t}, {M->ob1, M->nh / 32, 32, nullptr, nullptr},
+            {M->oW2, M->nh, M->nh, M->w2h, M->w2t},  {M->ob2, M->nh / 32, 32, nullptr, nullptr},
+            {M->oW3, M->nout, M->nh, M->w3h, M->w3t}, {M->ob3, M->nout / 32, 32, nullptr, nullptr}};
+        for (int i = 0; i < 6; ++i)
+            a.seg[i] = segs[i];
+        a.nseg = 6;
+        const TrainStep ts{batch, (const float *)dx, (const uint8_t *)dm, target_scale,
+                           (const int32_t *)dord, (double *)dl, &a};
+        const int unroll = std::min<int32_t>(nsteps, kStepsPerGraph);
+        const int full = nsteps / unroll, rest = nsteps % unroll;
+        hipGraphExec_t ex_full = nullptr, ex_rest = nullptr;
+        TRY(capture_steps(M, ts, unroll, &ex_full));
+        if (rest) {
+            const int rc = capture_steps(M, ts, rest, &ex_rest);
+            if (rc != OKA_OK) {
+                (void)hipGraphExecDestroy(ex_full);
+                return rc;
+            }
         }
         hipError_t el = hipSuccess;
-        for (int32_t st = 0; st < nsteps && el == hipSuccess; ++st)
-            el = hipGraphLaunch(exec, c->stream);
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
+        for (int i = 0; i < full && el == hipSuccess; ++i)
+            el = hipGraphLaunch(ex_full, c->stream);
+        if (rest && el == hipSuccess)
+            el = hipGraphLaunch(ex_rest, c->stream);
+        (void)hipGraphExecDestroy(ex_full);
+        if (ex_rest)
+            (void)hipGraphExecDestroy(ex_rest);
         HIPCHK(c, el);
         M->step += nsteps;
     }
@@ -445,7 +519,7 @@ static int infer(oka_mlp *M, int64_t n, const float *x, const uint8_t *marks, fl
     for (int64_t r0 = 0; r0 < n; r0 += chunk) {
         const int valid = (int)std::min<int64_t>(chunk, n - r0);
         const int Bp = (valid + 63) / 64 * 64;
-        TRY(hidden_forward(M, Bp, valid, (const float *)dx, nullptr, r0));
+        TRY(hidden_forward(M, Bp, valid, (const float *)dx, nullptr, r0, false));
         GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
         g.bias = M->p + M->ob3;
         g.mvalid = valid;
@@ -562,6 +636,36 @@ int oka_draw_realizations(oka_ctx *c, int32_t nwells, const oka_dist *dists, int
     return sync_unless_async(c, flags);
 }
 
+int oka_mark_moments(oka_ctx *c, int64_t n, int32_t ncell, const uint8_t *marks, uint64_t *sums,
+                     uint64_t *sums_sq, uint32_t flags)
+{
+    if (!c || !marks || !sums || n < 0 || ncell < 1)
+        return OKA_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const void *dm;
+    TRY(stage_in(c, kSlotMlpMarks, marks, (size_t)n * ncell, flags, &dm));
+    uint64_t *d_sums = sums, *d_sq = sums_sq;
+    if (!(flags & OKA_PTR_DEVICE)) {
+        void *base;
+        TRY(ensure(c, kSlotMlpOut2, sizeof(uint64_t) * 2 * ncell, &base));
+        d_sums = (uint64_t *)base;
+        d_sq = sums_sq ? d_sums + ncell : nullptr;
+        HIPCHK(c, hipMemcpyAsync(d_sums, sums, sizeof(uint64_t) * ncell, hipMemcpyHostToDevice, s));
+        if (sums_sq)
+            HIPCHK(c, hipMemcpyAsync(d_sq, sums_sq, sizeof(uint64_t) * ncell,
+                                     hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(c, launch_mark_moments((const uint8_t *)dm, n, ncell, d_sums, d_sq, s));
+    if (!(flags & OKA_PTR_DEVICE)) {
+        HIPCHK(c, hipMemcpyAsync(sums, d_sums, sizeof(uint64_t) * ncell, hipMemcpyDeviceToHost, s));
+        if (sums_sq)
+            HIPCHK(c, hipMemcpyAsync(sums_sq, d_sq, sizeof(uint64_t) * ncell,
+                                     hipMemcpyDeviceToHost, s));
+    }
+    return sync_unless_async(c, flags);
+}
+
 int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist *dists,
                           int64_t n, uint64_t seed, int64_t first, const double *feat_mean,
                           const double *feat_std, double *sums, double *sums_sq, uint32_t flags)
@@ -591,8 +695,9 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
     HIPCHK(c, hipMemcpyAsync(d_std, feat_std, sizeof(double) * nin, hipMemcpyHostToDevice, s));
     const void *dd;
     TRY(stage_in(c, kSlotCvDists, dists, sizeof(oka_dist) * (nw + 3), 0, &dd));
-    // per-chunk scratch
-    const int64_t chunk = M->max_batch;
+    // per-chunk scratch: draws, fits and features go 64 Ki realizations per
+    // launch (enough lanes to fill the GPU), the network max_batch rows per pass
+    const int64_t chunk = std::max<int64_t>(M->max_batch, 65536 / M->max_batch * M->max_batch);
     const size_t per = (size_t)nw + 3 + 6 + 6;   // q, K, por, thick, z, coef (doubles)
     void *wb;
     TRY(ensure(c, kSlotCvDraw, (sizeof(double) * per + sizeof(float) * nin + 4) * chunk + 256,
@@ -657,14 +762,17 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
         fe.std = d_std;
         fe.x = x;
         HIPCHK(c, launch_features(fe, s));
-        const int Bp = (valid + 63) / 64 * 64;
-        TRY(hidden_forward(M, Bp, valid, x, nullptr, 0));
-        GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
-        g.bias = M->p + M->ob3;
-        g.mvalid = valid;
-        g.sums = d_sums;
-        g.sums_sq = d_sq;
-        HIPCHK(c, launch_gemm(kEpiAccum, g, s));
+        for (int r1 = 0; r1 < valid; r1 += M->max_batch) {
+            const int nv = std::min(M->max_batch, valid - r1);
+            const int Bp = (nv + 63) / 64 * 64;
+            TRY(hidden_forward(M, Bp, nv, x, nullptr, r1, false));
+            GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
+            g.bias = M->p + M->ob3;
+            g.mvalid = nv;
+            g.sums = d_sums;
+            g.sums_sq = d_sq;
+            HIPCHK(c, launch_gemm(kEpiAccum, g, s));
+        }
     }
     if (!(flags & OKA_PTR_DEVICE)) {
         HIPCHK(c, hipMemcpyAsync(sums, d_sums, sizeof(double) * M->nout, hipMemcpyDeviceToHost, s));

@@ -123,7 +123,42 @@ __global__ void __launch_bounds__(256) k_features(FeatureArgs a)
     a.x[t] = (float)((f - a.mean[j]) / a.std[j]);
 }
 
+// Exact integer moments of the window marks: workgroup (x, y) sums cells
+// 256x .. 256x + 255 over rows y * rows_per .. + rows_per (partial sums stay
+// below 2^32: rows_per * 255^2 < 2^32).
+__global__ void __launch_bounds__(256) k_mark_moments(const uint8_t *marks, int64_t n, int ncell,
+                                                      int rows_per, unsigned long long *sums,
+                                                      unsigned long long *sums_sq)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= ncell)
+        return;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per;
+    const int64_t r1 = r0 + rows_per < n ? r0 + rows_per : n;
+    uint32_t s = 0, s2 = 0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const uint32_t v = marks[r * ncell + c];
+        s += v;
+        s2 += v * v;
+    }
+    atomicAdd(&sums[c], (unsigned long long)s);
+    if (sums_sq)
+        atomicAdd(&sums_sq[c], (unsigned long long)s2);
+}
+
 }  // namespace
+
+hipError_t launch_mark_moments(const uint8_t *marks, int64_t n, int32_t ncell, uint64_t *sums,
+                               uint64_t *sums_sq, hipStream_t s)
+{
+    constexpr int kRowsPer = 1024;
+    if (n <= 0 || ncell <= 0)
+        return hipSuccess;
+    const dim3 grid((unsigned)((ncell + 255) / 256), (unsigned)((n + kRowsPer - 1) / kRowsPer));
+    hipLaunchKernelGGL(k_mark_moments, grid, dim3(256), 0, s, marks, n, ncell, kRowsPer,
+                       (unsigned long long *)sums, (unsigned long long *)sums_sq);
+    return hipGetLastError();
+}
 
 hipError_t launch_draw(const DrawArgs &a, hipStream_t s)
 {

@@ -102,24 +102,27 @@ class SurrogateMLP:
         return g, loss.value
 
     def train(self, x, marks, target_scale, epochs, batch=512, lr=1e-3, beta1=0.9, beta2=0.999,
-              eps=1e-8, seed=0):
-        """Adam over shuffled minibatches; returns the per-step losses."""
+              eps=1e-8, seed=0, rows=None):
+        """Adam over shuffled minibatches of the samples `rows` (default: all);
+        returns the per-step losses."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         marks = np.ascontiguousarray(marks, dtype=np.uint8)
-        n = x.shape[0]
+        rows = np.arange(x.shape[0]) if rows is None else np.asarray(rows)
+        n = rows.shape[0]
         per_epoch = n // batch
         if per_epoch == 0:
             raise ValueError("fewer samples than one batch")
         rng = np.random.default_rng(seed)
-        losses = []
-        for _ in range(int(epochs)):
-            order = rng.permutation(n)[:per_epoch * batch].astype(np.int32)
-            out = np.zeros(per_epoch, dtype=np.float32)
-            self.ctx.check(self.ctx.lib.oka_mlp_train(
-                self.h, n, x.ctypes.data, marks.ctypes.data, float(target_scale),
-                order.ctypes.data, batch, per_epoch, lr, beta1, beta2, eps, out.ctypes.data, 0))
-            losses.append(out)
-        return np.concatenate(losses)
+        # every epoch's shuffle up front: the whole schedule is one library call
+        order = np.concatenate([rows[rng.permutation(n)[:per_epoch * batch]]
+                                for _ in range(int(epochs))]).astype(np.int32)
+        n = x.shape[0]
+        nsteps = int(epochs) * per_epoch
+        losses = np.zeros(nsteps, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.oka_mlp_train(
+            self.h, n, x.ctypes.data, marks.ctypes.data, float(target_scale), order.ctypes.data,
+            batch, nsteps, lr, beta1, beta2, eps, losses.ctypes.data, 0))
+        return losses
 
     def predict(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)
@@ -257,6 +260,23 @@ def cell_probabilities(counts, nreal, window):
     return out.reshape(nr, c, nc, c).mean(axis=(1, 3)).ravel() / float(nreal)
 
 
+def plain_estimate(marks, tscale):
+    """Mean of marks * tscale per cell and the variance of that mean, from the
+    exact integer moments of the marks (oka_mark_moments)."""
+    marks = np.ascontiguousarray(marks, dtype=np.uint8)
+    n, ncell = marks.shape
+    if n == 0:
+        raise ValueError("no realizations")
+    s1 = np.zeros(ncell, np.uint64)
+    s2 = np.zeros(ncell, np.uint64)
+    ctx = _lib.context()
+    ctx.check(ctx.lib.oka_mark_moments(ctx.h, n, ncell, marks.ctypes.data, s1.ctypes.data,
+                                       s2.ctypes.data, 0))
+    mean = s1.astype(np.float64) / n
+    var = np.maximum(s2.astype(np.float64) / n - mean ** 2, 0.0)
+    return mean * tscale, var * (tscale * tscale) / n
+
+
 class ControlVariateResult:
     def __init__(self, **kw):
         self.__dict__.update(kw)
@@ -290,33 +310,40 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     t0 = time.perf_counter()
     marks, st = trace_marks(site, d, coef, window, precision)
     t["trace"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
     tscale = 1.0 / (window.cell * window.cell)
-    std = Standardizer(features(d, coef))
-    X = std(features(d, coef))
-    fold_of = rng.permutation(n_traced) % folds
+    F = features(d, coef)
+    std = Standardizer(F)
+    X = std(F)
+    # folds are contiguous slices: the realizations are independent draws
+    bounds = np.linspace(0, n_traced, folds + 1).astype(np.int64)
     surr_seed = int(rng.integers(1, 2 ** 63))
     surr_split = np.linspace(0, n_surrogate, folds + 1).astype(np.int64)
     ncell = window.nrows * window.ncols
     sum_s = np.zeros(ncell)
     var_cv = np.zeros(ncell)
     sum_d = np.zeros(ncell)
+    net = SurrogateMLP(X.shape[1], n_hidden, ncell, seed=seed + 1)
+    t["setup"] = time.perf_counter() - t0
     t["train"] = 0.0
     t["surrogate"] = 0.0
     losses = []
     for f in range(folds):
-        tr = np.nonzero(fold_of != f)[0]
-        ho = np.nonzero(fold_of == f)[0]
-        net = SurrogateMLP(X.shape[1], n_hidden, ncell, seed=seed + 1 + f)
-        epochs = max(1, int(math.ceil(train_steps / max(1, len(tr) // batch))))
+        lo, hi = int(bounds[f]), int(bounds[f + 1])
+        tr = np.concatenate([np.arange(0, lo), np.arange(hi, n_traced)])
         t0 = time.perf_counter()
-        losses.append(net.train(X[tr], marks[tr], tscale, epochs, batch, lr, seed=seed + 100 + f))
+        if f > 0:   # a fresh network (and Adam state) per fold
+            net.set_params(net.initial_params(seed + 1 + f))
+        epochs = max(1, int(math.ceil(train_steps / max(1, len(tr) // batch))))
+        losses.append(net.train(X, marks, tscale, epochs, batch, lr, seed=seed + 100 + f,
+                                rows=tr))
         t["train"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         mf = int(surr_split[f + 1] - surr_split[f])
         a_s = net.surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std)
-        a_d = net.accumulate(X[ho], marks[ho], tscale)
+        a_d = net.accumulate(X[lo:hi], marks[lo:hi], tscale)
         t["surrogate"] += time.perf_counter() - t0
-        nf = len(ho)
+        nf = hi - lo
         w = nf / n_traced
         mean_s = a_s["s"] / mf
         var_s = np.maximum(a_s["s2"] / mf - mean_s ** 2, 0.0)
@@ -325,10 +352,10 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
         sum_s += w * mean_s
         sum_d += w * mean_d
         var_cv += w * w * (var_d / nf + var_s / mf)
-        del net
-    y = marks.astype(np.float64) * tscale
-    p_plain = y.mean(0)
-    var_plain = y.var(0) / n_traced
+    del net
+    t0 = time.perf_counter()
+    p_plain, var_plain = plain_estimate(marks, tscale)
+    t["plain"] = time.perf_counter() - t0
     return ControlVariateResult(p_cv=sum_s + sum_d, var_cv=var_cv, p_plain=p_plain,
                                 var_plain=var_plain, window=window, times=t, folds=folds,
                                 n_traced=n_traced, n_surrogate=n_surrogate, losses=losses,

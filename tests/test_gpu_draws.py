@@ -47,3 +47,29 @@ def test_distribution_moments(gpu_ctx):
     z = d["z"]
     assert np.abs(z.mean(0)).max() < 0.01 and np.abs(z.std(0) - 1.0).max() < 0.01
     assert np.abs(np.corrcoef(z.T) - np.eye(6)).max() < 0.01
+
+
+def test_mark_moments_are_exact(gpu_ctx):
+    from onekapy_amd.surrogate import plain_estimate
+    rng = np.random.default_rng(11)
+    # 2 row blocks and a ragged last one; a cell count that is not a multiple of 256
+    marks = rng.integers(0, 256, (2 * 1024 + 37, 300)).astype(np.uint8)
+    marks[:, 0] = 0        # never visited
+    marks[:, 1] = 255      # largest mark: the partial sums' range
+    p, v = plain_estimate(marks, 1.0)
+    m = marks.astype(np.int64)
+    n = marks.shape[0]
+    assert np.array_equal(p, m.sum(0).astype(np.float64) / n)
+    ref_var = (m * m).sum(0) / n - (m.sum(0) / n) ** 2
+    assert np.allclose(v * n, ref_var, rtol=1e-12, atol=1e-9)
+    assert p[0] == 0.0 and v[0] == 0.0 and p[1] == 255.0 and v[1] == 0.0
+    # the sums accumulate; an empty input leaves them untouched
+    s1 = np.full(300, 7, np.uint64)
+    s2 = np.full(300, 9, np.uint64)
+    empty = np.zeros((0, 300), np.uint8)
+    gpu_ctx.check(gpu_ctx.lib.oka_mark_moments(gpu_ctx.h, 0, 300, empty.ctypes.data,
+                                               s1.ctypes.data, s2.ctypes.data, 0))
+    assert (s1 == 7).all() and (s2 == 9).all()
+    gpu_ctx.check(gpu_ctx.lib.oka_mark_moments(gpu_ctx.h, n, 300, marks.ctypes.data,
+                                               s1.ctypes.data, None, 0))
+    assert np.array_equal(s1, m.sum(0).astype(np.uint64) + 7) and (s2 == 9).all()

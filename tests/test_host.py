@@ -183,3 +183,4 @@ def test_sites_are_well_posed():
     b = sites.draw(sites.config0(), seed=3)
     for k in a:
         assert np.array_equal(a[k], b[k])
+
