@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define OKA_ABI_VERSION 1
+#define OKA_ABI_VERSION 2
 
 enum {
     OKA_OK = 0,
@@ -131,7 +131,21 @@ typedef struct {
     double duration, tol, maxstep, umbra;
     int64_t max_steps;       /* attempted-step cap per trace; <= 0: 1,000,000        */
     const double *start_xy;  /* npaths x 2 start points (capturezone.py:114-115)     */
+    int32_t bitmap_mode;     /* visited-node bitmap of a realization: OKA_BITMAP_*   */
+    int32_t tile_slots;      /* OKA_BITMAP_TILED: LDS tile slots (<= 0: as many as fit) */
 } oka_trace_opts;
+
+/* Where a realization's visited-node bitmap lives.  AUTO picks LDS_FLAT when
+ * the whole bitmap fits in LDS, else TILED, else HBM_FLAT.  TILED keeps
+ * 32 x 32-node tiles in LDS slots claimed on first touch, with the tiles that
+ * find no free slot in HBM; the flush visits only the claimed tiles.  Every
+ * mode gives the same counts. */
+enum {
+    OKA_BITMAP_AUTO = 0,
+    OKA_BITMAP_LDS_FLAT = 1,
+    OKA_BITMAP_TILED = 2,
+    OKA_BITMAP_HBM_FLAT = 3
+};
 
 /* A node lattice: node (i, j) is at (xmin + j*dx, ymin + i*dy); row i is y.
  * xmax/ymax are the field's own attributes (probabilityfield.py:140-143,

@@ -74,7 +74,8 @@ class Site_(C.Structure):
 class TraceOpts_(C.Structure):
     _fields_ = [("npaths", _i32), ("precision", _i32), ("confined", _i32), ("reserved", _i32),
                 ("duration", _d), ("tol", _d), ("maxstep", _d), ("umbra", _d),
-                ("max_steps", _i64), ("start_xy", _vp)]
+                ("max_steps", _i64), ("start_xy", _vp), ("bitmap_mode", _i32),
+                ("tile_slots", _i32)]
 
 
 class Grid_(C.Structure):
@@ -267,8 +268,11 @@ def make_site(wells_xy, xo, yo, base=0.0, obs=None, wells_ptr=None, obs_ptr=None
     return s
 
 
+BITMAP_AUTO, BITMAP_LDS_FLAT, BITMAP_TILED, BITMAP_HBM_FLAT = 0, 1, 2, 3
+
+
 def make_trace_opts(npaths, duration, tol, maxstep, umbra, start_xy_ptr, precision=FP64_EXACT,
-                    confined=True, max_steps=0):
+                    confined=True, max_steps=0, bitmap_mode=BITMAP_AUTO, tile_slots=0):
     o = TraceOpts_()
     o.npaths = int(npaths)
     o.precision = int(precision)
@@ -276,6 +280,8 @@ def make_trace_opts(npaths, duration, tol, maxstep, umbra, start_xy_ptr, precisi
     o.duration, o.tol, o.maxstep, o.umbra = float(duration), float(tol), float(maxstep), float(umbra)
     o.max_steps = int(max_steps)
     o.start_xy = start_xy_ptr
+    o.bitmap_mode = int(bitmap_mode)
+    o.tile_slots = int(tile_slots)
     return o
 
 

@@ -174,6 +174,8 @@ int oka_ctx_create(oka_ctx **out, int device)
     c->stream = c->own;
     c->num_cu = prop.multiProcessorCount;
     c->max_lds = prop.sharedMemPerBlock;
+    c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                          : prop.sharedMemPerBlock;
     *out = c;
     return OKA_OK;
 }
@@ -464,7 +466,41 @@ static int trace_common(oka_ctx *c, const oka_site *site, int32_t nreal, const d
     return OKA_OK;
 }
 
-static int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+static // Where a realization's visited-node bitmap lives (OKA_BITMAP_*, see
+// include/oka.h): a flat LDS bitmap when the whole grid fits in the LDS
+// budget beside the wells and window counters, else 32 x 32-node tiles in LDS
+// slots, else a flat bitmap in HBM.
+int choose_bitmap(oka_ctx *c, const oka_trace_opts *o, size_t base_lds, TraceArgs &a, int *kind)
+{
+    const int mode = o->bitmap_mode;
+    if (mode < OKA_BITMAP_AUTO || mode > OKA_BITMAP_HBM_FLAT)
+        return fail(c, OKA_ERR_ARG, "unknown bitmap mode %d", mode);
+    const size_t budget = std::min<size_t>(kLdsBudget, c->max_lds);
+    const bool flat_fits = base_lds + (size_t)a.nwords * 4 <= budget;
+    a.tiles_x = (a.gncols + 31) / 32;
+    const int64_t ntiles = (int64_t)a.tiles_x * ((a.gnrows + 31) / 32);
+    int64_t slots = 0;
+    if (base_lds + (size_t)(ntiles + 1) * 4 < budget)
+        slots = (int64_t)(budget - base_lds - (size_t)(ntiles + 1) * 4) / (4 * 33);
+    if (o->tile_slots > 0)
+        slots = std::min<int64_t>(slots, o->tile_slots);
+    a.ntiles = (int32_t)ntiles;
+    a.tile_slots = (int32_t)slots;
+    if (mode == OKA_BITMAP_LDS_FLAT && !flat_fits)
+        return fail(c, OKA_ERR_ARG, "the grid's bitmap (%d words) does not fit in LDS", a.nwords);
+    if (mode == OKA_BITMAP_TILED && slots < 1)
+        return fail(c, OKA_ERR_ARG, "no LDS left for tile slots (%lld tiles)", (long long)ntiles);
+    if (mode == OKA_BITMAP_LDS_FLAT || (mode == OKA_BITMAP_AUTO && flat_fits))
+        *kind = kBmLdsFlat;
+    else if (mode == OKA_BITMAP_TILED ||
+             (mode == OKA_BITMAP_AUTO && slots >= kMinTileSlots))
+        *kind = kBmTiled;
+    else
+        *kind = kBmHbmFlat;
+    return OKA_OK;
+}
+
+int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
                           const double *n, const double *b, const double *q, const double *coef,
                           const oka_trace_opts *o, const oka_grid *g, int32_t nepochs,
                           const oka_epoch *epochs, int32_t rasterize, int32_t *counts,
@@ -542,25 +578,37 @@ static int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const
     }
 
     // launch shape: one realization per 256-lane workgroup, grid-stride
-    const bool lds_bitmap = (size_t)a.nwords * 4 <= (size_t)kLdsBitmapMaxWords * 4 &&
-                            trace_fp64_lds_bytes(a.nwells, a.nwords, true) + win_lds <= c->max_lds;
-    int blocks = nreal;
-    if (rasterize && !lds_bitmap) {
-        // per-block bitmaps in HBM: bound the resident set
-        const int cap_blocks = std::max(1, c->num_cu * 8);
-        blocks = std::min(nreal, cap_blocks);
-        TRY(ensure(c, kSlotBitmap, sizeof(uint32_t) * (size_t)a.nwords * blocks, &o2));
-        a.gbitmap = (uint32_t *)o2;
-    }
-    if (trace_fp64_lds_bytes(a.nwells, 0, false) + win_lds > c->max_lds)
+    const bool fp64 = o->precision == OKA_FP64_EXACT;
+    const size_t base_lds =
+        (fp64 ? trace_fp64_wells_lds(a.nwells) : trace_fp32_wells_lds(a.nwells)) + win_lds;
+    if (base_lds > c->max_lds)
         return fail(c, OKA_ERR_ARG, "too many wells for LDS (%d)", a.nwells);
+    int kind = kBmLdsFlat;
+    int blocks = nreal;
+    if (rasterize) {
+        TRY(choose_bitmap(c, o, base_lds, a, &kind));
+        if (kind != kBmLdsFlat) {
+            // per-workgroup HBM regions: bound the resident set to what the
+            // LDS lets run at once
+            const size_t lds = base_lds + (kind == kBmTiled
+                                               ? (size_t)tile_lds_words(a.ntiles, a.tile_slots) * 4
+                                               : 0);
+            const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, c->lds_per_cu / lds));
+            blocks = std::min(nreal, c->num_cu * per_cu);
+            const size_t words = kind == kBmTiled ? (size_t)a.ntiles * 32 : (size_t)a.nwords;
+            TRY(ensure(c, kSlotBitmap, sizeof(uint32_t) * words * blocks, &o2));
+            a.gbitmap = (uint32_t *)o2;
+            if (kind == kBmTiled)   // overflow tiles start empty; each flush leaves them so
+                TRY(fill_u32(c, a.gbitmap, (int64_t)words * blocks));
+        }
+    }
     const bool timed = stats != nullptr;
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if (o->precision == OKA_FP64_EXACT)
-        HIPCHK(c, launch_trace_fp64(a, blocks, lds_bitmap, c->stream));
+    if (fp64)
+        HIPCHK(c, launch_trace_fp64(a, blocks, kind, c->stream));
     else
-        HIPCHK(c, launch_trace_fp32(a, blocks, lds_bitmap, c->stream));
+        HIPCHK(c, launch_trace_fp32(a, blocks, kind, c->stream));
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     TRY(copy_back(c, rasterize ? counts : nullptr, a.counts, sizeof(int32_t) * nnodes, flags));

@@ -35,7 +35,8 @@ struct oka_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int num_cu = 0;
-    size_t max_lds = 0;
+    size_t max_lds = 0;      // per workgroup
+    size_t lds_per_cu = 0;
     std::string err;
     oka::DevBuf buf[oka::kSlotCount];
 };

@@ -15,6 +15,8 @@
 // probability is reported against the FP64-exact mode.
 #include <math.h>
 
+#include <type_traits>
+
 #include "oka_internal.h"
 
 namespace oka {
@@ -181,9 +183,10 @@ __device__ __forceinline__ Epoch32 find_epoch32(const TraceArgs &a, int64_t k)
     return {(float)(e.xmin - a.xo), (float)(e.ymin - a.yo), e.nrows, e.ncols, e.row_off, e.col_off};
 }
 
-__device__ __forceinline__ int raster32(uint32_t *bm, const Epoch32 &e, int ncols_final, float dx,
-                                         float dy, float idx_, float idy, float umbra, float u2,
-                                         float ax, float ay, float bx, float by)
+template <typename Bits>
+__device__ __forceinline__ int raster32(Bits &bits, const Epoch32 &e, float dx, float dy,
+                                         float idx_, float idy, float umbra, float u2, float ax,
+                                         float ay, float bx, float by)
 {
     const float lox = fminf(ax, bx), hix = fmaxf(ax, bx);
     const float loy = fminf(ay, by), hiy = fmaxf(ay, by);
@@ -207,9 +210,8 @@ __device__ __forceinline__ int raster32(uint32_t *bm, const Epoch32 &e, int ncol
         const float cax = fmaf((float)j, dx, e.xmin) - ax;
         const int col = j + e.coff;
         for (int i = i0; i < i1; ++i) {
-            const int idx = (i + e.roff) * ncols_final + col;
-            const uint32_t bit = 1u << (idx & 31);
-            if (bm[idx >> 5] & bit)
+            const int row = i + e.roff;
+            if (bits.test(row, col))
                 continue;
             const float cay = fmaf((float)i, dy, e.ymin) - ay;
             const float perp = bax * cay - bay * cax;
@@ -224,18 +226,19 @@ __device__ __forceinline__ int raster32(uint32_t *bm, const Epoch32 &e, int ncol
             else
                 d2 = al;
             if (d2 < u2)
-                atomicOr(&bm[idx >> 5], bit);
+                bits.set(row, col);
         }
     }
     return (j1 - j0) * (i1 - i0);
 }
 
-template <bool kLdsBitmap>
+template <int kKind>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
 {
     extern __shared__ float4 smem4[];
     float4 *s_w = smem4;
     uint32_t *s_bm = reinterpret_cast<uint32_t *>(smem4 + a.nwells);
+    using Bits = typename std::conditional<kKind == kBmTiled, TileBits, FlatBits>::type;
     __shared__ int s_next;
     __shared__ unsigned long long s_stat[kStatCount];
     __shared__ float s_reg[5];
@@ -255,10 +258,21 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
     const float igdx = (float)(1.0 / a.gdx), igdy = (float)(1.0 / a.gdy);
     const float gx0 = (float)(a.gxmin - a.xo), gx1 = (float)(a.gxmax - a.xo);
     const float gy0 = (float)(a.gymin - a.yo), gy1 = (float)(a.gymax - a.yo);
-    uint32_t *bm = kLdsBitmap ? s_bm : a.gbitmap + (int64_t)blockIdx.x * a.nwords;
+    Bits bits;
+    uint32_t *lds_tail = s_bm;
+    if constexpr (kKind == kBmTiled) {
+        bits = tile_bits(a, s_bm);
+        tile_init(bits);
+        lds_tail += tile_lds_words(a.ntiles, a.tile_slots);
+    } else if constexpr (kKind == kBmLdsFlat) {
+        bits = FlatBits{s_bm, a.gncols};
+        lds_tail += a.rasterize ? a.nwords : 0;
+    } else {
+        bits = FlatBits{a.gbitmap + (int64_t)blockIdx.x * a.nwords, a.gncols};
+    }
     unsigned int *s_win = nullptr;
     if (a.rasterize && a.wmarks) {
-        s_win = s_bm + ((kLdsBitmap && a.rasterize) ? a.nwords : 0);
+        s_win = lds_tail;
         for (int c = threadIdx.x; c < a.w_nr * a.w_nc; c += blockDim.x)
             s_win[c] = 0u;
     }
@@ -282,9 +296,11 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
             s_reg[4] = (float)(c[4] / bn);
             s_next = blockDim.x;
         }
-        if (a.rasterize)
-            for (int w = tid; w < a.nwords; w += blockDim.x)
-                bm[w] = 0u;
+        if constexpr (kKind != kBmTiled) {   // a tiled bitmap is left empty by its flush
+            if (a.rasterize)
+                for (int w = tid; w < a.nwords; w += blockDim.x)
+                    bits.bm[w] = 0u;
+        }
         __syncthreads();
 
         Field32 f;
@@ -318,8 +334,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
                 float ox, oy;
                 const int res = attempt32(f, I, s, ox, oy);
                 if ((res & 1) && a.rasterize)
-                    n_cand += (unsigned long long)raster32(bm, ep, a.gncols, gdx, gdy, igdx, igdy,
-                                                           umbra, u2, ox, oy, s.x, s.y);
+                    n_cand += (unsigned long long)raster32(bits, ep, gdx, gdy, igdx, igdy, umbra,
+                                                           u2, ox, oy, s.x, s.y);
                 if (res & 2) {
                     live = false;
                     ended = true;
@@ -353,8 +369,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
             if (n_st[j])
                 atomicAdd(&s_stat[kStatStatus0 + j], n_st[j]);
         __syncthreads();
-        if (a.rasterize)
-            atomicAdd(&s_stat[kStatVisited], flush_bitmap(a, bm, s_win));
+        if (a.rasterize) {
+            if constexpr (kKind == kBmTiled)
+                atomicAdd(&s_stat[kStatVisited], flush_tiles(a, bits, s_win));
+            else
+                atomicAdd(&s_stat[kStatVisited], flush_bitmap(a, bits.bm, s_win));
+        }
         __syncthreads();
         if (s_win) {
             store_window(a, r, s_win);
@@ -368,21 +388,21 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
 
 }  // namespace
 
-size_t trace_fp32_lds_bytes(int nwells, int nwords, bool lds_bitmap)
-{
-    return (size_t)nwells * sizeof(float4) + (lds_bitmap ? (size_t)nwords * 4 : 0);
-}
+size_t trace_fp32_wells_lds(int nwells) { return (size_t)nwells * sizeof(float4); }
 
-hipError_t launch_trace_fp32(const TraceArgs &a, int grid_blocks, bool lds_bitmap, hipStream_t s)
+hipError_t launch_trace_fp32(const TraceArgs &a, int grid_blocks, int kind, hipStream_t s)
 {
-    const bool use_lds = lds_bitmap || !a.rasterize;
-    const size_t lds = trace_fp32_lds_bytes(a.nwells, a.nwords, use_lds && a.rasterize) +
-                       (a.wmarks ? (size_t)a.w_nr * a.w_nc * 4 : 0);
+    size_t lds = trace_fp32_wells_lds(a.nwells) + (a.wmarks ? (size_t)a.w_nr * a.w_nc * 4 : 0);
     dim3 grid(grid_blocks), block(kTraceBlock);
-    if (use_lds)
-        hipLaunchKernelGGL(k_trace_fp32<true>, grid, block, lds, s, a);
-    else
-        hipLaunchKernelGGL(k_trace_fp32<false>, grid, block, lds, s, a);
+    if (!a.rasterize || kind == kBmLdsFlat) {
+        lds += a.rasterize ? (size_t)a.nwords * 4 : 0;
+        hipLaunchKernelGGL(k_trace_fp32<kBmLdsFlat>, grid, block, lds, s, a);
+    } else if (kind == kBmTiled) {
+        lds += (size_t)tile_lds_words(a.ntiles, a.tile_slots) * 4;
+        hipLaunchKernelGGL(k_trace_fp32<kBmTiled>, grid, block, lds, s, a);
+    } else {
+        hipLaunchKernelGGL(k_trace_fp32<kBmHbmFlat>, grid, block, lds, s, a);
+    }
     return hipGetLastError();
 }
 

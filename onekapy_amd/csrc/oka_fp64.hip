@@ -257,11 +257,13 @@ __device__ __forceinline__ EpochReg find_epoch(const TraceArgs &a, int64_t k)
     return {e.xmin, e.ymin, e.nrows, e.ncols, e.row_off, e.col_off};
 }
 
-// probabilityfield.py:292-310 for one segment, marking bits of the final grid
-// (ncols_final wide).  Already-marked nodes are skipped, as the reference does.
-__device__ __forceinline__ int raster_segment(uint32_t *bm, const EpochReg &e, int ncols_final,
-                                               double dx, double dy, double umbra, double u2,
-                                               double ax, double ay, double bx, double by)
+// probabilityfield.py:292-310 for one segment, marking nodes of the final grid
+// in `bits` (FlatBits or TileBits).  Already-marked nodes are skipped, as the
+// reference does.
+template <typename Bits>
+__device__ __forceinline__ int raster_segment(Bits &bits, const EpochReg &e, double dx, double dy,
+                                               double umbra, double u2, double ax, double ay,
+                                               double bx, double by)
 {
     const double lox = bx < ax ? bx : ax, hix = bx > ax ? bx : ax;
     const double loy = by < ay ? by : ay, hiy = by > ay ? by : ay;
@@ -283,9 +285,8 @@ __device__ __forceinline__ int raster_segment(uint32_t *bm, const EpochReg &e, i
         const double cax = cx - ax;
         const int col = j + e.coff;
         for (int i = i0; i < i1; ++i) {
-            const int idx = (i + e.roff) * ncols_final + col;
-            const uint32_t bit = 1u << (idx & 31);
-            if (bm[idx >> 5] & bit)
+            const int row = i + e.roff;
+            if (bits.test(row, col))
                 continue;
             const double cy = e.ymin + (double)i * dy;
             const double cay = cy - ay;
@@ -301,7 +302,7 @@ __device__ __forceinline__ int raster_segment(uint32_t *bm, const EpochReg &e, i
             else
                 d2 = al;
             if (d2 < u2)
-                atomicOr(&bm[idx >> 5], bit);
+                bits.set(row, col);
         }
     }
     return (j1 - j0) * (i1 - i0);
@@ -321,7 +322,45 @@ __device__ __forceinline__ Integ make_integ(const TraceArgs &a)
 
 
 // Trace kernel modes.
-enum { kModeLdsBitmap = 0, kModeHbmBitmap = 1, kModeVertices = 2 };
+enum { kModeLdsBitmap = 0, kModeHbmBitmap = 1, kModeVertices = 2, kModeTiledBitmap = 3 };
+
+// The chronicle step of one kernel mode: marks a segment, flushes a realization.
+template <int kMode>
+struct Chron;
+
+template <>
+struct Chron<kModeLdsBitmap> {
+    FlatBits bits;
+    __device__ void begin(const TraceArgs &a)
+    {
+        for (int w = threadIdx.x; w < a.nwords; w += blockDim.x)
+            bits.bm[w] = 0u;
+    }
+    __device__ unsigned long long flush(const TraceArgs &a, unsigned int *s_win)
+    {
+        return flush_bitmap(a, bits.bm, s_win);
+    }
+};
+
+template <>
+struct Chron<kModeHbmBitmap> : Chron<kModeLdsBitmap> {};
+
+template <>
+struct Chron<kModeTiledBitmap> {
+    TileBits bits;
+    __device__ void begin(const TraceArgs &) {}
+    __device__ unsigned long long flush(const TraceArgs &a, unsigned int *s_win)
+    {
+        return flush_tiles(a, bits, s_win);
+    }
+};
+
+template <>
+struct Chron<kModeVertices> {
+    FlatBits bits;   // unused
+    __device__ void begin(const TraceArgs &) {}
+    __device__ unsigned long long flush(const TraceArgs &, unsigned int *) { return 0; }
+};
 
 // One workgroup per realization (grid-stride), one trace per lane, lanes
 // refilled from the LDS counter s_next when their trace ends.
@@ -342,13 +381,17 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp64(TraceArgs a)
     const double umbra = a.umbra;
     const double u2 = umbra * umbra;
     const bool raster = !kVerts && a.rasterize;
-    uint32_t *bm = nullptr;
     uint32_t *lds_tail = reinterpret_cast<uint32_t *>(smem + 3 * a.nwells);
-    if (kMode == kModeLdsBitmap) {
-        bm = lds_tail;
+    Chron<kMode> ch;
+    if constexpr (kMode == kModeLdsBitmap) {
+        ch.bits = FlatBits{lds_tail, a.gncols};
         lds_tail += a.rasterize ? a.nwords : 0;
-    } else if (kMode == kModeHbmBitmap) {
-        bm = a.gbitmap + (int64_t)blockIdx.x * a.nwords;
+    } else if constexpr (kMode == kModeHbmBitmap) {
+        ch.bits = FlatBits{a.gbitmap + (int64_t)blockIdx.x * a.nwords, a.gncols};
+    } else if constexpr (kMode == kModeTiledBitmap) {
+        ch.bits = tile_bits(a, lds_tail);
+        tile_init(ch.bits);
+        lds_tail += tile_lds_words(a.ntiles, a.tile_slots);
     }
     unsigned int *s_win = (raster && a.wmarks) ? lds_tail : nullptr;
     if (s_win)
@@ -364,8 +407,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp64(TraceArgs a)
             s_nq[i] = -a.q[(int64_t)r * a.nwells + i];
         }
         if (raster)
-            for (int w = tid; w < a.nwords; w += blockDim.x)
-                bm[w] = 0u;
+            ch.begin(a);
         if (tid == 0)
             s_next = blockDim.x;
         __syncthreads();
@@ -399,9 +441,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp64(TraceArgs a)
                 const int res = dp5_attempt<kConfined>(f, I, s, ox, oy);
                 if (res & 1) {
                     if (raster)
-                        n_cand += (unsigned long long)raster_segment(bm, ep, a.gncols, a.gdx,
-                                                                     a.gdy, umbra, u2, ox, oy,
-                                                                     s.x, s.y);
+                        n_cand += (unsigned long long)raster_segment(ch.bits, ep, a.gdx, a.gdy,
+                                                                     umbra, u2, ox, oy, s.x, s.y);
                     if (kVerts && s.acc < a.vcap) {
                         a.verts[2 * (k * a.vcap + s.acc)] = s.x;
                         a.verts[2 * (k * a.vcap + s.acc) + 1] = s.y;
@@ -444,7 +485,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp64(TraceArgs a)
         __syncthreads();
 
         if (raster)
-            atomicAdd(&s_stat[kStatVisited], flush_bitmap(a, bm, s_win));
+            atomicAdd(&s_stat[kStatVisited], ch.flush(a, s_win));
         __syncthreads();
         if (s_win) {
             store_window(a, r, s_win);
@@ -541,8 +582,9 @@ __global__ void __launch_bounds__(256) k_raster_fp64(RasterArgs a)
     if (i >= a.nseg)
         return;
     EpochReg e{a.xmin, a.ymin, a.nrows, a.ncols, 0, 0};
-    raster_segment(a.bitmap, e, a.ncols, a.dx, a.dy, a.umbra, a.umbra * a.umbra, a.xy[2 * i],
-                   a.xy[2 * i + 1], a.xy[2 * i + 2], a.xy[2 * i + 3]);
+    FlatBits bits{a.bitmap, a.ncols};
+    raster_segment(bits, e, a.dx, a.dy, a.umbra, a.umbra * a.umbra, a.xy[2 * i], a.xy[2 * i + 1],
+                   a.xy[2 * i + 2], a.xy[2 * i + 3]);
 }
 
 template <int kMode>
@@ -558,24 +600,24 @@ hipError_t launch_mode(const TraceArgs &a, int grid_blocks, size_t lds, hipStrea
 
 }  // namespace
 
-size_t trace_fp64_lds_bytes(int nwells, int nwords, bool lds_bitmap)
-{
-    return (size_t)3 * nwells * sizeof(double) + (lds_bitmap ? (size_t)nwords * 4 : 0);
-}
+size_t trace_fp64_wells_lds(int nwells) { return (size_t)3 * nwells * sizeof(double); }
 
-hipError_t launch_trace_fp64(const TraceArgs &a, int grid_blocks, bool lds_bitmap, hipStream_t s)
+hipError_t launch_trace_fp64(const TraceArgs &a, int grid_blocks, int kind, hipStream_t s)
 {
-    const size_t lds = trace_fp64_lds_bytes(a.nwells, a.nwords, lds_bitmap && a.rasterize) +
-                       (a.wmarks ? (size_t)a.w_nr * a.w_nc * 4 : 0);
-    if (lds_bitmap || !a.rasterize)
-        return launch_mode<kModeLdsBitmap>(a, grid_blocks, lds, s);
+    size_t lds = trace_fp64_wells_lds(a.nwells) + (a.wmarks ? (size_t)a.w_nr * a.w_nc * 4 : 0);
+    if (!a.rasterize || kind == kBmLdsFlat)
+        return launch_mode<kModeLdsBitmap>(a, grid_blocks,
+                                           lds + (a.rasterize ? (size_t)a.nwords * 4 : 0), s);
+    if (kind == kBmTiled)
+        return launch_mode<kModeTiledBitmap>(
+            a, grid_blocks, lds + (size_t)tile_lds_words(a.ntiles, a.tile_slots) * 4, s);
     return launch_mode<kModeHbmBitmap>(a, grid_blocks, lds, s);
 }
 
 hipError_t launch_vertices_fp64(const TraceArgs &a, hipStream_t s)
 {
     const int blocks = a.nreal < 1 ? 1 : a.nreal;
-    return launch_mode<kModeVertices>(a, blocks, trace_fp64_lds_bytes(a.nwells, 0, false), s);
+    return launch_mode<kModeVertices>(a, blocks, trace_fp64_wells_lds(a.nwells), s);
 }
 
 hipError_t launch_eval_fp64(const EvalArgs &a, hipStream_t s)

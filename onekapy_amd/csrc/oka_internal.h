@@ -11,6 +11,12 @@ namespace oka {
 
 constexpr int kTraceBlock = 256;            // lanes per realization workgroup
 constexpr int kLdsBitmapMaxWords = 14336;   // 56 KiB: at most 2 realization blocks per CU
+constexpr int kLdsBudget = 57344;           // LDS bytes per trace workgroup (2 per CU)
+
+// Bitmap placement chosen by the launcher (OKA_BITMAP_* minus one).
+enum BitmapKind { kBmLdsFlat = 0, kBmTiled = 1, kBmHbmFlat = 2 };
+constexpr int kTileEmpty = -1, kTileOverflow = -2;
+constexpr int kMinTileSlots = 16;
 
 // Device-side statistics accumulators (unsigned 64-bit, atomically added).
 enum StatSlot {
@@ -45,8 +51,10 @@ struct TraceArgs {
     int32_t nepochs;            // >= 1 (a fixed grid is a single epoch)
     const oka_epoch *epochs;
     int32_t rasterize;
-    int32_t nwords;             // bitmap words per realization
-    uint32_t *gbitmap;          // per-block bitmaps in HBM when the LDS cannot hold one
+    int32_t nwords;             // bitmap words per realization (flat modes)
+    uint32_t *gbitmap;          // per-block bitmaps in HBM: flat, or the overflow tiles
+    int32_t tiles_x, ntiles;    // tiled mode: 32 x 32-node tiles of the final grid
+    int32_t tile_slots;         // tiled mode: LDS slots per workgroup
     int32_t *counts;            // gnrows x gncols
     // optional per-trace outputs
     double *bbox;               // ntraces x 4
@@ -62,11 +70,135 @@ struct TraceArgs {
     uint8_t *wmarks;            // nreal x (w_nr * w_nc)
 };
 
-// Flush of one realization's visited-node bitmap (probabilityfield.py:357-359):
-// one atomic add per set bit into the count grid, and, when a window is
-// requested, the per-cell number of visited fine nodes into LDS counters.
-// Called by every lane of the workgroup; returns this lane's share of the
-// visited count.
+// One marked node of a realization: its count (probabilityfield.py:357-359)
+// and, when a window is requested, its cell's LDS counter.
+__device__ inline void count_node(const TraceArgs &a, int row, int col, unsigned int *s_win)
+{
+    if (a.counts)
+        atomicAdd(&a.counts[(int64_t)row * a.gncols + col], 1);
+    if (s_win) {
+        const int ci = row - a.w_row0, cj = col - a.w_col0;
+        if (ci >= 0 && cj >= 0) {
+            const int r = ci / a.w_cell, c = cj / a.w_cell;
+            if (r < a.w_nr && c < a.w_nc)
+                atomicAdd(&s_win[r * a.w_nc + c], 1u);
+        }
+    }
+}
+
+// The nodes of one tile row: bit b of `bits` is column col0 + b of `row`.
+__device__ inline unsigned long long count_bits(const TraceArgs &a, int row, int col0,
+                                                uint32_t bits, unsigned int *s_win)
+{
+    const unsigned long long n = (unsigned long long)__popc(bits);
+    while (bits) {
+        const int col = col0 + __ffs(bits) - 1;
+        bits &= bits - 1u;
+        count_node(a, row, col, s_win);
+    }
+    return n;
+}
+
+// A realization's visited-node bitmap over the whole final grid (LDS or HBM):
+// node (row, col) is bit idx % 32 of word idx / 32, idx = row * ncols + col.
+struct FlatBits {
+    uint32_t *bm;
+    int ncols;
+    __device__ __forceinline__ bool test(int row, int col) const
+    {
+        const int idx = row * ncols + col;
+        return (bm[idx >> 5] >> (idx & 31)) & 1u;
+    }
+    __device__ __forceinline__ void set(int row, int col)
+    {
+        const int idx = row * ncols + col;
+        atomicOr(&bm[idx >> 5], 1u << (idx & 31));
+    }
+};
+
+// The bitmap in 32 x 32-node tiles (tile t = (row / 32) * tiles_x + col / 32;
+// row r of a tile is one word, column c its bit c).  A tile takes an LDS slot
+// the first time one of its nodes is marked; when the slots run out it lives
+// in the workgroup's HBM region instead.  Unmarked tiles cost nothing.
+struct TileBits {
+    int *tmap;          // LDS [ntiles]: slot index, kTileEmpty or kTileOverflow
+    int *slot_tile;     // LDS [slots]: tile of a claimed slot, or -1
+    uint32_t *slots;    // LDS [slots][32]
+    int *nslots;        // LDS: slots handed out (may exceed `slots`)
+    uint32_t *hbm;      // [ntiles][32] of this workgroup
+    int maxslots, tiles_x, ntiles;
+
+    __device__ __forceinline__ uint32_t *word(int t, int r) const
+    {
+        const int s = __hip_atomic_load(&tmap[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (s == kTileEmpty)
+            return nullptr;
+        return s == kTileOverflow ? hbm + (int64_t)t * 32 + r : slots + s * 32 + r;
+    }
+    __device__ __forceinline__ bool test(int row, int col) const
+    {
+        const uint32_t *w = word((row >> 5) * tiles_x + (col >> 5), row & 31);
+        return w != nullptr && ((*w >> (col & 31)) & 1u);
+    }
+    // The first lane to claim a tile maps it; a lane that loses the race uses
+    // the winner's slot, and its own slot stays unmapped (and zero).
+    __device__ __forceinline__ uint32_t *claim(int t)
+    {
+        const int s = atomicAdd(nslots, 1);
+        const int v = s < maxslots ? s : kTileOverflow;
+        const int prev = atomicCAS(&tmap[t], kTileEmpty, v);
+        const int got = prev == kTileEmpty ? v : prev;
+        if (prev == kTileEmpty && v >= 0)
+            slot_tile[v] = t;
+        return got == kTileOverflow ? hbm + (int64_t)t * 32 : slots + got * 32;
+    }
+    __device__ __forceinline__ void set(int row, int col)
+    {
+        const int t = (row >> 5) * tiles_x + (col >> 5);
+        uint32_t *w = word(t, row & 31);
+        if (w == nullptr)
+            w = claim(t) + (row & 31);
+        atomicOr(w, 1u << (col & 31));
+    }
+};
+
+// LDS words a tiled bitmap needs beside its slots.
+__host__ __device__ inline int tile_lds_words(int ntiles, int slots)
+{
+    return ntiles + slots + 1 + slots * 32;
+}
+
+// Carves a tiled bitmap out of LDS at `lds` (tile_lds_words words).
+__device__ inline TileBits tile_bits(const TraceArgs &a, uint32_t *lds)
+{
+    TileBits tb;
+    tb.tmap = reinterpret_cast<int *>(lds);
+    tb.slot_tile = tb.tmap + a.ntiles;
+    tb.nslots = tb.slot_tile + a.tile_slots;
+    tb.slots = reinterpret_cast<uint32_t *>(tb.nslots + 1);
+    tb.hbm = a.gbitmap + (int64_t)blockIdx.x * a.ntiles * 32;
+    tb.maxslots = a.tile_slots;
+    tb.tiles_x = a.tiles_x;
+    tb.ntiles = a.ntiles;
+    return tb;
+}
+
+// Empty tiled bitmap (the HBM region must already be zero).  Every lane calls
+// it; a barrier must follow before first use.
+__device__ inline void tile_init(TileBits &tb)
+{
+    for (int t = threadIdx.x; t < tb.ntiles; t += blockDim.x)
+        tb.tmap[t] = kTileEmpty;
+    for (int s = threadIdx.x; s < tb.maxslots; s += blockDim.x)
+        tb.slot_tile[s] = -1;
+    for (int w = threadIdx.x; w < tb.maxslots * 32; w += blockDim.x)
+        tb.slots[w] = 0u;
+    if (threadIdx.x == 0)
+        *tb.nslots = 0;
+}
+
+// Flush of one realization's flat bitmap; every lane calls it, after a
+// barrier.  Returns this lane's share of the visited count.
 __device__ inline unsigned long long flush_bitmap(const TraceArgs &a, const uint32_t *bm,
                                                   unsigned int *s_win)
 {
@@ -75,22 +207,52 @@ __device__ inline unsigned long long flush_bitmap(const TraceArgs &a, const uint
         uint32_t bits = bm[w];
         vis += (unsigned long long)__popc(bits);
         while (bits) {
-            const int bit = __ffs(bits) - 1;
+            const int idx = w * 32 + __ffs(bits) - 1;
             bits &= bits - 1u;
-            const int idx = w * 32 + bit;
-            if (a.counts)
-                atomicAdd(&a.counts[idx], 1);
-            if (s_win) {
-                const int i = idx / a.gncols, j = idx - i * a.gncols;
-                const int ci = i - a.w_row0, cj = j - a.w_col0;
-                if (ci >= 0 && cj >= 0) {
-                    const int r = ci / a.w_cell, c = cj / a.w_cell;
-                    if (r < a.w_nr && c < a.w_nc)
-                        atomicAdd(&s_win[r * a.w_nc + c], 1u);
-                }
-            }
+            const int row = idx / a.gncols;
+            count_node(a, row, idx - row * a.gncols, s_win);
         }
     }
+    return vis;
+}
+
+// Flush of one realization's tiled bitmap, leaving it empty for the next one.
+// Every lane calls it, after a barrier; it ends with a barrier.
+__device__ inline unsigned long long flush_tiles(const TraceArgs &a, TileBits &tb,
+                                                 unsigned int *s_win)
+{
+    unsigned long long vis = 0;
+    const int handed = *tb.nslots;
+    const int ns = handed < tb.maxslots ? handed : tb.maxslots;
+    for (int w = threadIdx.x; w < ns * 32; w += blockDim.x) {
+        const int t = tb.slot_tile[w >> 5];
+        const uint32_t bits = tb.slots[w];
+        if (t < 0 || !bits)
+            continue;
+        tb.slots[w] = 0u;
+        vis += count_bits(a, (t / tb.tiles_x) * 32 + (w & 31), (t % tb.tiles_x) * 32, bits, s_win);
+    }
+    if (handed > tb.maxslots) {   // some tiles went to HBM
+        for (int w = threadIdx.x; w < tb.ntiles * 32; w += blockDim.x) {
+            const int t = w >> 5;
+            if (tb.tmap[t] != kTileOverflow)
+                continue;
+            const uint32_t bits = tb.hbm[w];
+            if (!bits)
+                continue;
+            tb.hbm[w] = 0u;
+            vis += count_bits(a, (t / tb.tiles_x) * 32 + (w & 31), (t % tb.tiles_x) * 32, bits,
+                              s_win);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < tb.ntiles; t += blockDim.x)
+        tb.tmap[t] = kTileEmpty;
+    for (int s = threadIdx.x; s < ns; s += blockDim.x)
+        tb.slot_tile[s] = -1;
+    if (threadIdx.x == 0)
+        *tb.nslots = 0;
+    __syncthreads();
     return vis;
 }
 
@@ -162,17 +324,17 @@ hipError_t launch_mark_moments(const uint8_t *marks, int64_t n, int32_t ncell, u
 uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, int word);
 
 // oka_fp64.hip
-hipError_t launch_trace_fp64(const TraceArgs &a, int grid_blocks, bool lds_bitmap,
+hipError_t launch_trace_fp64(const TraceArgs &a, int grid_blocks, int kind,
                              hipStream_t s);
 hipError_t launch_vertices_fp64(const TraceArgs &a, hipStream_t s);
 hipError_t launch_eval_fp64(const EvalArgs &a, hipStream_t s);
 hipError_t launch_raster_fp64(const RasterArgs &a, hipStream_t s);
-size_t trace_fp64_lds_bytes(int nwells, int nwords, bool lds_bitmap);
+size_t trace_fp64_wells_lds(int nwells);
 
 // oka_fp32.hip
-hipError_t launch_trace_fp32(const TraceArgs &a, int grid_blocks, bool lds_bitmap,
+hipError_t launch_trace_fp32(const TraceArgs &a, int grid_blocks, int kind,
                              hipStream_t s);
-size_t trace_fp32_lds_bytes(int nwells, int nwords, bool lds_bitmap);
+size_t trace_fp32_wells_lds(int nwells);
 
 // oka_fit.hip
 hipError_t launch_fit_logs(const FitArgs &a, double *logr2, hipStream_t s);

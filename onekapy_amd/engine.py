@@ -66,12 +66,15 @@ def _launch(ctx, site, real, opts, grid, epochs, rasterize, counts, bbox, status
 
 
 def chronicle_into_field(pfield, wells_xy, xo, yo, real, starts, duration, tol, maxstep, umbra,
-                         confined=True, precision=_lib.FP64_EXACT, max_steps=0):
+                         confined=True, precision=_lib.FP64_EXACT, max_steps=0,
+                         bitmap_mode=_lib.BITMAP_AUTO, tile_slots=0):
     """Trace all realizations and rasterize them into ``pfield``'s lattice.
 
     Grows ``pfield`` exactly as the reference's per-trace ``rasterize`` calls
     would, and returns the per-node number of realizations that marked each
     node (the caller registers them).  Nothing is registered here.
+    bitmap_mode / tile_slots place the per-realization visited-node bitmap
+    (oka_trace_opts); every placement gives the same counts.
     """
     wells_xy = _lib.f64(wells_xy).reshape(-1, 2)
     starts = _lib.f64(starts).reshape(-1, 2)
@@ -80,7 +83,7 @@ def chronicle_into_field(pfield, wells_xy, xo, yo, real, starts, duration, tol, 
     ctx = _lib.context()
     site = _lib.make_site(wells_xy, xo, yo)
     opts = _lib.make_trace_opts(P, duration, tol, maxstep, umbra, _lib.ptr(starts), precision,
-                                confined, max_steps)
+                                confined, max_steps, bitmap_mode, tile_slots)
     bbox = np.zeros((ntr, 4))
     status = np.zeros(ntr, dtype=np.int32)
     stats = _lib.TraceStats_()

@@ -60,20 +60,64 @@ def test_empty_batches_are_noops(gpu_ctx):
     assert res.counts.sum() == 0
 
 
-def test_hbm_bitmap_path_large_grid(gpu_ctx):
-    """A 1200 x 1100 lattice (41250 bitmap words) exceeds the LDS budget: the
-    per-block bitmaps live in HBM; results are unchanged."""
+BITMAP_CASES = [
+    ("auto", _lib.BITMAP_AUTO, 0),           # 1200 x 1100 nodes: tiled (41250 flat words)
+    ("tiled_overflow", _lib.BITMAP_TILED, 3),   # 3 LDS slots: most tiles live in HBM
+    ("tiled_one_slot", _lib.BITMAP_TILED, 1),
+    ("hbm_flat", _lib.BITMAP_HBM_FLAT, 0),
+]
+
+
+@pytest.mark.parametrize("name,mode,slots", BITMAP_CASES, ids=[c[0] for c in BITMAP_CASES])
+@pytest.mark.parametrize("precision", [_lib.FP64_EXACT, _lib.FP32], ids=["fp64", "fp32"])
+def test_large_grid_bitmap_placements(gpu_ctx, name, mode, slots, precision):
+    """A 1200 x 1100 lattice (41250 bitmap words) exceeds the flat LDS budget.
+    Tiled bitmaps (with or without overflow into HBM) and flat HBM bitmaps
+    give the oracle's counts bit for bit (FP64) or the FP64 path's counts
+    within the FP32 tolerance."""
+    print(f"case {name} precision {precision}", flush=True)
     s, d = small_site(nreal=5, npaths=40)
     _, _, coef = oracle_fit_draw(s, d)
     pf = fixed_field(s.xt, s.yt, 5.0, 1100, 1200)
     real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
-    res = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real,
-                               start_points(s.xt, s.yt, s.rt, s.npaths), s.duration, s.tol,
-                               s.maxstep, 10.0)
-    s.umbra = 10.0
-    ocounts, ost = oracle_chronicle(s, d, coef, pf)
-    assert ost[6] == 0 and res.passes == 1
-    assert np.array_equal(res.counts, ocounts)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    res = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration, s.tol,
+                               s.maxstep, 10.0, precision=precision, bitmap_mode=mode,
+                               tile_slots=slots)
+    assert res.passes == 1
+    assert res.counts.sum() == res.stats.visited
+    if precision == _lib.FP64_EXACT:
+        s.umbra = 10.0
+        ocounts, ost = oracle_chronicle(s, d, coef, pf)
+        assert ost[6] == 0
+        assert np.array_equal(res.counts, ocounts)
+    else:
+        ref = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
+                                   s.tol, s.maxstep, 10.0, precision=precision,
+                                   bitmap_mode=_lib.BITMAP_HBM_FLAT)
+        assert np.array_equal(res.counts, ref.counts)
+
+
+def test_small_grid_tiled_matches_flat(gpu_ctx):
+    """The same realizations on the config-1 lattice (flat LDS bitmap by
+    default) through the tiled bitmap, with and without overflow into HBM:
+    identical counts."""
+    s = sites.config1(6)
+    d = sites.draw(s, seed=5)
+    _, _, coef = oracle_fit_draw(s, d)
+    pf = s.fixed_field()
+    real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    out = {}
+    for mode, slots in ((_lib.BITMAP_LDS_FLAT, 0), (_lib.BITMAP_TILED, 0), (_lib.BITMAP_TILED, 2)):
+        print(f"mode {mode} slots {slots}", flush=True)
+        out[(mode, slots)] = chronicle_into_field(
+            pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration, s.tol, s.maxstep, s.umbra,
+            precision=_lib.FP32, bitmap_mode=mode, tile_slots=slots).counts
+    base = out[(_lib.BITMAP_LDS_FLAT, 0)]
+    assert base.sum() > 0
+    for k, v in out.items():
+        assert np.array_equal(v, base), k
 
 
 def test_config1_subset_bit_exact_and_invariants(gpu_ctx):
