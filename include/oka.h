@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define OKA_ABI_VERSION 2
+#define OKA_ABI_VERSION 3
 
 enum {
     OKA_OK = 0,
@@ -127,13 +127,24 @@ typedef struct {
     int32_t npaths;          /* particles per realization (capturezone.py:113)      */
     int32_t precision;       /* OKA_FP64_EXACT or OKA_FP32                           */
     int32_t confined;        /* 1: compute_velocity_confined, 0: compute_velocity    */
-    int32_t reserved;
+    int32_t schedule;        /* OKA_SCHED_*                                          */
     double duration, tol, maxstep, umbra;
     int64_t max_steps;       /* attempted-step cap per trace; <= 0: 1,000,000        */
     const double *start_xy;  /* npaths x 2 start points (capturezone.py:114-115)     */
     int32_t bitmap_mode;     /* visited-node bitmap of a realization: OKA_BITMAP_*   */
     int32_t tile_slots;      /* OKA_BITMAP_TILED: LDS tile slots (<= 0: as many as fit) */
+    int32_t pool_slots;      /* OKA_SCHED_POOL: realizations in flight per wave (<= 0: 4) */
+    int32_t pool_waves_per_cu; /* OKA_SCHED_POOL: resident waves per CU (<= 0: 8)     */
 } oka_trace_opts;
+
+/* How particles are scheduled on the GPU.  WORKGROUP: one 256-lane workgroup
+ * per realization, its lanes refilled with the realization's particles.
+ * POOL (FP32, confined, no window marks): a persistent grid of single waves,
+ * each keeping pool_slots realizations in flight and refilling a lane with a
+ * particle of any of them, realizations taken from a queue shared by all
+ * waves -- for particle lifetimes that differ widely within a realization.
+ * AUTO is WORKGROUP.  Both give the same counts. */
+enum { OKA_SCHED_AUTO = 0, OKA_SCHED_WORKGROUP = 1, OKA_SCHED_POOL = 2 };
 
 /* Where a realization's visited-node bitmap lives.  AUTO picks LDS_FLAT when
  * the whole bitmap fits in LDS, else TILED, else HBM_FLAT.  TILED keeps

@@ -72,10 +72,10 @@ class Site_(C.Structure):
 
 
 class TraceOpts_(C.Structure):
-    _fields_ = [("npaths", _i32), ("precision", _i32), ("confined", _i32), ("reserved", _i32),
+    _fields_ = [("npaths", _i32), ("precision", _i32), ("confined", _i32), ("schedule", _i32),
                 ("duration", _d), ("tol", _d), ("maxstep", _d), ("umbra", _d),
                 ("max_steps", _i64), ("start_xy", _vp), ("bitmap_mode", _i32),
-                ("tile_slots", _i32)]
+                ("tile_slots", _i32), ("pool_slots", _i32), ("pool_waves_per_cu", _i32)]
 
 
 class Grid_(C.Structure):
@@ -269,10 +269,12 @@ def make_site(wells_xy, xo, yo, base=0.0, obs=None, wells_ptr=None, obs_ptr=None
 
 
 BITMAP_AUTO, BITMAP_LDS_FLAT, BITMAP_TILED, BITMAP_HBM_FLAT = 0, 1, 2, 3
+SCHED_AUTO, SCHED_WORKGROUP, SCHED_POOL = 0, 1, 2
 
 
 def make_trace_opts(npaths, duration, tol, maxstep, umbra, start_xy_ptr, precision=FP64_EXACT,
-                    confined=True, max_steps=0, bitmap_mode=BITMAP_AUTO, tile_slots=0):
+                    confined=True, max_steps=0, bitmap_mode=BITMAP_AUTO, tile_slots=0,
+                    schedule=SCHED_AUTO, pool_slots=0, pool_waves_per_cu=0):
     o = TraceOpts_()
     o.npaths = int(npaths)
     o.precision = int(precision)
@@ -282,6 +284,9 @@ def make_trace_opts(npaths, duration, tol, maxstep, umbra, start_xy_ptr, precisi
     o.start_xy = start_xy_ptr
     o.bitmap_mode = int(bitmap_mode)
     o.tile_slots = int(tile_slots)
+    o.schedule = int(schedule)
+    o.pool_slots = int(pool_slots)
+    o.pool_waves_per_cu = int(pool_waves_per_cu)
     return o
 
 

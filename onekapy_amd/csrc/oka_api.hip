@@ -141,6 +141,10 @@ int check_opts(oka_ctx *c, const oka_trace_opts *o)
         return fail(c, OKA_ERR_ARG, "unknown precision %d", o->precision);
     if (o->precision == OKA_FP32 && !o->confined)
         return fail(c, OKA_ERR_ARG, "FP32 tracing supports the confined velocity only");
+    if (o->schedule < OKA_SCHED_AUTO || o->schedule > OKA_SCHED_POOL)
+        return fail(c, OKA_ERR_ARG, "unknown schedule %d", o->schedule);
+    if (o->schedule == OKA_SCHED_POOL && o->precision != OKA_FP32)
+        return fail(c, OKA_ERR_ARG, "the pool schedule traces in FP32");
     return OKA_OK;
 }
 
@@ -466,7 +470,74 @@ static int trace_common(oka_ctx *c, const oka_site *site, int32_t nreal, const d
     return OKA_OK;
 }
 
-static // Where a realization's visited-node bitmap lives (OKA_BITMAP_*, see
+static int chronicle_finish(oka_ctx *c, const TraceArgs &a, int32_t rasterize, int64_t nnodes,
+                     int64_t ntr, int32_t *counts, double *bbox, int32_t *trace_status,
+                     int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags,
+                     const oka_window *win, uint8_t *marks);
+
+// Launches the persistent pool tracer over `nsites` sites (host copies of
+// their argument blocks, with device pointers inside); times it with the
+// context events when `timed`.
+int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opts *o, bool timed)
+{
+    std::vector<int64_t> first((size_t)nsites + 1);
+    int64_t total = 0;
+    int max_nw = 1, max_tiles = 1;
+    for (int i = 0; i < nsites; ++i) {
+        first[i] = total;
+        total += sites[i].nreal;
+        max_nw = std::max(max_nw, (int)sites[i].nwells);
+        if (sites[i].rasterize)
+            max_tiles = std::max(max_tiles, (int)sites[i].ntiles);
+        if (!sites[i].confined)
+            return fail(c, OKA_ERR_ARG, "the pool schedule traces the confined velocity only");
+    }
+    first[nsites] = total;
+    if (max_tiles > pool_max_tiles())
+        return fail(c, OKA_ERR_ARG, "grid too large for the pool schedule (%d tiles)", max_tiles);
+    const int slots = o->pool_slots > 0 ? std::min(o->pool_slots, 16) : 4;
+    const int wpc = o->pool_waves_per_cu > 0 ? std::min(o->pool_waves_per_cu, 32) : 8;
+    const size_t budget = std::min(c->lds_per_cu / wpc, c->max_lds);
+    const size_t fixed = pool_lds_bytes(slots, max_nw, 0);
+    if (fixed > budget)
+        return fail(c, OKA_ERR_ARG, "%d wells x %d slots do not fit in LDS at %d waves per CU",
+                    max_nw, slots, wpc);
+    const int pool_tiles =
+        (int)std::min<size_t>((budget - fixed) / ((size_t)slots * 128), pool_max_pool_tiles());
+    const int waves = c->num_cu * wpc;
+    void *p_sites, *p_first, *p_queue, *p_hbm;
+    TRY(ensure(c, kSlotPoolSites, sizeof(TraceArgs) * nsites, &p_sites));
+    TRY(ensure(c, kSlotPoolFirst, sizeof(int64_t) * first.size(), &p_first));
+    TRY(ensure(c, kSlotPoolQueue, sizeof(unsigned long long), &p_queue));
+    const int64_t hbm_words = (int64_t)waves * slots * max_tiles * 32;
+    TRY(ensure(c, kSlotPoolHbm, sizeof(uint32_t) * hbm_words, &p_hbm));
+    HIPCHK(c, hipMemcpyAsync(p_sites, sites, sizeof(TraceArgs) * nsites, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(p_first, first.data(), sizeof(int64_t) * first.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    TRY(fill_u32(c, p_queue, 2));
+    TRY(fill_u32(c, p_hbm, hbm_words));
+    PoolArgs p{};
+    p.sites = (const TraceArgs *)p_sites;
+    p.nsites = nsites;
+    p.site_first = (const int64_t *)p_first;
+    p.total = total;
+    p.queue = (unsigned long long *)p_queue;
+    p.hbm = (uint32_t *)p_hbm;
+    p.slots = slots;
+    p.max_nwells = max_nw;
+    p.pool_tiles = pool_tiles;
+    p.max_ntiles = max_tiles;
+    if (timed)
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (total > 0)
+        HIPCHK(c, launch_trace_pool32(p, waves, c->stream));
+    if (timed)
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    return OKA_OK;
+}
+
+// Where a realization's visited-node bitmap lives (OKA_BITMAP_*, see
 // include/oka.h): a flat LDS bitmap when the whole grid fits in the LDS
 // budget beside the wells and window counters, else 32 x 32-node tiles in LDS
 // slots, else a flat bitmap in HBM.
@@ -577,8 +648,19 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
         a.wmarks = (uint8_t *)o2;
     }
 
-    // launch shape: one realization per 256-lane workgroup, grid-stride
+    const bool timed = stats != nullptr;
     const bool fp64 = o->precision == OKA_FP64_EXACT;
+    if (o->schedule == OKA_SCHED_POOL) {
+        if (marks)
+            return fail(c, OKA_ERR_ARG, "window marks need the workgroup schedule");
+        a.tiles_x = (a.gncols + 31) / 32;
+        a.ntiles = a.tiles_x * ((a.gnrows + 31) / 32);
+        TRY(run_pool(c, &a, 1, o, timed));
+        return chronicle_finish(c, a, rasterize, nnodes, ntr, counts, bbox, trace_status,
+                                trace_steps, stats, flags, win, marks);
+    }
+
+    // launch shape: one realization per 256-lane workgroup, grid-stride
     const size_t base_lds =
         (fp64 ? trace_fp64_wells_lds(a.nwells) : trace_fp32_wells_lds(a.nwells)) + win_lds;
     if (base_lds > c->max_lds)
@@ -602,7 +684,6 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
                 TRY(fill_u32(c, a.gbitmap, (int64_t)words * blocks));
         }
     }
-    const bool timed = stats != nullptr;
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (fp64)
@@ -611,6 +692,17 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
         HIPCHK(c, launch_trace_fp32(a, blocks, kind, c->stream));
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    return chronicle_finish(c, a, rasterize, nnodes, ntr, counts, bbox, trace_status, trace_steps,
+                            stats, flags, win, marks);
+}
+
+// Copies a trace launch's outputs back (host buffers) and reads its statistics.
+int chronicle_finish(oka_ctx *c, const TraceArgs &a, int32_t rasterize, int64_t nnodes,
+                     int64_t ntr, int32_t *counts, double *bbox, int32_t *trace_status,
+                     int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags,
+                     const oka_window *win, uint8_t *marks)
+{
+    const int32_t nreal = a.nreal;
     TRY(copy_back(c, rasterize ? counts : nullptr, a.counts, sizeof(int32_t) * nnodes, flags));
     TRY(copy_back(c, bbox, a.bbox, sizeof(double) * 4 * ntr, flags));
     TRY(copy_back(c, trace_status, a.trace_status, sizeof(int32_t) * ntr, flags));

@@ -1,236 +1,19 @@
-// oka_fp32.hip -- FP32 particle tracing for gfx950 (confined velocity).
-//
-// The same Dormand-Prince 5(4) step control as the reference
-// (oneka/capturezone.py:198-253), the same velocity field (model.py:299-315,
-// 423-427) and the same segment rasterization (probabilityfield.py:292-310,
-// 407-427), computed in FP32:
-//   - coordinates are relative to the regional origin (the target well), so a
-//     float resolves millimetres anywhere on a 2000 x 2000 node grid;
-//   - particle positions are Kahan-compensated sums of their increments;
-//   - per realization, -1/(b n) and 1/(2 pi) are folded into the regional
-//     coefficients and the well strengths: k = -V = sum_w c_w d/r^2 + k_reg;
-//   - time t and step dt stay in FP64 (a handful of operations per step) so the
-//     termination rule |t| < |duration| behaves as in the reference.
-// Results are not bit-identical to the FP64 path; the difference in capture
-// probability is reported against the FP64-exact mode.
+// oka_fp32.hip -- FP32 particle tracing for gfx950 (confined velocity): one
+// 256-lane workgroup per realization, grid-stride over realizations.  The step,
+// field and rasterizer are in oka_fp32_dev.h.  Results are not bit-identical
+// to the FP64 path; the difference in capture probability is reported against
+// the FP64-exact mode.
 #include <math.h>
 
 #include <type_traits>
 
+#include "oka_fp32_dev.h"
 #include "oka_internal.h"
 
 namespace oka {
 namespace {
 
-constexpr float A21 = 1.0f / 5.0f;
-constexpr float A31 = 3.0f / 40.0f, A32 = 9.0f / 40.0f;
-constexpr float A41 = 44.0f / 45.0f, A42 = -56.0f / 15.0f, A43 = 32.0f / 9.0f;
-constexpr float A51 = 19372.0f / 6561.0f, A52 = -25360.0f / 2187.0f, A53 = 64448.0f / 6561.0f,
-                A54 = -212.0f / 729.0f;
-constexpr float A61 = 9017.0f / 3168.0f, A62 = -355.0f / 33.0f, A63 = 46732.0f / 5247.0f,
-                A64 = 49.0f / 176.0f, A65 = -5103.0f / 18656.0f;
-constexpr float B1 = 35.0f / 384.0f, B3 = 500.0f / 1113.0f, B4 = 125.0f / 192.0f,
-                B5 = -2187.0f / 6784.0f, B6 = 11.0f / 84.0f;
-constexpr float E1 = 71.0f / 57600.0f, E2 = -1.0f / 40.0f, E3 = -71.0f / 16695.0f,
-                E4 = 71.0f / 1920.0f, E5 = -17253.0f / 339200.0f, E6 = 22.0f / 525.0f;
-constexpr double kInv2Pi = 0.15915494309189535;
-
-struct Field32 {
-    const float4 *w;   // (xw - xo, yw - yo, q / (2 pi b n), 0)
-    int nw;
-    float a2, b2, c, d, e;  // regional: k_reg = ((a2 dx + c dy) + d, (b2 dy + c dx) + e)
-};
-
-__device__ __forceinline__ void feval32(const Field32 &f, float x, float y, float &kx, float &ky)
-{
-    float vx = fmaf(f.a2, x, fmaf(f.c, y, f.d));
-    float vy = fmaf(f.b2, y, fmaf(f.c, x, f.e));
-    for (int i = 0; i < f.nw; ++i) {
-        const float4 w = f.w[i];
-        const float ddx = x - w.x;
-        const float ddy = y - w.y;
-        const float r2 = fmaf(ddx, ddx, ddy * ddy);
-        const float s = w.z * __builtin_amdgcn_rcpf(r2);
-        vx = fmaf(s, ddx, vx);
-        vy = fmaf(s, ddy, vy);
-    }
-    kx = vx;
-    ky = vy;
-}
-
-struct Trace32 {
-    float x, y, cx, cy;   // Kahan sums (x, y) and their compensations (cx, cy)
-    float k1x, k1y;
-    float bx0, bx1, by0, by1;
-    double t, dt;
-    int att, acc;
-    int status;
-};
-
-__device__ __forceinline__ void kahan_add(float &s, float &c, float v)
-{
-    const float y = v - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}
-
-struct Integ32 {
-    double dur, absdur, sg;
-    float tol, maxstep;
-    int64_t cap;
-};
-
-__device__ __forceinline__ bool start32(const Field32 &f, const Integ32 &I, Trace32 &s, float xs,
-                                        float ys)
-{
-    s.x = xs;
-    s.y = ys;
-    s.cx = 0.f;
-    s.cy = 0.f;
-    s.t = 0.0;
-    s.dt = 0.1 * I.sg;
-    s.att = 0;
-    s.acc = 0;
-    s.bx0 = s.bx1 = xs;
-    s.by0 = s.by1 = ys;
-    s.status = OKA_TRACE_OK;
-    feval32(f, xs, ys, s.k1x, s.k1y);
-    return 0.0 < I.absdur;
-}
-
-__device__ __forceinline__ int attempt32(const Field32 &f, const Integ32 &I, Trace32 &s, float &ox,
-                                         float &oy)
-{
-    double dtd = s.dt;
-    if (fabs(s.t + dtd) > I.absdur)
-        dtd = I.dur - s.t;
-    s.att += 1;
-    const float dt = (float)dtd;
-    const float x = s.x, y = s.y, k1x = s.k1x, k1y = s.k1y;
-    float k2x, k2y, k3x, k3y, k4x, k4y, k5x, k5y, k6x, k6y, k7x, k7y;
-    feval32(f, x + dt * (A21 * k1x), y + dt * (A21 * k1y), k2x, k2y);
-    feval32(f, x + dt * (A31 * k1x + A32 * k2x), y + dt * (A31 * k1y + A32 * k2y), k3x, k3y);
-    feval32(f, x + dt * (A41 * k1x + A42 * k2x + A43 * k3x),
-            y + dt * (A41 * k1y + A42 * k2y + A43 * k3y), k4x, k4y);
-    feval32(f, x + dt * (A51 * k1x + A52 * k2x + A53 * k3x + A54 * k4x),
-            y + dt * (A51 * k1y + A52 * k2y + A53 * k3y + A54 * k4y), k5x, k5y);
-    feval32(f, x + dt * (A61 * k1x + A62 * k2x + A63 * k3x + A64 * k4x + A65 * k5x),
-            y + dt * (A61 * k1y + A62 * k2y + A63 * k3y + A64 * k4y + A65 * k5y), k6x, k6y);
-    const float dxs = dt * (B1 * k1x + B3 * k3x + B4 * k4x + B5 * k5x + B6 * k6x);
-    const float dys = dt * (B1 * k1y + B3 * k3y + B4 * k4y + B5 * k5y + B6 * k6y);
-    const float xn = x + dxs, yn = y + dys;
-    feval32(f, xn, yn, k7x, k7y);
-    const float ex = dt * (E1 * k1x + E2 * k7x + E3 * k3x + E4 * k4x + E5 * k5x + E6 * k6x);
-    const float ey = dt * (E1 * k1y + E2 * k7y + E3 * k3y + E4 * k4y + E5 * k5y + E6 * k6y);
-    const float est = fmaxf(fabsf(ex), fabsf(ey));
-    const float ds = sqrtf(fmaf(dxs, dxs, dys * dys));
-    int res = 0;
-    const bool finite = !(isnan(ex) || isnan(ey));
-    if (finite && est < I.tol && ds < I.maxstep) {
-        s.t = s.t + dtd;
-        s.k1x = k7x;
-        s.k1y = k7y;
-        ox = x;
-        oy = y;
-        kahan_add(s.x, s.cx, dxs);
-        kahan_add(s.y, s.cy, dys);
-        s.acc += 1;
-        s.bx0 = fminf(s.bx0, s.x);
-        s.bx1 = fmaxf(s.bx1, s.x);
-        s.by0 = fminf(s.by0, s.y);
-        s.by1 = fmaxf(s.by1, s.y);
-        res = 1;
-    }
-    float cf = __expf(0.2f * __logf(I.tol / (est + 2.220446049250313e-16f)));
-    const float c2 = I.maxstep / (ds + 2.220446049250313e-16f);
-    if (c2 < cf)
-        cf = c2;
-    if (10.0f < cf)
-        cf = 10.0f;
-    const double ndt = (double)(0.9f * cf) * dtd;
-    s.dt = ndt;
-    if (fabs(s.t) < I.absdur) {
-        if (!finite || isnan(ndt) || ndt == 0.0) {
-            s.status = OKA_TRACE_STALLED;
-            res |= 2;
-        } else if (I.cap > 0 && s.att >= I.cap) {
-            s.status = OKA_TRACE_STEP_CAP;
-            res |= 2;
-        }
-    } else {
-        res |= 2;
-    }
-    return res;
-}
-
-struct Epoch32 {
-    float xmin, ymin;   // relative to the regional origin
-    int nrows, ncols, roff, coff;
-};
-
-__device__ __forceinline__ Epoch32 find_epoch32(const TraceArgs &a, int64_t k)
-{
-    int lo = 0, hi = a.nepochs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.epochs[mid].first_trace <= k)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const oka_epoch &e = a.epochs[lo];
-    return {(float)(e.xmin - a.xo), (float)(e.ymin - a.yo), e.nrows, e.ncols, e.row_off, e.col_off};
-}
-
-template <typename Bits>
-__device__ __forceinline__ int raster32(Bits &bits, const Epoch32 &e, float dx, float dy,
-                                         float idx_, float idy, float umbra, float u2, float ax,
-                                         float ay, float bx, float by)
-{
-    const float lox = fminf(ax, bx), hix = fmaxf(ax, bx);
-    const float loy = fminf(ay, by), hiy = fmaxf(ay, by);
-    float jl = floorf((lox - umbra - e.xmin) * idx_);
-    float jr = floorf((hix + umbra - e.xmin) * idx_) + 1.0f;
-    float il = floorf((loy - umbra - e.ymin) * idy);
-    float ir = floorf((hiy + umbra - e.ymin) * idy) + 1.0f;
-    jl = fmaxf(jl, 0.0f);
-    il = fmaxf(il, 0.0f);
-    jr = fminf(jr, (float)e.ncols);
-    ir = fminf(ir, (float)e.nrows);
-    if (!(jl < jr) || !(il < ir))
-        return 0;
-    const int j0 = (int)jl, j1 = (int)jr, i0 = (int)il, i1 = (int)ir;
-    const float bax = bx - ax, bay = by - ay;
-    const float l2 = fmaf(bax, bax, bay * bay);
-    if (!(l2 > 0.0f))
-        return 0;
-    const float il2 = 1.0f / l2;
-    for (int j = j0; j < j1; ++j) {
-        const float cax = fmaf((float)j, dx, e.xmin) - ax;
-        const int col = j + e.coff;
-        for (int i = i0; i < i1; ++i) {
-            const int row = i + e.roff;
-            if (bits.test(row, col))
-                continue;
-            const float cay = fmaf((float)i, dy, e.ymin) - ay;
-            const float perp = bax * cay - bay * cax;
-            const float dot = fmaf(bax, cax, bay * cay);
-            const float al = perp * perp * il2;
-            const float be = dot * dot * il2;
-            float d2;
-            if (dot < 0.0f)
-                d2 = al + be;
-            else if (be > l2)
-                d2 = (al + be) - 2.0f * dot + l2;
-            else
-                d2 = al;
-            if (d2 < u2)
-                bits.set(row, col);
-        }
-    }
-    return (j1 - j0) * (i1 - i0);
-}
+using namespace fp32;
 
 template <int kKind>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)

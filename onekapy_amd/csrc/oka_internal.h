@@ -323,6 +323,22 @@ hipError_t launch_mark_moments(const uint8_t *marks, int64_t n, int32_t ncell, u
                                uint64_t *sums_sq, hipStream_t s);
 uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, int word);
 
+// Persistent FP32 tracer over a table of sites (oka_pool.hip).
+struct PoolArgs {
+    const TraceArgs *sites;     // device [nsites]
+    int32_t nsites;
+    const int64_t *site_first;  // device [nsites + 1]: first queue item of each site
+    int64_t total;              // realizations over all sites
+    unsigned long long *queue;  // next queue item; zero before the launch
+    uint32_t *hbm;              // [waves][slots][max_ntiles][32]; zero before the launch
+    int32_t slots;              // realizations in flight per wave
+    int32_t max_nwells, pool_tiles, max_ntiles;
+};
+size_t pool_lds_bytes(int slots, int max_nwells, int pool_tiles);
+int pool_max_tiles();
+int pool_max_pool_tiles();
+hipError_t launch_trace_pool32(const PoolArgs &p, int waves, hipStream_t s);
+
 // oka_fp64.hip
 hipError_t launch_trace_fp64(const TraceArgs &a, int grid_blocks, int kind,
                              hipStream_t s);

@@ -1,0 +1,440 @@
+// oka_pool.hip -- persistent FP32 tracer with work stealing over live particles.
+//
+// One wave (64 lanes) per workgroup, a persistent grid.  Each wave keeps up to
+// `slots` realizations in flight: when a lane's trace ends it takes the next
+// particle of the newest open realization, and when that realization has no
+// particles left the wave opens another one from a global queue shared by all
+// waves (possibly of another site).  A realization is flushed as soon as its
+// last particle ends, so a few long traces (particles circling an injection
+// well, a 50-year backtrace beside a 1-year one) no longer idle the lanes that
+// finished their own work.
+//
+// Per realization slot, the visited-node bitmap is kept as 32 x 32-node tiles:
+// a small LDS hash maps a tile to one of the slot's LDS pool tiles, or to the
+// slot's HBM region when the pool is full; tiles that find no hash entry live
+// in the HBM region unmapped, and the flush then scans that region.  All
+// coordination is within one wave; barriers separate the phases of an
+// iteration so that LDS writes by one lane are seen by the others.
+//
+// The step, field and rasterizer are those of the workgroup tracer
+// (oka_fp32_dev.h: oneka/capturezone.py:198-253, model.py:299-315, 423-427,
+// probabilityfield.py:292-310, 407-427).
+#include <math.h>
+
+#include "oka_fp32_dev.h"
+#include "oka_internal.h"
+
+namespace oka {
+namespace {
+
+using namespace fp32;
+
+constexpr int kWave = 64;
+constexpr int kHashBits = 7;
+constexpr int kHash = 1 << kHashBits;                 // map entries per slot
+constexpr int kValBits = 11;
+constexpr uint32_t kValMask = (1u << kValBits) - 1u;  // pool tile, or kValHbm
+constexpr uint32_t kValHbm = kValMask;
+constexpr uint32_t kKeyEmpty = 0xffffffffu;
+
+struct __align__(16) SlotHdr {   // a multiple of 16 bytes: the float4 wells follow
+    unsigned long long att, acc, esc, cand, vis, st[4];
+    int64_t r;          // realization within its site
+    int state;          // 0 free, 1 open
+    int site;
+    int next, np, rem;  // particles handed out, per realization, not yet ended
+    int pool_next;      // pool tiles handed out (may exceed the pool)
+    int hash_full;      // a tile found no map entry
+    float a2, b2, c, d, e;
+};
+
+__device__ __forceinline__ uint32_t tile_hash(uint32_t t)
+{
+    return (t * 2654435761u) >> (32 - kHashBits);
+}
+
+// One slot's tiled bitmap, as seen by one lane.
+struct PoolBits {
+    uint32_t *keys;     // LDS [kHash]: (tile << kValBits) | value, or kKeyEmpty
+    uint32_t *pool;     // LDS [pool_tiles][32]
+    int *pool_next, *hash_full;
+    int pool_tiles;
+    uint32_t *hbm;      // [ntiles][32]
+    int tiles_x;
+
+    __device__ __forceinline__ uint32_t *tile_words(uint32_t t, uint32_t v) const
+    {
+        return v == kValHbm ? hbm + (int64_t)t * 32 : pool + v * 32;
+    }
+    __device__ __forceinline__ uint32_t *find(uint32_t t) const
+    {
+        uint32_t h = tile_hash(t);
+        for (int k = 0; k < kHash; ++k) {
+            const uint32_t e = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (e == kKeyEmpty)
+                return nullptr;
+            if ((e >> kValBits) == t)
+                return tile_words(t, e & kValMask);
+            h = (h + 1) & (kHash - 1);
+        }
+        return nullptr;
+    }
+    __device__ __forceinline__ uint32_t *claim(uint32_t t)
+    {
+        const int v0 = atomicAdd(pool_next, 1);
+        const uint32_t v = v0 < pool_tiles ? (uint32_t)v0 : kValHbm;
+        const uint32_t want = (t << kValBits) | v;
+        uint32_t h = tile_hash(t);
+        for (int k = 0; k < kHash; ++k) {
+            const uint32_t prev = atomicCAS(&keys[h], kKeyEmpty, want);
+            if (prev == kKeyEmpty)
+                return tile_words(t, v);
+            if ((prev >> kValBits) == t)   // another lane mapped it first
+                return tile_words(t, prev & kValMask);
+            h = (h + 1) & (kHash - 1);
+        }
+        atomicOr(hash_full, 1);   // unmapped: the flush scans the HBM region
+        return hbm + (int64_t)t * 32;
+    }
+    __device__ __forceinline__ bool test(int row, int col) const
+    {
+        const uint32_t t = (uint32_t)((row >> 5) * tiles_x + (col >> 5));
+        const uint32_t *w = find(t);
+        if (w == nullptr) {
+            if (!__hip_atomic_load(hash_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                return false;
+            w = hbm + (int64_t)t * 32;
+        }
+        return (w[row & 31] >> (col & 31)) & 1u;
+    }
+    __device__ __forceinline__ void set(int row, int col)
+    {
+        const uint32_t t = (uint32_t)((row >> 5) * tiles_x + (col >> 5));
+        uint32_t *w = find(t);
+        if (w == nullptr)
+            w = claim(t);
+        atomicOr(&w[row & 31], 1u << (col & 31));
+    }
+};
+
+struct PoolLds {
+    SlotHdr *hdr;       // [slots]
+    float4 *wells;      // [slots][max_nwells]
+    uint32_t *keys;     // [slots][kHash]
+    uint32_t *pool;     // [slots][pool_tiles][32]
+};
+
+__device__ __forceinline__ PoolLds carve(const PoolArgs &P, void *lds)
+{
+    PoolLds L;
+    L.hdr = reinterpret_cast<SlotHdr *>(lds);
+    L.wells = reinterpret_cast<float4 *>(L.hdr + P.slots);
+    L.keys = reinterpret_cast<uint32_t *>(L.wells + (size_t)P.slots * P.max_nwells);
+    L.pool = L.keys + (size_t)P.slots * kHash;
+    return L;
+}
+
+__device__ __forceinline__ PoolBits slot_bits(const PoolArgs &P, const PoolLds &L, int s, int tiles_x)
+{
+    PoolBits b;
+    b.keys = L.keys + (size_t)s * kHash;
+    b.pool = L.pool + (size_t)s * P.pool_tiles * 32;
+    b.pool_next = &L.hdr[s].pool_next;
+    b.hash_full = &L.hdr[s].hash_full;
+    b.pool_tiles = P.pool_tiles;
+    b.hbm = P.hbm + ((int64_t)blockIdx.x * P.slots + s) * P.max_ntiles * 32;
+    b.tiles_x = tiles_x;
+    return b;
+}
+
+// Site and realization of queue item `item` (site_first: prefix sums of nreal).
+__device__ __forceinline__ int find_site(const PoolArgs &P, int64_t item)
+{
+    int lo = 0, hi = P.nsites - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (P.site_first[mid] <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Opens slot s for queue item `item`: wells, regional coefficients, header.
+// Wave-uniform; ends with a barrier.
+__device__ void open_slot(const PoolArgs &P, const PoolLds &L, int s, int64_t item)
+{
+    const int site = find_site(P, item);
+    const TraceArgs &A = P.sites[site];
+    const int64_t r = item - P.site_first[site];
+    const double bn = A.b[r] * A.n[r];
+    float4 *w = L.wells + (size_t)s * P.max_nwells;
+    for (int i = threadIdx.x; i < A.nwells; i += kWave) {
+        const double qw = A.q[r * A.nwells + i];
+        w[i] = make_float4((float)(A.wells_xy[2 * i] - A.xo), (float)(A.wells_xy[2 * i + 1] - A.yo),
+                           (float)(qw * 0.15915494309189535 / bn), 0.0f);
+    }
+    if (threadIdx.x == 0) {
+        SlotHdr &h = L.hdr[s];
+        const double *c = A.coef + 6 * r;
+        h.a2 = (float)(2.0 * c[0] / bn);
+        h.b2 = (float)(2.0 * c[1] / bn);
+        h.c = (float)(c[2] / bn);
+        h.d = (float)(c[3] / bn);
+        h.e = (float)(c[4] / bn);
+        h.att = h.acc = h.esc = h.cand = h.vis = 0;
+        for (int j = 0; j < 4; ++j)
+            h.st[j] = 0;
+        h.r = r;
+        h.site = site;
+        h.next = 0;
+        h.np = A.npaths;
+        h.rem = A.npaths;
+        h.pool_next = 0;
+        h.hash_full = 0;
+        h.state = 1;
+    }
+    __syncthreads();
+}
+
+// Flushes slot s into its site's counts and statistics and frees it.
+// Wave-uniform; ends with a barrier.
+__device__ void flush_slot(const PoolArgs &P, const PoolLds &L, int s)
+{
+    SlotHdr &h = L.hdr[s];
+    const TraceArgs &A = P.sites[h.site];
+    const PoolBits b = slot_bits(P, L, s, A.tiles_x);
+    unsigned long long vis = 0;
+    if (A.rasterize) {
+        for (int k = threadIdx.x; k < kHash * 32; k += kWave) {
+            const uint32_t e = b.keys[k >> 5];
+            if (e == kKeyEmpty)
+                continue;
+            const uint32_t t = e >> kValBits;
+            uint32_t *w = b.tile_words(t, e & kValMask) + (k & 31);
+            const uint32_t bits = *w;
+            if (!bits)
+                continue;
+            *w = 0u;
+            vis += count_bits(A, (int)(t / A.tiles_x) * 32 + (k & 31), (int)(t % A.tiles_x) * 32,
+                              bits, nullptr);
+        }
+        if (h.hash_full) {   // unmapped tiles: scan the whole HBM region
+            for (int k = threadIdx.x; k < A.ntiles * 32; k += kWave) {
+                const uint32_t bits = b.hbm[k];
+                if (!bits)
+                    continue;
+                b.hbm[k] = 0u;
+                const int t = k >> 5;
+                vis += count_bits(A, (t / A.tiles_x) * 32 + (k & 31), (t % A.tiles_x) * 32, bits,
+                                  nullptr);
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        vis += __shfl_xor(vis, off);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kHash; k += kWave)
+        b.keys[k] = kKeyEmpty;
+    if (threadIdx.x == 0) {
+        if (A.stats) {
+            atomicAdd(&A.stats[kStatAttempted], h.att);
+            atomicAdd(&A.stats[kStatAccepted], h.acc);
+            atomicAdd(&A.stats[kStatEscaped], h.esc);
+            atomicAdd(&A.stats[kStatCandidates], h.cand);
+            atomicAdd(&A.stats[kStatVisited], vis);
+            for (int j = 0; j < 4; ++j)
+                atomicAdd(&A.stats[kStatStatus0 + j], h.st[j]);
+        }
+        h.pool_next = 0;
+        h.hash_full = 0;
+        h.state = 0;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
+{
+    extern __shared__ __align__(16) unsigned char pool_lds[];
+    const PoolLds L = carve(P, pool_lds);
+    const int lane = threadIdx.x;
+    for (int s = 0; s < P.slots; ++s) {
+        if (lane == 0) {
+            L.hdr[s].state = 0;
+            L.hdr[s].pool_next = 0;
+            L.hdr[s].hash_full = 0;
+        }
+        for (int k = lane; k < kHash; k += kWave)
+            L.keys[(size_t)s * kHash + k] = kKeyEmpty;
+    }
+    for (int k = lane; k < P.slots * P.pool_tiles * 32; k += kWave)
+        L.pool[k] = 0u;
+    __syncthreads();
+
+    // the lane's trace and its context
+    bool live = false;
+    int slot = 0, pidx = 0;
+    unsigned long long cand = 0;
+    Trace32 tr;
+    Field32 f;
+    Integ32 I;
+    Epoch32 ep;
+    PoolBits bits;
+    float gdx = 0.f, gdy = 0.f, igdx = 0.f, igdy = 0.f, umbra = 0.f, u2 = 0.f;
+    bool raster = false;
+    // wave-uniform
+    bool queue_done = false;
+    int intake = -1;
+
+    for (;;) {
+        // A. idle lanes take particles from the intake slot, opening slots as needed
+        bool ended = false;
+        unsigned long long need = __ballot(!live);
+        while (need != 0ull) {
+            if (intake >= 0 && L.hdr[intake].next < L.hdr[intake].np) {
+                const int next = L.hdr[intake].next;
+                const int avail = L.hdr[intake].np - next;
+                const int rank = __popcll(need & ((1ull << lane) - 1ull));
+                const int took = min(__popcll(need), avail);
+                if (((need >> lane) & 1ull) && rank < avail) {
+                    slot = intake;
+                    pidx = next + rank;
+                    const SlotHdr &h = L.hdr[slot];
+                    const TraceArgs &A = P.sites[h.site];
+                    f.w = L.wells + (size_t)slot * P.max_nwells;
+                    f.nw = A.nwells;
+                    f.a2 = h.a2;
+                    f.b2 = h.b2;
+                    f.c = h.c;
+                    f.d = h.d;
+                    f.e = h.e;
+                    I.dur = A.duration;
+                    I.absdur = fabs(A.duration);
+                    I.sg = A.duration > 0.0 ? 1.0
+                                            : (A.duration < 0.0 ? -1.0 : (A.duration == 0.0 ? 0.0 : (double)NAN));
+                    I.tol = (float)A.tol;
+                    I.maxstep = (float)A.maxstep;
+                    I.cap = A.max_steps;
+                    raster = A.rasterize != 0;
+                    if (raster) {
+                        ep = find_epoch32(A, h.r * A.npaths + pidx);
+                        bits = slot_bits(P, L, slot, A.tiles_x);
+                        gdx = (float)A.gdx;
+                        gdy = (float)A.gdy;
+                        igdx = (float)(1.0 / A.gdx);
+                        igdy = (float)(1.0 / A.gdy);
+                        umbra = (float)A.umbra;
+                        u2 = umbra * umbra;
+                    }
+                    cand = 0;
+                    live = start32(f, I, tr, (float)(A.start_xy[2 * pidx] - A.xo),
+                                   (float)(A.start_xy[2 * pidx + 1] - A.yo));
+                    ended = !live;
+                }
+                __syncthreads();
+                if (lane == 0)
+                    L.hdr[intake].next = next + took;
+                __syncthreads();
+                need = __ballot(!live && !ended);
+                continue;
+            }
+            if (queue_done)
+                break;
+            int fs = -1;
+            for (int s = 0; s < P.slots; ++s)
+                if (fs < 0 && L.hdr[s].state == 0)
+                    fs = s;
+            if (fs < 0)
+                break;
+            unsigned long long item = 0ull;
+            if (lane == 0)
+                item = atomicAdd(P.queue, 1ull);
+            item = __shfl(item, 0);
+            if (item >= (unsigned long long)P.total) {
+                queue_done = true;
+                break;
+            }
+            open_slot(P, L, fs, (int64_t)item);
+            intake = fs;
+        }
+
+        // B. one attempted step per live lane
+        if (live) {
+            float ox, oy;
+            const int res = attempt32(f, I, tr, ox, oy);
+            if ((res & 1) && raster)
+                cand += (unsigned long long)raster32(bits, ep, gdx, gdy, igdx, igdy, umbra, u2, ox,
+                                                     oy, tr.x, tr.y);
+            if (res & 2) {
+                live = false;
+                ended = true;
+            }
+        }
+        __syncthreads();
+
+        // C. ended traces: statistics, per-trace outputs, the slot's count down
+        if (ended) {
+            SlotHdr &h = L.hdr[slot];
+            const TraceArgs &A = P.sites[h.site];
+            atomicAdd(&h.att, (unsigned long long)tr.att);
+            atomicAdd(&h.acc, (unsigned long long)tr.acc);
+            atomicAdd(&h.cand, cand);
+            atomicAdd(&h.st[tr.status & 3], 1ull);
+            const float gx0 = (float)(A.gxmin - A.xo), gx1 = (float)(A.gxmax - A.xo);
+            const float gy0 = (float)(A.gymin - A.yo), gy1 = (float)(A.gymax - A.yo);
+            if (!(tr.bx0 > gx0 && tr.bx1 < gx1 && tr.by0 > gy0 && tr.by1 < gy1))
+                atomicAdd(&h.esc, 1ull);
+            const int64_t k = h.r * A.npaths + pidx;
+            if (A.bbox) {
+                A.bbox[4 * k] = A.xo + (double)tr.bx0;
+                A.bbox[4 * k + 1] = A.xo + (double)tr.bx1;
+                A.bbox[4 * k + 2] = A.yo + (double)tr.by0;
+                A.bbox[4 * k + 3] = A.yo + (double)tr.by1;
+            }
+            if (A.trace_status)
+                A.trace_status[k] = tr.status;
+            if (A.trace_steps)
+                A.trace_steps[k] = tr.att;
+            atomicSub(&h.rem, 1);
+        }
+        __syncthreads();
+
+        // D. flush finished realizations
+        bool open = false;
+        for (int s = 0; s < P.slots; ++s) {
+            const SlotHdr &h = L.hdr[s];
+            if (h.state == 1 && h.next == h.np && h.rem == 0) {
+                flush_slot(P, L, s);
+                if (intake == s)
+                    intake = -1;
+            }
+            open = open || L.hdr[s].state == 1;
+        }
+        if (queue_done && !open && __ballot(live) == 0ull)
+            break;
+    }
+}
+
+static_assert(sizeof(SlotHdr) % 16 == 0, "slot headers keep the wells 16-byte aligned");
+
+}  // namespace
+
+size_t pool_lds_bytes(int slots, int max_nwells, int pool_tiles)
+{
+    return (size_t)slots * (sizeof(SlotHdr) + (size_t)max_nwells * sizeof(float4) +
+                            (size_t)kHash * 4 + (size_t)pool_tiles * 128);
+}
+
+int pool_max_tiles() { return (int)((1u << (32 - kValBits)) - 1u); }
+int pool_max_pool_tiles() { return (int)kValHbm; }
+
+hipError_t launch_trace_pool32(const PoolArgs &p, int waves, hipStream_t s)
+{
+    const size_t lds = pool_lds_bytes(p.slots, p.max_nwells, p.pool_tiles);
+    hipLaunchKernelGGL(k_trace_pool32, dim3(waves), dim3(kWave), lds, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace oka

@@ -172,3 +172,62 @@ def test_fp32_error_config1_subset(gpu_ctx):
     err = np.abs(p[_lib.FP32] - p[_lib.FP64_EXACT])
     assert err.max() <= 0.01
     assert err.sum() <= 0.01 * (p[_lib.FP64_EXACT] > 0).sum()
+
+
+def _pool_vs_workgroup(s, d, coef, pf, umbra, **pool):
+    real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    args = (pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration, s.tol, s.maxstep, umbra)
+    wg = chronicle_into_field(*args, precision=_lib.FP32)
+    pl = chronicle_into_field(*args, precision=_lib.FP32, schedule=_lib.SCHED_POOL, **pool)
+    assert np.array_equal(pl.counts, wg.counts)
+    assert np.array_equal(pl.trace_status, wg.trace_status)
+    for k in ("attempted", "accepted", "visited", "escaped", "candidates"):
+        assert getattr(pl.stats, k) == getattr(wg.stats, k), k
+    assert list(pl.stats.status_count) == list(wg.stats.status_count)
+    return pl
+
+
+POOL_CASES = [
+    ("default", {}),
+    ("one_slot", dict(pool_slots=1)),
+    ("tiny_pool", dict(pool_slots=4, pool_waves_per_cu=24)),   # ~5 LDS tiles per slot: HBM
+]
+
+
+@pytest.mark.parametrize("name,pool", POOL_CASES, ids=[c[0] for c in POOL_CASES])
+def test_pool_schedule_matches_workgroup_config1(gpu_ctx, name, pool):
+    """The persistent wave-pool schedule (work stealing over live particles)
+    traces the same particles, so counts, statuses and statistics equal the
+    workgroup schedule's."""
+    print(f"pool case {name}", flush=True)
+    s = sites.config1(40)
+    d = sites.draw(s, seed=7)
+    _, _, coef = oracle_fit_draw(s, d)
+    _pool_vs_workgroup(s, d, coef, s.fixed_field(), s.umbra, **pool)
+
+
+def test_pool_schedule_rejects_what_does_not_fit(gpu_ctx):
+    s = sites.config1(2)
+    d = sites.draw(s, seed=7)
+    _, _, coef = oracle_fit_draw(s, d)
+    real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    args = (s.fixed_field(), s.wells_xy(), s.xt, s.yt, real, starts, s.duration, s.tol,
+            s.maxstep, s.umbra)
+    with pytest.raises(_lib.OkaError, match="do not fit in LDS"):
+        chronicle_into_field(*args, precision=_lib.FP32, schedule=_lib.SCHED_POOL,
+                             pool_slots=8, pool_waves_per_cu=32)
+    with pytest.raises(_lib.OkaError, match="FP32"):
+        chronicle_into_field(*args, precision=_lib.FP64_EXACT, schedule=_lib.SCHED_POOL)
+
+
+def test_pool_schedule_many_tiles_and_uneven_lifetimes(gpu_ctx):
+    """A wide umbra on a fine 1200 x 1100 lattice marks more tiles than a
+    slot's map holds (unmapped tiles in HBM), and mixed realization sizes
+    (npaths 40) with a long duration give uneven lifetimes."""
+    s, d = small_site(nreal=12, npaths=40)
+    _, _, coef = oracle_fit_draw(s, d)
+    pf = fixed_field(s.xt, s.yt, 5.0, 1100, 1200)
+    pl = _pool_vs_workgroup(s, d, coef, pf, 300.0, pool_slots=3)
+    assert pl.counts.max() == 12
