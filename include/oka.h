@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define OKA_ABI_VERSION 3
+#define OKA_ABI_VERSION 4
 
 enum {
     OKA_OK = 0,
@@ -74,6 +74,15 @@ typedef struct oka_ctx oka_ctx;
 
 /* ---- context ------------------------------------------------------------ */
 int oka_abi_version(void);
+
+/* What the context's device offers the launch planners. */
+typedef struct {
+    int32_t num_cu;              /* compute units                                   */
+    int32_t warp_size;           /* lanes per wave                                  */
+    int64_t lds_per_workgroup;   /* largest LDS allocation of one workgroup (bytes)  */
+    int64_t lds_per_cu;          /* LDS shared by the workgroups of one CU (bytes)   */
+} oka_device_info;
+int oka_ctx_device_info(oka_ctx *ctx, oka_device_info *out);
 int oka_ctx_create(oka_ctx **out, int device);
 void oka_ctx_destroy(oka_ctx *ctx);
 const char *oka_last_error(const oka_ctx *ctx);
@@ -135,14 +144,18 @@ typedef struct {
     int32_t tile_slots;      /* OKA_BITMAP_TILED: LDS tile slots (<= 0: as many as fit) */
     int32_t pool_slots;      /* OKA_SCHED_POOL: realizations in flight per wave (<= 0: 4) */
     int32_t pool_waves_per_cu; /* OKA_SCHED_POOL: resident waves per CU (<= 0: 8)     */
+    int32_t pool_chunk;      /* OKA_SCHED_POOL: particles per work item (<= 0: 64)   */
+    int32_t reserved;
 } oka_trace_opts;
 
 /* How particles are scheduled on the GPU.  WORKGROUP: one 256-lane workgroup
  * per realization, its lanes refilled with the realization's particles.
- * POOL (FP32, confined, no window marks): a persistent grid of single waves,
- * each keeping pool_slots realizations in flight and refilling a lane with a
- * particle of any of them, realizations taken from a queue shared by all
- * waves -- for particle lifetimes that differ widely within a realization.
+ * POOL (FP32, confined, no window marks): a persistent grid of single waves
+ * working through a queue shared by all waves of chunks of pool_chunk
+ * particles; each wave keeps pool_slots chunks in flight and refills a lane
+ * with a particle of any of them, so uneven particle lifetimes and uneven
+ * realizations (even of different sites) balance across the GPU.  The chunks
+ * of one realization meet in an HBM record flushed by the last of them.
  * AUTO is WORKGROUP.  Both give the same counts. */
 enum { OKA_SCHED_AUTO = 0, OKA_SCHED_WORKGROUP = 1, OKA_SCHED_POOL = 2 };
 

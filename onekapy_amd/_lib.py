@@ -33,7 +33,7 @@ _vp = C.c_void_p
 
 # Every symbol include/oka.h declares (the CPU test checks the export table).
 EXPORTED = (
-    "oka_abi_version", "oka_ctx_create", "oka_ctx_destroy", "oka_last_error",
+    "oka_abi_version", "oka_ctx_device_info", "oka_ctx_create", "oka_ctx_destroy", "oka_last_error",
     "oka_ctx_set_stream", "oka_ctx_stream", "oka_ctx_synchronize", "oka_model_eval",
     "oka_fit_draw", "oka_trace_chronicle", "oka_grid_history", "oka_trace_vertices",
     "oka_rasterize", "oka_pow_fifth", "oka_log", "oka_trace_marks", "oka_mlp_create",
@@ -75,7 +75,13 @@ class TraceOpts_(C.Structure):
     _fields_ = [("npaths", _i32), ("precision", _i32), ("confined", _i32), ("schedule", _i32),
                 ("duration", _d), ("tol", _d), ("maxstep", _d), ("umbra", _d),
                 ("max_steps", _i64), ("start_xy", _vp), ("bitmap_mode", _i32),
-                ("tile_slots", _i32), ("pool_slots", _i32), ("pool_waves_per_cu", _i32)]
+                ("tile_slots", _i32), ("pool_slots", _i32), ("pool_waves_per_cu", _i32),
+                ("pool_chunk", _i32), ("reserved", _i32)]
+
+
+class DeviceInfo_(C.Structure):
+    _fields_ = [("num_cu", _i32), ("warp_size", _i32), ("lds_per_workgroup", _i64),
+                ("lds_per_cu", _i64)]
 
 
 class Grid_(C.Structure):
@@ -123,6 +129,8 @@ def load():
         raise OkaLibraryError(f"cannot load {LIB_PATH}: {e}") from e
     L.oka_abi_version.restype = C.c_int
     L.oka_abi_version.argtypes = []
+    L.oka_ctx_device_info.restype = C.c_int
+    L.oka_ctx_device_info.argtypes = [_vp, C.POINTER(DeviceInfo_)]
     L.oka_ctx_create.restype = C.c_int
     L.oka_ctx_create.argtypes = [C.POINTER(_vp), C.c_int]
     L.oka_ctx_destroy.restype = None
@@ -214,6 +222,12 @@ class Context:
     def set_stream(self, stream_ptr):
         self.check(self.lib.oka_ctx_set_stream(self.h, stream_ptr))
 
+    def device_info(self):
+        """dict(num_cu, warp_size, lds_per_workgroup, lds_per_cu) of the context's device."""
+        d = DeviceInfo_()
+        self.check(self.lib.oka_ctx_device_info(self.h, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in DeviceInfo_._fields_}
+
     def synchronize(self):
         self.check(self.lib.oka_ctx_synchronize(self.h))
 
@@ -274,7 +288,7 @@ SCHED_AUTO, SCHED_WORKGROUP, SCHED_POOL = 0, 1, 2
 
 def make_trace_opts(npaths, duration, tol, maxstep, umbra, start_xy_ptr, precision=FP64_EXACT,
                     confined=True, max_steps=0, bitmap_mode=BITMAP_AUTO, tile_slots=0,
-                    schedule=SCHED_AUTO, pool_slots=0, pool_waves_per_cu=0):
+                    schedule=SCHED_AUTO, pool_slots=0, pool_waves_per_cu=0, pool_chunk=0):
     o = TraceOpts_()
     o.npaths = int(npaths)
     o.precision = int(precision)
@@ -287,6 +301,7 @@ def make_trace_opts(npaths, duration, tol, maxstep, umbra, start_xy_ptr, precisi
     o.schedule = int(schedule)
     o.pool_slots = int(pool_slots)
     o.pool_waves_per_cu = int(pool_waves_per_cu)
+    o.pool_chunk = int(pool_chunk)
     return o
 
 

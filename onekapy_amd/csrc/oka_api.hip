@@ -114,15 +114,17 @@ __global__ void k_bits_to_u8(const uint32_t *bits, uint8_t *out, int64_t n)
 }
 
 // Zero-fill without hipMemsetAsync, so that the call can be graph-captured.
-int fill_u32(oka_ctx *c, void *p, int64_t nwords)
+int fill_u32v(oka_ctx *c, void *p, uint32_t value, int64_t nwords)
 {
     if (nwords <= 0)
         return OKA_OK;
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, c->stream,
-                       (uint32_t *)p, 0u, nwords);
+                       (uint32_t *)p, value, nwords);
     HIPCHK(c, hipGetLastError());
     return OKA_OK;
 }
+
+int fill_u32(oka_ctx *c, void *p, int64_t nwords) { return fill_u32v(c, p, 0u, nwords); }
 
 
 }  // namespace oka
@@ -156,6 +158,17 @@ extern "C" {
 
 int oka_abi_version(void) { return OKA_ABI_VERSION; }
 
+int oka_ctx_device_info(oka_ctx *c, oka_device_info *out)
+{
+    if (!c || !out)
+        return OKA_ERR_ARG;
+    out->num_cu = c->num_cu;
+    out->warp_size = c->warp_size;
+    out->lds_per_workgroup = (int64_t)c->max_lds;
+    out->lds_per_cu = (int64_t)c->lds_per_cu;
+    return OKA_OK;
+}
+
 int oka_ctx_create(oka_ctx **out, int device)
 {
     if (!out)
@@ -178,8 +191,12 @@ int oka_ctx_create(oka_ctx **out, int device)
     c->stream = c->own;
     c->num_cu = prop.multiProcessorCount;
     c->max_lds = prop.sharedMemPerBlock;
-    c->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
-                                                          : prop.sharedMemPerBlock;
+    c->warp_size = prop.warpSize;
+    int lds_cu = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                              device) != hipSuccess)
+        lds_cu = 0;
+    c->lds_per_cu = lds_cu > 0 ? (size_t)lds_cu : prop.sharedMemPerBlock;
     *out = c;
     return OKA_OK;
 }
@@ -480,12 +497,18 @@ static int chronicle_finish(oka_ctx *c, const TraceArgs &a, int32_t rasterize, i
 // context events when `timed`.
 int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opts *o, bool timed)
 {
-    std::vector<int64_t> first((size_t)nsites + 1);
-    int64_t total = 0;
+    const int chunk = o->pool_chunk > 0 ? o->pool_chunk : 64;
+    std::vector<int64_t> first((size_t)nsites + 1), rfirst((size_t)nsites + 1);
+    int64_t total = 0, nreal = 0, chunked = 0;
     int max_nw = 1, max_tiles = 1;
     for (int i = 0; i < nsites; ++i) {
+        const int nch = (sites[i].npaths + chunk - 1) / chunk;
         first[i] = total;
-        total += sites[i].nreal;
+        rfirst[i] = nreal;
+        total += (int64_t)sites[i].nreal * nch;
+        nreal += sites[i].nreal;
+        if (nch > 1)
+            chunked += sites[i].nreal;
         max_nw = std::max(max_nw, (int)sites[i].nwells);
         if (sites[i].rasterize)
             max_tiles = std::max(max_tiles, (int)sites[i].ntiles);
@@ -493,6 +516,7 @@ int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opt
             return fail(c, OKA_ERR_ARG, "the pool schedule traces the confined velocity only");
     }
     first[nsites] = total;
+    rfirst[nsites] = nreal;
     if (max_tiles > pool_max_tiles())
         return fail(c, OKA_ERR_ARG, "grid too large for the pool schedule (%d tiles)", max_tiles);
     const int slots = o->pool_slots > 0 ? std::min(o->pool_slots, 16) : 4;
@@ -505,22 +529,48 @@ int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opt
     const int pool_tiles =
         (int)std::min<size_t>((budget - fixed) / ((size_t)slots * 128), pool_max_pool_tiles());
     const int waves = c->num_cu * wpc;
-    void *p_sites, *p_first, *p_queue, *p_hbm;
+    // realization records: enough for every chunked realization in flight
+    // (a chunk waits while an older realization holds its record), at most 8 GiB
+    const int dirty_words = (max_tiles + 31) / 32;
+    const int64_t rec_bytes = (int64_t)max_tiles * 128 + dirty_words * 4 + 12;
+    int64_t nrec = 0;
+    if (chunked > 0) {
+        nrec = std::min<int64_t>(chunked, std::max<int64_t>(2LL * waves * slots, 64));
+        nrec = std::max<int64_t>(1, std::min<int64_t>(nrec, (8LL << 30) / rec_bytes));
+    }
+    void *p_sites, *p_first, *p_rfirst, *p_queue, *p_hbm, *p_rec = nullptr;
     TRY(ensure(c, kSlotPoolSites, sizeof(TraceArgs) * nsites, &p_sites));
-    TRY(ensure(c, kSlotPoolFirst, sizeof(int64_t) * first.size(), &p_first));
+    TRY(ensure(c, kSlotPoolFirst, sizeof(int64_t) * 2 * first.size(), &p_first));
+    p_rfirst = (int64_t *)p_first + first.size();
     TRY(ensure(c, kSlotPoolQueue, sizeof(unsigned long long), &p_queue));
     const int64_t hbm_words = (int64_t)waves * slots * max_tiles * 32;
     TRY(ensure(c, kSlotPoolHbm, sizeof(uint32_t) * hbm_words, &p_hbm));
+    // per record: owner (2 words), done and a pad word, the tiles, the dirty bits
+    const int64_t rec_words = nrec * ((int64_t)max_tiles * 32 + dirty_words + 4);
+    if (nrec > 0)
+        TRY(ensure(c, kSlotPoolRec, sizeof(uint32_t) * rec_words, &p_rec));
     HIPCHK(c, hipMemcpyAsync(p_sites, sites, sizeof(TraceArgs) * nsites, hipMemcpyHostToDevice,
                              c->stream));
     HIPCHK(c, hipMemcpyAsync(p_first, first.data(), sizeof(int64_t) * first.size(),
                              hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(p_rfirst, rfirst.data(), sizeof(int64_t) * rfirst.size(),
+                             hipMemcpyHostToDevice, c->stream));
     TRY(fill_u32(c, p_queue, 2));
     TRY(fill_u32(c, p_hbm, hbm_words));
     PoolArgs p{};
+    if (nrec > 0) {
+        // [owner int64][done int32][pad int32][bits][dirty]
+        p.rec_owner = (int64_t *)p_rec;
+        p.rec_done = (int32_t *)(p.rec_owner + nrec);
+        p.rec_bits = (uint32_t *)(p.rec_done + 2 * nrec);
+        p.rec_dirty = p.rec_bits + nrec * (int64_t)max_tiles * 32;
+        TRY(fill_u32v(c, p.rec_owner, 0xffffffffu, 2 * nrec));
+        TRY(fill_u32(c, p.rec_done, 2 * nrec + nrec * ((int64_t)max_tiles * 32 + dirty_words)));
+    }
     p.sites = (const TraceArgs *)p_sites;
     p.nsites = nsites;
     p.site_first = (const int64_t *)p_first;
+    p.real_first = (const int64_t *)p_rfirst;
     p.total = total;
     p.queue = (unsigned long long *)p_queue;
     p.hbm = (uint32_t *)p_hbm;
@@ -528,6 +578,9 @@ int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opt
     p.max_nwells = max_nw;
     p.pool_tiles = pool_tiles;
     p.max_ntiles = max_tiles;
+    p.chunk = chunk;
+    p.dirty_words = dirty_words;
+    p.nrec = std::max<int64_t>(nrec, 1);
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (total > 0)

@@ -19,7 +19,7 @@ enum Slot {
     kSlotVerts, kSlotNVerts, kSlotObs, kSlotLogs, kSlotZ, kSlotEv, kSlotCov, kSlotFStatus,
     kSlotXY, kSlotOut, kSlotEStatus, kSlotRgrid, kSlotMarks, kSlotMlpX, kSlotMlpMarks,
     kSlotMlpOrder, kSlotMlpOut, kSlotMlpOut2, kSlotMlpLoss, kSlotCvDists, kSlotCvSite,
-    kSlotCvDraw, kSlotPoolSites, kSlotPoolFirst, kSlotPoolQueue, kSlotPoolHbm, kSlotCount
+    kSlotCvDraw, kSlotPoolSites, kSlotPoolFirst, kSlotPoolQueue, kSlotPoolHbm, kSlotPoolRec, kSlotCount
 };
 
 struct DevBuf {
@@ -37,6 +37,7 @@ struct oka_ctx {
     int num_cu = 0;
     size_t max_lds = 0;      // per workgroup
     size_t lds_per_cu = 0;
+    int warp_size = 64;
     std::string err;
     oka::DevBuf buf[oka::kSlotCount];
 };
@@ -50,6 +51,7 @@ int stage_out(oka_ctx *c, Slot s, void *dst, size_t bytes, uint32_t flags, bool 
               void **out);
 int copy_back(oka_ctx *c, void *dst, const void *dev, size_t bytes, uint32_t flags);
 int fill_u32(oka_ctx *c, void *p, int64_t nwords);
+int fill_u32v(oka_ctx *c, void *p, uint32_t value, int64_t nwords);
 
 }  // namespace oka
 

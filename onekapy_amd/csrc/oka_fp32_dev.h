@@ -38,19 +38,35 @@ struct Field32 {
     float a2, b2, c, d, e;  // regional: k_reg = ((a2 dx + c dy) + d, (b2 dy + c dx) + e)
 };
 
+__device__ __forceinline__ void well_term(const float4 &w, float x, float y, float &vx, float &vy)
+{
+    const float ddx = x - w.x;
+    const float ddy = y - w.y;
+    const float r2 = fmaf(ddx, ddx, ddy * ddy);
+    const float s = w.z * __builtin_amdgcn_rcpf(r2);
+    vx = fmaf(s, ddx, vx);
+    vy = fmaf(s, ddy, vy);
+}
+
+// Wells are summed in index order; their LDS reads are issued 8 at a time so
+// that one trace alone (a long trace after its realization's others ended)
+// does not wait a full LDS round trip per well.
 __device__ __forceinline__ void feval32(const Field32 &f, float x, float y, float &kx, float &ky)
 {
     float vx = fmaf(f.a2, x, fmaf(f.c, y, f.d));
     float vy = fmaf(f.b2, y, fmaf(f.c, x, f.e));
-    for (int i = 0; i < f.nw; ++i) {
-        const float4 w = f.w[i];
-        const float ddx = x - w.x;
-        const float ddy = y - w.y;
-        const float r2 = fmaf(ddx, ddx, ddy * ddy);
-        const float s = w.z * __builtin_amdgcn_rcpf(r2);
-        vx = fmaf(s, ddx, vx);
-        vy = fmaf(s, ddy, vy);
+    int i = 0;
+    for (; i + 8 <= f.nw; i += 8) {
+        float4 w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            w[k] = f.w[i + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            well_term(w[k], x, y, vx, vy);
     }
+    for (; i < f.nw; ++i)
+        well_term(f.w[i], x, y, vx, vy);
     kx = vx;
     ky = vy;
 }

@@ -324,15 +324,27 @@ hipError_t launch_mark_moments(const uint8_t *marks, int64_t n, int32_t ncell, u
 uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, int word);
 
 // Persistent FP32 tracer over a table of sites (oka_pool.hip).
+// A queue item is a chunk of `chunk` particles of one realization.  A
+// realization of several chunks is gathered in a realization record in HBM
+// (its visited-node tiles and dirty-tile bits), which the wave that merges
+// the last chunk flushes.
 struct PoolArgs {
     const TraceArgs *sites;     // device [nsites]
     int32_t nsites;
     const int64_t *site_first;  // device [nsites + 1]: first queue item of each site
-    int64_t total;              // realizations over all sites
+    const int64_t *real_first;  // device [nsites + 1]: first realization id of each site
+    int64_t total;              // queue items over all sites
     unsigned long long *queue;  // next queue item; zero before the launch
     uint32_t *hbm;              // [waves][slots][max_ntiles][32]; zero before the launch
-    int32_t slots;              // realizations in flight per wave
+    int32_t slots;              // chunks in flight per wave
     int32_t max_nwells, pool_tiles, max_ntiles;
+    int32_t chunk;              // particles per queue item
+    int32_t dirty_words;        // ceil(max_ntiles / 32)
+    int64_t nrec;               // realization records
+    int64_t *rec_owner;         // [nrec]: realization id holding the record, or -1
+    int32_t *rec_done;          // [nrec]: chunks merged; zero before the launch
+    uint32_t *rec_bits;         // [nrec][max_ntiles][32]; zero before the launch
+    uint32_t *rec_dirty;        // [nrec][dirty_words]; zero before the launch
 };
 size_t pool_lds_bytes(int slots, int max_nwells, int pool_tiles);
 int pool_max_tiles();

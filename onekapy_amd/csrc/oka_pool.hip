@@ -40,9 +40,11 @@ constexpr uint32_t kKeyEmpty = 0xffffffffu;
 struct __align__(16) SlotHdr {   // a multiple of 16 bytes: the float4 wells follow
     unsigned long long att, acc, esc, cand, vis, st[4];
     int64_t r;          // realization within its site
-    int state;          // 0 free, 1 open
+    int64_t g;          // realization id over all sites
+    int state;          // 0 free, 1 open, 2 waiting for its realization record
     int site;
-    int next, np, rem;  // particles handed out, per realization, not yet ended
+    int next, np, rem;  // next particle to hand out, end of the chunk, not yet ended
+    int nchunks, rec;   // chunks of the realization; its record when nchunks > 1
     int pool_next;      // pool tiles handed out (may exceed the pool)
     int hash_full;      // a tile found no map entry
     float a2, b2, c, d, e;
@@ -61,6 +63,9 @@ struct PoolBits {
     int pool_tiles;
     uint32_t *hbm;      // [ntiles][32]
     int tiles_x;
+    // the last tile this lane found mapped (a mapping lasts until the flush)
+    uint32_t last_t;
+    uint32_t *last_w;
 
     __device__ __forceinline__ uint32_t *tile_words(uint32_t t, uint32_t v) const
     {
@@ -97,10 +102,21 @@ struct PoolBits {
         atomicOr(hash_full, 1);   // unmapped: the flush scans the HBM region
         return hbm + (int64_t)t * 32;
     }
-    __device__ __forceinline__ bool test(int row, int col) const
+    __device__ __forceinline__ uint32_t *lookup(uint32_t t)
+    {
+        if (t == last_t)
+            return last_w;
+        uint32_t *w = find(t);
+        if (w != nullptr) {
+            last_t = t;
+            last_w = w;
+        }
+        return w;
+    }
+    __device__ __forceinline__ bool test(int row, int col)
     {
         const uint32_t t = (uint32_t)((row >> 5) * tiles_x + (col >> 5));
-        const uint32_t *w = find(t);
+        const uint32_t *w = lookup(t);
         if (w == nullptr) {
             if (!__hip_atomic_load(hash_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
                 return false;
@@ -111,9 +127,12 @@ struct PoolBits {
     __device__ __forceinline__ void set(int row, int col)
     {
         const uint32_t t = (uint32_t)((row >> 5) * tiles_x + (col >> 5));
-        uint32_t *w = find(t);
-        if (w == nullptr)
+        uint32_t *w = lookup(t);
+        if (w == nullptr) {
             w = claim(t);
+            last_t = t;
+            last_w = w;
+        }
         atomicOr(&w[row & 31], 1u << (col & 31));
     }
 };
@@ -145,6 +164,8 @@ __device__ __forceinline__ PoolBits slot_bits(const PoolArgs &P, const PoolLds &
     b.pool_tiles = P.pool_tiles;
     b.hbm = P.hbm + ((int64_t)blockIdx.x * P.slots + s) * P.max_ntiles * 32;
     b.tiles_x = tiles_x;
+    b.last_t = kKeyEmpty;
+    b.last_w = nullptr;
     return b;
 }
 
@@ -162,13 +183,30 @@ __device__ __forceinline__ int find_site(const PoolArgs &P, int64_t item)
     return lo;
 }
 
+// Claims realization record h.rec for h.g (lane 0; the result is broadcast).
+__device__ __forceinline__ bool claim_record(const PoolArgs &P, const SlotHdr &h)
+{
+    int ok = 0;
+    if (threadIdx.x == 0) {
+        const unsigned long long prev = atomicCAS((unsigned long long *)&P.rec_owner[h.rec],
+                                                  ~0ull, (unsigned long long)h.g);
+        ok = prev == ~0ull || prev == (unsigned long long)h.g;
+    }
+    return __shfl(ok, 0) != 0;
+}
+
 // Opens slot s for queue item `item`: wells, regional coefficients, header.
-// Wave-uniform; ends with a barrier.
+// A chunk of a realization of several chunks also claims the realization's
+// record (state 2 while an older realization still holds it).  Wave-uniform;
+// ends with a barrier.
 __device__ void open_slot(const PoolArgs &P, const PoolLds &L, int s, int64_t item)
 {
     const int site = find_site(P, item);
     const TraceArgs &A = P.sites[site];
-    const int64_t r = item - P.site_first[site];
+    const int nch = (A.npaths + P.chunk - 1) / P.chunk;
+    const int64_t local = item - P.site_first[site];
+    const int64_t r = nch > 0 ? local / nch : 0;
+    const int ch = nch > 0 ? (int)(local - r * nch) : 0;
     const double bn = A.b[r] * A.n[r];
     float4 *w = L.wells + (size_t)s * P.max_nwells;
     for (int i = threadIdx.x; i < A.nwells; i += kWave) {
@@ -188,14 +226,20 @@ __device__ void open_slot(const PoolArgs &P, const PoolLds &L, int s, int64_t it
         for (int j = 0; j < 4; ++j)
             h.st[j] = 0;
         h.r = r;
+        h.g = P.real_first[site] + r;
         h.site = site;
-        h.next = 0;
-        h.np = A.npaths;
-        h.rem = A.npaths;
+        h.next = ch * P.chunk;
+        h.np = min(A.npaths, h.next + P.chunk);
+        h.rem = h.np - h.next;
+        h.nchunks = nch;
+        h.rec = nch > 1 ? (int)(h.g % P.nrec) : -1;
         h.pool_next = 0;
         h.hash_full = 0;
-        h.state = 1;
     }
+    __syncthreads();
+    const bool ready = L.hdr[s].nchunks <= 1 || claim_record(P, L.hdr[s]);
+    if (threadIdx.x == 0)
+        L.hdr[s].state = ready ? 1 : 2;
     __syncthreads();
 }
 
@@ -255,6 +299,106 @@ __device__ void flush_slot(const PoolArgs &P, const PoolLds &L, int s)
     __syncthreads();
 }
 
+// A finished chunk of a realization of several chunks: ORs the chunk's marked
+// words into the realization record and sets its dirty-tile bits; the wave
+// that merges the last chunk flushes the record into the counts and frees
+// it.  Wave-uniform; ends with a barrier.
+__device__ void merge_slot(const PoolArgs &P, const PoolLds &L, int s)
+{
+    SlotHdr &h = L.hdr[s];
+    const TraceArgs &A = P.sites[h.site];
+    const PoolBits b = slot_bits(P, L, s, A.tiles_x);
+    uint32_t *rb = P.rec_bits + (int64_t)h.rec * P.max_ntiles * 32;
+    uint32_t *rd = P.rec_dirty + (int64_t)h.rec * P.dirty_words;
+    if (A.rasterize) {
+        for (int k = threadIdx.x; k < kHash * 32; k += kWave) {
+            const uint32_t e = b.keys[k >> 5];
+            if (e == kKeyEmpty)
+                continue;
+            const uint32_t t = e >> kValBits;
+            uint32_t *w = b.tile_words(t, e & kValMask) + (k & 31);
+            const uint32_t bits = *w;
+            if (!bits)
+                continue;
+            *w = 0u;
+            atomicOr(&rb[(int64_t)t * 32 + (k & 31)], bits);
+            atomicOr(&rd[t >> 5], 1u << (t & 31));
+        }
+        if (h.hash_full) {
+            for (int k = threadIdx.x; k < A.ntiles * 32; k += kWave) {
+                const uint32_t bits = b.hbm[k];
+                if (!bits)
+                    continue;
+                b.hbm[k] = 0u;
+                atomicOr(&rb[k], bits);
+                atomicOr(&rd[k >> 10], 1u << ((k >> 5) & 31));
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kHash; k += kWave)
+        b.keys[k] = kKeyEmpty;
+    if (threadIdx.x == 0) {
+        if (A.stats) {
+            atomicAdd(&A.stats[kStatAttempted], h.att);
+            atomicAdd(&A.stats[kStatAccepted], h.acc);
+            atomicAdd(&A.stats[kStatEscaped], h.esc);
+            atomicAdd(&A.stats[kStatCandidates], h.cand);
+            for (int j = 0; j < 4; ++j)
+                atomicAdd(&A.stats[kStatStatus0 + j], h.st[j]);
+        }
+        h.pool_next = 0;
+        h.hash_full = 0;
+    }
+    // publish the chunk's marks before counting it
+    __threadfence();
+    int last = 0;
+    if (threadIdx.x == 0)
+        last = atomicAdd(&P.rec_done[h.rec], 1) == h.nchunks - 1;
+    last = __shfl(last, 0);
+    if (last) {
+        __threadfence();
+        unsigned long long vis = 0;
+        if (A.rasterize) {
+            const int dwords = (A.ntiles + 31) / 32;
+            for (int dw = 0; dw < dwords; ++dw) {
+                uint32_t dbits = __hip_atomic_load(&rd[dw], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                while (dbits) {
+                    const int t = dw * 32 + __ffs(dbits) - 1;
+                    dbits &= dbits - 1u;
+                    if (threadIdx.x < 32) {
+                        uint32_t *w = &rb[(int64_t)t * 32 + threadIdx.x];
+                        const uint32_t bits = __hip_atomic_load(w, __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT);
+                        if (bits) {
+                            __hip_atomic_store(w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            vis += count_bits(A, (t / A.tiles_x) * 32 + (int)threadIdx.x,
+                                              (t % A.tiles_x) * 32, bits, nullptr);
+                        }
+                    }
+                }
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&rd[dw], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1)
+            vis += __shfl_xor(vis, off);
+        if (threadIdx.x == 0) {
+            if (A.stats)
+                atomicAdd(&A.stats[kStatVisited], vis);
+            __hip_atomic_store(&P.rec_done[h.rec], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // the record is clean before another realization may claim it
+        __threadfence();
+        if (threadIdx.x == 0)
+            atomicExch((unsigned long long *)&P.rec_owner[h.rec], ~0ull);
+    }
+    if (threadIdx.x == 0)
+        h.state = 0;
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
 {
     extern __shared__ __align__(16) unsigned char pool_lds[];
@@ -293,6 +437,14 @@ __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
         bool ended = false;
         unsigned long long need = __ballot(!live);
         while (need != 0ull) {
+            if (intake >= 0 && L.hdr[intake].state == 2) {   // its record is still busy
+                const bool ok = claim_record(P, L.hdr[intake]);
+                if (threadIdx.x == 0 && ok)
+                    L.hdr[intake].state = 1;
+                __syncthreads();
+                if (!ok)
+                    break;
+            }
             if (intake >= 0 && L.hdr[intake].next < L.hdr[intake].np) {
                 const int next = L.hdr[intake].next;
                 const int avail = L.hdr[intake].np - next;
@@ -406,11 +558,14 @@ __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
         for (int s = 0; s < P.slots; ++s) {
             const SlotHdr &h = L.hdr[s];
             if (h.state == 1 && h.next == h.np && h.rem == 0) {
-                flush_slot(P, L, s);
+                if (h.nchunks > 1)
+                    merge_slot(P, L, s);
+                else
+                    flush_slot(P, L, s);
                 if (intake == s)
                     intake = -1;
             }
-            open = open || L.hdr[s].state == 1;
+            open = open || L.hdr[s].state != 0;
         }
         if (queue_done && !open && __ballot(live) == 0ull)
             break;

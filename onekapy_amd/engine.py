@@ -68,14 +68,14 @@ def _launch(ctx, site, real, opts, grid, epochs, rasterize, counts, bbox, status
 def chronicle_into_field(pfield, wells_xy, xo, yo, real, starts, duration, tol, maxstep, umbra,
                          confined=True, precision=_lib.FP64_EXACT, max_steps=0,
                          bitmap_mode=_lib.BITMAP_AUTO, tile_slots=0, schedule=_lib.SCHED_AUTO,
-                         pool_slots=0, pool_waves_per_cu=0):
+                         pool_slots=0, pool_waves_per_cu=0, pool_chunk=0):
     """Trace all realizations and rasterize them into ``pfield``'s lattice.
 
     Grows ``pfield`` exactly as the reference's per-trace ``rasterize`` calls
     would, and returns the per-node number of realizations that marked each
     node (the caller registers them).  Nothing is registered here.
     bitmap_mode / tile_slots place the per-realization visited-node bitmap
-    and schedule / pool_slots / pool_waves_per_cu choose how particles are
+    and schedule / pool_slots / pool_waves_per_cu / pool_chunk choose how particles are
     scheduled (oka_trace_opts); every choice gives the same counts.
     """
     wells_xy = _lib.f64(wells_xy).reshape(-1, 2)
@@ -86,7 +86,7 @@ def chronicle_into_field(pfield, wells_xy, xo, yo, real, starts, duration, tol, 
     site = _lib.make_site(wells_xy, xo, yo)
     opts = _lib.make_trace_opts(P, duration, tol, maxstep, umbra, _lib.ptr(starts), precision,
                                 confined, max_steps, bitmap_mode, tile_slots, schedule,
-                                pool_slots, pool_waves_per_cu)
+                                pool_slots, pool_waves_per_cu, pool_chunk)
     bbox = np.zeros((ntr, 4))
     status = np.zeros(ntr, dtype=np.int32)
     stats = _lib.TraceStats_()

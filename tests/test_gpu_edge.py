@@ -189,8 +189,9 @@ def _pool_vs_workgroup(s, d, coef, pf, umbra, **pool):
 
 
 POOL_CASES = [
-    ("default", {}),
-    ("one_slot", dict(pool_slots=1)),
+    ("default", {}),                                            # chunks of 64 particles
+    ("whole_realizations", dict(pool_chunk=1000)),              # one chunk: direct flush
+    ("odd_chunks_one_slot", dict(pool_chunk=7, pool_slots=1)),
     ("tiny_pool", dict(pool_slots=4, pool_waves_per_cu=24)),   # ~5 LDS tiles per slot: HBM
 ]
 
@@ -229,5 +230,7 @@ def test_pool_schedule_many_tiles_and_uneven_lifetimes(gpu_ctx):
     s, d = small_site(nreal=12, npaths=40)
     _, _, coef = oracle_fit_draw(s, d)
     pf = fixed_field(s.xt, s.yt, 5.0, 1100, 1200)
-    pl = _pool_vs_workgroup(s, d, coef, pf, 300.0, pool_slots=3)
-    assert pl.counts.max() == 12
+    for chunk in (40, 16, 5):
+        print(f"chunk {chunk}", flush=True)
+        pl = _pool_vs_workgroup(s, d, coef, pf, 300.0, pool_slots=3, pool_chunk=chunk)
+        assert pl.counts.max() == 12
