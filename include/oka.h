@@ -244,6 +244,41 @@ int oka_trace_marks(oka_ctx *ctx, const oka_site *site, int32_t nreal, const dou
                     const oka_trace_opts *opts, const oka_grid *grid, const oka_window *win,
                     uint8_t *marks, int32_t *counts, oka_trace_stats *stats, uint32_t flags);
 
+/* ---- batches: many sites in one persistent, graph-captured run ------------
+ * (the realization loop of stochastic.py:187-207 for many independent sites
+ * at once; no reference counterpart for the batching itself)
+ * oka_batch_create stages every site on the device and captures one run as a
+ * HIP graph: per site the fit and MVN draw of every realization (as
+ * oka_fit_draw), then every particle of every site in one launch of the pool
+ * tracer (OKA_SCHED_POOL: FP32, confined velocity), each site rasterized into
+ * its own fixed lattice.  All site arrays are host memory, read at create. */
+typedef struct {
+    const oka_site *site;    /* wells, origin (the target), observations, base    */
+    int32_t nreal;
+    const double *K, *n, *b; /* nreal                                              */
+    const double *q;         /* nreal x nwells                                     */
+    const double *z;         /* nreal x 6 standard normals, or NULL: coef = ev     */
+    oka_trace_opts opts;     /* npaths, duration, tol, maxstep, umbra, start_xy    */
+    oka_grid grid;           /* the site's fixed lattice                          */
+} oka_batch_site;
+
+typedef struct oka_batch oka_batch;
+
+/* tune: the pool knobs (pool_slots, pool_waves_per_cu, pool_chunk) or NULL. */
+int oka_batch_create(oka_ctx *ctx, int32_t nsites, const oka_batch_site *sites,
+                     const oka_trace_opts *tune, oka_batch **out);
+/* Re-captures the run with other pool knobs; the staged sites are kept. */
+int oka_batch_tune(oka_batch *batch, const oka_trace_opts *tune);
+/* Replays the captured run (counts start from zero); *ms (may be NULL)
+ * receives its device time. */
+int oka_batch_run(oka_batch *batch, float *ms);
+/* Site s after the last run: counts (nrows x ncols), coef (nreal x 6) and
+ * stats, each may be NULL.  Returns OKA_ERR_RANGE or OKA_ERR_LINALG (after
+ * copying) when a realization's fit failed, as oka_fit_draw would. */
+int oka_batch_results(oka_batch *batch, int32_t s, int32_t *counts, double *coef,
+                      oka_trace_stats *stats);
+void oka_batch_destroy(oka_batch *batch);
+
 /* ---- expansion history (probabilityfield.py:175-261 applied per trace in
  *      order, capturezone.py:117-120) -- host computation ---------------------
  * Given the field before the batch (init; init->nrows == 0 means an empty

@@ -39,7 +39,8 @@ EXPORTED = (
     "oka_rasterize", "oka_pow_fifth", "oka_log", "oka_trace_marks", "oka_mlp_create",
     "oka_mlp_destroy", "oka_mlp_param_count", "oka_mlp_set_params", "oka_mlp_get_params",
     "oka_mlp_grad", "oka_mlp_train", "oka_mlp_predict", "oka_mlp_accumulate",
-    "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_mark_moments",
+    "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_mark_moments", "oka_batch_create",
+    "oka_batch_tune", "oka_batch_run", "oka_batch_results", "oka_batch_destroy",
 )
 
 DIST_CONST, DIST_UNIFORM, DIST_TRIANGULAR, DIST_LOGNORMAL = 0, 1, 2, 3
@@ -87,6 +88,11 @@ class DeviceInfo_(C.Structure):
 class Grid_(C.Structure):
     _fields_ = [("xmin", _d), ("ymin", _d), ("xmax", _d), ("ymax", _d), ("dx", _d), ("dy", _d),
                 ("nrows", _i32), ("ncols", _i32)]
+
+
+class BatchSite_(C.Structure):
+    _fields_ = [("site", C.POINTER(Site_)), ("nreal", _i32), ("K", _vp), ("n", _vp), ("b", _vp),
+                ("q", _vp), ("z", _vp), ("opts", TraceOpts_), ("grid", Grid_)]
 
 
 class Epoch_(C.Structure):
@@ -189,6 +195,16 @@ def load():
     L.oka_cv_surrogate_sums.restype = C.c_int
     L.oka_cv_surrogate_sums.argtypes = [_vp, _vp, C.POINTER(Site_), _vp, _i64, C.c_uint64, _i64,
                                         _vp, _vp, _vp, _vp, _u32]
+    L.oka_batch_create.restype = C.c_int
+    L.oka_batch_create.argtypes = [_vp, _i32, _vp, C.POINTER(TraceOpts_), C.POINTER(_vp)]
+    L.oka_batch_tune.restype = C.c_int
+    L.oka_batch_tune.argtypes = [_vp, C.POINTER(TraceOpts_)]
+    L.oka_batch_run.restype = C.c_int
+    L.oka_batch_run.argtypes = [_vp, C.POINTER(C.c_float)]
+    L.oka_batch_results.restype = C.c_int
+    L.oka_batch_results.argtypes = [_vp, _i32, _vp, _vp, C.POINTER(TraceStats_)]
+    L.oka_batch_destroy.restype = None
+    L.oka_batch_destroy.argtypes = [_vp]
     L.oka_mark_moments.restype = C.c_int
     L.oka_mark_moments.argtypes = [_vp, _i64, _i32, _vp, _vp, _vp, _u32]
     L.oka_pow_fifth.restype = _d

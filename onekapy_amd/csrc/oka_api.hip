@@ -127,6 +127,103 @@ int fill_u32v(oka_ctx *c, void *p, uint32_t value, int64_t nwords)
 int fill_u32(oka_ctx *c, void *p, int64_t nwords) { return fill_u32v(c, p, 0u, nwords); }
 
 
+
+int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opts *o,
+               PoolShape *out)
+{
+    PoolShape &sh = *out;
+    sh.chunk = o->pool_chunk > 0 ? o->pool_chunk : 64;
+    sh.nsites = nsites;
+    // first[]: queue items, then realizations, per site (2 x (nsites + 1))
+    sh.first.assign(2 * ((size_t)nsites + 1), 0);
+    int64_t total = 0, nreal = 0, chunked = 0;
+    sh.max_nw = 1;
+    sh.max_tiles = 1;
+    for (int i = 0; i < nsites; ++i) {
+        const int nch = (sites[i].npaths + sh.chunk - 1) / sh.chunk;
+        sh.first[i] = total;
+        sh.first[nsites + 1 + i] = nreal;
+        total += (int64_t)sites[i].nreal * nch;
+        nreal += sites[i].nreal;
+        if (nch > 1)
+            chunked += sites[i].nreal;
+        sh.max_nw = std::max(sh.max_nw, (int)sites[i].nwells);
+        if (sites[i].rasterize)
+            sh.max_tiles = std::max(sh.max_tiles, (int)sites[i].ntiles);
+        if (!sites[i].confined)
+            return fail(c, OKA_ERR_ARG, "the pool schedule traces the confined velocity only");
+    }
+    sh.first[nsites] = total;
+    sh.first[2 * (size_t)nsites + 1] = nreal;
+    sh.total = total;
+    if (sh.max_tiles > pool_max_tiles())
+        return fail(c, OKA_ERR_ARG, "grid too large for the pool schedule (%d tiles)",
+                    sh.max_tiles);
+    sh.slots = o->pool_slots > 0 ? std::min(o->pool_slots, 16) : 4;
+    sh.wpc = o->pool_waves_per_cu > 0 ? std::min(o->pool_waves_per_cu, 32) : 8;
+    const size_t budget = std::min(c->lds_per_cu / sh.wpc, c->max_lds);
+    const size_t fixed = pool_lds_bytes(sh.slots, sh.max_nw, 0);
+    if (fixed > budget)
+        return fail(c, OKA_ERR_ARG, "%d wells x %d slots do not fit in LDS at %d waves per CU",
+                    sh.max_nw, sh.slots, sh.wpc);
+    sh.pool_tiles = (int)std::min<size_t>((budget - fixed) / ((size_t)sh.slots * 128),
+                                          pool_max_pool_tiles());
+    sh.waves = c->num_cu * sh.wpc;
+    sh.hbm_words = (int64_t)sh.waves * sh.slots * sh.max_tiles * 32;
+    // realization records: enough for every chunked realization in flight (a
+    // chunk waits while an older realization holds its record), at most 8 GiB;
+    // per record: owner (2 words), done and a pad word, the tiles, dirty bits
+    sh.dirty_words = (sh.max_tiles + 31) / 32;
+    const int64_t per_rec = (int64_t)sh.max_tiles * 32 + sh.dirty_words + 4;
+    sh.nrec = 0;
+    if (chunked > 0) {
+        sh.nrec = std::min<int64_t>(chunked, std::max<int64_t>(2LL * sh.waves * sh.slots, 64));
+        sh.nrec = std::max<int64_t>(1, std::min<int64_t>(sh.nrec, (2LL << 30) / per_rec));
+    }
+    sh.rec_words = sh.nrec * per_rec;
+    return OKA_OK;
+}
+
+PoolArgs pool_args(const PoolShape &sh, void *sites, void *first, void *queue, void *hbm,
+                   void *rec)
+{
+    PoolArgs p{};
+    p.sites = (const TraceArgs *)sites;
+    p.nsites = sh.nsites;
+    p.site_first = (const int64_t *)first;
+    p.real_first = (const int64_t *)first + sh.nsites + 1;
+    p.total = sh.total;
+    p.queue = (unsigned long long *)queue;
+    p.hbm = (uint32_t *)hbm;
+    p.slots = sh.slots;
+    p.max_nwells = sh.max_nw;
+    p.pool_tiles = sh.pool_tiles;
+    p.max_ntiles = sh.max_tiles;
+    p.chunk = sh.chunk;
+    p.dirty_words = sh.dirty_words;
+    p.nrec = std::max<int64_t>(sh.nrec, 1);
+    if (sh.nrec > 0) {
+        p.rec_owner = (int64_t *)rec;
+        p.rec_done = (int32_t *)(p.rec_owner + sh.nrec);
+        p.rec_bits = (uint32_t *)(p.rec_done + 2 * sh.nrec);
+        p.rec_dirty = p.rec_bits + sh.nrec * (int64_t)sh.max_tiles * 32;
+    }
+    return p;
+}
+
+int pool_launch(oka_ctx *c, const PoolShape &sh, const PoolArgs &p)
+{
+    TRY(fill_u32(c, p.queue, 2));
+    TRY(fill_u32(c, p.hbm, sh.hbm_words));
+    if (sh.nrec > 0) {
+        TRY(fill_u32v(c, p.rec_owner, 0xffffffffu, 2 * sh.nrec));
+        TRY(fill_u32(c, p.rec_done, sh.rec_words - 2 * sh.nrec));
+    }
+    if (sh.total > 0)
+        HIPCHK(c, launch_trace_pool32(p, sh.waves, c->stream));
+    return OKA_OK;
+}
+
 }  // namespace oka
 
 namespace {
@@ -493,98 +590,27 @@ static int chronicle_finish(oka_ctx *c, const TraceArgs &a, int32_t rasterize, i
                      const oka_window *win, uint8_t *marks);
 
 // Launches the persistent pool tracer over `nsites` sites (host copies of
-// their argument blocks, with device pointers inside); times it with the
-// context events when `timed`.
+// their argument blocks, with device pointers inside), with its buffers in the
+// context's slots; times it with the context events when `timed`.
 int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opts *o, bool timed)
 {
-    const int chunk = o->pool_chunk > 0 ? o->pool_chunk : 64;
-    std::vector<int64_t> first((size_t)nsites + 1), rfirst((size_t)nsites + 1);
-    int64_t total = 0, nreal = 0, chunked = 0;
-    int max_nw = 1, max_tiles = 1;
-    for (int i = 0; i < nsites; ++i) {
-        const int nch = (sites[i].npaths + chunk - 1) / chunk;
-        first[i] = total;
-        rfirst[i] = nreal;
-        total += (int64_t)sites[i].nreal * nch;
-        nreal += sites[i].nreal;
-        if (nch > 1)
-            chunked += sites[i].nreal;
-        max_nw = std::max(max_nw, (int)sites[i].nwells);
-        if (sites[i].rasterize)
-            max_tiles = std::max(max_tiles, (int)sites[i].ntiles);
-        if (!sites[i].confined)
-            return fail(c, OKA_ERR_ARG, "the pool schedule traces the confined velocity only");
-    }
-    first[nsites] = total;
-    rfirst[nsites] = nreal;
-    if (max_tiles > pool_max_tiles())
-        return fail(c, OKA_ERR_ARG, "grid too large for the pool schedule (%d tiles)", max_tiles);
-    const int slots = o->pool_slots > 0 ? std::min(o->pool_slots, 16) : 4;
-    const int wpc = o->pool_waves_per_cu > 0 ? std::min(o->pool_waves_per_cu, 32) : 8;
-    const size_t budget = std::min(c->lds_per_cu / wpc, c->max_lds);
-    const size_t fixed = pool_lds_bytes(slots, max_nw, 0);
-    if (fixed > budget)
-        return fail(c, OKA_ERR_ARG, "%d wells x %d slots do not fit in LDS at %d waves per CU",
-                    max_nw, slots, wpc);
-    const int pool_tiles =
-        (int)std::min<size_t>((budget - fixed) / ((size_t)slots * 128), pool_max_pool_tiles());
-    const int waves = c->num_cu * wpc;
-    // realization records: enough for every chunked realization in flight
-    // (a chunk waits while an older realization holds its record), at most 8 GiB
-    const int dirty_words = (max_tiles + 31) / 32;
-    const int64_t rec_bytes = (int64_t)max_tiles * 128 + dirty_words * 4 + 12;
-    int64_t nrec = 0;
-    if (chunked > 0) {
-        nrec = std::min<int64_t>(chunked, std::max<int64_t>(2LL * waves * slots, 64));
-        nrec = std::max<int64_t>(1, std::min<int64_t>(nrec, (8LL << 30) / rec_bytes));
-    }
-    void *p_sites, *p_first, *p_rfirst, *p_queue, *p_hbm, *p_rec = nullptr;
+    PoolShape sh;
+    TRY(pool_shape(c, sites, nsites, o, &sh));
+    void *p_sites, *p_first, *p_queue, *p_hbm, *p_rec = nullptr;
     TRY(ensure(c, kSlotPoolSites, sizeof(TraceArgs) * nsites, &p_sites));
-    TRY(ensure(c, kSlotPoolFirst, sizeof(int64_t) * 2 * first.size(), &p_first));
-    p_rfirst = (int64_t *)p_first + first.size();
+    TRY(ensure(c, kSlotPoolFirst, sizeof(int64_t) * sh.first.size(), &p_first));
     TRY(ensure(c, kSlotPoolQueue, sizeof(unsigned long long), &p_queue));
-    const int64_t hbm_words = (int64_t)waves * slots * max_tiles * 32;
-    TRY(ensure(c, kSlotPoolHbm, sizeof(uint32_t) * hbm_words, &p_hbm));
-    // per record: owner (2 words), done and a pad word, the tiles, the dirty bits
-    const int64_t rec_words = nrec * ((int64_t)max_tiles * 32 + dirty_words + 4);
-    if (nrec > 0)
-        TRY(ensure(c, kSlotPoolRec, sizeof(uint32_t) * rec_words, &p_rec));
+    TRY(ensure(c, kSlotPoolHbm, sizeof(uint32_t) * sh.hbm_words, &p_hbm));
+    if (sh.rec_words > 0)
+        TRY(ensure(c, kSlotPoolRec, sizeof(uint32_t) * sh.rec_words, &p_rec));
     HIPCHK(c, hipMemcpyAsync(p_sites, sites, sizeof(TraceArgs) * nsites, hipMemcpyHostToDevice,
                              c->stream));
-    HIPCHK(c, hipMemcpyAsync(p_first, first.data(), sizeof(int64_t) * first.size(),
+    HIPCHK(c, hipMemcpyAsync(p_first, sh.first.data(), sizeof(int64_t) * sh.first.size(),
                              hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(p_rfirst, rfirst.data(), sizeof(int64_t) * rfirst.size(),
-                             hipMemcpyHostToDevice, c->stream));
-    TRY(fill_u32(c, p_queue, 2));
-    TRY(fill_u32(c, p_hbm, hbm_words));
-    PoolArgs p{};
-    if (nrec > 0) {
-        // [owner int64][done int32][pad int32][bits][dirty]
-        p.rec_owner = (int64_t *)p_rec;
-        p.rec_done = (int32_t *)(p.rec_owner + nrec);
-        p.rec_bits = (uint32_t *)(p.rec_done + 2 * nrec);
-        p.rec_dirty = p.rec_bits + nrec * (int64_t)max_tiles * 32;
-        TRY(fill_u32v(c, p.rec_owner, 0xffffffffu, 2 * nrec));
-        TRY(fill_u32(c, p.rec_done, 2 * nrec + nrec * ((int64_t)max_tiles * 32 + dirty_words)));
-    }
-    p.sites = (const TraceArgs *)p_sites;
-    p.nsites = nsites;
-    p.site_first = (const int64_t *)p_first;
-    p.real_first = (const int64_t *)p_rfirst;
-    p.total = total;
-    p.queue = (unsigned long long *)p_queue;
-    p.hbm = (uint32_t *)p_hbm;
-    p.slots = slots;
-    p.max_nwells = max_nw;
-    p.pool_tiles = pool_tiles;
-    p.max_ntiles = max_tiles;
-    p.chunk = chunk;
-    p.dirty_words = dirty_words;
-    p.nrec = std::max<int64_t>(nrec, 1);
+    const PoolArgs p = pool_args(sh, p_sites, p_first, p_queue, p_hbm, p_rec);
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if (total > 0)
-        HIPCHK(c, launch_trace_pool32(p, waves, c->stream));
+    TRY(pool_launch(c, sh, p));
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     return OKA_OK;

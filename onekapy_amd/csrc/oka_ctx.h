@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "oka_internal.h"
 
@@ -52,6 +53,23 @@ int stage_out(oka_ctx *c, Slot s, void *dst, size_t bytes, uint32_t flags, bool 
 int copy_back(oka_ctx *c, void *dst, const void *dev, size_t bytes, uint32_t flags);
 int fill_u32(oka_ctx *c, void *p, int64_t nwords);
 int fill_u32v(oka_ctx *c, void *p, uint32_t value, int64_t nwords);
+
+// Shape of a persistent pool launch (oka_pool.hip) over a table of sites.
+struct PoolShape {
+    int nsites = 0, chunk = 64, slots = 4, wpc = 8, waves = 0, pool_tiles = 0;
+    int max_nw = 1, max_tiles = 1, dirty_words = 1;
+    int64_t total = 0, nrec = 0, hbm_words = 0, rec_words = 0;
+    std::vector<int64_t> first;   // items, then realizations, per site: 2 x (nsites + 1)
+};
+int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opts *o,
+               PoolShape *out);
+// Device buffers: sites [nsites] TraceArgs, first [2 (nsites + 1)] int64
+// (uploaded by the caller), queue (8 bytes), hbm [hbm_words], rec [rec_words].
+PoolArgs pool_args(const PoolShape &sh, void *sites, void *first, void *queue, void *hbm,
+                   void *rec);
+// Clears the launch's queue, bitmaps and records and launches it: kernel
+// launches only, so a caller may capture it in a graph.
+int pool_launch(oka_ctx *c, const PoolShape &sh, const PoolArgs &p);
 
 }  // namespace oka
 

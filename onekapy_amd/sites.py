@@ -156,6 +156,28 @@ def config4(nreal=1000000):
                  200, 256, 30, nreal, 10.0, 2000)
 
 
+def config2(nsites=64, scale=1.0, seed=20260002):
+    """64 independent sites: wells U{1..200} (the target first; q ~ U(100,
+    1500)), observations U{10..100}, a square lattice of U{100..800} nodes a
+    side at 20 m, realizations U{100..5000} (times `scale`, at least 1) x 256
+    particles, backtraces of U(1, 50) years.  Each site's domain grows with its
+    well count (8 km for 25 wells, at least 4 km) so that wells stay 50 m and
+    observations 100 m apart."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nsites):
+        nw = int(rng.integers(1, 201))
+        nobs = int(rng.integers(10, 101))
+        nside = int(rng.integers(100, 801))
+        nreal = max(1, int(round(int(rng.integers(100, 5001)) * scale)))
+        years = float(rng.uniform(1.0, 50.0))
+        side = max(4000.0, 8000.0 * math.sqrt(nw / 25.0))
+        out.append(_make(f"cfg2 site {k}: {nw} wells, {nobs} obs, {nreal} x 256, {years:.1f} y, "
+                         f"{nside}^2 @ 20 m", seed * 100 + k, (0.0, side), nw,
+                         lambda i: (100.0, 1500.0), nobs, 256, years, nreal, 20.0, nside))
+    return out
+
+
 def draw(site, nreal=None, seed=0):
     """Seeded realization parameters for a site: dict(q, K, n, b, z)."""
     from .stochastic import draw_realizations

@@ -1,0 +1,75 @@
+"""Many sites in one graph-captured run (oka_batch_*) against per-site runs,
+and the autotuner of its launch shape."""
+import numpy as np
+import pytest
+
+from onekapy_amd import _lib, fit_and_draw, sites
+from onekapy_amd.autotune import cached, tune_batch
+from onekapy_amd.batch import SiteBatch
+from onekapy_amd.surrogate import trace_counts
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_batch():
+    ss = sites.config2(nsites=6, scale=0.004, seed=7)
+    draws = [sites.draw(s, seed=11 + i) for i, s in enumerate(ss)]
+    return ss, draws
+
+
+def test_batch_matches_per_site_runs(gpu_ctx):
+    ss, draws = _small_batch()
+    b = SiteBatch(ss, draws)
+    b.run()
+    for i, (s, d) in enumerate(zip(ss, draws)):
+        print(f"site {i}: {s.name}", flush=True)
+        counts, coef, st = b.results(i)
+        _, _, ref_coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"],
+                                      d["b"], d["q"], d["z"])
+        assert np.array_equal(coef, ref_coef)
+        # the same tracer, one site at a time, on the site's fixed lattice (the
+        # batch drops what falls outside it, as a pre-expanded field does)
+        ref_counts, ref_st = trace_counts(s, d, ref_coef, _lib.FP32)
+        assert np.array_equal(counts, ref_counts)
+        assert (st.attempted, st.accepted, st.visited, st.escaped) == (
+            ref_st.attempted, ref_st.accepted, ref_st.visited, ref_st.escaped)
+        assert counts.max() <= len(d["K"])
+    fields = b.fields()
+    assert fields[0].total_weight == len(draws[0]["K"])
+    b.close()
+
+
+def test_batch_replays_and_retunes_identically(gpu_ctx):
+    ss, draws = _small_batch()
+    b = SiteBatch(ss, draws)
+    b.run()
+    first = [b.results(i)[0] for i in range(len(ss))]
+    b.run()                                   # counts restart from zero on every replay
+    for i in range(len(ss)):
+        assert np.array_equal(b.results(i)[0], first[i])
+    for knobs in (dict(pool_chunk=16, pool_slots=2), dict(pool_chunk=256, pool_waves_per_cu=4)):
+        print(f"knobs {knobs}", flush=True)
+        b.tune(**knobs)
+        b.run()
+        for i in range(len(ss)):
+            assert np.array_equal(b.results(i)[0], first[i])
+    b.close()
+
+
+def test_autotuner_picks_a_fitting_shape_and_caches_it(gpu_ctx, tmp_path):
+    ss, draws = _small_batch()
+    b = SiteBatch(ss, draws)
+    path = str(tmp_path / "tune.json")
+    search = (("pool_chunk", (64, 256)), ("pool_slots", (2, 16)))
+    best, trials = tune_batch(b, cache_path=path, search=search)
+    times = [ms for _, ms in trials if ms is not None]
+    assert times and min(times) > 0
+    assert cached(b, path)["pool_chunk"] == best["pool_chunk"]
+    # the tuned batch still gives the same counts
+    b.run()
+    ref = SiteBatch(ss, draws)
+    ref.run()
+    for i in range(len(ss)):
+        assert np.array_equal(b.results(i)[0], ref.results(i)[0])
+    b.close()
+    ref.close()
