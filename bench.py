@@ -48,7 +48,17 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nreal", type=int, default=10000)
-    ap.add_argument("--config", default="cfg1", choices=["cfg0", "cfg1"])
+    ap.add_argument("--config", default="cfg1", choices=["cfg0", "cfg1", "cfg2", "cfg4"],
+                    help="cfg1 (default): the single-site headline workload; cfg2: 64 uneven "
+                         "sites in one graph-captured pool launch; cfg4: the 200-well "
+                         "2000^2 stress site on a realization subset")
+    ap.add_argument("--cfg2-scale", type=float, default=1.0,
+                    help="cfg2: fraction of each site's realizations")
+    ap.add_argument("--cfg4-nreal", type=int, default=20000,
+                    help="cfg4: realizations traced (of the workload's 10^6)")
+    ap.add_argument("--cfg4-tune-nreal", type=int, default=6000,
+                    help="cfg4: realizations of the autotuning runs (small runs are bound by "
+                         "their single longest trace)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU time budget of each CPU baseline")
@@ -180,8 +190,9 @@ class DeviceRun:
         self.site_s.nobs = self.obs.shape[0]
         torch.cuda.synchronize()
 
-    def step(self, precision):
-        """fit + draw, then trace + chronicle; returns the trace launch's stats."""
+    def step(self, precision, **sched):
+        """fit + draw, then trace + chronicle; returns the trace launch's stats.
+        sched: scheduling options of oka_trace_opts (schedule, pool_*)."""
         L, c = self.lib, self.ctx
         self.counts.zero_()
         c.check(c.lib.oka_fit_draw(c.h, self.site_s, self.R, self.K.data_ptr(), self.b.data_ptr(),
@@ -190,7 +201,7 @@ class DeviceRun:
                                    L.PTR_DEVICE | L.ASYNC))
         s = self.site
         opts = L.make_trace_opts(s.npaths, s.duration, s.tol, s.maxstep, s.umbra,
-                                 self.starts.data_ptr(), precision, True)
+                                 self.starts.data_ptr(), precision, True, **sched)
         stats = L.TraceStats_()
         c.check(c.lib.oka_trace_chronicle(c.h, self.site_s, self.R, self.K.data_ptr(),
                                           self.n.data_ptr(), self.b.data_ptr(), self.q.data_ptr(),
@@ -200,14 +211,14 @@ class DeviceRun:
         return stats
 
 
-def time_mode(run, precision, steps, warmup, torch):
+def time_mode(run, precision, steps, warmup, torch, **sched):
     for _ in range(warmup):
-        run.step(precision)
+        run.step(precision, **sched)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     kms, st = [], None
     for _ in range(steps):
-        st = run.step(precision)
+        st = run.step(precision, **sched)
         kms.append(st.kernel_ms)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
@@ -274,6 +285,163 @@ def cpu_baselines(site, d, coef, budget_s):
     return out
 
 
+def oracle_sample_rate(site, d, coef, budget_s, max_real=None):
+    """The C FP64 oracle (OpenMP, all host threads) on the first realizations
+    of a site, growing the sample until it takes a sizeable part of budget_s."""
+    import oracle as O
+    from onekapy_amd.capturezone import start_points
+    pf = site.fixed_field()
+    starts = start_points(site.xt, site.yt, site.rt, site.npaths)
+    g = O.make_grid(pf.xmin, pf.ymin, pf.xmax, pf.ymax, pf.deltax, pf.deltay, pf.nrows, pf.ncols)
+    opts = O.trace_opts(site.duration, site.tol, site.maxstep, True, O.MATH_GLIBC)
+    threads = O.max_threads()
+    cap = len(d["K"]) if max_real is None else min(max_real, len(d["K"]))
+    nr = min(cap, max(1, threads // 4))
+    while True:
+        t0 = time.perf_counter()
+        _, st = O.chronicle_fixed(site.wells_xy(), site.xt, site.yt, site.base, d["K"][:nr],
+                                  d["n"][:nr], d["b"][:nr], d["q"][:nr], coef[:nr], starts,
+                                  site.umbra, opts, g, nthreads=threads)
+        el = time.perf_counter() - t0
+        if el > 0.3 * budget_s or nr >= cap:
+            return nr, el, int(st[0]), threads
+        nr = min(cap, int(nr * max(2.0, 0.5 * budget_s / max(el, 1e-3))))
+
+
+def canonical_flops(stats, nwells):
+    return stats.attempted * (207 + 60 * nwells) + 10 * stats.accepted + 14 * stats.candidates
+
+
+def bench_cfg2(args):
+    """64 uneven sites in one persistent, graph-captured pool launch (autotuned)."""
+    from onekapy_amd import _lib, sites
+    from onekapy_amd.autotune import tune_batch
+    from onekapy_amd.batch import SiteBatch
+    from onekapy_amd.surrogate import trace_counts
+    log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
+    small = sites.config2(scale=0.05)
+    sdraws = [sites.draw(s, seed=100 + i) for i, s in enumerate(small)]
+    tb = SiteBatch(small, sdraws)
+    log("cfg2: autotuning the pool launch on a 5 % copy")
+    best, trials = tune_batch(tb, log=log)
+    tb.close()
+    ss = sites.config2(scale=args.cfg2_scale)
+    draws = [sites.draw(s, seed=100 + i) for i, s in enumerate(ss)]
+    b = SiteBatch(ss, draws, knobs=best)
+    R = sum(b.nreal)
+    log(f"cfg2: 64 sites, {R} realizations x 256 particles: {args.warmup} warmup + "
+        f"{args.steps} timed replays")
+    for _ in range(args.warmup):
+        b.run()
+    t0 = time.perf_counter()
+    kms = [b.run() for _ in range(args.steps)]
+    dt = time.perf_counter() - t0
+    res = [b.results(i) for i in range(len(ss))]
+    att = sum(r[2].attempted for r in res)
+    flops = sum(canonical_flops(r[2], len(s.wells)) for r, s in zip(res, ss))
+    kernel_ms = float(np.mean(kms))
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    # FP32 (batch) against the FP64-exact tracer on the first 64 realizations of 6 sites
+    errs = []
+    for i in range(6):
+        n = min(64, b.nreal[i])
+        dd = {k: v[:n] for k, v in draws[i].items()}
+        coef = res[i][1][:n]
+        c64, _ = trace_counts(ss[i], dd, coef, _lib.FP64_EXACT)
+        c32, _ = trace_counts(ss[i], dd, coef, _lib.FP32)
+        errs.append(np.abs(c32 - c64) / n)
+    err_max = float(max(e.max() for e in errs))
+    err_l1 = float(sum(e.sum() for e in errs))
+    cpu = None
+    if not args.no_cpu:
+        log("cfg2: CPU baseline (C oracle) on site 0")
+        nr, el, st0, threads = oracle_sample_rate(ss[0], draws[0], res[0][1], args.cpu_seconds)
+        cpu = {"value": nr / el, "unit": "realizations/s", "cores": threads, "kind": "port",
+               "particle_steps_per_s": st0 / el,
+               "sample": f"site 0 ({len(ss[0].wells)} wells), first {nr} realizations x 256 "
+                         f"particles ({st0} attempted steps, {el:.1f} s), C FP64 oracle, OpenMP"}
+    peak = PEAK_TFLOPS["fp32"]
+    return {
+        "metric": "realizations/sec and particle-steps/sec at max capture-prob error <=0.01 vs FP64",
+        "value": R * args.steps / dt, "unit": "realizations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded sites and realization parameters)",
+        "config": {"workload": "cfg2: 64 sites, wells U{1..200}, obs U{10..100}, grids "
+                               "U{100..800}^2 @ 20 m, realizations U{100..5000} x 256, "
+                               "backtraces U(1, 50) y", "sites": len(ss), "realizations": R,
+                   "particles": 256, "scale": args.cfg2_scale, "autotuned": best},
+        "particle_steps_per_s": att * args.steps / dt,
+        "error_vs_fp64": {"max": err_max, "l1": err_l1,
+                          "sample": "first 64 realizations of sites 0-5, FP64-exact tracer"},
+        "autotune": [{"knobs": k, "ms": ms} for k, ms in trials],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "pool tracer (fp32), whole batch", "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_cfg4(args, torch):
+    """The 200-well, 2000^2 stress site on a realization subset, FP32 pool tracer."""
+    from onekapy_amd import _lib, sites
+    from onekapy_amd.autotune import tune_batch
+    from onekapy_amd.batch import SiteBatch
+    from onekapy_amd.surrogate import trace_counts
+    log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
+    site = sites.config4(args.cfg4_nreal)
+    d = sites.draw(site, seed=4)
+    small = sites.config4(args.cfg4_tune_nreal)
+    tb = SiteBatch([small], [sites.draw(small, seed=40)])
+    log(f"cfg4: autotuning the pool launch on {args.cfg4_tune_nreal} realizations")
+    best, trials = tune_batch(tb, log=log)
+    tb.close()
+    run = DeviceRun(site, d, torch)
+    log(f"cfg4: {args.cfg4_nreal} realizations: {args.warmup} warmup + {args.steps} timed steps")
+    m = time_mode(run, _lib.FP32, args.steps, args.warmup, torch, schedule=_lib.SCHED_POOL,
+                  **best)
+    R = run.R
+    st = m["stats"]
+    flops = canonical_flops(st, len(site.wells))
+    achieved = flops / (m["kernel_ms"] * 1e-3) / 1e12
+    # FP32 against FP64-exact on the first 1000 realizations
+    n = min(1000, R)
+    coef = run.coef.cpu().numpy()
+    dd = {k: v[:n] for k, v in d.items()}
+    log(f"cfg4: FP64-exact subset of {n} realizations")
+    c64, st64 = trace_counts(site, dd, coef[:n], _lib.FP64_EXACT)
+    c32, _ = trace_counts(site, dd, coef[:n], _lib.FP32)
+    err = np.abs(c32 - c64) / n
+    cpu = None
+    if not args.no_cpu:
+        log("cfg4: CPU baseline (C oracle)")
+        nr, el, st0, threads = oracle_sample_rate(site, d, coef, args.cpu_seconds, max_real=64)
+        cpu = {"value": nr / el, "unit": "realizations/s", "cores": threads, "kind": "port",
+               "particle_steps_per_s": st0 / el,
+               "sample": f"first {nr} realizations x 256 particles ({st0} attempted steps, "
+                         f"{el:.1f} s), C FP64 oracle, OpenMP"}
+    peak = PEAK_TFLOPS["fp32"]
+    return {
+        "metric": "realizations/sec and particle-steps/sec at max capture-prob error <=0.01 vs FP64",
+        "value": R * args.steps / m["seconds"], "unit": "realizations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded site and realization parameters)",
+        "config": {"workload": site.name, "realizations": R, "of": 1000000, "particles": 256,
+                   "wells": len(site.wells), "grid": [site.nrows, site.ncols],
+                   "schedule": "pool", "autotuned": best},
+        "particle_steps_per_s": st.attempted * args.steps / m["seconds"],
+        "status_count": [int(v) for v in st.status_count],
+        "error_vs_fp64": {"max": float(err.max()), "l1": float(err.sum()),
+                          "sample": f"first {n} realizations, FP64-exact tracer"},
+        "autotune": [{"knobs": k, "ms": ms} for k, ms in trials],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "pool tracer (fp32)", "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -288,6 +456,12 @@ def main():
     from onekapy_amd import sites
     from onekapy_amd import _lib
 
+    if args.config == "cfg2":
+        print(json.dumps(bench_cfg2(args)))
+        return
+    if args.config == "cfg4":
+        print(json.dumps(bench_cfg4(args, torch)))
+        return
     site = sites.config1(args.nreal) if args.config == "cfg1" else sites.config0()
     if args.only_cv:
         cv = control_variate_leg(site, args, None)
