@@ -380,6 +380,14 @@ int oka_draw_realizations(oka_ctx *ctx, int32_t nwells, const oka_dist *dists, i
 int oka_mark_moments(oka_ctx *ctx, int64_t n, int32_t ncell, const uint8_t *marks, uint64_t *sums,
                      uint64_t *sums_sq, uint32_t flags);
 
+/* Histogram of a count grid (probabilityfield.py pgrid with weight 1:
+ * realizations that marked each node), for the impact curve and deciles of
+ * visualize.py:382-385 and oneka.py:279-288: hist[k] += number of the n
+ * counts equal to k, 0 <= k <= max_count.  Counts outside [0, max_count] are
+ * an error (OKA_ERR_RANGE).  counts and hist follow flags. */
+int oka_count_histogram(oka_ctx *ctx, int64_t n, const int32_t *counts, int32_t max_count,
+                        int64_t *hist, uint32_t flags);
+
 /* The surrogate term of the control variate for n fresh realizations of the
  * stream `seed` (starting at realization `first`), computed entirely on the
  * device in chunks: draws, fit + MVN draw (as oka_fit_draw), standardized

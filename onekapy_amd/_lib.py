@@ -41,6 +41,7 @@ EXPORTED = (
     "oka_mlp_grad", "oka_mlp_train", "oka_mlp_predict", "oka_mlp_accumulate",
     "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_mark_moments", "oka_batch_create",
     "oka_batch_tune", "oka_batch_run", "oka_batch_results", "oka_batch_destroy",
+    "oka_count_histogram",
 )
 
 DIST_CONST, DIST_UNIFORM, DIST_TRIANGULAR, DIST_LOGNORMAL = 0, 1, 2, 3
@@ -205,6 +206,8 @@ def load():
     L.oka_batch_results.argtypes = [_vp, _i32, _vp, _vp, C.POINTER(TraceStats_)]
     L.oka_batch_destroy.restype = None
     L.oka_batch_destroy.argtypes = [_vp]
+    L.oka_count_histogram.restype = C.c_int
+    L.oka_count_histogram.argtypes = [_vp, _i64, _vp, _i32, _vp, _u32]
     L.oka_mark_moments.restype = C.c_int
     L.oka_mark_moments.argtypes = [_vp, _i64, _i32, _vp, _vp, _vp, _u32]
     L.oka_pow_fifth.restype = _d

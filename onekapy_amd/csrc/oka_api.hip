@@ -106,6 +106,29 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n)
         p[i] = v;
 }
 
+// hist[k] += counts equal to k.  The empty nodes (usually most of a grid) are
+// counted per lane and added once per wave; *bad is set when a count lies
+// outside [0, max_count].
+__global__ void k_count_histogram(const int32_t *counts, int64_t n, int32_t max_count,
+                                  unsigned long long *hist, int32_t *bad)
+{
+    unsigned long long zeros = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = counts[i];
+        if (v < 0 || v > max_count)
+            atomicOr(bad, 1);
+        else if (v == 0)
+            zeros += 1;
+        else
+            atomicAdd(&hist[v], 1ull);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        zeros += __shfl_xor(zeros, off);
+    if ((threadIdx.x & 63) == 0 && zeros)
+        atomicAdd(&hist[0], zeros);
+}
+
 __global__ void k_bits_to_u8(const uint32_t *bits, uint8_t *out, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -254,6 +277,34 @@ constexpr int64_t kDefaultMaxSteps = 1000000;
 extern "C" {
 
 int oka_abi_version(void) { return OKA_ABI_VERSION; }
+
+int oka_count_histogram(oka_ctx *c, int64_t n, const int32_t *counts, int32_t max_count,
+                        int64_t *hist, uint32_t flags)
+{
+    if (!c || !hist || n < 0 || max_count < 0 || (n > 0 && !counts))
+        return OKA_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    const void *dc;
+    TRY(stage_in(c, kSlotCounts, counts, sizeof(int32_t) * n, flags, &dc));
+    void *dh, *db;
+    TRY(stage_out(c, kSlotOut, hist, sizeof(int64_t) * ((size_t)max_count + 1), flags, true, &dh));
+    TRY(ensure(c, kSlotEStatus, sizeof(int32_t), &db));
+    TRY(fill_u32(c, db, 1));
+    if (n > 0) {
+        const int64_t blocks = std::min<int64_t>((n + 255) / 256, (int64_t)c->num_cu * 16);
+        hipLaunchKernelGGL(k_count_histogram, dim3((unsigned)blocks), dim3(256), 0, c->stream,
+                           (const int32_t *)dc, n, max_count, (unsigned long long *)dh,
+                           (int32_t *)db);
+        HIPCHK(c, hipGetLastError());
+    }
+    int32_t bad = 0;
+    HIPCHK(c, hipMemcpyAsync(&bad, db, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+    TRY(copy_back(c, hist, dh, sizeof(int64_t) * ((size_t)max_count + 1), flags));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bad)
+        return fail(c, OKA_ERR_RANGE, "a count lies outside [0, %d]", max_count);
+    return OKA_OK;
+}
 
 int oka_ctx_device_info(oka_ctx *c, oka_device_info *out)
 {
