@@ -56,6 +56,9 @@ def parse():
                     help="cfg2: fraction of each site's realizations")
     ap.add_argument("--cfg4-nreal", type=int, default=20000,
                     help="cfg4: realizations traced (of the workload's 10^6)")
+    ap.add_argument("--knobs", default=None,
+                    help="cfg2/cfg4: pool launch shape SLOTS,WAVES_PER_CU,CHUNK instead of "
+                         "autotuning")
     ap.add_argument("--cfg4-tune-nreal", type=int, default=6000,
                     help="cfg4: realizations of the autotuning runs (small runs are bound by "
                          "their single longest trace)")
@@ -308,6 +311,14 @@ def oracle_sample_rate(site, d, coef, budget_s, max_real=None):
         nr = min(cap, int(nr * max(2.0, 0.5 * budget_s / max(el, 1e-3))))
 
 
+def fixed_knobs(args):
+    """(knobs, []) from --knobs, or (None, None) to autotune."""
+    if not args.knobs:
+        return None, None
+    sl, wpc, ch = (int(v) for v in args.knobs.split(","))
+    return {"pool_slots": sl, "pool_waves_per_cu": wpc, "pool_chunk": ch}, []
+
+
 def canonical_flops(stats, nwells):
     return stats.attempted * (207 + 60 * nwells) + 10 * stats.accepted + 14 * stats.candidates
 
@@ -319,12 +330,14 @@ def bench_cfg2(args):
     from onekapy_amd.batch import SiteBatch
     from onekapy_amd.surrogate import trace_counts
     log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
-    small = sites.config2(scale=0.05)
-    sdraws = [sites.draw(s, seed=100 + i) for i, s in enumerate(small)]
-    tb = SiteBatch(small, sdraws)
-    log("cfg2: autotuning the pool launch on a 5 % copy")
-    best, trials = tune_batch(tb, log=log)
-    tb.close()
+    best, trials = fixed_knobs(args)
+    if best is None:
+        small = sites.config2(scale=0.05)
+        sdraws = [sites.draw(s, seed=100 + i) for i, s in enumerate(small)]
+        tb = SiteBatch(small, sdraws)
+        log("cfg2: autotuning the pool launch on a 5 % copy")
+        best, trials = tune_batch(tb, log=log)
+        tb.close()
     ss = sites.config2(scale=args.cfg2_scale)
     draws = [sites.draw(s, seed=100 + i) for i, s in enumerate(ss)]
     b = SiteBatch(ss, draws, knobs=best)
@@ -391,11 +404,13 @@ def bench_cfg4(args, torch):
     log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
     site = sites.config4(args.cfg4_nreal)
     d = sites.draw(site, seed=4)
-    small = sites.config4(args.cfg4_tune_nreal)
-    tb = SiteBatch([small], [sites.draw(small, seed=40)])
-    log(f"cfg4: autotuning the pool launch on {args.cfg4_tune_nreal} realizations")
-    best, trials = tune_batch(tb, log=log)
-    tb.close()
+    best, trials = fixed_knobs(args)
+    if best is None:
+        small = sites.config4(args.cfg4_tune_nreal)
+        tb = SiteBatch([small], [sites.draw(small, seed=40)])
+        log(f"cfg4: autotuning the pool launch on {args.cfg4_tune_nreal} realizations")
+        best, trials = tune_batch(tb, log=log)
+        tb.close()
     run = DeviceRun(site, d, torch)
     log(f"cfg4: {args.cfg4_nreal} realizations: {args.warmup} warmup + {args.steps} timed steps")
     m = time_mode(run, _lib.FP32, args.steps, args.warmup, torch, schedule=_lib.SCHED_POOL,
