@@ -22,6 +22,8 @@ struct oka_batch {
     oka_trace_opts tune{};
     PoolShape shape;
     void *d_sites = nullptr, *d_first = nullptr, *d_queue = nullptr;
+    void *d_fits = nullptr, *d_fit_first = nullptr;   // fit table, its two prefix arrays
+    int64_t fit_logs = 0, fit_reals = 0;
     void *d_hbm = nullptr, *d_rec = nullptr;
     size_t hbm_cap = 0, rec_cap = 0, first_cap = 0;
     hipGraphExec_t exec = nullptr;
@@ -157,13 +159,16 @@ int capture(oka_batch *B)
     hipGraph_t graph = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = fill_u32(c, B->stats, 2 * (int64_t)kStatCount * B->nsites);
-    for (int s = 0; s < B->nsites && rc == OKA_OK; ++s) {
+    for (int s = 0; s < B->nsites && rc == OKA_OK; ++s)
         rc = fill_u32(c, B->args[s].counts, B->nnodes[s]);
-        if (rc == OKA_OK && B->fits[s].nreal > 0 &&
-            (launch_fit_logs(B->fits[s], (double *)B->fits[s].logr2, c->stream) != hipSuccess ||
-             launch_fit_draw(B->fits[s], c->stream) != hipSuccess))
-            rc = fail(c, OKA_ERR_DEVICE, "site %d: fit launch failed", s);
-    }
+    // every site's fits in two launches
+    const int64_t *ff = (const int64_t *)B->d_fit_first;
+    if (rc == OKA_OK &&
+        (launch_fit_logs_multi((const FitArgs *)B->d_fits, B->nsites, ff, B->fit_logs,
+                               c->stream) != hipSuccess ||
+         launch_fit_draw_multi((const FitArgs *)B->d_fits, B->nsites, ff + B->nsites + 1,
+                               B->fit_reals, c->stream) != hipSuccess))
+        rc = fail(c, OKA_ERR_DEVICE, "fit launch failed");
     if (rc == OKA_OK)
         rc = pool_launch(c, sh, p);
     const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
@@ -208,6 +213,26 @@ int oka_batch_create(oka_ctx *c, int32_t nsites, const oka_batch_site *sites,
                                   hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(c, OKA_ERR_DEVICE, "batch staging failed");
     rc = rc ? rc : balloc(B, sizeof(unsigned long long), &B->d_queue);
+    if (rc == OKA_OK) {
+        // fit table and the prefix sums of its (observation, well) pairs and realizations
+        std::vector<int64_t> ff(2 * ((size_t)nsites + 1), 0);
+        for (int s = 0; s < nsites; ++s) {
+            const FitArgs &f = B->fits[s];
+            const int64_t logs = f.nreal > 0 ? (int64_t)f.nobs * f.nwells : 0;
+            ff[s + 1] = ff[s] + logs;
+            ff[nsites + 1 + s + 1] = ff[nsites + 1 + s] + f.nreal;
+        }
+        B->fit_logs = ff[nsites];
+        B->fit_reals = ff[2 * (size_t)nsites + 1];
+        rc = balloc(B, sizeof(FitArgs) * nsites, &B->d_fits);
+        rc = rc ? rc : balloc(B, sizeof(int64_t) * ff.size(), &B->d_fit_first);
+        if (rc == OKA_OK &&
+            (hipMemcpy(B->d_fits, B->fits.data(), sizeof(FitArgs) * nsites,
+                       hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(B->d_fit_first, ff.data(), sizeof(int64_t) * ff.size(),
+                       hipMemcpyHostToDevice) != hipSuccess))
+            rc = fail(c, OKA_ERR_DEVICE, "batch staging failed");
+    }
     rc = rc ? rc : capture(B);
     if (rc != OKA_OK) {
         oka_batch_destroy(B);

@@ -119,11 +119,9 @@ __device__ __forceinline__ void chol6_solve(const double *L, const double *s, co
         out[j] = s[j] * y[j];
 }
 
-__global__ void __launch_bounds__(64) k_fit_draw(FitArgs a)
+// Fit and MVN draw of realization r of the site `a`.
+__device__ __forceinline__ void fit_one(const FitArgs &a, int64_t r)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.nreal)
-        return;
     const double K = a.K[r], b = a.b[r];
     const double *q = a.q + (int64_t)r * a.nwells;
     double N[36], rhs[6], row[6], wb;
@@ -257,6 +255,54 @@ __global__ void __launch_bounds__(64) k_fit_draw(FitArgs a)
         a.status[r] = st;
 }
 
+__global__ void __launch_bounds__(64) k_fit_draw(FitArgs a)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < a.nreal)
+        fit_one(a, r);
+}
+
+// Site of work item t of a table whose item counts have prefix sums `first`.
+__device__ __forceinline__ int table_site(const int64_t *first, int nsites, int64_t t)
+{
+    int lo = 0, hi = nsites - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// log(r^2) of every (observation, well) pair of every site of a table.
+__global__ void __launch_bounds__(256) k_fit_logs_multi(const FitArgs *sites, int nsites,
+                                                        const int64_t *first, int64_t total)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total)
+        return;
+    const int site = table_site(first, nsites, t);
+    const FitArgs &a = sites[site];
+    const int64_t k = t - first[site];
+    const int i = (int)(k / a.nwells), w = (int)(k % a.nwells);
+    const double dx = a.obs[4 * i] - a.wells_xy[2 * w];
+    const double dy = a.obs[4 * i + 1] - a.wells_xy[2 * w + 1];
+    ((double *)a.logr2)[k] = cr_log(dx * dx + dy * dy);
+}
+
+// Fit and MVN draw of every realization of every site of a table.
+__global__ void __launch_bounds__(64) k_fit_draw_multi(const FitArgs *sites, int nsites,
+                                                       const int64_t *first, int64_t total)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total)
+        return;
+    const int site = table_site(first, nsites, t);
+    fit_one(sites[site], t - first[site]);
+}
+
 }  // namespace
 
 hipError_t launch_fit_logs(const FitArgs &a, double *logr2, hipStream_t s)
@@ -265,6 +311,26 @@ hipError_t launch_fit_logs(const FitArgs &a, double *logr2, hipStream_t s)
     if (n == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_fit_logs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, logr2);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_logs_multi(const FitArgs *sites, int nsites, const int64_t *first,
+                                int64_t total, hipStream_t s)
+{
+    if (total <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fit_logs_multi, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       sites, nsites, first, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_draw_multi(const FitArgs *sites, int nsites, const int64_t *first,
+                                int64_t total, hipStream_t s)
+{
+    if (total <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fit_draw_multi, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, s,
+                       sites, nsites, first, total);
     return hipGetLastError();
 }
 

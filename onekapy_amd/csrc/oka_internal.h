@@ -366,6 +366,12 @@ size_t trace_fp32_wells_lds(int nwells);
 
 // oka_fit.hip
 hipError_t launch_fit_logs(const FitArgs &a, double *logr2, hipStream_t s);
+// Over a device table of sites; `first` holds the prefix sums of the item
+// counts (nobs x nwells for the logs, nreal for the fits), total the last.
+hipError_t launch_fit_logs_multi(const FitArgs *sites, int nsites, const int64_t *first,
+                                 int64_t total, hipStream_t s);
+hipError_t launch_fit_draw_multi(const FitArgs *sites, int nsites, const int64_t *first,
+                                 int64_t total, hipStream_t s);
 hipError_t launch_fit_draw(const FitArgs &a, hipStream_t s);
 
 double host_pow_fifth(double x);
