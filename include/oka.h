@@ -69,6 +69,7 @@ enum {
 
 #define OKA_PTR_DEVICE 0x1u
 #define OKA_ASYNC 0x2u
+#define OKA_RASTER_FP32 0x4u   /* oka_rasterize: the FP32 tracers' rasterizer */
 
 typedef struct oka_ctx oka_ctx;
 
@@ -302,7 +303,11 @@ int oka_trace_vertices(oka_ctx *ctx, const oka_site *site, int32_t nreal, const 
 /* ---- one polyline into a registration grid: ProbabilityField.rasterize's
  *      insert loop (probabilityfield.py:264-339) on a fixed grid -------------
  * Marks rgrid (nrows x ncols uint8, nonzero = visited) for every node strictly
- * within umbra of any segment of xy (nverts x 2).  No expansion is done. */
+ * within umbra of any segment of xy (nverts x 2).  No expansion is done.
+ * With OKA_RASTER_FP32 in flags the segments go through the FP32 tracers'
+ * rasterizer (float coordinates relative to the grid's centre node), which
+ * may differ from the FP64-exact one for nodes within rounding of the umbra
+ * circle; otherwise the FP64-exact rasterizer is used. */
 int oka_rasterize(oka_ctx *ctx, const oka_grid *grid, int64_t nverts, const double *xy,
                   double umbra, uint8_t *rgrid, uint32_t flags);
 

@@ -24,6 +24,7 @@ FP64_EXACT, FP32 = 0, 1
  EVAL_VELOCITY_CONFINED) = range(6)
 PTR_DEVICE = 0x1
 ASYNC = 0x2
+RASTER_FP32 = 0x4      # oka_rasterize: the FP32 tracers' rasterizer
 
 _d = C.c_double
 _i32 = C.c_int32

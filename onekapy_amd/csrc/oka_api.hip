@@ -955,7 +955,8 @@ int oka_rasterize(oka_ctx *c, const oka_grid *g, int64_t nverts, const double *x
     TRY(ensure(c, kSlotBitmap, sizeof(uint32_t) * nwords, &bm));
     a.bitmap = (uint32_t *)bm;
     TRY(fill_u32(c, bm, nwords));
-    HIPCHK(c, launch_raster_fp64(a, c->stream));
+    HIPCHK(c, (flags & OKA_RASTER_FP32) ? launch_raster_fp32(a, c->stream)
+                                        : launch_raster_fp64(a, c->stream));
     void *rg;
     TRY(stage_out(c, kSlotRgrid, rgrid, (size_t)nnodes, flags, true, &rg));
     hipLaunchKernelGGL(k_bits_to_u8, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0,

@@ -15,8 +15,12 @@ namespace {
 
 using namespace fp32;
 
+// At most 128 VGPRs, so that 4 waves share a SIMD: the tracer is bound by
+// VALU issue, and with 3 waves (the compiler's own choice, 129 VGPRs) the
+// SIMDs idle about 12 % of the time on LDS and scalar waits.
 template <int kKind>
-__global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_trace_fp32(TraceArgs a)
 {
     extern __shared__ float4 smem4[];
     float4 *s_w = smem4;
@@ -94,14 +98,15 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
         f.c = s_reg[2];
         f.d = s_reg[3];
         f.e = s_reg[4];
-        unsigned long long n_att = 0, n_acc = 0, n_esc = 0, n_cand = 0;
-        unsigned long long n_st[4] = {0, 0, 0, 0};
+        // per-trace statistics go to LDS when the trace ends (few registers)
+        unsigned long long n_cand = 0;   // candidate nodes of the lane's trace
 
         Trace32 s;
         Epoch32 ep;
         int p = tid;
         bool have = p < P;
-        bool live = false;
+        bool live = false, fresh = false;
+        bool a_done = false;   // the disk around the trace's current point is marked
         for (;;) {
             bool ended = false;
             const int64_t k = (int64_t)r * P + p;
@@ -110,26 +115,42 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
                     break;
                 if (a.rasterize)
                     ep = find_epoch32(a, k);
-                live = start32(f, I, s, (float)(a.start_xy[2 * p] - a.xo),
-                               (float)(a.start_xy[2 * p + 1] - a.yo));
-                ended = !live;
+                open32(s, I, (float)(a.start_xy[2 * p] - a.xo),
+                       (float)(a.start_xy[2 * p + 1] - a.yo));
+                live = fresh = true;
+                a_done = false;
+            }
+            // one field evaluation for every lane: k1 of a fresh trace, else
+            // the first stage of an attempted step
+            double dtd;
+            float dt, px, py, kx, ky;
+            first_point(I, s, fresh, dtd, dt, px, py);
+            feval32(f, px, py, kx, ky);
+            if (fresh) {
+                fresh = false;
+                if (!take_k1(I, s, kx, ky)) {
+                    live = false;
+                    ended = true;
+                }
             } else {
                 float ox, oy;
-                const int res = attempt32(f, I, s, ox, oy);
+                const int res = finish32(f, I, s, dtd, dt, kx, ky, ox, oy);
                 if ((res & 1) && a.rasterize)
                     n_cand += (unsigned long long)raster32(bits, ep, gdx, gdy, igdx, igdy, umbra,
-                                                           u2, ox, oy, s.x, s.y);
+                                                           u2, ox, oy, s.x, s.y, !a_done, a_done);
                 if (res & 2) {
                     live = false;
                     ended = true;
                 }
             }
             if (ended) {
-                n_att += (unsigned long long)s.att;
-                n_acc += (unsigned long long)s.acc;
-                n_st[s.status & 3] += 1;
+                atomicAdd(&s_stat[kStatAttempted], (unsigned long long)s.att);
+                atomicAdd(&s_stat[kStatAccepted], (unsigned long long)s.acc);
+                atomicAdd(&s_stat[kStatStatus0 + (s.status & 3)], 1ull);
+                atomicAdd(&s_stat[kStatCandidates], n_cand);
+                n_cand = 0;
                 if (!(s.bx0 > gx0 && s.bx1 < gx1 && s.by0 > gy0 && s.by1 < gy1))
-                    n_esc += 1;
+                    atomicAdd(&s_stat[kStatEscaped], 1ull);
                 if (a.bbox) {
                     a.bbox[4 * k] = a.xo + (double)s.bx0;
                     a.bbox[4 * k + 1] = a.xo + (double)s.bx1;
@@ -144,13 +165,6 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
                 have = p < P;
             }
         }
-        atomicAdd(&s_stat[kStatAttempted], n_att);
-        atomicAdd(&s_stat[kStatAccepted], n_acc);
-        atomicAdd(&s_stat[kStatEscaped], n_esc);
-        atomicAdd(&s_stat[kStatCandidates], n_cand);
-        for (int j = 0; j < 4; ++j)
-            if (n_st[j])
-                atomicAdd(&s_stat[kStatStatus0 + j], n_st[j]);
         __syncthreads();
         if (a.rasterize) {
             if constexpr (kKind == kBmTiled)
@@ -167,6 +181,24 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_fp32(TraceArgs a)
             atomicAdd(&a.stats[tid], s_stat[tid]);
         __syncthreads();
     }
+}
+
+// The FP32 tracers' segment rasterizer over the segments of one polyline (one
+// per lane), in coordinates relative to the grid's centre node.
+__global__ void __launch_bounds__(256) k_raster_fp32(RasterArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nseg)
+        return;
+    const double ox = a.xmin + (double)(a.ncols / 2) * a.dx;
+    const double oy = a.ymin + (double)(a.nrows / 2) * a.dy;
+    const Epoch32 e{(float)(a.xmin - ox), (float)(a.ymin - oy), a.nrows, a.ncols, 0, 0};
+    FlatBits bits{a.bitmap, a.ncols};
+    const float umbra = (float)a.umbra;
+    bool covered;
+    raster32(bits, e, (float)a.dx, (float)a.dy, (float)(1.0 / a.dx), (float)(1.0 / a.dy), umbra,
+             umbra * umbra, (float)(a.xy[2 * i] - ox), (float)(a.xy[2 * i + 1] - oy),
+             (float)(a.xy[2 * i + 2] - ox), (float)(a.xy[2 * i + 3] - oy), true, covered);
 }
 
 }  // namespace
@@ -186,6 +218,13 @@ hipError_t launch_trace_fp32(const TraceArgs &a, int grid_blocks, int kind, hipS
     } else {
         hipLaunchKernelGGL(k_trace_fp32<kBmHbmFlat>, grid, block, lds, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_raster_fp32(const RasterArgs &a, hipStream_t s)
+{
+    const int64_t blocks = (a.nseg + 255) / 256;
+    hipLaunchKernelGGL(k_raster_fp32, dim3((unsigned)blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

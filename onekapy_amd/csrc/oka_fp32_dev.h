@@ -94,8 +94,12 @@ struct Integ32 {
     int64_t cap;
 };
 
-__device__ __forceinline__ bool start32(const Field32 &f, const Integ32 &I, Trace32 &s, float xs,
-                                        float ys)
+// Opens a trace at (xs, ys).  Its k1 = f(start) is not evaluated here: the
+// trace is `fresh`, and the caller's next field evaluation, the one other
+// lanes spend on their first Dormand-Prince stage, is at the start point
+// (first_point) and gives k1 (take_k1).  A lane that opens a trace therefore
+// costs its wave no evaluation of its own.
+__device__ __forceinline__ void open32(Trace32 &s, const Integ32 &I, float xs, float ys)
 {
     s.x = xs;
     s.y = ys;
@@ -108,21 +112,41 @@ __device__ __forceinline__ bool start32(const Field32 &f, const Integ32 &I, Trac
     s.bx0 = s.bx1 = xs;
     s.by0 = s.by1 = ys;
     s.status = OKA_TRACE_OK;
-    feval32(f, xs, ys, s.k1x, s.k1y);
+}
+
+// The point of this iteration's first field evaluation and the step it
+// starts (capturezone.py:221-231): the start of a fresh trace, else the first
+// stage with dt clamped to the remaining duration.
+__device__ __forceinline__ void first_point(const Integ32 &I, const Trace32 &s, bool fresh,
+                                            double &dtd, float &dt, float &px, float &py)
+{
+    dtd = s.dt;
+    if (fabs(s.t + dtd) > I.absdur)
+        dtd = I.dur - s.t;
+    dt = (float)dtd;
+    px = fresh ? s.x : s.x + dt * (A21 * s.k1x);
+    py = fresh ? s.y : s.y + dt * (A21 * s.k1y);
+}
+
+// A fresh trace's k1; false when its duration is zero (the trace is only its
+// start vertex, capturezone.py:221).
+__device__ __forceinline__ bool take_k1(const Integ32 &I, Trace32 &s, float kx, float ky)
+{
+    s.k1x = kx;
+    s.k1y = ky;
     return 0.0 < I.absdur;
 }
 
-__device__ __forceinline__ int attempt32(const Field32 &f, const Integ32 &I, Trace32 &s, float &ox,
-                                         float &oy)
+// The rest of one attempted step (capturezone.py:221-247) given k2 = f(first
+// stage): bit 0 of the result = accepted (the segment is (ox, oy) -> (s.x,
+// s.y)), bit 1 = the trace ended.
+__device__ __forceinline__ int finish32(const Field32 &f, const Integ32 &I, Trace32 &s,
+                                        double dtd, float dt, float k2x, float k2y, float &ox,
+                                        float &oy)
 {
-    double dtd = s.dt;
-    if (fabs(s.t + dtd) > I.absdur)
-        dtd = I.dur - s.t;
     s.att += 1;
-    const float dt = (float)dtd;
     const float x = s.x, y = s.y, k1x = s.k1x, k1y = s.k1y;
-    float k2x, k2y, k3x, k3y, k4x, k4y, k5x, k5y, k6x, k6y, k7x, k7y;
-    feval32(f, x + dt * (A21 * k1x), y + dt * (A21 * k1y), k2x, k2y);
+    float k3x, k3y, k4x, k4y, k5x, k5y, k6x, k6y, k7x, k7y;
     feval32(f, x + dt * (A31 * k1x + A32 * k2x), y + dt * (A31 * k1y + A32 * k2y), k3x, k3y);
     feval32(f, x + dt * (A41 * k1x + A42 * k2x + A43 * k3x),
             y + dt * (A41 * k1y + A42 * k2y + A43 * k3y), k4x, k4y);
@@ -137,7 +161,7 @@ __device__ __forceinline__ int attempt32(const Field32 &f, const Integ32 &I, Tra
     const float ex = dt * (E1 * k1x + E2 * k7x + E3 * k3x + E4 * k4x + E5 * k5x + E6 * k6x);
     const float ey = dt * (E1 * k1y + E2 * k7y + E3 * k3y + E4 * k4y + E5 * k5y + E6 * k6y);
     const float est = fmaxf(fabsf(ex), fabsf(ey));
-    const float ds = sqrtf(fmaf(dxs, dxs, dys * dys));
+    const float ds = __builtin_amdgcn_sqrtf(fmaf(dxs, dxs, dys * dys));
     int res = 0;
     const bool finite = !(isnan(ex) || isnan(ey));
     if (finite && est < I.tol && ds < I.maxstep) {
@@ -155,8 +179,9 @@ __device__ __forceinline__ int attempt32(const Field32 &f, const Integ32 &I, Tra
         s.by1 = fmaxf(s.by1, s.y);
         res = 1;
     }
-    float cf = __expf(0.2f * __logf(I.tol / (est + 2.220446049250313e-16f)));
-    const float c2 = I.maxstep / (ds + 2.220446049250313e-16f);
+    // step-size factors with hardware reciprocal and square root (about 1 ulp)
+    float cf = __expf(0.2f * __logf(I.tol * __builtin_amdgcn_rcpf(est + 2.220446049250313e-16f)));
+    const float c2 = I.maxstep * __builtin_amdgcn_rcpf(ds + 2.220446049250313e-16f);
     if (c2 < cf)
         cf = c2;
     if (10.0f < cf)
@@ -196,28 +221,14 @@ __device__ __forceinline__ Epoch32 find_epoch32(const TraceArgs &a, int64_t k)
     return {(float)(e.xmin - a.xo), (float)(e.ymin - a.yo), e.nrows, e.ncols, e.row_off, e.col_off};
 }
 
+// Per-node form of the segment test (probabilityfield.py:296-310, 407-427),
+// used for segments within 1e-20 of an axis direction, where the row-span
+// form below would divide by a vanishing component.
 template <typename Bits>
-__device__ __forceinline__ int raster32(Bits &bits, const Epoch32 &e, float dx, float dy,
-                                         float idx_, float idy, float umbra, float u2, float ax,
-                                         float ay, float bx, float by)
+__device__ __forceinline__ void raster32_nodes(Bits &bits, const Epoch32 &e, float dx, float dy,
+                                               float u2, float ax, float ay, float bax, float bay,
+                                               float l2, int j0, int j1, int i0, int i1)
 {
-    const float lox = fminf(ax, bx), hix = fmaxf(ax, bx);
-    const float loy = fminf(ay, by), hiy = fmaxf(ay, by);
-    float jl = floorf((lox - umbra - e.xmin) * idx_);
-    float jr = floorf((hix + umbra - e.xmin) * idx_) + 1.0f;
-    float il = floorf((loy - umbra - e.ymin) * idy);
-    float ir = floorf((hiy + umbra - e.ymin) * idy) + 1.0f;
-    jl = fmaxf(jl, 0.0f);
-    il = fmaxf(il, 0.0f);
-    jr = fminf(jr, (float)e.ncols);
-    ir = fminf(ir, (float)e.nrows);
-    if (!(jl < jr) || !(il < ir))
-        return 0;
-    const int j0 = (int)jl, j1 = (int)jr, i0 = (int)il, i1 = (int)ir;
-    const float bax = bx - ax, bay = by - ay;
-    const float l2 = fmaf(bax, bax, bay * bay);
-    if (!(l2 > 0.0f))
-        return 0;
     const float il2 = 1.0f / l2;
     for (int j = j0; j < j1; ++j) {
         const float cax = fmaf((float)j, dx, e.xmin) - ax;
@@ -241,6 +252,90 @@ __device__ __forceinline__ int raster32(Bits &bits, const Epoch32 &e, float dx, 
             if (d2 < u2)
                 bits.set(row, col);
         }
+    }
+}
+
+// Marks the nodes strictly within umbra of segment A-B among the reference's
+// candidate window (probabilityfield.py:296-301) and returns the window's
+// size.  The reference's distance (probabilityfield.py:407-427) is the
+// distance to the segment, so the marked nodes of a row are those inside the
+// row's chord of the capsule {P : dist(P, AB) < umbra}: an open interval, the
+// hull of the chords of the disks around A and B and of the band
+// {0 <= (P - A).(B - A) <= |B - A|^2, |(B - A) x (P - A)| < umbra |B - A|}.
+// Each row costs its chord and a word-wise mark instead of a test per node;
+// the two forms differ only for nodes within rounding of the umbra circle.
+// The band with the disk around B is convex too, so leaving out a disk around
+// A that is marked already still leaves one interval per row.
+// disk_a = false skips the disk around A: the caller's previous segment of
+// the same trace ended at A and marked that disk already (its `covered` came
+// back true).  covered is set when the disk around B is marked (or lies off
+// the grid), i.e. unless the segment has zero length.
+template <typename Bits>
+__device__ __forceinline__ int raster32(Bits &bits, const Epoch32 &e, float dx, float dy,
+                                         float idx_, float idy, float umbra, float u2, float ax,
+                                         float ay, float bx, float by, bool disk_a, bool &covered)
+{
+    covered = true;
+    const float lox = fminf(ax, bx), hix = fmaxf(ax, bx);
+    const float loy = fminf(ay, by), hiy = fmaxf(ay, by);
+    float jl = floorf((lox - umbra - e.xmin) * idx_);
+    float jr = floorf((hix + umbra - e.xmin) * idx_) + 1.0f;
+    float il = floorf((loy - umbra - e.ymin) * idy);
+    float ir = floorf((hiy + umbra - e.ymin) * idy) + 1.0f;
+    jl = fmaxf(jl, 0.0f);
+    il = fmaxf(il, 0.0f);
+    jr = fminf(jr, (float)e.ncols);
+    ir = fminf(ir, (float)e.nrows);
+    if (!(jl < jr) || !(il < ir))
+        return 0;
+    const int j0 = (int)jl, j1 = (int)jr, i0 = (int)il, i1 = (int)ir;
+    const float bax = bx - ax, bay = by - ay;
+    const float l2 = fmaf(bax, bax, bay * bay);
+    if (!(l2 > 0.0f)) {
+        covered = false;
+        return 0;
+    }
+    if (!(fabsf(bax) > 1e-20f && fabsf(bay) > 1e-20f)) {
+        raster32_nodes(bits, e, dx, dy, u2, ax, ay, bax, bay, l2, j0, j1, i0, i1);
+        return (j1 - j0) * (i1 - i0);
+    }
+    // In row coordinates s = x - ax, c = y_row - ay the band's chord is
+    // [kb c, kb c + l2 / bax] (either order) cut by ka c -+ wr.
+    const float ibax = __builtin_amdgcn_rcpf(bax), ibay = __builtin_amdgcn_rcpf(bay);
+    const float kb = -bay * ibax, l2b = l2 * ibax;
+    const float ka = bax * ibay, wr = umbra * __builtin_amdgcn_sqrtf(l2) * fabsf(ibay);
+    const float ox = (ax - e.xmin) * idx_;   // A's column coordinate
+    const float jlast = jr - 1.0f;
+    for (int i = i0; i < i1; ++i) {
+        const float c = fmaf((float)i, dy, e.ymin) - ay;
+        const float t1 = kb * c, t2 = t1 + l2b, m = ka * c;
+        float lo = fmaxf(fminf(t1, t2), m - wr);
+        float hi = fminf(fmaxf(t1, t2), m + wr);
+        if (!(lo < hi)) {
+            lo = INFINITY;
+            hi = -INFINITY;
+        }
+        if (disk_a) {
+            const float ha = fmaf(-c, c, u2);
+            if (ha > 0.0f) {
+                const float r = __builtin_amdgcn_sqrtf(ha);
+                lo = fminf(lo, -r);
+                hi = fmaxf(hi, r);
+            }
+        }
+        const float cb = c - bay;
+        const float hb = fmaf(-cb, cb, u2);
+        if (hb > 0.0f) {
+            const float r = __builtin_amdgcn_sqrtf(hb);
+            lo = fminf(lo, bax - r);
+            hi = fmaxf(hi, bax + r);
+        }
+        // nodes with lo < x_j - ax < hi, clamped to the window (both bounds
+        // are finite and inside it whenever fl <= fh)
+        const float fl = fmaxf(floorf(fmaf(lo, idx_, ox)) + 1.0f, jl);
+        const float fh = fminf(ceilf(fmaf(hi, idx_, ox)) - 1.0f, jlast);
+        if (fl <= fh)
+            bits.mark(i + e.roff, (int)fl + e.coff, (int)fh + e.coff);
     }
     return (j1 - j0) * (i1 - i0);
 }

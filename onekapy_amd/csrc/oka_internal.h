@@ -99,6 +99,13 @@ __device__ inline unsigned long long count_bits(const TraceArgs &a, int row, int
     return n;
 }
 
+// n consecutive bits starting at bit b of a word (1 <= n <= 32 - b; for
+// n = 32, 2 << 31 wraps to 0 and the mask is all ones).
+__device__ __forceinline__ uint32_t span_mask(int n, int b)
+{
+    return ((2u << (n - 1)) - 1u) << b;
+}
+
 // A realization's visited-node bitmap over the whole final grid (LDS or HBM):
 // node (row, col) is bit idx % 32 of word idx / 32, idx = row * ncols + col.
 struct FlatBits {
@@ -113,6 +120,24 @@ struct FlatBits {
     {
         const int idx = row * ncols + col;
         atomicOr(&bm[idx >> 5], 1u << (idx & 31));
+    }
+    // Marks columns c0..c1 (inclusive) of one row, a word at a time; a word
+    // whose bits are all set already is only read.
+    __device__ __forceinline__ void mark(int row, int c0, int c1)
+    {
+        int k = row * ncols + c0;
+        const int end = row * ncols + c1;
+        for (;;) {
+            const int b = k & 31;
+            const int n = min(end - k + 1, 32 - b);
+            const uint32_t m = span_mask(n, b);
+            uint32_t *w = &bm[k >> 5];
+            if ((*w & m) != m)
+                atomicOr(w, m);
+            k += n;
+            if (k > end)
+                break;
+        }
     }
 };
 
@@ -159,6 +184,23 @@ struct TileBits {
         if (w == nullptr)
             w = claim(t) + (row & 31);
         atomicOr(w, 1u << (col & 31));
+    }
+    // Marks columns c0..c1 (inclusive) of one row, one tile word at a time.
+    __device__ __forceinline__ void mark(int row, int c0, int c1)
+    {
+        const int tr = (row >> 5) * tiles_x, r = row & 31;
+        for (int c = c0; c <= c1;) {
+            const int b = c & 31;
+            const int n = min(c1 - c + 1, 32 - b);
+            const uint32_t m = span_mask(n, b);
+            const int t = tr + (c >> 5);
+            uint32_t *w = word(t, r);
+            if (w == nullptr)
+                w = claim(t) + r;
+            if ((*w & m) != m)
+                atomicOr(w, m);
+            c += n;
+        }
     }
 };
 
@@ -357,6 +399,7 @@ hipError_t launch_trace_fp64(const TraceArgs &a, int grid_blocks, int kind,
 hipError_t launch_vertices_fp64(const TraceArgs &a, hipStream_t s);
 hipError_t launch_eval_fp64(const EvalArgs &a, hipStream_t s);
 hipError_t launch_raster_fp64(const RasterArgs &a, hipStream_t s);
+hipError_t launch_raster_fp32(const RasterArgs &a, hipStream_t s);
 size_t trace_fp64_wells_lds(int nwells);
 
 // oka_fp32.hip

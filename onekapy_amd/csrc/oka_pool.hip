@@ -135,6 +135,26 @@ struct PoolBits {
         }
         atomicOr(&w[row & 31], 1u << (col & 31));
     }
+    // Marks columns c0..c1 (inclusive) of one row, one tile word at a time.
+    __device__ __forceinline__ void mark(int row, int c0, int c1)
+    {
+        const int tr = (row >> 5) * tiles_x, r = row & 31;
+        for (int c = c0; c <= c1;) {
+            const int b = c & 31;
+            const int n = min(c1 - c + 1, 32 - b);
+            const uint32_t m = span_mask(n, b);
+            const uint32_t t = (uint32_t)(tr + (c >> 5));
+            uint32_t *w = lookup(t);
+            if (w == nullptr) {
+                w = claim(t);
+                last_t = t;
+                last_w = w;
+            }
+            if ((w[r] & m) != m)
+                atomicOr(&w[r], m);
+            c += n;
+        }
+    }
 };
 
 struct PoolLds {
@@ -418,7 +438,8 @@ __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
     __syncthreads();
 
     // the lane's trace and its context
-    bool live = false;
+    bool live = false, fresh = false;
+    bool a_done = false;   // the disk around the trace's current point is marked
     int slot = 0, pidx = 0;
     unsigned long long cand = 0;
     Trace32 tr;
@@ -481,9 +502,10 @@ __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
                         u2 = umbra * umbra;
                     }
                     cand = 0;
-                    live = start32(f, I, tr, (float)(A.start_xy[2 * pidx] - A.xo),
-                                   (float)(A.start_xy[2 * pidx + 1] - A.yo));
-                    ended = !live;
+                    open32(tr, I, (float)(A.start_xy[2 * pidx] - A.xo),
+                           (float)(A.start_xy[2 * pidx + 1] - A.yo));
+                    live = fresh = true;
+                    a_done = false;
                 }
                 __syncthreads();
                 if (lane == 0)
@@ -512,16 +534,29 @@ __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
             intake = fs;
         }
 
-        // B. one attempted step per live lane
+        // B. one field evaluation per live lane (k1 of a fresh trace, else the
+        // first stage of an attempted step), then the rest of the step
         if (live) {
-            float ox, oy;
-            const int res = attempt32(f, I, tr, ox, oy);
-            if ((res & 1) && raster)
-                cand += (unsigned long long)raster32(bits, ep, gdx, gdy, igdx, igdy, umbra, u2, ox,
-                                                     oy, tr.x, tr.y);
-            if (res & 2) {
-                live = false;
-                ended = true;
+            double dtd;
+            float dt, px, py, kx, ky;
+            first_point(I, tr, fresh, dtd, dt, px, py);
+            feval32(f, px, py, kx, ky);
+            if (fresh) {
+                fresh = false;
+                if (!take_k1(I, tr, kx, ky)) {
+                    live = false;
+                    ended = true;
+                }
+            } else {
+                float ox, oy;
+                const int res = finish32(f, I, tr, dtd, dt, kx, ky, ox, oy);
+                if ((res & 1) && raster)
+                    cand += (unsigned long long)raster32(bits, ep, gdx, gdy, igdx, igdy, umbra,
+                                                         u2, ox, oy, tr.x, tr.y, !a_done, a_done);
+                if (res & 2) {
+                    live = false;
+                    ended = true;
+                }
             }
         }
         __syncthreads();

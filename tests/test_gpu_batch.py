@@ -40,13 +40,25 @@ def test_batch_matches_per_site_runs(gpu_ctx):
 
 
 def test_batch_replays_and_retunes_identically(gpu_ctx):
+    """Every replay of the captured graph (fills, fits, pool launch) equals the
+    eager per-site runs, with eager fits and traces on the device between the
+    replays."""
     ss, draws = _small_batch()
+    eager = []
+    for s, d in zip(ss, draws):
+        _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"],
+                                  d["b"], d["q"], d["z"])
+        eager.append(trace_counts(s, d, coef, _lib.FP32)[0])
     b = SiteBatch(ss, draws)
-    b.run()
-    first = [b.results(i)[0] for i in range(len(ss))]
-    b.run()                                   # counts restart from zero on every replay
-    for i in range(len(ss)):
-        assert np.array_equal(b.results(i)[0], first[i])
+    for rep in range(4):                      # counts restart from zero on every replay
+        print(f"replay {rep}", flush=True)
+        b.run()
+        for i in range(len(ss)):
+            assert np.array_equal(b.results(i)[0], eager[i]), (rep, i)
+        trace_counts(ss[rep], draws[rep], fit_and_draw(
+            ss[rep].wells_xy(), ss[rep].xt, ss[rep].yt, ss[rep].base, ss[rep].observations,
+            draws[rep]["K"], draws[rep]["b"], draws[rep]["q"], draws[rep]["z"])[2], _lib.FP32)
+    first = eager
     for knobs in (dict(pool_chunk=16, pool_slots=2), dict(pool_chunk=256, pool_waves_per_cu=4)):
         print(f"knobs {knobs}", flush=True)
         b.tune(**knobs)

@@ -234,3 +234,60 @@ def test_pool_schedule_many_tiles_and_uneven_lifetimes(gpu_ctx):
         print(f"chunk {chunk}", flush=True)
         pl = _pool_vs_workgroup(s, d, coef, pf, 300.0, pool_slots=3, pool_chunk=chunk)
         assert pl.counts.max() == 12
+
+
+def _raster_polyline(grid, xy, umbra, flags):
+    ctx = _lib.context()
+    marks = np.zeros((grid.nrows, grid.ncols), dtype=np.uint8)
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    ctx.check(ctx.lib.oka_rasterize(ctx.h, grid, xy.shape[0], _lib.ptr(xy), float(umbra),
+                                    _lib.ptr(marks), flags))
+    return marks.astype(bool)
+
+
+@pytest.mark.parametrize("spacing,umbra", [(20.0, 40.0), (10.0, 20.0), (40.0, 15.0)])
+def test_fp32_rasterizer_matches_exact_but_at_the_umbra_circle(gpu_ctx, spacing, umbra):
+    """The FP32 tracers' rasterizer (row chords of the umbra capsule) marks the
+    same nodes as the FP64-exact one (per-node test, itself bit-exact against
+    the oracle), except nodes whose exact distance to the polyline is within
+    rounding of umbra.  The polyline mixes steps shorter than a cell, long
+    jumps (> 2 umbra), exact axis-parallel and zero-length segments, and parts
+    outside the grid."""
+    rng = np.random.default_rng(11)
+    nrows, ncols = 180, 170
+    x0, y0 = -1234.5, 777.25
+    grid = _lib.make_grid(x0, y0, x0 + (ncols - 1) * spacing, y0 + (nrows - 1) * spacing,
+                          spacing, spacing, nrows, ncols)
+    pts = [(x0 + 0.5 * ncols * spacing, y0 + 0.5 * nrows * spacing)]
+    for k in range(6000):
+        x, y = pts[-1]
+        kind = k % 6
+        if kind == 0:                      # long jump anywhere, partly off the grid
+            x = x0 + rng.uniform(-0.1, 1.1) * ncols * spacing
+            y = y0 + rng.uniform(-0.1, 1.1) * nrows * spacing
+        elif kind == 1:                    # exactly horizontal or vertical
+            if rng.random() < 0.5:
+                x = x + rng.uniform(-3, 3) * spacing
+            else:
+                y = y + rng.uniform(-3, 3) * spacing
+        elif kind == 2:                    # zero length
+            pass
+        else:                              # tracer-like steps up to 10 m
+            th = rng.uniform(0, 2 * np.pi)
+            ds = rng.uniform(0, 10.0) if kind < 5 else rng.uniform(0, 1e-3)
+            x, y = x + ds * np.cos(th), y + ds * np.sin(th)
+        pts.append((x, y))
+    xy = np.array(pts)
+    exact = _raster_polyline(grid, xy, umbra, 0)
+    fp32 = _raster_polyline(grid, xy, umbra, _lib.RASTER_FP32)
+    assert exact.sum() > 10000
+    diff = np.argwhere(exact != fp32)
+    assert len(diff) <= 1e-4 * exact.sum()
+    # float coordinates relative to the grid centre resolve ~5e-4 m here
+    tol = 64 * np.spacing(np.float32(0.6 * max(nrows, ncols) * spacing))
+    for i, j in diff:
+        cx, cy = x0 + j * spacing, y0 + i * spacing
+        d = min(np.sqrt(O.distancesquared(ax, ay, bx, by, cx, cy))
+                for (ax, ay), (bx, by) in zip(xy[:-1], xy[1:])
+                if np.hypot(ax - bx, ay - by) > 0)
+        assert abs(d - umbra) <= tol, (i, j, d)

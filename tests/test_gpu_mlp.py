@@ -75,3 +75,37 @@ def test_control_variate_is_unbiased_and_reduces_variance(gpu_ctx):
     assert np.sqrt(np.mean(z[live] ** 2)) < 1.5
     gain = res.var_plain.sum() / res.var_cv.sum()
     assert gain > 3.0, gain
+
+
+def test_replayed_training_graph_matches_one_step_calls(gpu_ctx):
+    """oka_mlp_train captures its steps 16 at a time as a HIP graph and replays
+    it.  Forty steps in one call (two replays of the 16-step graph, one of an
+    8-step graph) must leave the parameters bit-identical to forty one-step
+    calls (each its own graph, launched once) with other network work on the
+    device between them.  Gradients are reduced without atomics, so the only
+    difference allowed is the loss's per-workgroup atomic order."""
+    rng = np.random.default_rng(9)
+    nin, nout, batch, nsteps = 34, 256, 128, 40
+    x = rng.normal(0, 1, (1024, nin)).astype(np.float32)
+    marks = rng.integers(0, 26, (1024, nout)).astype(np.uint8)
+    order = rng.integers(0, 1024, nsteps * batch).astype(np.int32)
+    nets = [SurrogateMLP(nin, 128, nout, max_batch=256, seed=4) for _ in range(2)]
+    assert np.array_equal(nets[0].get_params(), nets[1].get_params())
+
+    def train(net, first, k):
+        losses = np.zeros(k, dtype=np.float32)
+        o = np.ascontiguousarray(order[first * batch:(first + k) * batch])
+        net.ctx.check(net.ctx.lib.oka_mlp_train(
+            net.h, x.shape[0], x.ctypes.data, marks.ctypes.data, 1.0 / 25.0, o.ctypes.data,
+            batch, k, 2e-3, 0.9, 0.999, 1e-8, losses.ctypes.data, 0))
+        return losses
+
+    whole = train(nets[0], 0, nsteps)
+    steps = []
+    for i in range(nsteps):
+        steps.append(train(nets[1], i, 1)[0])
+        nets[1].predict(x[:64])            # eager forward passes between the graphs
+        nets[1].accumulate(x[:128], marks[:128], 1.0 / 25.0)
+    assert np.array_equal(nets[0].get_params(), nets[1].get_params())
+    np.testing.assert_allclose(whole, np.array(steps), rtol=1e-5)
+    assert whole[-8:].mean() < whole[:8].mean()
