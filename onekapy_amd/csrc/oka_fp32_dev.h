@@ -300,17 +300,19 @@ __device__ __forceinline__ int raster32(Bits &bits, const Epoch32 &e, float dx, 
         return (j1 - j0) * (i1 - i0);
     }
     // In row coordinates s = x - ax, c = y_row - ay the band's chord is
-    // [kb c, kb c + l2 / bax] (either order) cut by ka c -+ wr.
+    // kb c + [min(0, l2 / bax), max(0, l2 / bax)] cut by ka c -+ wr.
     const float ibax = __builtin_amdgcn_rcpf(bax), ibay = __builtin_amdgcn_rcpf(bay);
     const float kb = -bay * ibax, l2b = l2 * ibax;
+    const float blo = fminf(l2b, 0.0f), bhi = fmaxf(l2b, 0.0f);
     const float ka = bax * ibay, wr = umbra * __builtin_amdgcn_sqrtf(l2) * fabsf(ibay);
     const float ox = (ax - e.xmin) * idx_;   // A's column coordinate
     const float jlast = jr - 1.0f;
-    for (int i = i0; i < i1; ++i) {
+    const int stride = bits.stride();
+    int base = (i0 + e.roff) * stride;       // flat index of the row's first node
+    for (int i = i0; i < i1; ++i, base += stride) {
         const float c = fmaf((float)i, dy, e.ymin) - ay;
-        const float t1 = kb * c, t2 = t1 + l2b, m = ka * c;
-        float lo = fmaxf(fminf(t1, t2), m - wr);
-        float hi = fminf(fmaxf(t1, t2), m + wr);
+        float lo = fmaxf(fmaf(kb, c, blo), fmaf(ka, c, -wr));
+        float hi = fminf(fmaf(kb, c, bhi), fmaf(ka, c, wr));
         if (!(lo < hi)) {
             lo = INFINITY;
             hi = -INFINITY;
@@ -332,10 +334,10 @@ __device__ __forceinline__ int raster32(Bits &bits, const Epoch32 &e, float dx, 
         }
         // nodes with lo < x_j - ax < hi, clamped to the window (both bounds
         // are finite and inside it whenever fl <= fh)
-        const float fl = fmaxf(floorf(fmaf(lo, idx_, ox)) + 1.0f, jl);
-        const float fh = fminf(ceilf(fmaf(hi, idx_, ox)) - 1.0f, jlast);
+        const float fl = fmaxf(floorf(fmaf(lo, idx_, ox + 1.0f)), jl);
+        const float fh = fminf(ceilf(fmaf(hi, idx_, ox - 1.0f)), jlast);
         if (fl <= fh)
-            bits.mark(i + e.roff, (int)fl + e.coff, (int)fh + e.coff);
+            bits.mark(i + e.roff, base, (int)fl + e.coff, (int)fh + e.coff);
     }
     return (j1 - j0) * (i1 - i0);
 }

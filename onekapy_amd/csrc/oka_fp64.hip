@@ -80,6 +80,38 @@ __device__ __forceinline__ Field64 make_field(const TraceArgs &a, int r, const d
     return f;
 }
 
+// a1 / b and a2 / b, each correctly rounded (IEEE division, as numpy's).
+// gfx950 has no division instruction; the compiler expands every FP64 `/`
+// into one fixed sequence:
+//   d = div_scale(b; b, a)  r = rcp(d)  r += r (1 - d r)  r += r (1 - d r)
+//   n, s = div_scale(a; b, a)  m = n r  q = div_fmas(fma(-d, m, n), r, m; s)
+//   q = div_fixup(q, b, a)
+// Only d and r depend on the numerator, through d's scaling of extreme
+// exponents.  When both numerators leave d the same (always, in the normal
+// range), this runs that sequence once for d and r and twice for the rest,
+// so each result has the bits `a / b` would have; otherwise it divides twice.
+__device__ __forceinline__ void div_pair(double a1, double a2, double b, double &q1, double &q2)
+{
+    bool s1, s2;
+    const double d = __builtin_amdgcn_div_scale(a1, b, false, &s1);
+    const double d2 = __builtin_amdgcn_div_scale(a2, b, false, &s2);
+    if (__double_as_longlong(d) != __double_as_longlong(d2)) {
+        q1 = a1 / b;
+        q2 = a2 / b;
+        return;
+    }
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    const double n1 = __builtin_amdgcn_div_scale(a1, b, true, &s1);
+    const double n2 = __builtin_amdgcn_div_scale(a2, b, true, &s2);
+    const double m1 = n1 * r, m2 = n2 * r;
+    q1 = __builtin_amdgcn_div_fixup(
+        __builtin_amdgcn_div_fmas(fma(-d, m1, n1), r, m1, s1), b, a1);
+    q2 = __builtin_amdgcn_div_fixup(
+        __builtin_amdgcn_div_fmas(fma(-d, m2, n2), r, m2, s2), b, a2);
+}
+
 // feval(xy) = -V(xy) (stochastic.py:253-260).  Returns false where the
 // reference raises AquiferError (unconfined only).
 template <bool kConfined>
@@ -95,14 +127,18 @@ __device__ __forceinline__ bool feval(const Field64 &f, double x, double y, doub
         const double ddy = y - f.yw[w];
         const double r2 = ddx * ddx + ddy * ddy;
         const double nq = f.nq[w];
-        Qx = Qx + ((nq * ddx) / r2) * kC2Pi;
-        Qy = Qy + ((nq * ddy) / r2) * kC2Pi;
+        double sx, sy;
+        div_pair(nq * ddx, nq * ddy, r2, sx, sy);
+        Qx = Qx + sx * kC2Pi;
+        Qy = Qy + sy * kC2Pi;
         if (!kConfined)
             pw = pw + ((-nq) * cr_log(r2)) * kC4Pi;
     }
     if (kConfined) {
-        kx = -(Qx / f.bn);
-        ky = -(Qy / f.bn);
+        double vx, vy;
+        div_pair(Qx, Qy, f.bn, vx, vy);
+        kx = -vx;
+        ky = -vy;
         return true;
     }
     double pot = f.A * (dx * dx) + f.B * (dy * dy);
@@ -523,8 +559,10 @@ __global__ void __launch_bounds__(256) k_eval_fp64(EvalArgs a)
         const double r2 = ddx * ddx + ddy * ddy;
         if (need_q) {
             const double nq = -wl[3];
-            Qx = Qx + ((nq * ddx) / r2) * kC2Pi;
-            Qy = Qy + ((nq * ddy) / r2) * kC2Pi;
+            double sx, sy;
+            div_pair(nq * ddx, nq * ddy, r2, sx, sy);
+            Qx = Qx + sx * kC2Pi;
+            Qy = Qy + sy * kC2Pi;
         }
         if (need_p)
             pw = pw + (wl[3] * cr_log(r2)) * kC4Pi;

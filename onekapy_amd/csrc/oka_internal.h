@@ -121,12 +121,15 @@ struct FlatBits {
         const int idx = row * ncols + col;
         atomicOr(&bm[idx >> 5], 1u << (idx & 31));
     }
-    // Marks columns c0..c1 (inclusive) of one row, a word at a time; a word
-    // whose bits are all set already is only read.
-    __device__ __forceinline__ void mark(int row, int c0, int c1)
+    // Row stride of the flat index (the span markers' `base` = row * stride).
+    __device__ __forceinline__ int stride() const { return ncols; }
+    // Marks columns c0..c1 (inclusive) of one row, whose first node has flat
+    // index base = row * ncols, a word at a time; a word whose bits are all
+    // set already is only read.
+    __device__ __forceinline__ void mark(int /*row*/, int base, int c0, int c1)
     {
-        int k = row * ncols + c0;
-        const int end = row * ncols + c1;
+        int k = base + c0;
+        const int end = base + c1;
         for (;;) {
             const int b = k & 31;
             const int n = min(end - k + 1, 32 - b);
@@ -185,8 +188,9 @@ struct TileBits {
             w = claim(t) + (row & 31);
         atomicOr(w, 1u << (col & 31));
     }
+    __device__ __forceinline__ int stride() const { return 0; }
     // Marks columns c0..c1 (inclusive) of one row, one tile word at a time.
-    __device__ __forceinline__ void mark(int row, int c0, int c1)
+    __device__ __forceinline__ void mark(int row, int /*base*/, int c0, int c1)
     {
         const int tr = (row >> 5) * tiles_x, r = row & 31;
         for (int c = c0; c <= c1;) {

@@ -135,8 +135,9 @@ struct PoolBits {
         }
         atomicOr(&w[row & 31], 1u << (col & 31));
     }
+    __device__ __forceinline__ int stride() const { return 0; }
     // Marks columns c0..c1 (inclusive) of one row, one tile word at a time.
-    __device__ __forceinline__ void mark(int row, int c0, int c1)
+    __device__ __forceinline__ void mark(int row, int /*base*/, int c0, int c1)
     {
         const int tr = (row >> 5) * tiles_x, r = row & 31;
         for (int c = c0; c <= c1;) {

@@ -50,6 +50,44 @@ def test_model_eval_bit_exact_vs_oracle(gpu_ctx):
         assert pot[i] == om.potential(x[i], y[i], O.MATH_CR)
 
 
+def _same_bits(a, b):
+    return np.float64(a).tobytes() == np.float64(b).tobytes() or (math.isnan(a) and math.isnan(b))
+
+
+def test_model_eval_edge_points_bit_exact_vs_oracle(gpu_ctx):
+    """The FP64 discharge divides each well's x and y terms by the same r^2
+    with one shared reciprocal refinement when the operands allow it, and by
+    two plain divisions otherwise.  Both paths must give the oracle's bits.
+    One well per model and no regional gradient, so each result is that
+    well's quotient: zero numerators (points level with the well, q = 0),
+    extreme exponents (q = 1e-300 and 1e300, points 1e-150 m from the well or
+    1e7 m away, denormal coordinates), every sign, and the confined
+    velocity's second shared division by b n."""
+    rng = np.random.default_rng(8)
+    base_pts = [(0.0, 7.0), (13.0, 0.0), (1e-150, 0.0), (0.0, 1e-150), (1e-150, 2e-150),
+                (1e7, -1e7), (-3.5, 1e-9), (5e-324, 250.0), (1e-300, -1e-300), (2.0, 2.0)]
+    nchecked = 0
+    for q in (500.0, 0.0, -730.5, 1e-300, 1e300, -2.5e-20, 6e150):
+        for xw, yw in ((0.0, 0.0), (100.0, -250.0)):
+            m = Model(0.0, 10.0, 0.25, 20.0, [(xw, yw, 0.25, q)], 0.0, 0.0,
+                      np.array([0.0, 0.0, 0.0, 0.0, 0.0, 2000.0]))
+            om = O.OracleModel(m.base, m.conductivity, m.porosity, m.thickness, m.wells,
+                               m.xo, m.yo, m.coef)
+            pts = [(xw + a, yw + b) for a, b in base_pts]
+            pts += [(xw + a, yw + b) for a, b in rng.uniform(-400, 400, (20, 2))]
+            x = np.array([p[0] for p in pts])
+            y = np.array([p[1] for p in pts])
+            qx, qy = m.compute_discharge(x, y)
+            vx, vy = m.compute_velocity_confined(x, y)
+            for i in range(len(x)):
+                ox, oy = om.discharge(x[i], y[i])
+                assert _same_bits(qx[i], ox) and _same_bits(qy[i], oy), (q, xw, x[i], y[i])
+                ox, oy = om.velocity_confined(x[i], y[i])
+                assert _same_bits(vx[i], ox) and _same_bits(vy[i], oy), (q, xw, x[i], y[i])
+                nchecked += 1
+    assert nchecked == 7 * 2 * 30
+
+
 def test_fit_kat_and_library_on_gpu(gpu_ctx):
     g = golden("reference_kats.json")["fit"]
     m = _kat_model()
