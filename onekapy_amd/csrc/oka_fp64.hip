@@ -303,10 +303,13 @@ __device__ __forceinline__ int raster_segment(Bits &bits, const EpochReg &e, dou
 {
     const double lox = bx < ax ? bx : ax, hix = bx > ax ? bx : ax;
     const double loy = by < ay ? by : ay, hiy = by > ay ? by : ay;
-    double jl = floor(((lox - umbra) - e.xmin) / dx);
-    double jr = floor(((hix + umbra) - e.xmin) / dx) + 1.0;
-    double il = floor(((loy - umbra) - e.ymin) / dy);
-    double ir = floor(((hiy + umbra) - e.ymin) / dy) + 1.0;
+    double qxl, qxr, qyl, qyr;
+    div_pair((lox - umbra) - e.xmin, (hix + umbra) - e.xmin, dx, qxl, qxr);
+    div_pair((loy - umbra) - e.ymin, (hiy + umbra) - e.ymin, dy, qyl, qyr);
+    double jl = floor(qxl);
+    double jr = floor(qxr) + 1.0;
+    double il = floor(qyl);
+    double ir = floor(qyr) + 1.0;
     if (jl < 0.0) jl = 0.0;
     if (il < 0.0) il = 0.0;
     if (jr > (double)e.ncols) jr = (double)e.ncols;
@@ -328,8 +331,8 @@ __device__ __forceinline__ int raster_segment(Bits &bits, const EpochReg &e, dou
             const double cay = cy - ay;
             const double perp = bax * cay - bay * cax;
             const double dot = bax * cax + bay * cay;
-            const double al = (perp * perp) / l2;
-            const double be = (dot * dot) / l2;
+            double al, be;
+            div_pair(perp * perp, dot * dot, l2, al, be);
             double d2;
             if (dot < 0.0)
                 d2 = al + be;

@@ -24,6 +24,7 @@ kernels (oka_mlp_*).
 """
 import ctypes as C
 import math
+import threading
 import time
 
 import numpy as np
@@ -36,8 +37,11 @@ from .stochastic import draw_realizations, fit_and_draw
 class SurrogateMLP:
     """n_in -> n_hidden -> n_hidden -> n_out perceptron on the GPU (oka_mlp_*)."""
 
-    def __init__(self, n_in, n_hidden=512, n_out=4096, max_batch=8192, seed=0, device=0):
-        self.ctx = _lib.context(device)
+    def __init__(self, n_in, n_hidden=512, n_out=4096, max_batch=8192, seed=0, device=0,
+                 ctx=None):
+        """ctx: the liboka context (and so the stream) the network lives on;
+        default the process-wide context of `device`."""
+        self.ctx = ctx if ctx is not None else _lib.context(device)
         self.n_in, self.n_hidden, self.n_out = int(n_in), int(n_hidden), int(n_out)
         self.max_batch = int(max_batch)
         h = C.c_void_p()
@@ -292,6 +296,8 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     Returns a ControlVariateResult with p_cv, its estimated variance var_cv,
     the plain estimate p_plain from the same traced realizations with its
     variance var_plain, the window, and the time of each phase (seconds).
+    The folds run concurrently: times["folds"] is their wall time, "train"
+    and "surrogate" the longest fold's time in each of its two phases.
     """
     t = {}
     rng = np.random.default_rng(seed)
@@ -320,30 +326,55 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     surr_seed = int(rng.integers(1, 2 ** 63))
     surr_split = np.linspace(0, n_surrogate, folds + 1).astype(np.int64)
     ncell = window.nrows * window.ncols
+    # one network per fold, each on its own context and stream: the folds'
+    # training and surrogate passes are independent and run concurrently
+    # (one host thread per context), which fills the GPU better than the
+    # small launches of one network alone
+    ctxs = [_lib.context()] + [_lib.Context(_lib.context().device) for _ in range(folds - 1)]
+    nets = [SurrogateMLP(X.shape[1], n_hidden, ncell, seed=seed + 1 + f, ctx=ctxs[f])
+            for f in range(folds)]
+    t["setup"] = time.perf_counter() - t0
+    out = [None] * folds
+
+    def run_fold(f):
+        lo, hi = int(bounds[f]), int(bounds[f + 1])
+        tr = np.concatenate([np.arange(0, lo), np.arange(hi, n_traced)])
+        t1 = time.perf_counter()
+        epochs = max(1, int(math.ceil(train_steps / max(1, len(tr) // batch))))
+        loss = nets[f].train(X, marks, tscale, epochs, batch, lr, seed=seed + 100 + f, rows=tr)
+        t2 = time.perf_counter()
+        mf = int(surr_split[f + 1] - surr_split[f])
+        a_s = nets[f].surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std)
+        a_d = nets[f].accumulate(X[lo:hi], marks[lo:hi], tscale)
+        out[f] = (loss, a_s, a_d, mf, hi - lo, t2 - t1, time.perf_counter() - t2)
+
+    t0 = time.perf_counter()
+    errors = []
+
+    def guarded(f):
+        try:
+            run_fold(f)
+        except BaseException as e:   # re-raised in the calling thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=guarded, args=(f,)) for f in range(1, folds)]
+    for th in threads:
+        th.start()
+    guarded(0)
+    for th in threads:
+        th.join()
+    if errors:
+        raise errors[0]
+    t["folds"] = time.perf_counter() - t0
+    t["train"] = max(o[5] for o in out)
+    t["surrogate"] = max(o[6] for o in out)
     sum_s = np.zeros(ncell)
     var_cv = np.zeros(ncell)
     sum_d = np.zeros(ncell)
-    net = SurrogateMLP(X.shape[1], n_hidden, ncell, seed=seed + 1)
-    t["setup"] = time.perf_counter() - t0
-    t["train"] = 0.0
-    t["surrogate"] = 0.0
     losses = []
     for f in range(folds):
-        lo, hi = int(bounds[f]), int(bounds[f + 1])
-        tr = np.concatenate([np.arange(0, lo), np.arange(hi, n_traced)])
-        t0 = time.perf_counter()
-        if f > 0:   # a fresh network (and Adam state) per fold
-            net.set_params(net.initial_params(seed + 1 + f))
-        epochs = max(1, int(math.ceil(train_steps / max(1, len(tr) // batch))))
-        losses.append(net.train(X, marks, tscale, epochs, batch, lr, seed=seed + 100 + f,
-                                rows=tr))
-        t["train"] += time.perf_counter() - t0
-        t0 = time.perf_counter()
-        mf = int(surr_split[f + 1] - surr_split[f])
-        a_s = net.surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std)
-        a_d = net.accumulate(X[lo:hi], marks[lo:hi], tscale)
-        t["surrogate"] += time.perf_counter() - t0
-        nf = hi - lo
+        loss, a_s, a_d, mf, nf, _, _ = out[f]
+        losses.append(loss)
         w = nf / n_traced
         mean_s = a_s["s"] / mf
         var_s = np.maximum(a_s["s2"] / mf - mean_s ** 2, 0.0)
@@ -352,7 +383,7 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
         sum_s += w * mean_s
         sum_d += w * mean_d
         var_cv += w * w * (var_d / nf + var_s / mf)
-    del net
+    del nets
     t0 = time.perf_counter()
     p_plain, var_plain = plain_estimate(marks, tscale)
     t["plain"] = time.perf_counter() - t0
