@@ -59,9 +59,10 @@ def parse():
     ap.add_argument("--knobs", default=None,
                     help="cfg2/cfg4: pool launch shape SLOTS,WAVES_PER_CU,CHUNK instead of "
                          "autotuning")
-    ap.add_argument("--cfg4-tune-nreal", type=int, default=6000,
-                    help="cfg4: realizations of the autotuning runs (small runs are bound by "
-                         "their single longest trace)")
+    ap.add_argument("--cfg4-tune-nreal", type=int, default=20000,
+                    help="cfg4: realizations of the autotuning runs (runs much smaller than "
+                         "the timed one are bound by their single longest trace and mislead "
+                         "the tuner)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU time budget of each CPU baseline")
@@ -411,7 +412,10 @@ def bench_cfg4(args, torch):
         small = sites.config4(args.cfg4_tune_nreal)
         tb = SiteBatch([small], [sites.draw(small, seed=40)])
         log(f"cfg4: autotuning the pool launch on {args.cfg4_tune_nreal} realizations")
-        best, trials = tune_batch(tb, log=log)
+        # one sweep over the knobs that matter here (each replay takes seconds)
+        search = (("pool_waves_per_cu", (8, 12, 16)), ("pool_slots", (2, 3)),
+                  ("pool_chunk", (32, 64)))
+        best, trials = tune_batch(tb, log=log, search=search, passes=1)
         tb.close()
     run = DeviceRun(site, d, torch)
     log(f"cfg4: {args.cfg4_nreal} realizations: {args.warmup} warmup + {args.steps} timed steps")

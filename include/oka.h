@@ -143,8 +143,9 @@ typedef struct {
     const double *start_xy;  /* npaths x 2 start points (capturezone.py:114-115)     */
     int32_t bitmap_mode;     /* visited-node bitmap of a realization: OKA_BITMAP_*   */
     int32_t tile_slots;      /* OKA_BITMAP_TILED: LDS tile slots (<= 0: as many as fit) */
-    int32_t pool_slots;      /* OKA_SCHED_POOL: realizations in flight per wave (<= 0: 4) */
-    int32_t pool_waves_per_cu; /* OKA_SCHED_POOL: resident waves per CU (<= 0: 8)     */
+    int32_t pool_slots;      /* OKA_SCHED_POOL: realizations in flight per wave (<= 0: 2) */
+    int32_t pool_waves_per_cu; /* OKA_SCHED_POOL: resident waves per CU (<= 0: the most
+                                  of 16, 12, 8, 6, 4 whose LDS share fits the slots) */
     int32_t pool_chunk;      /* OKA_SCHED_POOL: particles per work item (<= 0: 64)   */
     int32_t reserved;
 } oka_trace_opts;

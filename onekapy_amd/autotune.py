@@ -4,8 +4,9 @@ The knobs are the realization chunks in flight per wave (pool_slots), the
 resident waves per CU (pool_waves_per_cu, which also sets each wave's LDS
 share and so its tile pool) and the particles per queue item (pool_chunk).
 The search is coordinate descent from the library's defaults, one knob at a
-time, timing replays of the batch's captured run; knob sets that do not fit
-the LDS are skipped.  Results are cached per batch signature and device in a
+time, timing replays of the batch's captured run, repeated until a sweep over
+all knobs changes nothing (the knobs interact: fewer slots leave room for
+more waves); knob sets that do not fit the LDS are skipped.  Results are cached per batch signature and device in a
 JSON file.
 """
 import json
@@ -15,10 +16,10 @@ from . import _lib
 
 SEARCH = (
     ("pool_chunk", (32, 64, 128, 256)),
-    ("pool_waves_per_cu", (4, 6, 8, 12)),
+    ("pool_waves_per_cu", (4, 6, 8, 12, 16)),
     ("pool_slots", (2, 3, 4, 6)),
 )
-DEFAULT = {"pool_slots": 4, "pool_waves_per_cu": 8, "pool_chunk": 64}
+DEFAULT = {"pool_slots": 2, "pool_waves_per_cu": 16, "pool_chunk": 64}
 
 
 def signature(batch):
@@ -45,9 +46,12 @@ def cached(batch, path):
     return _load(path).get(signature(batch)) if path else None
 
 
-def tune_batch(batch, repeats=1, cache_path=None, search=SEARCH, log=None):
-    """Tunes `batch` in place; returns (best knobs, [(knobs, ms or None)])."""
-    best = dict(DEFAULT)
+def tune_batch(batch, repeats=1, cache_path=None, search=SEARCH, log=None, passes=3,
+               start=None):
+    """Tunes `batch` in place; returns (best knobs, [(knobs, ms or None)]).
+    The search starts from `start` (default DEFAULT); at most `passes`
+    sweeps; a knob set is timed once."""
+    best = dict(DEFAULT if start is None else start)
     trials = []
     seen = {}
 
@@ -67,12 +71,16 @@ def tune_batch(batch, repeats=1, cache_path=None, search=SEARCH, log=None):
         return ms
 
     best_ms = timed(best)
-    for name, values in search:
-        for v in values:
-            cand = dict(best, **{name: v})
-            ms = timed(cand)
-            if ms is not None and (best_ms is None or ms < best_ms):
-                best, best_ms = cand, ms
+    for _ in range(max(1, passes)):
+        start = dict(best)
+        for name, values in search:
+            for v in values:
+                cand = dict(best, **{name: v})
+                ms = timed(cand)
+                if ms is not None and (best_ms is None or ms < best_ms):
+                    best, best_ms = cand, ms
+        if best == start:
+            break
     batch.tune(**best)
     if cache_path:
         data = _load(cache_path)

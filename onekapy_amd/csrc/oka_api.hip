@@ -182,8 +182,22 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
     if (sh.max_tiles > pool_max_tiles())
         return fail(c, OKA_ERR_ARG, "grid too large for the pool schedule (%d tiles)",
                     sh.max_tiles);
-    sh.slots = o->pool_slots > 0 ? std::min(o->pool_slots, 16) : 4;
-    sh.wpc = o->pool_waves_per_cu > 0 ? std::min(o->pool_waves_per_cu, 32) : 8;
+    sh.slots = o->pool_slots > 0 ? std::min(o->pool_slots, 16) : 2;
+    if (o->pool_waves_per_cu > 0) {
+        sh.wpc = std::min(o->pool_waves_per_cu, 32);
+    } else {
+        // default: the most waves per CU (of 16, 12, 8, 6, 4) whose LDS share
+        // still holds the slots' wells and maps and 8 pool tiles per slot
+        const int choice[] = {16, 12, 8, 6, 4};
+        sh.wpc = 4;
+        for (int w : choice) {
+            if (pool_lds_bytes(sh.slots, sh.max_nw, 8) <=
+                std::min(c->lds_per_cu / (size_t)w, c->max_lds)) {
+                sh.wpc = w;
+                break;
+            }
+        }
+    }
     const size_t budget = std::min(c->lds_per_cu / sh.wpc, c->max_lds);
     const size_t fixed = pool_lds_bytes(sh.slots, sh.max_nw, 0);
     if (fixed > budget)

@@ -35,6 +35,7 @@ constexpr int kHash = 1 << kHashBits;                 // map entries per slot
 constexpr int kValBits = 11;
 constexpr uint32_t kValMask = (1u << kValBits) - 1u;  // pool tile, or kValHbm
 constexpr uint32_t kValHbm = kValMask;
+constexpr uint32_t kValNone = 0xffffffffu;              // no map entry (find)
 constexpr uint32_t kKeyEmpty = 0xffffffffu;
 
 struct __align__(16) SlotHdr {   // a multiple of 16 bytes: the float4 wells follow
@@ -48,6 +49,12 @@ struct __align__(16) SlotHdr {   // a multiple of 16 bytes: the float4 wells fol
     int pool_next;      // pool tiles handed out (may exceed the pool)
     int hash_full;      // a tile found no map entry
     float a2, b2, c, d, e;
+    // the site's integrator and raster constants, read by the lanes when
+    // they need them rather than kept in registers per lane
+    double dur, absdur, sg;
+    long long cap;
+    float tol, maxstep, gdx, gdy, igdx, igdy, umbra, u2;
+    int nw, raster, tiles_x;
 };
 
 __device__ __forceinline__ uint32_t tile_hash(uint32_t t)
@@ -63,29 +70,32 @@ struct PoolBits {
     int pool_tiles;
     uint32_t *hbm;      // [ntiles][32]
     int tiles_x;
-    // the last tile this lane found mapped (a mapping lasts until the flush)
-    uint32_t last_t;
-    uint32_t *last_w;
+    // the last tile this lane found mapped and its value (a mapping lasts
+    // until the flush); kept by the lane between steps
+    uint32_t last_t, last_v;
 
     __device__ __forceinline__ uint32_t *tile_words(uint32_t t, uint32_t v) const
     {
         return v == kValHbm ? hbm + (int64_t)t * 32 : pool + v * 32;
     }
-    __device__ __forceinline__ uint32_t *find(uint32_t t) const
+    // Tile t's value (pool tile or kValHbm), or kValNone when unmapped.
+    __device__ __forceinline__ uint32_t find(uint32_t t) const
     {
         uint32_t h = tile_hash(t);
         for (int k = 0; k < kHash; ++k) {
             const uint32_t e = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
             if (e == kKeyEmpty)
-                return nullptr;
+                return kValNone;
             if ((e >> kValBits) == t)
-                return tile_words(t, e & kValMask);
+                return e & kValMask;
             h = (h + 1) & (kHash - 1);
         }
-        return nullptr;
+        return kValNone;
     }
-    __device__ __forceinline__ uint32_t *claim(uint32_t t)
+    // Maps tile t (or finds the mapping another lane made first); kValHbm
+    // when it lives in the HBM region, unmapped if the map is full.
+    __device__ __forceinline__ uint32_t claim(uint32_t t)
     {
         const int v0 = atomicAdd(pool_next, 1);
         const uint32_t v = v0 < pool_tiles ? (uint32_t)v0 : kValHbm;
@@ -94,46 +104,46 @@ struct PoolBits {
         for (int k = 0; k < kHash; ++k) {
             const uint32_t prev = atomicCAS(&keys[h], kKeyEmpty, want);
             if (prev == kKeyEmpty)
-                return tile_words(t, v);
+                return v;
             if ((prev >> kValBits) == t)   // another lane mapped it first
-                return tile_words(t, prev & kValMask);
+                return prev & kValMask;
             h = (h + 1) & (kHash - 1);
         }
         atomicOr(hash_full, 1);   // unmapped: the flush scans the HBM region
-        return hbm + (int64_t)t * 32;
+        return kValHbm;
     }
-    __device__ __forceinline__ uint32_t *lookup(uint32_t t)
+    __device__ __forceinline__ uint32_t lookup(uint32_t t)
     {
         if (t == last_t)
-            return last_w;
-        uint32_t *w = find(t);
-        if (w != nullptr) {
+            return last_v;
+        const uint32_t v = find(t);
+        if (v != kValNone) {
             last_t = t;
-            last_w = w;
+            last_v = v;
         }
-        return w;
+        return v;
     }
     __device__ __forceinline__ bool test(int row, int col)
     {
         const uint32_t t = (uint32_t)((row >> 5) * tiles_x + (col >> 5));
-        const uint32_t *w = lookup(t);
-        if (w == nullptr) {
+        uint32_t v = lookup(t);
+        if (v == kValNone) {
             if (!__hip_atomic_load(hash_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
                 return false;
-            w = hbm + (int64_t)t * 32;
+            v = kValHbm;
         }
-        return (w[row & 31] >> (col & 31)) & 1u;
+        return (tile_words(t, v)[row & 31] >> (col & 31)) & 1u;
     }
     __device__ __forceinline__ void set(int row, int col)
     {
         const uint32_t t = (uint32_t)((row >> 5) * tiles_x + (col >> 5));
-        uint32_t *w = lookup(t);
-        if (w == nullptr) {
-            w = claim(t);
+        uint32_t v = lookup(t);
+        if (v == kValNone) {
+            v = claim(t);
             last_t = t;
-            last_w = w;
+            last_v = v;
         }
-        atomicOr(&w[row & 31], 1u << (col & 31));
+        atomicOr(&tile_words(t, v)[row & 31], 1u << (col & 31));
     }
     __device__ __forceinline__ int stride() const { return 0; }
     // Marks columns c0..c1 (inclusive) of one row, one tile word at a time.
@@ -145,14 +155,15 @@ struct PoolBits {
             const int n = min(c1 - c + 1, 32 - b);
             const uint32_t m = span_mask(n, b);
             const uint32_t t = (uint32_t)(tr + (c >> 5));
-            uint32_t *w = lookup(t);
-            if (w == nullptr) {
-                w = claim(t);
+            uint32_t v = lookup(t);
+            if (v == kValNone) {
+                v = claim(t);
                 last_t = t;
-                last_w = w;
+                last_v = v;
             }
-            if ((w[r] & m) != m)
-                atomicOr(&w[r], m);
+            uint32_t *w = tile_words(t, v) + r;
+            if ((*w & m) != m)
+                atomicOr(w, m);
             c += n;
         }
     }
@@ -186,8 +197,35 @@ __device__ __forceinline__ PoolBits slot_bits(const PoolArgs &P, const PoolLds &
     b.hbm = P.hbm + ((int64_t)blockIdx.x * P.slots + s) * P.max_ntiles * 32;
     b.tiles_x = tiles_x;
     b.last_t = kKeyEmpty;
-    b.last_w = nullptr;
+    b.last_v = kValNone;
     return b;
+}
+
+// Slot s's field and integrator, read from its header.
+__device__ __forceinline__ Field32 slot_field(const PoolArgs &P, const PoolLds &L, int s)
+{
+    const SlotHdr &h = L.hdr[s];
+    Field32 f;
+    f.w = L.wells + (size_t)s * P.max_nwells;
+    f.nw = h.nw;
+    f.a2 = h.a2;
+    f.b2 = h.b2;
+    f.c = h.c;
+    f.d = h.d;
+    f.e = h.e;
+    return f;
+}
+
+__device__ __forceinline__ Integ32 slot_integ(const SlotHdr &h)
+{
+    Integ32 I;
+    I.dur = h.dur;
+    I.absdur = h.absdur;
+    I.sg = h.sg;
+    I.tol = h.tol;
+    I.maxstep = h.maxstep;
+    I.cap = h.cap;
+    return I;
 }
 
 // Site and realization of queue item `item` (site_first: prefix sums of nreal).
@@ -256,6 +294,22 @@ __device__ void open_slot(const PoolArgs &P, const PoolLds &L, int s, int64_t it
         h.rec = nch > 1 ? (int)(h.g % P.nrec) : -1;
         h.pool_next = 0;
         h.hash_full = 0;
+        h.dur = A.duration;
+        h.absdur = fabs(A.duration);
+        h.sg = A.duration > 0.0 ? 1.0
+                                : (A.duration < 0.0 ? -1.0 : (A.duration == 0.0 ? 0.0 : (double)NAN));
+        h.cap = A.max_steps;
+        h.tol = (float)A.tol;
+        h.maxstep = (float)A.maxstep;
+        h.gdx = (float)A.gdx;
+        h.gdy = (float)A.gdy;
+        h.igdx = (float)(1.0 / A.gdx);
+        h.igdy = (float)(1.0 / A.gdy);
+        h.umbra = (float)A.umbra;
+        h.u2 = h.umbra * h.umbra;
+        h.nw = A.nwells;
+        h.raster = A.rasterize != 0;
+        h.tiles_x = A.tiles_x;
     }
     __syncthreads();
     const bool ready = L.hdr[s].nchunks <= 1 || claim_record(P, L.hdr[s]);
@@ -420,7 +474,10 @@ __device__ void merge_slot(const PoolArgs &P, const PoolLds &L, int s)
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
+// At most 128 VGPRs (one 64-bit value spills, in the flush's scan of an
+// overfull map), so that 16 waves fit a CU when the LDS allows them.
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
+k_trace_pool32(PoolArgs P)
 {
     extern __shared__ __align__(16) unsigned char pool_lds[];
     const PoolLds L = carve(P, pool_lds);
@@ -438,18 +495,14 @@ __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
         L.pool[k] = 0u;
     __syncthreads();
 
-    // the lane's trace and its context
+    // the lane's trace; its slot's constants stay in the slot header
     bool live = false, fresh = false;
     bool a_done = false;   // the disk around the trace's current point is marked
     int slot = 0, pidx = 0;
     unsigned long long cand = 0;
     Trace32 tr;
-    Field32 f;
-    Integ32 I;
     Epoch32 ep;
-    PoolBits bits;
-    float gdx = 0.f, gdy = 0.f, igdx = 0.f, igdy = 0.f, umbra = 0.f, u2 = 0.f;
-    bool raster = false;
+    uint32_t last_t = kKeyEmpty, last_v = kValNone;   // the lane's tile cache
     // wave-uniform
     bool queue_done = false;
     int intake = -1;
@@ -477,33 +530,12 @@ __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
                     pidx = next + rank;
                     const SlotHdr &h = L.hdr[slot];
                     const TraceArgs &A = P.sites[h.site];
-                    f.w = L.wells + (size_t)slot * P.max_nwells;
-                    f.nw = A.nwells;
-                    f.a2 = h.a2;
-                    f.b2 = h.b2;
-                    f.c = h.c;
-                    f.d = h.d;
-                    f.e = h.e;
-                    I.dur = A.duration;
-                    I.absdur = fabs(A.duration);
-                    I.sg = A.duration > 0.0 ? 1.0
-                                            : (A.duration < 0.0 ? -1.0 : (A.duration == 0.0 ? 0.0 : (double)NAN));
-                    I.tol = (float)A.tol;
-                    I.maxstep = (float)A.maxstep;
-                    I.cap = A.max_steps;
-                    raster = A.rasterize != 0;
-                    if (raster) {
+                    if (h.raster)
                         ep = find_epoch32(A, h.r * A.npaths + pidx);
-                        bits = slot_bits(P, L, slot, A.tiles_x);
-                        gdx = (float)A.gdx;
-                        gdy = (float)A.gdy;
-                        igdx = (float)(1.0 / A.gdx);
-                        igdy = (float)(1.0 / A.gdy);
-                        umbra = (float)A.umbra;
-                        u2 = umbra * umbra;
-                    }
                     cand = 0;
-                    open32(tr, I, (float)(A.start_xy[2 * pidx] - A.xo),
+                    last_t = kKeyEmpty;
+                    last_v = kValNone;
+                    open32(tr, slot_integ(h), (float)(A.start_xy[2 * pidx] - A.xo),
                            (float)(A.start_xy[2 * pidx + 1] - A.yo));
                     live = fresh = true;
                     a_done = false;
@@ -538,6 +570,9 @@ __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
         // B. one field evaluation per live lane (k1 of a fresh trace, else the
         // first stage of an attempted step), then the rest of the step
         if (live) {
+            const SlotHdr &h = L.hdr[slot];
+            const Field32 f = slot_field(P, L, slot);
+            const Integ32 I = slot_integ(h);
             double dtd;
             float dt, px, py, kx, ky;
             first_point(I, tr, fresh, dtd, dt, px, py);
@@ -551,9 +586,16 @@ __global__ void __launch_bounds__(kWave) k_trace_pool32(PoolArgs P)
             } else {
                 float ox, oy;
                 const int res = finish32(f, I, tr, dtd, dt, kx, ky, ox, oy);
-                if ((res & 1) && raster)
-                    cand += (unsigned long long)raster32(bits, ep, gdx, gdy, igdx, igdy, umbra,
-                                                         u2, ox, oy, tr.x, tr.y, !a_done, a_done);
+                if ((res & 1) && h.raster) {
+                    PoolBits bits = slot_bits(P, L, slot, h.tiles_x);
+                    bits.last_t = last_t;
+                    bits.last_v = last_v;
+                    cand += (unsigned long long)raster32(bits, ep, h.gdx, h.gdy, h.igdx, h.igdy,
+                                                         h.umbra, h.u2, ox, oy, tr.x, tr.y,
+                                                         !a_done, a_done);
+                    last_t = bits.last_t;
+                    last_v = bits.last_v;
+                }
                 if (res & 2) {
                     live = false;
                     ended = true;
