@@ -134,7 +134,7 @@ k_trace_fp32(TraceArgs a)
                 }
             } else {
                 float ox, oy;
-                const int res = finish32(f, I, s, dtd, dt, kx, ky, ox, oy);
+                const int res = finish32(LaneEval{f}, I, s, dtd, dt, kx, ky, ox, oy);
                 if ((res & 1) && a.rasterize)
                     n_cand += (unsigned long long)raster32(bits, ep, gdx, gdy, igdx, igdy, umbra,
                                                            u2, ox, oy, s.x, s.y, !a_done, a_done);

@@ -71,6 +71,15 @@ __device__ __forceinline__ void feval32(const Field32 &f, float x, float y, floa
     ky = vy;
 }
 
+// The lane's own field evaluation, as finish32's evaluator.
+struct LaneEval {
+    const Field32 &f;
+    __device__ __forceinline__ void operator()(float x, float y, float &kx, float &ky) const
+    {
+        feval32(f, x, y, kx, ky);
+    }
+};
+
 struct Trace32 {
     float x, y, cx, cy;   // Kahan sums (x, y) and their compensations (cx, cy)
     float k1x, k1y;
@@ -139,25 +148,27 @@ __device__ __forceinline__ bool take_k1(const Integ32 &I, Trace32 &s, float kx, 
 
 // The rest of one attempted step (capturezone.py:221-247) given k2 = f(first
 // stage): bit 0 of the result = accepted (the segment is (ox, oy) -> (s.x,
-// s.y)), bit 1 = the trace ended.
-__device__ __forceinline__ int finish32(const Field32 &f, const Integ32 &I, Trace32 &s,
+// s.y)), bit 1 = the trace ended.  ev(x, y, kx, ky) evaluates the field (the
+// lane's own evaluation, or one that the whole wave shares).
+template <typename Eval>
+__device__ __forceinline__ int finish32(const Eval &ev, const Integ32 &I, Trace32 &s,
                                         double dtd, float dt, float k2x, float k2y, float &ox,
                                         float &oy)
 {
     s.att += 1;
     const float x = s.x, y = s.y, k1x = s.k1x, k1y = s.k1y;
     float k3x, k3y, k4x, k4y, k5x, k5y, k6x, k6y, k7x, k7y;
-    feval32(f, x + dt * (A31 * k1x + A32 * k2x), y + dt * (A31 * k1y + A32 * k2y), k3x, k3y);
-    feval32(f, x + dt * (A41 * k1x + A42 * k2x + A43 * k3x),
-            y + dt * (A41 * k1y + A42 * k2y + A43 * k3y), k4x, k4y);
-    feval32(f, x + dt * (A51 * k1x + A52 * k2x + A53 * k3x + A54 * k4x),
-            y + dt * (A51 * k1y + A52 * k2y + A53 * k3y + A54 * k4y), k5x, k5y);
-    feval32(f, x + dt * (A61 * k1x + A62 * k2x + A63 * k3x + A64 * k4x + A65 * k5x),
-            y + dt * (A61 * k1y + A62 * k2y + A63 * k3y + A64 * k4y + A65 * k5y), k6x, k6y);
+    ev(x + dt * (A31 * k1x + A32 * k2x), y + dt * (A31 * k1y + A32 * k2y), k3x, k3y);
+    ev(x + dt * (A41 * k1x + A42 * k2x + A43 * k3x),
+       y + dt * (A41 * k1y + A42 * k2y + A43 * k3y), k4x, k4y);
+    ev(x + dt * (A51 * k1x + A52 * k2x + A53 * k3x + A54 * k4x),
+       y + dt * (A51 * k1y + A52 * k2y + A53 * k3y + A54 * k4y), k5x, k5y);
+    ev(x + dt * (A61 * k1x + A62 * k2x + A63 * k3x + A64 * k4x + A65 * k5x),
+       y + dt * (A61 * k1y + A62 * k2y + A63 * k3y + A64 * k4y + A65 * k5y), k6x, k6y);
     const float dxs = dt * (B1 * k1x + B3 * k3x + B4 * k4x + B5 * k5x + B6 * k6x);
     const float dys = dt * (B1 * k1y + B3 * k3y + B4 * k4y + B5 * k5y + B6 * k6y);
     const float xn = x + dxs, yn = y + dys;
-    feval32(f, xn, yn, k7x, k7y);
+    ev(xn, yn, k7x, k7y);
     const float ex = dt * (E1 * k1x + E2 * k7x + E3 * k3x + E4 * k4x + E5 * k5x + E6 * k6x);
     const float ey = dt * (E1 * k1y + E2 * k7y + E3 * k3y + E4 * k4y + E5 * k5y + E6 * k6y);
     const float est = fmaxf(fabsf(ex), fabsf(ey));

@@ -585,7 +585,7 @@ k_trace_pool32(PoolArgs P)
                 }
             } else {
                 float ox, oy;
-                const int res = finish32(f, I, tr, dtd, dt, kx, ky, ox, oy);
+                const int res = finish32(LaneEval{f}, I, tr, dtd, dt, kx, ky, ox, oy);
                 if ((res & 1) && h.raster) {
                     PoolBits bits = slot_bits(P, L, slot, h.tiles_x);
                     bits.last_t = last_t;
