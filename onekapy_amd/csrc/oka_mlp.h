@@ -67,7 +67,12 @@ struct AdamArgs {
 
 constexpr int kMaxSplitSlices = 8;   // split-K partial slices summed by launch_gelu_bwd
 
-__host__ __device__ inline int gemm_tiles(const GemmArgs &g) { return (g.N / 64) * (g.M / 64) * g.splits; }
+// Block tiles of a product at 64 x 64 (big = 0) or 128 x 128 (big = 1).
+__host__ __device__ inline int gemm_tiles(const GemmArgs &g, int big = 0)
+{
+    const int t = big ? 128 : 64;
+    return (g.N / t) * (g.M / t) * g.splits;
+}
 
 // One launch of up to two independent products (the second with e2 != kEpiNone).
 hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s);

@@ -58,41 +58,60 @@ __device__ __forceinline__ float gelu_grad(float z)
     return cdf + z * pdf;
 }
 
-__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// Logistic function with the hardware reciprocal (about 1 ulp; exp(-z) = inf
+// gives 0 and exp(-z) = 0 gives 1).
+__device__ __forceinline__ float sigmoid(float z)
+{
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 
-constexpr int kTile = 64;       // block tile M and N
+constexpr int kTile = 64;       // the smaller block tile (M and N); products are multiples of it
 constexpr int kBK = 64;         // K per LDS stage (two MFMA K steps of 32)
 constexpr int kLd = kBK + 8;    // LDS row stride in BF16 elements (16-byte pad)
 
+// Block tile shapes: kBig = 0 is 64 x 64 (each of the 4 waves 32 x 32, 2 x 2
+// MFMA tiles), kBig = 1 is 128 x 128 (each wave 64 x 64, 4 x 4 MFMA tiles:
+// twice the MFMAs per LDS fragment and half the operand traffic per flop).
+template <int kBig>
+struct Shape {
+    static constexpr int TM = kBig ? 4 : 2, TN = kBig ? 4 : 2;   // MFMA tiles per wave
+    static constexpr int BM = 32 * TM, BN = 32 * TN;             // block tile
+    static constexpr int RA = BM / 32, RB = BN / 32;             // rows loaded per lane
+};
+
+template <int kBig>
 struct GemmSmem {
-    uint16_t a[2][kTile][kLd];  // double-buffered A and B slices
-    uint16_t b[2][kTile][kLd];
+    uint16_t a[2][Shape<kBig>::BM][kLd];  // double-buffered A and B slices
+    uint16_t b[2][Shape<kBig>::BN][kLd];
     double red[4];
-    float col[4][kTile];        // kEpiAccum: the workgroup's column sums
+    float col[4][Shape<kBig>::BN];        // kEpiAccum: the workgroup's column sums
 };
 
 __device__ __forceinline__ uint4 ld16(const uint16_t *p) { return *reinterpret_cast<const uint4 *>(p); }
 __device__ __forceinline__ void st16(uint16_t *p, uint4 v) { *reinterpret_cast<uint4 *>(p) = v; }
 
+template <int kBig>
 struct Stage {
-    uint4 a0, a1, b0, b1;   // rows lr and lr + 32 of the A and B slices
+    uint4 a[Shape<kBig>::RA], b[Shape<kBig>::RB];   // rows lr + 32 r of the A and B slices
 };
 
-// One 64 x 64 tile of C (tile index = (z * M/64 + row tile) * N/64 + column
-// tile).  256 lanes: wave w computes the 32 x 32 quadrant (w / 2, w % 2);
-// lane t loads 16 bytes of rows t/8 and t/8 + 32 of each operand per K stage
-// of 64.  Global loads run two stages ahead of the MFMAs (two register sets)
-// and LDS is double buffered, with one barrier per stage.
-template <int kEpi>
-__device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem &sm)
+// One BM x BN tile of C (tile index = (z * M/BM + row tile) * N/BN + column
+// tile).  256 lanes: wave w computes the quadrant (w / 2, w % 2); lane t
+// loads 16 bytes of rows t/8 + 32 r of each operand per K stage of 64.
+// Global loads run two stages ahead of the MFMAs (two register sets) and LDS
+// is double buffered, with one barrier per stage.
+template <int kEpi, int kBig>
+__device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<kBig> &sm)
 {
+    using S = Shape<kBig>;
+    constexpr int TM = S::TM, TN = S::TN;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int gx = g.N / kTile, gxy = gx * (g.M / kTile);
+    const int gx = g.N / S::BN, gxy = gx * (g.M / S::BM);
     const int bz = tile / gxy;
     const int rem = tile - bz * gxy;
-    const int m0 = (rem / gx) * kTile, n0 = (rem - (rem / gx) * gx) * kTile;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int m0 = (rem / gx) * S::BM, n0 = (rem - (rem / gx) * gx) * S::BN;
+    const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
     const int lr = tid >> 3, lc = (tid & 7) * 8;
     const int kspan = g.K / g.splits;
     const int nst = kspan / kBK;
@@ -100,67 +119,76 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem 
     const uint16_t *pa = g.A + (int64_t)(m0 + lr) * g.lda + lc + (int64_t)bz * kspan;
     const uint16_t *pb = g.B + (int64_t)(n0 + lr) * g.ldb + lc + (int64_t)bz * kspan;
     const bool rowsum = kEpi == kEpiStoreF32 && g.dbias != nullptr && n0 == 0;
-    float rs0 = 0.f, rs1 = 0.f;
+    float rs[S::RA];
+#pragma unroll
+    for (int r = 0; r < S::RA; ++r)
+        rs[r] = 0.f;
 
-    // loss-gradient targets: fetched before the K loop (16 bytes per lane)
-    uint8_t ymk[2][2][4];
+    // loss-gradient targets: fetched before the K loop
+    uint8_t ymk[TN][TM][4];
     const int64_t step = g.dstate ? g.dstate[0] : 0;
     if constexpr (kEpi == kEpiLossGrad) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TN; ++j)
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
                     ymk[j][i][e] = g.marks[(int64_t)(m0 + wm + i * 16 + (lane >> 4) * 4 + e) * g.N +
                                            n0 + wn + j * 16 + (lane & 15)];
     }
 
-    f32x4 acc[2][2];
+    f32x4 acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TN; ++j)
             acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    auto load = [&](Stage &r, int st) {
+    auto load = [&](Stage<kBig> &r, int st) {
         const int64_t o = (int64_t)st * kBK;
-        r.a0 = ld16(pa + o);
-        r.a1 = ld16(pa + a32 + o);
-        r.b0 = ld16(pb + o);
-        r.b1 = ld16(pb + b32 + o);
+#pragma unroll
+        for (int q = 0; q < S::RA; ++q)
+            r.a[q] = ld16(pa + q * a32 + o);
+#pragma unroll
+        for (int q = 0; q < S::RB; ++q)
+            r.b[q] = ld16(pb + q * b32 + o);
     };
-    auto put = [&](const Stage &r, int b) {
-        st16(&sm.a[b][lr][lc], r.a0);
-        st16(&sm.a[b][lr + 32][lc], r.a1);
-        st16(&sm.b[b][lr][lc], r.b0);
-        st16(&sm.b[b][lr + 32][lc], r.b1);
+    auto put = [&](const Stage<kBig> &r, int b) {
+#pragma unroll
+        for (int q = 0; q < S::RA; ++q)
+            st16(&sm.a[b][lr + 32 * q][lc], r.a[q]);
+#pragma unroll
+        for (int q = 0; q < S::RB; ++q)
+            st16(&sm.b[b][lr + 32 * q][lc], r.b[q]);
         if (rowsum) {
-            rs0 += sum8(r.a0);
-            rs1 += sum8(r.a1);
+#pragma unroll
+            for (int q = 0; q < S::RA; ++q)
+                rs[q] += sum8(r.a[q]);
         }
     };
     auto mma = [&](int b) {
 #pragma unroll
         for (int ks = 0; ks < kBK; ks += 32) {
-            bf16x8 a[2], bb[2];
+            bf16x8 a[TM], bb[TN];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < TM; ++i)
                 a[i] = *reinterpret_cast<const bf16x8 *>(
                     &sm.a[b][wm + i * 16 + (lane & 15)][ks + (lane >> 4) * 8]);
-                bb[i] = *reinterpret_cast<const bf16x8 *>(
-                    &sm.b[b][wn + i * 16 + (lane & 15)][ks + (lane >> 4) * 8]);
-            }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < TN; ++j)
+                bb[j] = *reinterpret_cast<const bf16x8 *>(
+                    &sm.b[b][wn + j * 16 + (lane & 15)][ks + (lane >> 4) * 8]);
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
         }
     };
     // stage s is computed from LDS buffer s % 2; register set s % 2 carries
     // stage s + 1 (then s + 3) while stage s is computed
-    Stage r0, r1;
+    Stage<kBig> r0, r1;
     load(r0, 0);
     if (nst > 1)
         load(r1, 1);
@@ -187,30 +215,29 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem 
     if (rowsum) {
         // the 8 lanes loading one row are consecutive
 #pragma unroll
-        for (int off = 1; off < 8; off <<= 1) {
-            rs0 += __shfl_xor(rs0, off);
-            rs1 += __shfl_xor(rs1, off);
-        }
-        if ((tid & 7) == 0) {
-            g.dbias[m0 + lr] = rs0;
-            g.dbias[m0 + lr + 32] = rs1;
+        for (int q = 0; q < S::RA; ++q) {
+#pragma unroll
+            for (int off = 1; off < 8; off <<= 1)
+                rs[q] += __shfl_xor(rs[q], off);
+            if ((tid & 7) == 0)
+                g.dbias[m0 + lr + 32 * q] = rs[q];
         }
     }
 
     // epilogue: lane holds C[m = mb + 4*(lane/16) + e][n = nb + lane%16], e = 0..3
     const bool want_loss = kEpi == kEpiLossGrad && g.loss != nullptr;
-    double part_l = 0.0;
+    float part_l = 0.f;   // the lane's 16 loss terms (monitoring only)
     float *cf = g.Cf;
     if (kEpi == kEpiPartialF32)
         cf += (int64_t)bz * g.M * g.ldc;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < TN; ++j) {
         const int n = n0 + wn + j * 16 + (lane & 15);
         const float bias = (kEpi == kEpiGeluFwd || kEpi == kEpiLogits || kEpi == kEpiLossGrad ||
                             kEpi == kEpiAccum || kEpi == kEpiProbs) ? g.bias[n] : 0.f;
         float col_s = 0.f, col_d = 0.f, col_s2 = 0.f, col_d2 = 0.f;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < TM; ++i) {
             const int mb = m0 + wm + i * 16 + (lane >> 4) * 4;
             uint16_t hv[4];
 #pragma unroll
@@ -235,8 +262,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem 
                     const float y = (float)ymk[j][i][e] * g.tscale;
                     hv[e] = f2bf((sigmoid(z) - y) * g.gscale);
                     g.Ch[(int64_t)m * g.N + n] = hv[e];
-                    if (want_loss)
-                        part_l += (double)(fmaxf(z, 0.f) - z * y + log1pf(__expf(-fabsf(z))));
+                    if (want_loss)   // BCE with logits: max(z, 0) - z y + log(1 + e^-|z|)
+                        part_l += fmaxf(z, 0.f) - z * y + __logf(1.0f + __expf(-fabsf(z)));
                 } else if (kEpi == kEpiAccum || kEpi == kEpiProbs) {
                     if (m < g.mvalid) {
                         const float s = sigmoid(v + bias);
@@ -284,10 +311,11 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem 
         }
     }
     if (want_loss) {   // one atomic per workgroup
+        double pl = (double)part_l;
         for (int off = 32; off > 0; off >>= 1)
-            part_l += __shfl_xor(part_l, off);
+            pl += __shfl_xor(pl, off);
         if (lane == 0)
-            sm.red[wave] = part_l;
+            sm.red[wave] = pl;
         __syncthreads();
         if (tid == 0)
             atomicAdd(g.loss + step, sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3]);
@@ -295,37 +323,39 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem 
 }
 
 // Tiles [0, tiles(g1)) belong to g1, the rest to g2.
-template <int kEpi1, int kEpi2>
+template <int kEpi1, int kEpi2, int kBig>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g1, GemmArgs g2)
 {
-    __shared__ GemmSmem sm;
+    __shared__ GemmSmem<kBig> sm;
     if constexpr (kEpi2 == kEpiNone) {
-        gemm_tile<kEpi1>(g1, blockIdx.x, sm);
+        gemm_tile<kEpi1, kBig>(g1, blockIdx.x, sm);
     } else {
-        const int n1 = gemm_tiles(g1);
+        const int n1 = gemm_tiles(g1, kBig);
         if ((int)blockIdx.x < n1)
-            gemm_tile<kEpi1>(g1, blockIdx.x, sm);
+            gemm_tile<kEpi1, kBig>(g1, blockIdx.x, sm);
         else
-            gemm_tile<kEpi2>(g2, blockIdx.x - n1, sm);
+            gemm_tile<kEpi2, kBig>(g2, blockIdx.x - n1, sm);
     }
 }
 
-// kEpiAccum: workgroup b sums the tiles of column tile b % (N/64) in row
-// tiles [rt * (b / (N/64)), + rt) in LDS, then adds them to the FP64 sums.
+// kEpiAccum: workgroup b sums the tiles of column tile b % (N/BN) in row
+// tiles [rt * (b / (N/BN)), + rt) in LDS, then adds them to the FP64 sums.
+template <int kBig>
 __global__ void __launch_bounds__(256) k_gemm_accum(GemmArgs g, int rt)
 {
-    __shared__ GemmSmem sm;
-    const int gx = g.N / kTile, gy = g.M / kTile;
+    using S = Shape<kBig>;
+    __shared__ GemmSmem<kBig> sm;
+    const int gx = g.N / S::BN, gy = g.M / S::BM;
     const int ct = blockIdx.x % gx, rg = blockIdx.x / gx;
-    for (int i = threadIdx.x; i < 4 * kTile; i += blockDim.x)
-        sm.col[i / kTile][i % kTile] = 0.f;
+    for (int i = threadIdx.x; i < 4 * S::BN; i += blockDim.x)
+        sm.col[i / S::BN][i % S::BN] = 0.f;
     __syncthreads();
     const int end = min(gy, (rg + 1) * rt);
     for (int r = rg * rt; r < end; ++r)
-        gemm_tile<kEpiAccum>(g, r * gx + ct, sm);
+        gemm_tile<kEpiAccum, kBig>(g, r * gx + ct, sm);
     __syncthreads();
-    if (threadIdx.x < kTile) {
-        const int n = ct * kTile + threadIdx.x;
+    if (threadIdx.x < S::BN) {
+        const int n = ct * S::BN + threadIdx.x;
         atomicAdd(&g.sums[n], (double)sm.col[0][threadIdx.x]);
         if (g.sums_sq)
             atomicAdd(&g.sums_sq[n], (double)sm.col[1][threadIdx.x]);
@@ -521,6 +551,24 @@ __global__ void k_zero_f64(double *p, int64_t n)
 
 using namespace mlp;
 
+// 128 x 128 tiles when every product of the launch divides into them and
+// they still give the GPU at least one workgroup per CU; 64 x 64 otherwise.
+static bool use_big_tiles(const GemmArgs &g1, const GemmArgs *g2)
+{
+    static int num_cu = 0;
+    if (num_cu == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            n = 256;
+        num_cu = n > 0 ? n : 256;
+    }
+    auto fits = [](const GemmArgs &g) { return g.M % 128 == 0 && g.N % 128 == 0; };
+    if (!fits(g1) || (g2 && !fits(*g2)))
+        return false;
+    return gemm_tiles(g1, 1) + (g2 ? gemm_tiles(*g2, 1) : 0) >= num_cu;
+}
+
 hipError_t launch_gemm2(int e1, const GemmArgs &g1, int e2, const GemmArgs &g2, hipStream_t s)
 {
     if (g1.M % kTile || g1.N % kTile || g1.splits < 1 || g1.K % (g1.splits * kBK))
@@ -528,18 +576,26 @@ hipError_t launch_gemm2(int e1, const GemmArgs &g1, int e2, const GemmArgs &g2, 
     if (e2 != kEpiNone && (g2.M % kTile || g2.N % kTile || g2.splits < 1 ||
                            g2.K % (g2.splits * kBK)))
         return hipErrorInvalidValue;
+    const bool big = use_big_tiles(g1, e2 == kEpiNone ? nullptr : &g2);
     if (e1 == kEpiAccum && e2 == kEpiNone) {
         constexpr int kRowTiles = 8;
-        const int gy = g1.M / kTile;
-        const dim3 grid((g1.N / kTile) * ((gy + kRowTiles - 1) / kRowTiles));
-        hipLaunchKernelGGL(k_gemm_accum, grid, dim3(256), 0, s, g1, kRowTiles);
+        const int bt = big ? 128 : 64;
+        const int gy = g1.M / bt;
+        const dim3 grid((g1.N / bt) * ((gy + kRowTiles - 1) / kRowTiles));
+        if (big)
+            hipLaunchKernelGGL(k_gemm_accum<1>, grid, dim3(256), 0, s, g1, kRowTiles);
+        else
+            hipLaunchKernelGGL(k_gemm_accum<0>, grid, dim3(256), 0, s, g1, kRowTiles);
         return hipGetLastError();
     }
-    const int n = gemm_tiles(g1) + (e2 == kEpiNone ? 0 : gemm_tiles(g2));
+    const int n = gemm_tiles(g1, big) + (e2 == kEpiNone ? 0 : gemm_tiles(g2, big));
     const dim3 grid(n), block(256);
 #define OKA_GEMM_CASE(E1, E2)                                                                 \
     if (e1 == (E1) && e2 == (E2)) {                                                           \
-        hipLaunchKernelGGL((k_gemm<(E1), (E2)>), grid, block, 0, s, g1, g2);                  \
+        if (big)                                                                              \
+            hipLaunchKernelGGL((k_gemm<(E1), (E2), 1>), grid, block, 0, s, g1, g2);           \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_gemm<(E1), (E2), 0>), grid, block, 0, s, g1, g2);           \
         return hipGetLastError();                                                             \
     }
     OKA_GEMM_CASE(kEpiGeluFwd, kEpiNone)

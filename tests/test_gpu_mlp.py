@@ -42,6 +42,28 @@ def test_gradient_matches_oracle(gpu_ctx):
         assert a @ b / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.9995, name
 
 
+def test_full_size_products_match_oracle(gpu_ctx):
+    """At the control variate's own shape (34 -> 512 -> 512 -> 4096, batch
+    1024) the large products run on 128 x 128 block tiles (the small tests
+    above use 64 x 64): forward probabilities, loss and every gradient
+    against the CPU oracle, with the same bounds."""
+    net = SurrogateMLP(34, 512, 4096, max_batch=1024, seed=11)
+    rng = np.random.default_rng(12)
+    x = rng.normal(0, 1, (1024, 34)).astype(np.float32)
+    marks = rng.integers(0, 26, (1024, 4096)).astype(np.uint8)
+    p = {k: v.astype(np.float64) for k, v in net.split(net.get_params()).items()}
+    ref_loss, ref, ref_probs = mlp_ref.loss_and_grad(p, x, marks / 25.0, bf16=True)
+    assert np.abs(net.predict(x) - ref_probs).max() < 2e-3
+    g, loss = net.grad(x, marks, 1.0 / 25.0)
+    assert abs(loss - ref_loss) < 1e-4 * abs(ref_loss) + 1e-6
+    gs = net.split(g)
+    for name in ("W1", "b1", "W2", "b2", "W3", "b3"):
+        a, b = gs[name].astype(np.float64).ravel(), ref[name].ravel()
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert rel < 2e-2, (name, rel)
+        assert a @ b / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.9995, name
+
+
 def test_training_fits_a_smooth_target(gpu_ctx):
     rng = np.random.default_rng(5)
     nin, nout = 34, 256
