@@ -555,14 +555,12 @@ using namespace mlp;
 // they still give the GPU at least one workgroup per CU; 64 x 64 otherwise.
 static bool use_big_tiles(const GemmArgs &g1, const GemmArgs *g2)
 {
-    static int num_cu = 0;
-    if (num_cu == 0) {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            n = 256;
-        num_cu = n > 0 ? n : 256;
-    }
+    // compute units of the current device (the caller has set it)
+    int dev = 0, num_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        num_cu <= 0)
+        num_cu = 256;
     auto fits = [](const GemmArgs &g) { return g.M % 128 == 0 && g.N % 128 == 0; };
     if (!fits(g1) || (g2 && !fits(*g2)))
         return false;
