@@ -76,9 +76,9 @@ def parse():
     ap.add_argument("--cv-surrogate", type=int, default=10000000)
     ap.add_argument("--cv-judge", type=int, default=1000000,
                     help="realizations of the FP64 plain run that judges the estimates")
-    ap.add_argument("--cv-steps", type=int, default=3000)
-    ap.add_argument("--cv-batch", type=int, default=1024)
-    ap.add_argument("--cv-lr", type=float, default=1.5e-3)
+    ap.add_argument("--cv-steps", type=int, default=1200)
+    ap.add_argument("--cv-batch", type=int, default=2048)
+    ap.add_argument("--cv-lr", type=float, default=5e-3)
     return ap.parse_args()
 
 
