@@ -87,18 +87,27 @@ def control_variate_leg(site, args, fp32_rate):
     + M surrogate evaluations, cross-fitted over 2 folds, judged against an
     FP64 plain run.  Effective realizations/s = the plain-run size with the
     same maximum (over window cells) standard error, divided by the
-    pipeline's wall time (draws, fits, pilot, tracing, training, surrogate)."""
+    pipeline's wall time (draws, fits, pilot, tracing, training, surrogate).
+    Two estimators share the judge: the tuned one (N = --cv-traced, M =
+    --cv-surrogate) and the workload's own sizes (100,000 traced realizations
+    to train on, 10^6 surrogate evaluations)."""
     from onekapy_amd import _lib
     from onekapy_amd.stochastic import draw_realizations, fit_and_draw
     from onekapy_amd.surrogate import cell_probabilities, control_variate, trace_counts
-    print(f"bench: control variate: {args.cv_traced} traced + {args.cv_surrogate} surrogate",
-          file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    res = control_variate(site, args.cv_traced, args.cv_surrogate, folds=2,
-                          train_steps=args.cv_steps, batch=args.cv_batch, lr=args.cv_lr, seed=123,
-                          precision=_lib.FP32)
-    wall = time.perf_counter() - t0
-    print(f"bench: control variate {wall:.2f} s {res.times}", file=sys.stderr, flush=True)
+
+    def run(n_traced, n_surrogate):
+        print(f"bench: control variate: {n_traced} traced + {n_surrogate} surrogate",
+              file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        res = control_variate(site, n_traced, n_surrogate, folds=2, train_steps=args.cv_steps,
+                              batch=args.cv_batch, lr=args.cv_lr, seed=123,
+                              precision=_lib.FP32)
+        wall = time.perf_counter() - t0
+        print(f"bench: control variate {wall:.2f} s {res.times}", file=sys.stderr, flush=True)
+        return res, wall
+
+    tuned = run(args.cv_traced, args.cv_surrogate)
+    configured = run(100000, 1000000)
     rng = np.random.default_rng(321)
     d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, args.cv_judge, rng)
     _, _, coef = fit_and_draw(site.wells_xy(), site.xt, site.yt, site.base, site.observations,
@@ -107,30 +116,37 @@ def control_variate_leg(site, args, fp32_rate):
     tj = time.perf_counter()
     counts, _ = trace_counts(site, d, coef, _lib.FP64_EXACT)
     tj = time.perf_counter() - tj
-    p_ref = cell_probabilities(counts, args.cv_judge, res.window)
-    err_cv = np.abs(res.p_cv - p_ref)
-    err_plain = np.abs(res.p_plain - p_ref)
-    n_eff = args.cv_traced * res.var_plain.max() / res.var_cv.max()
-    eff_rate = n_eff / wall
-    return {
-        "traced": args.cv_traced, "surrogate_evals": args.cv_surrogate, "folds": 2,
-        "train_steps_per_fold": args.cv_steps, "batch": args.cv_batch, "lr": args.cv_lr,
-        "network": "34->512->512->4096 GELU, BF16 MFMA",
-        "window": {"row0": res.window.row0, "col0": res.window.col0, "cell_nodes": res.window.cell,
-                   "cells": [res.window.nrows, res.window.ncols]},
-        "judge": {"realizations": args.cv_judge, "tracer": "fp64", "seconds": tj},
-        "max_error_cv": float(err_cv.max()), "l1_error_cv": float(err_cv.sum()),
-        "max_error_plain_same_traced": float(err_plain.max()),
-        "l1_error_plain_same_traced": float(err_plain.sum()),
-        "max_std_cv": float(np.sqrt(res.var_cv.max())),
-        "max_std_plain_same_traced": float(np.sqrt(res.var_plain.max())),
-        "variance_ratio_at_max": float(res.var_plain.max() / res.var_cv.max()),
-        "variance_ratio_sum": float(res.var_plain.sum() / res.var_cv.sum()),
-        "effective_realizations": float(n_eff),
-        "wall_seconds": wall, "phase_seconds": res.times,
-        "effective_realizations_per_s": float(eff_rate),
-        "vs_fp32_plain_rate": float(eff_rate / fp32_rate) if fp32_rate else None,
-    }
+
+    def summary(res, wall, n_traced, n_surrogate):
+        p_ref = cell_probabilities(counts, args.cv_judge, res.window)
+        err_cv = np.abs(res.p_cv - p_ref)
+        err_plain = np.abs(res.p_plain - p_ref)
+        n_eff = n_traced * res.var_plain.max() / res.var_cv.max()
+        eff_rate = n_eff / wall
+        return {
+            "traced": n_traced, "surrogate_evals": n_surrogate,
+            "window": {"row0": res.window.row0, "col0": res.window.col0,
+                       "cell_nodes": res.window.cell,
+                       "cells": [res.window.nrows, res.window.ncols]},
+            "max_error_cv": float(err_cv.max()), "l1_error_cv": float(err_cv.sum()),
+            "max_error_plain_same_traced": float(err_plain.max()),
+            "l1_error_plain_same_traced": float(err_plain.sum()),
+            "max_std_cv": float(np.sqrt(res.var_cv.max())),
+            "max_std_plain_same_traced": float(np.sqrt(res.var_plain.max())),
+            "variance_ratio_at_max": float(res.var_plain.max() / res.var_cv.max()),
+            "variance_ratio_sum": float(res.var_plain.sum() / res.var_cv.sum()),
+            "effective_realizations": float(n_eff),
+            "wall_seconds": wall, "phase_seconds": res.times,
+            "effective_realizations_per_s": float(eff_rate),
+            "vs_fp32_plain_rate": float(eff_rate / fp32_rate) if fp32_rate else None,
+        }
+
+    out = summary(*tuned, args.cv_traced, args.cv_surrogate)
+    out.update({"folds": 2, "train_steps_per_fold": args.cv_steps, "batch": args.cv_batch,
+                "lr": args.cv_lr, "network": "34->512->512->4096 GELU, BF16 MFMA",
+                "judge": {"realizations": args.cv_judge, "tracer": "fp64", "seconds": tj}})
+    out["as_configured"] = summary(*configured, 100000, 1000000)
+    return out
 
 
 def copy_calibration(torch):
