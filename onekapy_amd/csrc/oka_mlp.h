@@ -48,6 +48,10 @@ struct AdamSeg {
     int64_t off;
     int rows, cols;
     uint16_t *wh, *wt;
+    // nslices > 0: the segment's gradient is the sum, in slice order, of
+    // nslices split-K partial slices [rows x cols] at `slices`, not grad + off
+    const float *slices;
+    int nslices;
 };
 
 struct AdamArgs {

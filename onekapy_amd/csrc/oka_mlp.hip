@@ -118,7 +118,9 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
     const int64_t a32 = 32 * (int64_t)g.lda, b32 = 32 * (int64_t)g.ldb;
     const uint16_t *pa = g.A + (int64_t)(m0 + lr) * g.lda + lc + (int64_t)bz * kspan;
     const uint16_t *pb = g.B + (int64_t)(n0 + lr) * g.ldb + lc + (int64_t)bz * kspan;
-    const bool rowsum = kEpi == kEpiStoreF32 && g.dbias != nullptr && n0 == 0;
+    // the bias gradient: row sums of the A slices (per K split for partials)
+    const bool rowsum = (kEpi == kEpiStoreF32 || kEpi == kEpiPartialF32) && g.dbias != nullptr &&
+                        n0 == 0;
     float rs[S::RA];
 #pragma unroll
     for (int r = 0; r < S::RA; ++r)
@@ -220,7 +222,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
             for (int off = 1; off < 8; off <<= 1)
                 rs[q] += __shfl_xor(rs[q], off);
             if ((tid & 7) == 0)
-                g.dbias[m0 + lr + 32 * q] = rs[q];
+                g.dbias[(kEpi == kEpiPartialF32 ? (int64_t)bz * g.M : 0) + m0 + lr + 32 * q] =
+                    rs[q];
         }
     }
 
@@ -441,7 +444,15 @@ __global__ void __launch_bounds__(256) k_adam(AdamArgs a)
         const int r = r0 + k, c = c0 + tx;
         if (r < sg.rows && c < sg.cols) {
             const int64_t i = sg.off + (int64_t)r * sg.cols + c;
-            const float gi = a.grad[i];
+            float gi;
+            if (sg.nslices > 0) {
+                const int64_t e = (int64_t)r * sg.cols + c, sz = (int64_t)sg.rows * sg.cols;
+                gi = sg.slices[e];
+                for (int k = 1; k < sg.nslices; ++k)
+                    gi += sg.slices[k * sz + e];
+            } else {
+                gi = a.grad[i];
+            }
             const float mi = a.b1 * a.m[i] + (1.f - a.b1) * gi;
             const float vi = a.b2 * a.v[i] + (1.f - a.b2) * gi * gi;
             a.m[i] = mi;

@@ -27,6 +27,8 @@ struct oka_mlp {
     uint8_t *ym = nullptr;       // the batch's target rows [max_batch x n_out]
     float *acc = nullptr;        // split-K partial slices, acc_cap floats
     size_t acc_cap = 0;
+    float *w1s = nullptr;        // training: dW1 and db1 split-K slices (kMaxSplitSlices each)
+    float *b1s = nullptr;
     int64_t *dstate = nullptr;   // graph replay: [0] step within the call, [1] steps begun
     int64_t step = 0;
     std::vector<void *> allocs;
@@ -128,9 +130,21 @@ int hidden_forward(oka_mlp *M, int Bp, int valid, const float *x, const int32_t 
 // Loss (when `loss` is set) and full gradient (into M->grad) for one batch of
 // B rows: 7 launches, each dW product sharing its launch with the layer's dX
 // product and forming its bias gradient.
+// dW1 (the first layer's weight gradient: 8 output tiles over a K of B) in
+// `w1_splits` split-K slices of M->w1s / M->b1s when above 1, for Adam to sum.
+int w1_split_plan(int B)
+{
+    int s = 1;
+    while (s < kMaxSplitSlices && B % (2 * s * 64) == 0 && B / (2 * s) >= 256)
+        s *= 2;
+    return s;
+}
+
+// Loss (when `loss` is set) and full gradient for one batch of B rows: into
+// M->grad, except dW1 and db1 when w1_splits > 1 (their slices, for Adam).
 int forward_backward(oka_mlp *M, int B, const float *x, const uint8_t *marks, float tscale,
                      const int32_t *rowidx, int64_t row0, double *loss,
-                     int64_t *dstate = nullptr)
+                     int64_t *dstate = nullptr, int w1_splits = 1)
 {
     oka_ctx *c = M->ctx;
     hipStream_t s = c->stream;
@@ -165,7 +179,14 @@ int forward_backward(oka_mlp *M, int B, const float *x, const uint8_t *marks, fl
         dw.ldc = L.nin_l;
         dw.dbias = M->grad + L.ob;
         if (!L.g) {
-            HIPCHK(c, launch_gemm(kEpiStoreF32, dw, s));
+            if (w1_splits > 1) {
+                dw.Cf = M->w1s;
+                dw.dbias = M->b1s;
+                dw.splits = w1_splits;
+                HIPCHK(c, launch_gemm(kEpiPartialF32, dw, s));
+            } else {
+                HIPCHK(c, launch_gemm(kEpiStoreF32, dw, s));
+            }
             break;
         }
         GemmArgs dx;
@@ -188,6 +209,7 @@ struct TrainStep {
     const int32_t *order;
     double *loss;
     const AdamArgs *adam;
+    int w1_splits;   // dW1 / db1 as split-K slices that Adam sums
 };
 
 constexpr int kStepsPerGraph = 16;
@@ -202,7 +224,8 @@ int capture_steps(oka_mlp *M, const TrainStep &t, int steps, hipGraphExec_t *out
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = OKA_OK;
     for (int k = 0; k < steps && rc == OKA_OK; ++k) {
-        rc = forward_backward(M, t.batch, t.x, t.marks, t.tscale, t.order, 0, t.loss, M->dstate);
+        rc = forward_backward(M, t.batch, t.x, t.marks, t.tscale, t.order, 0, t.loss, M->dstate,
+                              t.w1_splits);
         if (rc == OKA_OK && launch_adam(*t.adam, c->stream) != hipSuccess)
             rc = fail(c, OKA_ERR_DEVICE, "adam launch failed");
     }
@@ -317,6 +340,8 @@ int oka_mlp_create(oka_ctx *c, int32_t n_in, int32_t n_hidden, int32_t n_out, in
     rc = rc ? rc : alloc_t(M, B * M->nout, &M->ym);
     M->acc_cap = std::max(B * M->nh, (size_t)4 * c->num_cu * 64 * 64);
     rc = rc ? rc : alloc_t(M, M->acc_cap, &M->acc);
+    rc = rc ? rc : alloc_t(M, (size_t)kMaxSplitSlices * M->nh * M->kin, &M->w1s);
+    rc = rc ? rc : alloc_t(M, (size_t)kMaxSplitSlices * M->nh, &M->b1s);
     rc = rc ? rc : alloc_t(M, 2, &M->dstate);
     if (rc == OKA_OK) {
         // zero state (padding columns of W1 must stay exactly zero)
@@ -442,15 +467,21 @@ int oka_mlp_train(oka_mlp *M, int64_t nsamples, const float *x, const uint8_t *m
         a.eps = eps;
         a.tbase = M->step;
         a.dstate = M->dstate;
+        const int w1s = w1_split_plan(batch);
+        const float *sw = w1s > 1 ? M->w1s : nullptr, *sb = w1s > 1 ? M->b1s : nullptr;
+        const int ns = w1s > 1 ? w1s : 0;
         const AdamSeg segs[6] = {
-            {M->oW1, M->nh, M->kin, M->w1h, M->w1t}, {M->ob1, M->nh / 32, 32, nullptr, nullptr},
-            {M->oW2, M->nh, M->nh, M->w2h, M->w2t},  {M->ob2, M->nh / 32, 32, nullptr, nullptr},
-            {M->oW3, M->nout, M->nh, M->w3h, M->w3t}, {M->ob3, M->nout / 32, 32, nullptr, nullptr}};
+            {M->oW1, M->nh, M->kin, M->w1h, M->w1t, sw, ns},
+            {M->ob1, M->nh / 32, 32, nullptr, nullptr, sb, ns},
+            {M->oW2, M->nh, M->nh, M->w2h, M->w2t, nullptr, 0},
+            {M->ob2, M->nh / 32, 32, nullptr, nullptr, nullptr, 0},
+            {M->oW3, M->nout, M->nh, M->w3h, M->w3t, nullptr, 0},
+            {M->ob3, M->nout / 32, 32, nullptr, nullptr, nullptr, 0}};
         for (int i = 0; i < 6; ++i)
             a.seg[i] = segs[i];
         a.nseg = 6;
         const TrainStep ts{batch, (const float *)dx, (const uint8_t *)dm, target_scale,
-                           (const int32_t *)dord, (double *)dl, &a};
+                           (const int32_t *)dord, (double *)dl, &a, w1s};
         const int unroll = std::min<int32_t>(nsteps, kStepsPerGraph);
         const int full = nsteps / unroll, rest = nsteps % unroll;
         hipGraphExec_t ex_full = nullptr, ex_rest = nullptr;

@@ -64,6 +64,34 @@ def test_full_size_products_match_oracle(gpu_ctx):
         assert a @ b / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.9995, name
 
 
+def test_training_step_gradient_matches_oracle(gpu_ctx):
+    """The captured training step computes dW1 and db1 in split-K slices that
+    the Adam kernel sums (the gradient call does not).  With eps far above
+    every gradient, one Adam step moves each parameter by -lr g / eps (m = g,
+    v-hat = g^2 at step 1), so the step's own gradient can be read back and
+    checked against the CPU oracle like the gradient call's."""
+    nin, nh, nout, B = 34, 512, 4096, 1024
+    net = SurrogateMLP(nin, nh, nout, max_batch=B, seed=21)
+    rng = np.random.default_rng(22)
+    x = rng.normal(0, 1, (B, nin)).astype(np.float32)
+    marks = rng.integers(0, 26, (B, nout)).astype(np.uint8)
+    p0 = net.get_params().astype(np.float64)
+    order = np.arange(B, dtype=np.int32)
+    losses = np.zeros(1, dtype=np.float32)
+    eps, lr = 1e3, 1e3     # the step is g / (1 + |g| / eps): g itself, to 1e-3
+    net.ctx.check(net.ctx.lib.oka_mlp_train(
+        net.h, B, x.ctypes.data, marks.ctypes.data, 1.0 / 25.0, order.ctypes.data, B, 1, lr,
+        0.9, 0.999, eps, losses.ctypes.data, 0))
+    step_grad = (p0 - net.get_params().astype(np.float64)) * eps / lr
+    p = {k: v for k, v in net.split(p0).items()}
+    _, ref, _ = mlp_ref.loss_and_grad(p, x, marks / 25.0, bf16=True)
+    gs = net.split(step_grad)
+    for name in ("W1", "b1", "W2", "b2", "W3", "b3"):
+        a, b = gs[name].ravel(), ref[name].ravel()
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert rel < 2e-2, (name, rel)
+
+
 def test_training_fits_a_smooth_target(gpu_ctx):
     rng = np.random.default_rng(5)
     nin, nout = 34, 256
