@@ -70,6 +70,7 @@ enum {
 #define OKA_PTR_DEVICE 0x1u
 #define OKA_ASYNC 0x2u
 #define OKA_RASTER_FP32 0x4u   /* oka_rasterize: the FP32 tracers' rasterizer */
+#define OKA_ANTITHETIC 0x8u    /* oka_draw_realizations, oka_cv_surrogate_sums: antithetic pairs */
 
 typedef struct oka_ctx oka_ctx;
 
@@ -373,7 +374,11 @@ typedef struct {
 
 /* Realizations first .. first + n - 1 of the stream `seed`: q (n x nwells),
  * K, por (porosity), thick (n) and z (n x 6 standard normals).  dists holds
- * nwells discharge distributions then conductivity, porosity, thickness. */
+ * nwells discharge distributions then conductivity, porosity, thickness.
+ * With OKA_ANTITHETIC in flags (n and first even), realization 2k + 1 is the
+ * antithetic partner of 2k: it takes 2k's Philox uniforms u as 1 - u (the
+ * inverse-CDF and sign variates) and 2k's standard normals (z and the
+ * lognormal's) negated; realization 2k is drawn as without the flag. */
 int oka_draw_realizations(oka_ctx *ctx, int32_t nwells, const oka_dist *dists, int64_t n,
                           uint64_t seed, int64_t first, double *q, double *K, double *por,
                           double *thick, double *z, uint32_t flags);
@@ -400,7 +405,10 @@ int oka_count_histogram(oka_ctx *ctx, int64_t n, const int32_t *counts, int32_t 
  * features (q, K, porosity, thickness, A..F) -> (f - mean) / std, network, and
  * per output c: sums[c] += sum of s_c, sums_sq[c] += sum of s_c^2.  site and
  * dists are host memory; feat_mean / feat_std (n_in doubles) too; sums and
- * sums_sq follow flags. */
+ * sums_sq follow flags.  With OKA_ANTITHETIC (n and first even) the
+ * realizations are drawn in antithetic pairs as in oka_draw_realizations,
+ * and sums_sq[c] += sum over the n / 2 pairs of ((s_c,2k + s_c,2k+1) / 2)^2,
+ * so that the variance of the sums' mean is that of the pair means / (n / 2). */
 int oka_cv_surrogate_sums(oka_ctx *ctx, oka_mlp *mlp, const oka_site *site, const oka_dist *dists,
                           int64_t n, uint64_t seed, int64_t first, const double *feat_mean,
                           const double *feat_std, double *sums, double *sums_sq, uint32_t flags);

@@ -25,6 +25,7 @@ FP64_EXACT, FP32 = 0, 1
 PTR_DEVICE = 0x1
 ASYNC = 0x2
 RASTER_FP32 = 0x4      # oka_rasterize: the FP32 tracers' rasterizer
+ANTITHETIC = 0x8       # draws / surrogate sums in antithetic pairs (2k, 2k + 1)
 
 _d = C.c_double
 _i32 = C.c_int32

@@ -351,6 +351,7 @@ struct DrawArgs {
     int64_t n, first;
     uint64_t seed;
     double *q, *K, *por, *thick, *z;
+    int32_t antithetic;         // odd realizations reflect their even partner's draws
 };
 
 struct FeatureArgs {

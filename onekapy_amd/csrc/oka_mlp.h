@@ -23,6 +23,7 @@ struct GemmArgs {
     const uint16_t *B;       // [N, K] BF16, row stride ldb
     int M, N, K, lda, ldb;
     int mvalid;              // rows >= mvalid are padding (kEpiAccum, kEpiProbs)
+    int pairs;               // kEpiAccum: sums_sq takes ((s_2k + s_2k+1) / 2)^2 per row pair
     int splits;              // K ranges (kEpiPartialF32), else 1
     const float *bias;       // [N]
     float *Cf;               // FP32 output, row stride ldc

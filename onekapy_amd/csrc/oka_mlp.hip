@@ -239,6 +239,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
         const float bias = (kEpi == kEpiGeluFwd || kEpi == kEpiLogits || kEpi == kEpiLossGrad ||
                             kEpi == kEpiAccum || kEpi == kEpiProbs) ? g.bias[n] : 0.f;
         float col_s = 0.f, col_d = 0.f, col_s2 = 0.f, col_d2 = 0.f;
+        float s_prev = 0.f;   // kEpiAccum with pairs: the pair's even row
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
             const int mb = m0 + wm + i * 16 + (lane >> 4) * 4;
@@ -274,7 +275,13 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
                             cf[(int64_t)m * g.ldc + n] = s;
                         } else {
                             col_s += s;
-                            col_s2 += s * s;
+                            if (!g.pairs) {
+                                col_s2 += s * s;
+                            } else if (e & 1) {   // rows mb + e - 1, mb + e: one pair
+                                const float pm = 0.5f * (s_prev + s);
+                                col_s2 += pm * pm;
+                            }
+                            s_prev = s;
                             if (g.marks) {
                                 const int64_t row = g.row0 + m;
                                 const float dv = (float)g.marks[row * g.N + n] * g.tscale - s;

@@ -634,6 +634,8 @@ int oka_draw_realizations(oka_ctx *c, int32_t nwells, const oka_dist *dists, int
     TRY(check_dists(c, nwells, dists));
     if (n < 0 || first < 0)
         return fail(c, OKA_ERR_ARG, "negative count or offset");
+    if ((flags & OKA_ANTITHETIC) && ((n | first) & 1))
+        return fail(c, OKA_ERR_ARG, "antithetic draws need an even count and offset");
     if (n == 0)
         return OKA_OK;
     if ((nwells > 0 && !q) || !K || !por || !thick || !z)
@@ -644,6 +646,7 @@ int oka_draw_realizations(oka_ctx *c, int32_t nwells, const oka_dist *dists, int
     a.n = n;
     a.first = first;
     a.seed = seed;
+    a.antithetic = (flags & OKA_ANTITHETIC) != 0;
     const void *dd;
     TRY(stage_in(c, kSlotCvDists, dists, sizeof(oka_dist) * (nwells + 3), 0, &dd));
     a.dists = (const oka_dist *)dd;
@@ -711,6 +714,11 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
         return fail(c, OKA_ERR_ARG, "site needs wells and at least 6 observations");
     if (n < 0 || first < 0)
         return fail(c, OKA_ERR_ARG, "negative count or offset");
+    const bool pairs = (flags & OKA_ANTITHETIC) != 0;
+    // pairs (2k, 2k + 1) never straddle a chunk or a network pass: both are
+    // multiples of max_batch (a multiple of 64) from an even `first`
+    if (pairs && ((n | first) & 1))
+        return fail(c, OKA_ERR_ARG, "antithetic draws need an even count and offset");
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
     // site, distributions, standardization (host memory)
@@ -778,6 +786,7 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
         da.por = por;
         da.thick = thick;
         da.z = z;
+        da.antithetic = pairs;
         HIPCHK(c, launch_draw(da, s));
         fa.nreal = valid;
         HIPCHK(c, launch_fit_draw(fa, s));
@@ -800,6 +809,7 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
             GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
             g.bias = M->p + M->ob3;
             g.mvalid = nv;
+            g.pairs = pairs;
             g.sums = d_sums;
             g.sums_sq = d_sq;
             HIPCHK(c, launch_gemm(kEpiAccum, g, s));

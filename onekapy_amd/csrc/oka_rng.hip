@@ -45,25 +45,31 @@ __device__ inline void uniforms(uint64_t seed, int64_t r, uint32_t slot, uint32_
     u2 = ((double)(b & ((1ull << 53) - 1)) + 0.5) * 0x1p-53;
 }
 
-__device__ inline double sample(const oka_dist &d, uint64_t seed, int64_t r, uint32_t slot)
+// With `flip` the draw is the antithetic partner of realization r's: every
+// uniform u becomes 1 - u (exact: u = (k + 1/2) 2^-53 maps to the lattice
+// point of 2^53 - 1 - k) and the lognormal's normal g becomes -g, so the
+// marginal distribution is the same and the pair is negatively correlated.
+__device__ inline double sample(const oka_dist &d, uint64_t seed, int64_t r, uint32_t slot,
+                                bool flip)
 {
     double u1, u2;
     uniforms(seed, r, slot, 0, u1, u2);
+    const double ui = flip ? 1.0 - u1 : u1;   // the inverse-CDF variate
     double v, us = u2;
     switch (d.kind) {
     case OKA_DIST_UNIFORM:
-        v = d.a + (d.b - d.a) * u1;
+        v = d.a + (d.b - d.a) * ui;
         break;
     case OKA_DIST_TRIANGULAR: {
         // inverse CDF of triangular(lo = a, mode = b, hi = c)
         const double fc = (d.b - d.a) / (d.c - d.a);
-        v = u1 < fc ? d.a + sqrt(u1 * (d.c - d.a) * (d.b - d.a))
-                    : d.c - sqrt((1.0 - u1) * (d.c - d.a) * (d.c - d.b));
+        v = ui < fc ? d.a + sqrt(ui * (d.c - d.a) * (d.b - d.a))
+                    : d.c - sqrt((1.0 - ui) * (d.c - d.a) * (d.c - d.b));
         break;
     }
     case OKA_DIST_LOGNORMAL: {
         const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-        v = exp(d.a + d.b * g);
+        v = exp(d.a + d.b * (flip ? -g : g));
         double w;
         uniforms(seed, r, slot, 1, us, w);
         break;
@@ -72,6 +78,8 @@ __device__ inline double sample(const oka_dist &d, uint64_t seed, int64_t r, uin
         v = d.a;
         break;
     }
+    if (flip)
+        us = 1.0 - us;
     if (d.p_negative > 0.0 && us < d.p_negative)
         v = -v;
     return v;
@@ -82,21 +90,24 @@ __global__ void __launch_bounds__(256) k_draw(DrawArgs a)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n)
         return;
-    const int64_t r = a.first + i;
+    // antithetic pairs (2k, 2k + 1): the odd member reuses counter 2k, reflected
+    const bool flip = a.antithetic && (a.first + i) % 2 == 1;
+    const int64_t r = flip ? a.first + i - 1 : a.first + i;
     const int nw = a.nwells;
     for (int w = 0; w < nw; ++w)
-        a.q[i * nw + w] = sample(a.dists[w], a.seed, r, (uint32_t)w);
-    a.K[i] = sample(a.dists[nw], a.seed, r, (uint32_t)nw);
-    a.por[i] = sample(a.dists[nw + 1], a.seed, r, (uint32_t)nw + 1);
-    a.thick[i] = sample(a.dists[nw + 2], a.seed, r, (uint32_t)nw + 2);
+        a.q[i * nw + w] = sample(a.dists[w], a.seed, r, (uint32_t)w, flip);
+    a.K[i] = sample(a.dists[nw], a.seed, r, (uint32_t)nw, flip);
+    a.por[i] = sample(a.dists[nw + 1], a.seed, r, (uint32_t)nw + 1, flip);
+    a.thick[i] = sample(a.dists[nw + 2], a.seed, r, (uint32_t)nw + 2, flip);
+    const double sg = flip ? -1.0 : 1.0;
     for (int j = 0; j < 3; ++j) {
         double u1, u2;
         uniforms(a.seed, r, (uint32_t)(nw + 3 + j), 0, u1, u2);
         const double rad = sqrt(-2.0 * log(u1));
         double sn, cs;
         sincos(6.283185307179586 * u2, &sn, &cs);
-        a.z[i * 6 + 2 * j] = rad * cs;
-        a.z[i * 6 + 2 * j + 1] = rad * sn;
+        a.z[i * 6 + 2 * j] = sg * (rad * cs);
+        a.z[i * 6 + 2 * j + 1] = sg * (rad * sn);
     }
 }
 

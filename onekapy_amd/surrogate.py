@@ -149,9 +149,11 @@ class SurrogateMLP:
             acc["d2"].ctypes.data if mp else None, 0))
         return acc
 
-    def surrogate_sums(self, site, n, seed, first, std):
+    def surrogate_sums(self, site, n, seed, first, std, antithetic=False):
         """Sums of s and s^2 over n fresh realizations drawn, fitted and
-        featurized on the device (oka_cv_surrogate_sums)."""
+        featurized on the device (oka_cv_surrogate_sums).  With antithetic
+        the realizations come in reflected pairs and "s2" sums the squared
+        pair means (n / 2 of them)."""
         from .stochastic import oka_dists
         acc = {"s": np.zeros(self.n_out), "s2": np.zeros(self.n_out)}
         wxy = site.wells_xy()
@@ -163,7 +165,7 @@ class SurrogateMLP:
         self.ctx.check(self.ctx.lib.oka_cv_surrogate_sums(
             self.ctx.h, self.h, s, C.addressof(dists), int(n), int(seed) & (2 ** 64 - 1),
             int(first), mean.ctypes.data, sd.ctypes.data, acc["s"].ctypes.data,
-            acc["s2"].ctypes.data, 0))
+            acc["s2"].ctypes.data, _lib.ANTITHETIC if antithetic else 0))
         if not np.all(np.isfinite(acc["s"])):
             raise ArithmeticError("the fit failed for some surrogate draws")
         return acc
@@ -287,11 +289,14 @@ class ControlVariateResult:
 
 
 def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batch=512, lr=1e-3,
-                    seed=0, precision=_lib.FP32, pilot=2000, window=None, n_hidden=512):
+                    seed=0, precision=_lib.FP32, pilot=2000, window=None, n_hidden=512,
+                    antithetic=False):
     """Cross-fitted control-variate estimate of the window cells' capture probability.
 
     Each fold's network takes train_steps Adam steps of `batch` samples (whole
-    shuffled epochs, rounded up).
+    shuffled epochs, rounded up).  With antithetic, the surrogate term draws
+    its realizations in antithetic pairs and its variance is that of the pair
+    means over the number of pairs.
 
     Returns a ControlVariateResult with p_cv, its estimated variance var_cv,
     the plain estimate p_plain from the same traced realizations with its
@@ -325,6 +330,8 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     bounds = np.linspace(0, n_traced, folds + 1).astype(np.int64)
     surr_seed = int(rng.integers(1, 2 ** 63))
     surr_split = np.linspace(0, n_surrogate, folds + 1).astype(np.int64)
+    if antithetic:   # pairs (2k, 2k + 1) stay within one fold
+        surr_split = surr_split // 2 * 2
     ncell = window.nrows * window.ncols
     # one network per fold, each on its own context and stream: the folds'
     # training and surrogate passes are independent and run concurrently
@@ -344,7 +351,7 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
         loss = nets[f].train(X, marks, tscale, epochs, batch, lr, seed=seed + 100 + f, rows=tr)
         t2 = time.perf_counter()
         mf = int(surr_split[f + 1] - surr_split[f])
-        a_s = nets[f].surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std)
+        a_s = nets[f].surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std, antithetic)
         a_d = nets[f].accumulate(X[lo:hi], marks[lo:hi], tscale)
         out[f] = (loss, a_s, a_d, mf, hi - lo, t2 - t1, time.perf_counter() - t2)
 
@@ -377,12 +384,15 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
         losses.append(loss)
         w = nf / n_traced
         mean_s = a_s["s"] / mf
-        var_s = np.maximum(a_s["s2"] / mf - mean_s ** 2, 0.0)
+        # the variance of mean_s: that of one surrogate value (or of one
+        # antithetic pair's mean) over the number of values (or pairs)
+        ms = mf // 2 if antithetic else mf
+        var_s = np.maximum(a_s["s2"] / ms - mean_s ** 2, 0.0)
         mean_d = a_d["d"] / nf
         var_d = np.maximum(a_d["d2"] / nf - mean_d ** 2, 0.0)
         sum_s += w * mean_s
         sum_d += w * mean_d
-        var_cv += w * w * (var_d / nf + var_s / mf)
+        var_cv += w * w * (var_d / nf + var_s / ms)
     del nets
     t0 = time.perf_counter()
     p_plain, var_plain = plain_estimate(marks, tscale)

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import philox
-from onekapy_amd import Lognormal, Signed
+from onekapy_amd import Lognormal, Signed, _lib
 from onekapy_amd.stochastic import draw_realizations_device
 
 pytestmark = pytest.mark.gpu
@@ -26,6 +26,43 @@ def test_uniform_draws_are_the_philox_stream(gpu_ctx):
     # realizations are reproducible one by one, whatever the batch they are in
     d2 = draw_realizations_device(wells, (2.0, 5.0), 0.25, (20.0, 30.0), 8, seed, first=1010)
     assert np.array_equal(d2["K"], d["K"][10:18]) and np.array_equal(d2["z"], d["z"][10:18])
+
+
+def test_antithetic_pairs_reflect_the_even_draw(gpu_ctx):
+    """Realization 2k + 1 takes 2k's uniforms u as 1 - u and its normals
+    negated; realization 2k is the plain draw."""
+    mu = math.log(800.0)
+    wells = [(0.0, 0.0, 0.25, (300.0, 450.0, 700.0)), (1.0, 0.0, 0.25, Signed(Lognormal(mu, 0.6), 0.3))]
+    seed, first, n = 0x0DDB_A11, 2000, 256
+    args = (wells, (2.0, 5.0), 0.25, (20.0, 30.0), n, seed)
+    a = draw_realizations_device(*args, first=first, antithetic=True)
+    p = draw_realizations_device(*args, first=first)
+    for k in ("q", "K", "n", "b", "z"):
+        assert np.array_equal(a[k][0::2], p[k][0::2]), k
+    assert np.array_equal(a["z"][1::2], -a["z"][0::2])
+    flips = 0
+    for i in range(1, n, 2):
+        r = first + i - 1
+        # uniform: the reflected uniform, bit for bit
+        assert a["K"][i] == 2.0 + 3.0 * (1.0 - philox.uniforms(seed, r, 2)[0])
+        assert a["b"][i] == 20.0 + 10.0 * (1.0 - philox.uniforms(seed, r, 4)[0])
+        # triangular: CDF(v_odd) = 1 - CDF(v_even)
+        for v, w in ((a["q"][i, 0], a["q"][i - 1, 0]),):
+            def cdf(x):
+                return ((x - 300.0) ** 2 / (400.0 * 150.0) if x < 450.0
+                        else 1.0 - (700.0 - x) ** 2 / (400.0 * 250.0))
+            assert abs(cdf(v) + cdf(w) - 1.0) < 1e-9
+        # lognormal: log v_odd + log v_even = 2 mu; the sign variate reflected
+        lo, le = a["q"][i, 1], a["q"][i - 1, 1]
+        assert abs(math.log(abs(lo)) + math.log(abs(le)) - 2.0 * mu) < 1e-12
+        us = philox.uniforms(seed, r, 1, 1)[0]
+        assert (le < 0) == (us < 0.3) and (lo < 0) == (1.0 - us < 0.3)
+        flips += (lo < 0) != (le < 0)
+    assert flips > 0
+    with pytest.raises(_lib.OkaError):
+        draw_realizations_device(*args[:4], n - 1, seed, first=first, antithetic=True)
+    with pytest.raises(_lib.OkaError):
+        draw_realizations_device(*args, first=first + 1, antithetic=True)
 
 
 def test_distribution_moments(gpu_ctx):

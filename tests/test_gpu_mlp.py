@@ -127,6 +127,36 @@ def test_control_variate_is_unbiased_and_reduces_variance(gpu_ctx):
     assert gain > 3.0, gain
 
 
+@pytest.mark.parametrize("antithetic", [False, True])
+def test_surrogate_sums_compose_draws_fit_features_and_network(gpu_ctx, antithetic):
+    """oka_cv_surrogate_sums against its parts called one by one: the device
+    draws, the fit + MVN draw, the standardized features and the network's
+    probabilities, summed on the host (squared pair means when antithetic)."""
+    from onekapy_amd.stochastic import draw_realizations_device
+    from onekapy_amd.surrogate import Standardizer, features
+    s = sites.config0()
+    net = SurrogateMLP(len(s.wells) + 9, 128, 128, max_batch=256, seed=3)
+    seed, first, n = 0xBEEF, 4096, 1000   # four network passes, the last one ragged
+    d = draw_realizations_device(s.wells, s.c_dist, s.p_dist, s.t_dist, n, seed, first,
+                                 antithetic)
+    _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
+                              d["q"], d["z"], moments=False)
+    feats = features(d, coef)
+    std = Standardizer(feats)
+    prob = net.predict(std(feats)).astype(np.float64)
+    acc = net.surrogate_sums(s, n, seed, first, std, antithetic)
+    # FP32 partial sums per lane, FP64 across workgroups
+    np.testing.assert_allclose(acc["s"], prob.sum(0), rtol=1e-5, atol=1e-6)
+    if antithetic:
+        pm = 0.5 * (prob[0::2] + prob[1::2])
+        np.testing.assert_allclose(acc["s2"], (pm * pm).sum(0), rtol=1e-5, atol=1e-6)
+        # reflected pairs are negatively correlated over the outputs as a whole:
+        # var(pair mean) < var(s) / 2, i.e. sum pm^2 < (sum s^2 + (sum s)^2 / n) / 4
+        assert (pm * pm).sum() < 0.25 * ((prob * prob).sum() + (prob.sum(0) ** 2).sum() / n)
+    else:
+        np.testing.assert_allclose(acc["s2"], (prob * prob).sum(0), rtol=1e-5, atol=1e-6)
+
+
 def test_replayed_training_graph_matches_one_step_calls(gpu_ctx):
     """oka_mlp_train captures its steps 16 at a time as a HIP graph and replays
     it.  Forty steps in one call (two replays of the 16-step graph, one of an
