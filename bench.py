@@ -72,8 +72,10 @@ def parse():
                     help="only run a plain 1 GiB device-to-device copy (to scale HBM counters)")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--only-cv", action="store_true", help="run only the control-variate leg")
-    ap.add_argument("--cv-traced", type=int, default=40000)
+    ap.add_argument("--cv-traced", type=int, default=80000)
     ap.add_argument("--cv-surrogate", type=int, default=10000000)
+    ap.add_argument("--cv-antithetic", action=argparse.BooleanOptionalAction, default=True,
+                    help="draw the surrogate realizations in antithetic pairs")
     ap.add_argument("--cv-judge", type=int, default=1000000,
                     help="realizations of the FP64 plain run that judges the estimates")
     ap.add_argument("--cv-steps", type=int, default=1200)
@@ -101,7 +103,7 @@ def control_variate_leg(site, args, fp32_rate):
         t0 = time.perf_counter()
         res = control_variate(site, n_traced, n_surrogate, folds=2, train_steps=args.cv_steps,
                               batch=args.cv_batch, lr=args.cv_lr, seed=123,
-                              precision=_lib.FP32)
+                              precision=_lib.FP32, antithetic=args.cv_antithetic)
         wall = time.perf_counter() - t0
         print(f"bench: control variate {wall:.2f} s {res.times}", file=sys.stderr, flush=True)
         return res, wall
@@ -143,7 +145,7 @@ def control_variate_leg(site, args, fp32_rate):
 
     out = summary(*tuned, args.cv_traced, args.cv_surrogate)
     out.update({"folds": 2, "train_steps_per_fold": args.cv_steps, "batch": args.cv_batch,
-                "lr": args.cv_lr, "network": "34->512->512->4096 GELU, BF16 MFMA",
+                "lr": args.cv_lr, "antithetic_surrogate_draws": args.cv_antithetic, "network": "34->512->512->4096 GELU, BF16 MFMA",
                 "judge": {"realizations": args.cv_judge, "tracer": "fp64", "seconds": tj}})
     out["as_configured"] = summary(*configured, 100000, 1000000)
     return out

@@ -106,17 +106,26 @@ def test_training_fits_a_smooth_target(gpu_ctx):
     assert np.abs(pred - marks[:512] / 25.0).mean() < 0.08
 
 
-def test_control_variate_is_unbiased_and_reduces_variance(gpu_ctx):
+@pytest.fixture(scope="module")
+def cfg0_judge(gpu_ctx):
+    """A large plain run on configuration 0's site (FP32 tracer): node counts
+    over R realizations."""
     s = sites.config0()
-    res = control_variate(s, n_traced=16384, n_surrogate=131072, folds=2, train_steps=1500,
-                          batch=512, lr=1e-3, seed=7, n_hidden=256)
-    # the judge: a large plain run (FP32 tracer, same window)
     rng = np.random.default_rng(99)
     R = 40000
     d = draw_realizations(s.wells, s.c_dist, s.p_dist, s.t_dist, R, rng)
     _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
                               d["q"], d["z"])
     counts, _ = trace_counts(s, d, coef, _lib.FP32)
+    return s, counts, R
+
+
+@pytest.mark.parametrize("antithetic", [False, True])
+def test_control_variate_is_unbiased_and_reduces_variance(cfg0_judge, antithetic):
+    s, counts, R = cfg0_judge
+    res = control_variate(s, n_traced=16384, n_surrogate=131072, folds=2, train_steps=1500,
+                          batch=512, lr=1e-3, seed=7, n_hidden=256, antithetic=antithetic)
+    # the judge: the large plain run over the same window
     p_ref = cell_probabilities(counts, R, res.window)
     var_ref = p_ref * (1 - p_ref) / R
     z = (res.p_cv - p_ref) / np.sqrt(res.var_cv + var_ref + 1e-12)
