@@ -126,19 +126,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
     for (int r = 0; r < S::RA; ++r)
         rs[r] = 0.f;
 
-    // loss-gradient targets: fetched before the K loop
-    uint8_t ymk[TN][TM][4];
     const int64_t step = g.dstate ? g.dstate[0] : 0;
-    if constexpr (kEpi == kEpiLossGrad) {
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    ymk[j][i][e] = g.marks[(int64_t)(m0 + wm + i * 16 + (lane >> 4) * 4 + e) * g.N +
-                                           n0 + wn + j * 16 + (lane & 15)];
-    }
 
     f32x4 acc[TM][TN];
 #pragma unroll
@@ -198,21 +186,24 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
     if (nst > 2)
         load(r0, 2);
     __syncthreads();
-    for (int s = 0; s < nst; s += 2) {
+    // pairs of stages, then an odd last one: no MFMA is conditional inside
+    // the loop, so the accumulators keep their registers across iterations
+    for (int s = 0; s + 1 < nst; s += 2) {
         mma(0);
-        if (s + 1 < nst)
-            put(r1, 1);   // buffer 1 was last read before the previous barrier
+        put(r1, 1);   // buffer 1 was last read before the previous barrier
         if (s + 3 < nst)
             load(r1, s + 3);
         __syncthreads();
-        if (s + 1 < nst) {
-            mma(1);
-            if (s + 2 < nst)
-                put(r0, 0);
-            if (s + 4 < nst)
-                load(r0, s + 4);
-            __syncthreads();
-        }
+        mma(1);
+        if (s + 2 < nst)
+            put(r0, 0);
+        if (s + 4 < nst)
+            load(r0, s + 4);
+        __syncthreads();
+    }
+    if (nst & 1) {
+        mma(0);
+        __syncthreads();   // buffer 0 is refilled by the workgroup's next tile
     }
     if (rowsum) {
         // the 8 lanes loading one row are consecutive
@@ -228,6 +219,73 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
     }
 
     // epilogue: lane holds C[m = mb + 4*(lane/16) + e][n = nb + lane%16], e = 0..3
+    if constexpr (kEpi == kEpiAccum) {
+        // s = sigmoid(acc + b); per column the sums of s and of s^2 (of the
+        // squared pair means when pairs is set: rows mb, mb + 1 and mb + 2,
+        // mb + 3 of one lane are pairs) and, with marks, of d = y - s and d^2.
+        // Rows at or beyond mvalid count as s = 0 (and d = 0); mvalid is even
+        // when pairs is set, so a pair never straddles it.
+        const bool full = m0 + S::BM <= g.mvalid;
+        const bool with_marks = g.marks != nullptr;
+        const int mlane = m0 + wm + (lane >> 4) * 4;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int n = n0 + wn + j * 16 + (lane & 15);
+            const float bias = g.bias[n];
+            float cs = 0.f, cs2 = 0.f, cd = 0.f, cd2 = 0.f;
+#pragma unroll
+            for (int i = 0; i < TM; ++i) {
+                const int mb = mlane + i * 16;
+                float sv[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float sg = sigmoid(acc[i][j][e] + bias);
+                    sv[e] = (full || mb + e < g.mvalid) ? sg : 0.f;
+                }
+                cs += (sv[0] + sv[1]) + (sv[2] + sv[3]);
+                if (g.pairs) {
+                    const float p0 = 0.5f * (sv[0] + sv[1]), p1 = 0.5f * (sv[2] + sv[3]);
+                    cs2 = fmaf(p0, p0, fmaf(p1, p1, cs2));
+                } else {
+                    cs2 = fmaf(sv[0], sv[0], fmaf(sv[1], sv[1], cs2));
+                    cs2 = fmaf(sv[2], sv[2], fmaf(sv[3], sv[3], cs2));
+                }
+                if (with_marks) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        if (full || mb + e < g.mvalid) {
+                            const float y = (float)g.marks[(g.row0 + mb + e) * g.N + n];
+                            const float dv = y * g.tscale - sv[e];
+                            cd += dv;
+                            cd2 = fmaf(dv, dv, cd2);
+                        }
+                    }
+                }
+            }
+            // lanes l, l^16, l^32, l^48 hold the same column; two waves share
+            // each column, so the workgroup's sums meet in LDS
+            cs += __shfl_xor(cs, 16);
+            cs += __shfl_xor(cs, 32);
+            cs2 += __shfl_xor(cs2, 16);
+            cs2 += __shfl_xor(cs2, 32);
+            if (with_marks) {
+                cd += __shfl_xor(cd, 16);
+                cd += __shfl_xor(cd, 32);
+                cd2 += __shfl_xor(cd2, 16);
+                cd2 += __shfl_xor(cd2, 32);
+            }
+            if (lane < 16) {
+                const int cl = n - n0;
+                atomicAdd(&sm.col[0][cl], cs);
+                atomicAdd(&sm.col[1][cl], cs2);
+                if (with_marks) {
+                    atomicAdd(&sm.col[2][cl], cd);
+                    atomicAdd(&sm.col[3][cl], cd2);
+                }
+            }
+        }
+        return;
+    }
     const bool want_loss = kEpi == kEpiLossGrad && g.loss != nullptr;
     float part_l = 0.f;   // the lane's 16 loss terms (monitoring only)
     float *cf = g.Cf;
@@ -237,9 +295,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
     for (int j = 0; j < TN; ++j) {
         const int n = n0 + wn + j * 16 + (lane & 15);
         const float bias = (kEpi == kEpiGeluFwd || kEpi == kEpiLogits || kEpi == kEpiLossGrad ||
-                            kEpi == kEpiAccum || kEpi == kEpiProbs) ? g.bias[n] : 0.f;
-        float col_s = 0.f, col_d = 0.f, col_s2 = 0.f, col_d2 = 0.f;
-        float s_prev = 0.f;   // kEpiAccum with pairs: the pair's even row
+                            kEpi == kEpiProbs) ? g.bias[n] : 0.f;
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
             const int mb = m0 + wm + i * 16 + (lane >> 4) * 4;
@@ -263,33 +319,14 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
                     cf[(int64_t)m * g.ldc + n] = v;
                 } else if (kEpi == kEpiLossGrad) {
                     const float z = v + bias;
-                    const float y = (float)ymk[j][i][e] * g.tscale;
+                    const float y = (float)g.marks[(int64_t)m * g.N + n] * g.tscale;
                     hv[e] = f2bf((sigmoid(z) - y) * g.gscale);
                     g.Ch[(int64_t)m * g.N + n] = hv[e];
                     if (want_loss)   // BCE with logits: max(z, 0) - z y + log(1 + e^-|z|)
                         part_l += fmaxf(z, 0.f) - z * y + __logf(1.0f + __expf(-fabsf(z)));
-                } else if (kEpi == kEpiAccum || kEpi == kEpiProbs) {
-                    if (m < g.mvalid) {
-                        const float s = sigmoid(v + bias);
-                        if (kEpi == kEpiProbs) {
-                            cf[(int64_t)m * g.ldc + n] = s;
-                        } else {
-                            col_s += s;
-                            if (!g.pairs) {
-                                col_s2 += s * s;
-                            } else if (e & 1) {   // rows mb + e - 1, mb + e: one pair
-                                const float pm = 0.5f * (s_prev + s);
-                                col_s2 += pm * pm;
-                            }
-                            s_prev = s;
-                            if (g.marks) {
-                                const int64_t row = g.row0 + m;
-                                const float dv = (float)g.marks[row * g.N + n] * g.tscale - s;
-                                col_d += dv;
-                                col_d2 += dv * dv;
-                            }
-                        }
-                    }
+                } else if (kEpi == kEpiProbs) {
+                    if (m < g.mvalid)
+                        cf[(int64_t)m * g.ldc + n] = sigmoid(v + bias);
                 }
             }
             if ((kEpi == kEpiGeluFwd || kEpi == kEpiGeluBwd || kEpi == kEpiLossGrad) && g.Cht) {
@@ -298,25 +335,6 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
                 packed.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
                 packed.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
                 *reinterpret_cast<uint2 *>(&g.Cht[(int64_t)n * g.M + mb]) = packed;
-            }
-        }
-        if (kEpi == kEpiAccum) {
-            // lanes l, l^16, l^32, l^48 hold the same column; two waves share
-            // each column, so the workgroup's sums meet in LDS
-            col_s += __shfl_xor(col_s, 16);
-            col_s += __shfl_xor(col_s, 32);
-            col_s2 += __shfl_xor(col_s2, 16);
-            col_s2 += __shfl_xor(col_s2, 32);
-            col_d += __shfl_xor(col_d, 16);
-            col_d += __shfl_xor(col_d, 32);
-            col_d2 += __shfl_xor(col_d2, 16);
-            col_d2 += __shfl_xor(col_d2, 32);
-            if (lane < 16) {
-                const int cl = n - n0;
-                atomicAdd(&sm.col[0][cl], col_s);
-                atomicAdd(&sm.col[1][cl], col_s2);
-                atomicAdd(&sm.col[2][cl], col_d);
-                atomicAdd(&sm.col[3][cl], col_d2);
             }
         }
     }
@@ -332,9 +350,13 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
     }
 }
 
-// Tiles [0, tiles(g1)) belong to g1, the rest to g2.
+// Tiles [0, tiles(g1)) belong to g1, the rest to g2.  Register budgets match
+// the workgroups that fit a CU's LDS: 128 x 128 kernels two waves per SIMD
+// (256 VGPRs), 64 x 64 kernels four (128 VGPRs); the accumulators then stay
+// in architectural registers with no copies between MFMAs.
 template <int kEpi1, int kEpi2, int kBig>
-__global__ void __launch_bounds__(256) k_gemm(GemmArgs g1, GemmArgs g2)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kBig ? 2 : 4)))
+k_gemm(GemmArgs g1, GemmArgs g2)
 {
     __shared__ GemmSmem<kBig> sm;
     if constexpr (kEpi2 == kEpiNone) {
@@ -351,7 +373,8 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g1, GemmArgs g2)
 // kEpiAccum: workgroup b sums the tiles of column tile b % (N/BN) in row
 // tiles [rt * (b / (N/BN)), + rt) in LDS, then adds them to the FP64 sums.
 template <int kBig>
-__global__ void __launch_bounds__(256) k_gemm_accum(GemmArgs g, int rt)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kBig ? 2 : 4)))
+k_gemm_accum(GemmArgs g, int rt)
 {
     using S = Shape<kBig>;
     __shared__ GemmSmem<kBig> sm;

@@ -28,6 +28,23 @@ def test_forward_matches_oracle(gpu_ctx):
     assert np.abs(probs - ref).max() < 2e-3
 
 
+def test_accumulate_sums_match_the_predictions(gpu_ctx):
+    """oka_mlp_accumulate's column sums (s, s^2, d = y - s, d^2) against the
+    network's own probabilities summed on the host: two passes of max_batch
+    rows, the second ragged (rows beyond the input count nothing)."""
+    net, _, _ = _net_and_batch(seed=4)
+    rng = np.random.default_rng(4)
+    n = 300
+    x = rng.normal(0, 1, (n, 34)).astype(np.float32)
+    marks = rng.integers(0, 26, (n, 256)).astype(np.uint8)
+    acc = net.accumulate(x, marks, 1.0 / 25.0)
+    p = net.predict(x).astype(np.float64)
+    d = marks / np.float32(25.0) - p
+    for key, ref in (("s", p.sum(0)), ("s2", (p * p).sum(0)), ("d", d.sum(0)),
+                     ("d2", (d * d).sum(0))):
+        np.testing.assert_allclose(acc[key], ref, rtol=1e-5, atol=1e-5, err_msg=key)
+
+
 def test_gradient_matches_oracle(gpu_ctx):
     net, x, marks = _net_and_batch(seed=3)
     g, loss = net.grad(x, marks, 1.0 / 25.0)
