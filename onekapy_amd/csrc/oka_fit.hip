@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(256) k_fit_logs(FitArgs a, double *logr2)
 
 // One weighted row (model.py:545-565).  Returns 0 or OKA_ERR_RANGE.
 __device__ __forceinline__ int fit_row(const FitArgs &a, int i, double K, double b,
-                                       const double *q, double *row, double &wb)
+                                       const double *q, int64_t qs, double *row, double &wb)
 {
     const double x = a.obs[4 * i], y = a.obs[4 * i + 1];
     const double z_ev = a.obs[4 * i + 2], z_std = a.obs[4 * i + 3];
@@ -60,7 +60,7 @@ __device__ __forceinline__ int fit_row(const FitArgs &a, int i, double K, double
     const double *L = a.logr2 + (int64_t)i * a.nwells;
     double pw = 0.0;
     for (int w = 0; w < a.nwells; ++w)
-        pw = pw + (q[w] * L[w]) * kC4Pi;
+        pw = pw + (q[w * qs] * L[w]) * kC4Pi;
     wb = (pot_ev - pw) / pot_std;
     return 0;
 }
@@ -123,7 +123,9 @@ __device__ __forceinline__ void chol6_solve(const double *L, const double *s, co
 __device__ __forceinline__ void fit_one(const FitArgs &a, int64_t r)
 {
     const double K = a.K[r], b = a.b[r];
-    const double *q = a.q + (int64_t)r * a.nwells;
+    // the realization's discharges: q[w * qs]
+    const int64_t qs = a.q_well_stride ? a.q_well_stride : 1;
+    const double *q = a.q + (a.q_well_stride ? r : (int64_t)r * a.nwells);
     double N[36], rhs[6], row[6], wb;
 #pragma unroll
     for (int t = 0; t < 36; ++t)
@@ -133,7 +135,7 @@ __device__ __forceinline__ void fit_one(const FitArgs &a, int64_t r)
         rhs[j] = 0.0;
     int st = 0;
     for (int i = 0; i < a.nobs && st == 0; ++i) {
-        st = fit_row(a, i, K, b, q, row, wb);
+        st = fit_row(a, i, K, b, q, qs, row, wb);
         if (st)
             break;
 #pragma unroll
@@ -162,7 +164,7 @@ __device__ __forceinline__ void fit_one(const FitArgs &a, int64_t r)
         chol6_solve(L, s, rhs, ev);
         double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, d[6];
         for (int i = 0; i < a.nobs; ++i) {
-            fit_row(a, i, K, b, q, row, wb);
+            fit_row(a, i, K, b, q, qs, row, wb);
             double res = wb;
 #pragma unroll
             for (int j = 0; j < 6; ++j)

@@ -343,6 +343,9 @@ struct FitArgs {
     const double *K, *b, *q, *z;
     double *ev, *cov, *coef;
     int32_t *status;
+    // q layout: 0 = [nreal x nwells]; else well w of realization r is at
+    // q[w * q_well_stride + r] (well-major: a wave's loads are contiguous)
+    int64_t q_well_stride;
 };
 
 struct DrawArgs {
@@ -352,6 +355,7 @@ struct DrawArgs {
     uint64_t seed;
     double *q, *K, *por, *thick, *z;
     int32_t antithetic;         // odd realizations reflect their even partner's draws
+    int64_t q_well_stride;      // q layout, as FitArgs::q_well_stride
 };
 
 struct FeatureArgs {
@@ -360,6 +364,7 @@ struct FeatureArgs {
     const double *q, *K, *por, *thick, *coef;
     const double *mean, *std;   // device: nwells + 9
     float *x;                   // n x (nwells + 9)
+    int64_t q_well_stride;      // q layout, as FitArgs::q_well_stride
 };
 
 // oka_rng.hip

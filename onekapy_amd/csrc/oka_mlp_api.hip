@@ -772,6 +772,7 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
     fa.z = z;
     fa.coef = coef;
     fa.status = status;
+    fa.q_well_stride = chunk;
     HIPCHK(c, launch_fit_logs(fa, d_logr2, s));
     for (int64_t r0 = 0; r0 < n; r0 += chunk) {
         const int valid = (int)std::min<int64_t>(chunk, n - r0);
@@ -787,6 +788,7 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
         da.thick = thick;
         da.z = z;
         da.antithetic = pairs;
+        da.q_well_stride = chunk;   // well-major: the fit's loads are contiguous
         HIPCHK(c, launch_draw(da, s));
         fa.nreal = valid;
         HIPCHK(c, launch_fit_draw(fa, s));
@@ -801,6 +803,7 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
         fe.mean = d_mean;
         fe.std = d_std;
         fe.x = x;
+        fe.q_well_stride = chunk;
         HIPCHK(c, launch_features(fe, s));
         for (int r1 = 0; r1 < valid; r1 += M->max_batch) {
             const int nv = std::min(M->max_batch, valid - r1);

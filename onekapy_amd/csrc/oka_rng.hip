@@ -94,8 +94,10 @@ __global__ void __launch_bounds__(256) k_draw(DrawArgs a)
     const bool flip = a.antithetic && (a.first + i) % 2 == 1;
     const int64_t r = flip ? a.first + i - 1 : a.first + i;
     const int nw = a.nwells;
+    double *q = a.q + (a.q_well_stride ? i : i * nw);
+    const int64_t qs = a.q_well_stride ? a.q_well_stride : 1;
     for (int w = 0; w < nw; ++w)
-        a.q[i * nw + w] = sample(a.dists[w], a.seed, r, (uint32_t)w, flip);
+        q[w * qs] = sample(a.dists[w], a.seed, r, (uint32_t)w, flip);
     a.K[i] = sample(a.dists[nw], a.seed, r, (uint32_t)nw, flip);
     a.por[i] = sample(a.dists[nw + 1], a.seed, r, (uint32_t)nw + 1, flip);
     a.thick[i] = sample(a.dists[nw + 2], a.seed, r, (uint32_t)nw + 2, flip);
@@ -122,7 +124,7 @@ __global__ void __launch_bounds__(256) k_features(FeatureArgs a)
     const int j = (int)(t % nin);
     double f;
     if (j < a.nwells)
-        f = a.q[i * a.nwells + j];
+        f = a.q_well_stride ? a.q[j * a.q_well_stride + i] : a.q[i * a.nwells + j];
     else if (j == a.nwells)
         f = a.K[i];
     else if (j == a.nwells + 1)
