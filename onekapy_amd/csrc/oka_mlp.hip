@@ -24,13 +24,10 @@ namespace mlp {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32; NaN stays NaN)
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u)
-        return (uint16_t)((u >> 16) | 0x40u);           // quiet NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                    // round to nearest even
-    return (uint16_t)(u >> 16);
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
@@ -320,10 +317,16 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
                 } else if (kEpi == kEpiLossGrad) {
                     const float z = v + bias;
                     const float y = (float)g.marks[(int64_t)m * g.N + n] * g.tscale;
-                    hv[e] = f2bf((sigmoid(z) - y) * g.gscale);
+                    // ez = e^-|z| in (0, 1] serves the logistic function and the
+                    // loss (hardware exp2, log2 and reciprocal, about 1 ulp each)
+                    const float ez = __builtin_amdgcn_exp2f(-1.4426950408889634f * fabsf(z));
+                    const float r = __builtin_amdgcn_rcpf(1.0f + ez);
+                    const float sg = z >= 0.f ? r : ez * r;
+                    hv[e] = f2bf((sg - y) * g.gscale);
                     g.Ch[(int64_t)m * g.N + n] = hv[e];
                     if (want_loss)   // BCE with logits: max(z, 0) - z y + log(1 + e^-|z|)
-                        part_l += fmaxf(z, 0.f) - z * y + __logf(1.0f + __expf(-fabsf(z)));
+                        part_l += fmaxf(z, 0.f) - z * y +
+                                  0.6931471805599453f * __builtin_amdgcn_logf(1.0f + ez);
                 } else if (kEpi == kEpiProbs) {
                     if (m < g.mvalid)
                         cf[(int64_t)m * g.ldc + n] = sigmoid(v + bias);
