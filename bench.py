@@ -75,7 +75,10 @@ def parse():
     ap.add_argument("--cv-traced", type=int, default=80000)
     ap.add_argument("--cv-surrogate", type=int, default=10000000)
     ap.add_argument("--cv-antithetic", action=argparse.BooleanOptionalAction, default=True,
-                    help="draw the surrogate realizations in antithetic pairs")
+                    help="draw the tuned estimator's surrogate realizations in antithetic pairs")
+    ap.add_argument("--cv-config-qmc", type=int, default=32,
+                    help="QMC replicates of the estimator at the workload's own sizes (scrambled "
+                         "Sobol' points; 0: antithetic pairs like the tuned one)")
     ap.add_argument("--cv-judge", type=int, default=1000000,
                     help="realizations of the FP64 plain run that judges the estimates")
     ap.add_argument("--cv-steps", type=int, default=1200)
@@ -91,25 +94,32 @@ def control_variate_leg(site, args, fp32_rate):
     same maximum (over window cells) standard error, divided by the
     pipeline's wall time (draws, fits, pilot, tracing, training, surrogate).
     Two estimators share the judge: the tuned one (N = --cv-traced, M =
-    --cv-surrogate) and the workload's own sizes (100,000 traced realizations
-    to train on, 10^6 surrogate evaluations)."""
+    --cv-surrogate, surrogate realizations in antithetic pairs) and the
+    workload's own sizes (100,000 traced realizations to train on, 10^6
+    surrogate evaluations as --cv-config-qmc scrambled Sobol' replicates)."""
     from onekapy_amd import _lib
     from onekapy_amd.stochastic import draw_realizations, fit_and_draw
     from onekapy_amd.surrogate import cell_probabilities, control_variate, trace_counts
 
-    def run(n_traced, n_surrogate):
+    def run(n_traced, n_surrogate, antithetic, qmc):
         print(f"bench: control variate: {n_traced} traced + {n_surrogate} surrogate",
               file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         res = control_variate(site, n_traced, n_surrogate, folds=2, train_steps=args.cv_steps,
                               batch=args.cv_batch, lr=args.cv_lr, seed=123,
-                              precision=_lib.FP32, antithetic=args.cv_antithetic)
+                              precision=_lib.FP32, antithetic=antithetic, qmc_replicates=qmc)
         wall = time.perf_counter() - t0
         print(f"bench: control variate {wall:.2f} s {res.times}", file=sys.stderr, flush=True)
         return res, wall
 
-    tuned = run(args.cv_traced, args.cv_surrogate)
-    configured = run(100000, 1000000)
+    def sampling(antithetic, qmc):
+        if qmc:
+            return f"scrambled Sobol' points, {qmc} replicates per fold"
+        return "antithetic pairs" if antithetic else "independent draws"
+
+    tuned = run(args.cv_traced, args.cv_surrogate, args.cv_antithetic, 0)
+    configured = run(100000, 1000000, args.cv_antithetic and not args.cv_config_qmc,
+                     args.cv_config_qmc)
     rng = np.random.default_rng(321)
     d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, args.cv_judge, rng)
     _, _, coef = fit_and_draw(site.wells_xy(), site.xt, site.yt, site.base, site.observations,
@@ -144,10 +154,13 @@ def control_variate_leg(site, args, fp32_rate):
         }
 
     out = summary(*tuned, args.cv_traced, args.cv_surrogate)
+    out["surrogate_sampling"] = sampling(args.cv_antithetic, 0)
     out.update({"folds": 2, "train_steps_per_fold": args.cv_steps, "batch": args.cv_batch,
-                "lr": args.cv_lr, "antithetic_surrogate_draws": args.cv_antithetic, "network": "34->512->512->4096 GELU, BF16 MFMA",
+                "lr": args.cv_lr, "network": "34->512->512->4096 GELU, BF16 MFMA",
                 "judge": {"realizations": args.cv_judge, "tracer": "fp64", "seconds": tj}})
     out["as_configured"] = summary(*configured, 100000, 1000000)
+    out["as_configured"]["surrogate_sampling"] = sampling(
+        args.cv_antithetic and not args.cv_config_qmc, args.cv_config_qmc)
     return out
 
 

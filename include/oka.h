@@ -71,6 +71,7 @@ enum {
 #define OKA_ASYNC 0x2u
 #define OKA_RASTER_FP32 0x4u   /* oka_rasterize: the FP32 tracers' rasterizer */
 #define OKA_ANTITHETIC 0x8u    /* oka_draw_realizations, oka_cv_surrogate_sums: antithetic pairs */
+#define OKA_QMC 0x10u          /* oka_draw_realizations, oka_cv_surrogate_sums: scrambled Sobol' points */
 
 typedef struct oka_ctx oka_ctx;
 
@@ -378,7 +379,15 @@ typedef struct {
  * With OKA_ANTITHETIC in flags (n and first even), realization 2k + 1 is the
  * antithetic partner of 2k: it takes 2k's Philox uniforms u as 1 - u (the
  * inverse-CDF and sign variates) and 2k's standard normals (z and the
- * lognormal's) negated; realization 2k is drawn as without the flag. */
+ * lognormal's) negated; realization 2k is drawn as without the flag.
+ * With OKA_QMC instead (first + n <= 2^32), realization j takes its uniforms
+ * from point j of a Sobol' sequence (natural order; direction numbers from
+ * the primitive polynomials over GF(2) by degree) under a nested (Owen-type)
+ * hash scramble keyed by seed: each realization is distributed as without
+ * the flag, and runs with different seeds are independent.  Dimensions,
+ * most influential first: K, porosity, thickness, the 6 normals, the
+ * discharges, then the lognormals' second uniforms and the sign uniforms (up
+ * to 1024; constants take none). */
 int oka_draw_realizations(oka_ctx *ctx, int32_t nwells, const oka_dist *dists, int64_t n,
                           uint64_t seed, int64_t first, double *q, double *K, double *por,
                           double *thick, double *z, uint32_t flags);
@@ -408,7 +417,9 @@ int oka_count_histogram(oka_ctx *ctx, int64_t n, const int32_t *counts, int32_t 
  * sums_sq follow flags.  With OKA_ANTITHETIC (n and first even) the
  * realizations are drawn in antithetic pairs as in oka_draw_realizations,
  * and sums_sq[c] += sum over the n / 2 pairs of ((s_c,2k + s_c,2k+1) / 2)^2,
- * so that the variance of the sums' mean is that of the pair means / (n / 2). */
+ * so that the variance of the sums' mean is that of the pair means / (n / 2).
+ * With OKA_QMC the realizations are Sobol' points as in oka_draw_realizations;
+ * the variance of the mean then comes from independent scrambles (seeds). */
 int oka_cv_surrogate_sums(oka_ctx *ctx, oka_mlp *mlp, const oka_site *site, const oka_dist *dists,
                           int64_t n, uint64_t seed, int64_t first, const double *feat_mean,
                           const double *feat_std, double *sums, double *sums_sq, uint32_t flags);

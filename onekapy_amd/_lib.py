@@ -26,6 +26,7 @@ PTR_DEVICE = 0x1
 ASYNC = 0x2
 RASTER_FP32 = 0x4      # oka_rasterize: the FP32 tracers' rasterizer
 ANTITHETIC = 0x8       # draws / surrogate sums in antithetic pairs (2k, 2k + 1)
+QMC = 0x10             # draws / surrogate sums from scrambled Sobol' points
 
 _d = C.c_double
 _i32 = C.c_int32

@@ -20,7 +20,8 @@ enum Slot {
     kSlotVerts, kSlotNVerts, kSlotObs, kSlotLogs, kSlotZ, kSlotEv, kSlotCov, kSlotFStatus,
     kSlotXY, kSlotOut, kSlotEStatus, kSlotRgrid, kSlotMarks, kSlotMlpX, kSlotMlpMarks,
     kSlotMlpOrder, kSlotMlpOut, kSlotMlpOut2, kSlotMlpLoss, kSlotCvDists, kSlotCvSite,
-    kSlotCvDraw, kSlotPoolSites, kSlotPoolFirst, kSlotPoolQueue, kSlotPoolHbm, kSlotPoolRec, kSlotCount
+    kSlotCvDraw, kSlotPoolSites, kSlotPoolFirst, kSlotPoolQueue, kSlotPoolHbm, kSlotPoolRec, kSlotQmc,
+    kSlotCount
 };
 
 struct DevBuf {

@@ -356,6 +356,7 @@ struct DrawArgs {
     double *q, *K, *por, *thick, *z;
     int32_t antithetic;         // odd realizations reflect their even partner's draws
     int64_t q_well_stride;      // q layout, as FitArgs::q_well_stride
+    const uint32_t *sobol;      // QMC: device copy of sobol_directions() (>= qmc_dims x 32)
 };
 
 struct FeatureArgs {
@@ -374,6 +375,13 @@ hipError_t launch_features(const FeatureArgs &a, hipStream_t s);
 hipError_t launch_mark_moments(const uint8_t *marks, int64_t n, int32_t ncell, uint64_t *sums,
                                uint64_t *sums_sq, hipStream_t s);
 uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, int word);
+
+// Sobol' direction numbers of kSobolMaxDims dimensions, 32 per dimension
+// (host memory, built once); qmc_dims is the number of dimensions a QMC draw
+// of these distributions uses.
+constexpr int kSobolMaxDims = 1024;
+const uint32_t *sobol_directions();
+int qmc_dims(int32_t nwells, const oka_dist *dists);
 
 // Persistent FP32 tracer over a table of sites (oka_pool.hip).
 // A queue item is a chunk of `chunk` particles of one realization.  A

@@ -625,6 +625,35 @@ int check_dists(oka_ctx *c, int32_t nwells, const oka_dist *dists)
 
 extern "C" {
 
+// OKA_QMC preconditions: not with OKA_ANTITHETIC, point indices below 2^32,
+// at most kSobolMaxDims dimensions.
+static int check_qmc(oka_ctx *c, int32_t nwells, const oka_dist *dists, int64_t n, int64_t first,
+                     uint32_t flags)
+{
+    if (!(flags & OKA_QMC))
+        return OKA_OK;
+    if (flags & OKA_ANTITHETIC)
+        return fail(c, OKA_ERR_ARG, "OKA_QMC and OKA_ANTITHETIC exclude each other");
+    if (first + n > (int64_t(1) << 32))
+        return fail(c, OKA_ERR_ARG, "QMC point indices must stay below 2^32");
+    if (qmc_dims(nwells, dists) > kSobolMaxDims)
+        return fail(c, OKA_ERR_ARG, "QMC draws need %d dimensions, at most %d",
+                    qmc_dims(nwells, dists), kSobolMaxDims);
+    return OKA_OK;
+}
+
+// The direction numbers a QMC draw of these distributions uses, on the
+// device (the host table lives for the whole process).
+static int stage_sobol(oka_ctx *c, int32_t nwells, const oka_dist *dists, const uint32_t **out)
+{
+    const size_t bytes = sizeof(uint32_t) * 32 * (size_t)qmc_dims(nwells, dists);
+    void *p;
+    TRY(ensure(c, kSlotQmc, bytes, &p));
+    HIPCHK(c, hipMemcpyAsync(p, sobol_directions(), bytes, hipMemcpyHostToDevice, c->stream));
+    *out = (const uint32_t *)p;
+    return OKA_OK;
+}
+
 int oka_draw_realizations(oka_ctx *c, int32_t nwells, const oka_dist *dists, int64_t n,
                           uint64_t seed, int64_t first, double *q, double *K, double *por,
                           double *thick, double *z, uint32_t flags)
@@ -636,6 +665,7 @@ int oka_draw_realizations(oka_ctx *c, int32_t nwells, const oka_dist *dists, int
         return fail(c, OKA_ERR_ARG, "negative count or offset");
     if ((flags & OKA_ANTITHETIC) && ((n | first) & 1))
         return fail(c, OKA_ERR_ARG, "antithetic draws need an even count and offset");
+    TRY(check_qmc(c, nwells, dists, n, first, flags));
     if (n == 0)
         return OKA_OK;
     if ((nwells > 0 && !q) || !K || !por || !thick || !z)
@@ -647,6 +677,8 @@ int oka_draw_realizations(oka_ctx *c, int32_t nwells, const oka_dist *dists, int
     a.first = first;
     a.seed = seed;
     a.antithetic = (flags & OKA_ANTITHETIC) != 0;
+    if (flags & OKA_QMC)
+        TRY(stage_sobol(c, nwells, dists, &a.sobol));
     const void *dd;
     TRY(stage_in(c, kSlotCvDists, dists, sizeof(oka_dist) * (nwells + 3), 0, &dd));
     a.dists = (const oka_dist *)dd;
@@ -719,8 +751,12 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
     // multiples of max_batch (a multiple of 64) from an even `first`
     if (pairs && ((n | first) & 1))
         return fail(c, OKA_ERR_ARG, "antithetic draws need an even count and offset");
+    TRY(check_qmc(c, nw, dists, n, first, flags));
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    const uint32_t *sobol = nullptr;
+    if (flags & OKA_QMC)
+        TRY(stage_sobol(c, nw, dists, &sobol));
     // site, distributions, standardization (host memory)
     const size_t site_doubles = (size_t)2 * nw + 4 * nobs + (size_t)nobs * nw + 2 * nin;
     void *sb;
@@ -788,6 +824,7 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
         da.thick = thick;
         da.z = z;
         da.antithetic = pairs;
+        da.sobol = sobol;
         da.q_well_stride = chunk;   // well-major: the fit's loads are contiguous
         HIPCHK(c, launch_draw(da, s));
         fa.nreal = valid;

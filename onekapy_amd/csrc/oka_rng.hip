@@ -5,7 +5,20 @@
 // counter (realization, slot, sub) and key (seed) needs no state, so every
 // realization's draws are reproducible on their own and the launch shape does
 // not matter.
+//
+// Randomized quasi-Monte Carlo (OKA_QMC): realization j takes its uniforms
+// from point j of a Sobol' sequence (Bratley & Fox, ACM TOMS 14, 1988; the
+// direction-number recurrence as in Joe & Kuo, ACM TOMS 29, 2003) under a
+// nested (Owen-type) scramble keyed by the seed: the hash-based scramble of
+// Burley ("Practical hash-based Owen scrambling", JCGT 9(4), 2020), where
+// every bit of a coordinate flips as a function of the more significant bits
+// only.  Each scrambled point is uniform on the unit cube (a random 32-bit
+// offset enters before a bijection), the nets' stratification survives, and
+// independent seeds give independent unbiased estimates.
 #include <math.h>
+
+#include <mutex>
+#include <vector>
 
 #include "oka_internal.h"
 
@@ -45,6 +58,37 @@ __device__ inline void uniforms(uint64_t seed, int64_t r, uint32_t slot, uint32_
     u2 = ((double)(b & ((1ull << 53) - 1)) + 0.5) * 0x1p-53;
 }
 
+// The value of distribution d from its uniforms: u (the inverse-CDF variate,
+// or the Box-Muller radius variate of a lognormal), w (the lognormal's angle
+// variate) and us (the sign variate); neg negates the lognormal's normal.
+__device__ inline double dist_value(const oka_dist &d, double u, double w, double us, bool neg)
+{
+    double v;
+    switch (d.kind) {
+    case OKA_DIST_UNIFORM:
+        v = d.a + (d.b - d.a) * u;
+        break;
+    case OKA_DIST_TRIANGULAR: {
+        // inverse CDF of triangular(lo = a, mode = b, hi = c)
+        const double fc = (d.b - d.a) / (d.c - d.a);
+        v = u < fc ? d.a + sqrt(u * (d.c - d.a) * (d.b - d.a))
+                   : d.c - sqrt((1.0 - u) * (d.c - d.a) * (d.c - d.b));
+        break;
+    }
+    case OKA_DIST_LOGNORMAL: {
+        const double g = sqrt(-2.0 * log(u)) * cos(6.283185307179586 * w);
+        v = exp(d.a + d.b * (neg ? -g : g));
+        break;
+    }
+    default:
+        v = d.a;
+        break;
+    }
+    if (d.p_negative > 0.0 && us < d.p_negative)
+        v = -v;
+    return v;
+}
+
 // With `flip` the draw is the antithetic partner of realization r's: every
 // uniform u becomes 1 - u (exact: u = (k + 1/2) 2^-53 maps to the lattice
 // point of 2^53 - 1 - k) and the lognormal's normal g becomes -g, so the
@@ -54,35 +98,90 @@ __device__ inline double sample(const oka_dist &d, uint64_t seed, int64_t r, uin
 {
     double u1, u2;
     uniforms(seed, r, slot, 0, u1, u2);
-    const double ui = flip ? 1.0 - u1 : u1;   // the inverse-CDF variate
-    double v, us = u2;
-    switch (d.kind) {
-    case OKA_DIST_UNIFORM:
-        v = d.a + (d.b - d.a) * ui;
-        break;
-    case OKA_DIST_TRIANGULAR: {
-        // inverse CDF of triangular(lo = a, mode = b, hi = c)
-        const double fc = (d.b - d.a) / (d.c - d.a);
-        v = ui < fc ? d.a + sqrt(ui * (d.c - d.a) * (d.b - d.a))
-                    : d.c - sqrt((1.0 - ui) * (d.c - d.a) * (d.c - d.b));
-        break;
-    }
-    case OKA_DIST_LOGNORMAL: {
-        const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-        v = exp(d.a + d.b * (flip ? -g : g));
+    const bool ln = d.kind == OKA_DIST_LOGNORMAL;
+    double us = u2;
+    if (ln) {
         double w;
         uniforms(seed, r, slot, 1, us, w);
-        break;
-    }
-    default:
-        v = d.a;
-        break;
     }
     if (flip)
         us = 1.0 - us;
-    if (d.p_negative > 0.0 && us < d.p_negative)
-        v = -v;
-    return v;
+    return dist_value(d, ln || !flip ? u1 : 1.0 - u1, u2, us, flip);
+}
+
+// Nested scramble of a 32-bit coordinate: in bit-reversed order, a keyed
+// offset and four steps x ^= x * c (c even), each of which flips a bit as a
+// function of the bits below it only (Burley 2020, the Laine-Karras form).
+__host__ __device__ inline uint32_t nested_scramble(uint32_t x, uint32_t key)
+{
+    x = __builtin_bitreverse32(x);
+    x += key;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return __builtin_bitreverse32(x);
+}
+
+// Uniform in (0, 1) of dimension d of Sobol' point j under the scramble key
+// of `seed` for that dimension (Philox counter (d, 2^32 - 1, kQmcKeySlot, 2),
+// a counter the Philox draws never use).
+constexpr uint32_t kQmcKeySlot = 0x50B0150u;
+
+__device__ inline double qmc_u(const DrawArgs &a, uint32_t j, int d)
+{
+    const uint32_t *v = a.sobol + 32 * d;
+    uint32_t x = 0;
+    for (int k = 0; k < 32 && (j >> k) != 0u; ++k)
+        if ((j >> k) & 1u)
+            x ^= v[k];
+    const uint32_t key = philox4x32_10(U4{(uint32_t)d, 0xFFFFFFFFu, kQmcKeySlot, 2u},
+                                       (uint32_t)a.seed, (uint32_t)(a.seed >> 32)).x;
+    return ((double)nested_scramble(x, key) + 0.5) * 0x1p-32;
+}
+
+// Realization j of the QMC stream.  Dimensions, most influential first: the
+// inverse-CDF variates of K, porosity and thickness, the 6 uniforms of the
+// MVN draw's normals, those of the discharges (wells in order), then the
+// lognormals' angle variates and the sign variates (slots K, porosity,
+// thickness, then the wells).  A constant takes no dimension.
+__device__ inline void draw_qmc(const DrawArgs &a, int64_t i)
+{
+    const uint32_t j = (uint32_t)(a.first + i);
+    const int nw = a.nwells;
+    int ncore = 0, nq = 0;
+    for (int s = 0; s < 3; ++s)
+        ncore += a.dists[nw + s].kind != OKA_DIST_CONST;
+    for (int w = 0; w < nw; ++w)
+        nq += a.dists[w].kind != OKA_DIST_CONST;
+    int prim = 0, sec = ncore + 6 + nq;   // next primary and secondary dimension
+    auto value = [&](int slot) {
+        const oka_dist &d = a.dists[slot];
+        double u = 0.5, w = 0.5, us = 1.0;
+        if (d.kind != OKA_DIST_CONST)
+            u = qmc_u(a, j, prim++);
+        if (d.kind == OKA_DIST_LOGNORMAL)
+            w = qmc_u(a, j, sec++);
+        if (d.p_negative > 0.0)
+            us = qmc_u(a, j, sec++);
+        return dist_value(d, u, w, us, false);
+    };
+    a.K[i] = value(nw);
+    a.por[i] = value(nw + 1);
+    a.thick[i] = value(nw + 2);
+    for (int p = 0; p < 3; ++p) {
+        const double u1 = qmc_u(a, j, prim), u2 = qmc_u(a, j, prim + 1);
+        prim += 2;
+        const double rad = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincos(6.283185307179586 * u2, &sn, &cs);
+        a.z[i * 6 + 2 * p] = rad * cs;
+        a.z[i * 6 + 2 * p + 1] = rad * sn;
+    }
+    double *q = a.q + (a.q_well_stride ? i : i * nw);
+    const int64_t qs = a.q_well_stride ? a.q_well_stride : 1;
+    for (int w = 0; w < nw; ++w)
+        q[w * qs] = value(w);
 }
 
 __global__ void __launch_bounds__(256) k_draw(DrawArgs a)
@@ -90,6 +189,10 @@ __global__ void __launch_bounds__(256) k_draw(DrawArgs a)
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n)
         return;
+    if (a.sobol) {
+        draw_qmc(a, i);
+        return;
+    }
     // antithetic pairs (2k, 2k + 1): the odd member reuses counter 2k, reflected
     const bool flip = a.antithetic && (a.first + i) % 2 == 1;
     const int64_t r = flip ? a.first + i - 1 : a.first + i;
@@ -188,6 +291,112 @@ hipError_t launch_features(const FeatureArgs &a, hipStream_t s)
         return hipSuccess;
     hipLaunchKernelGGL(k_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
+}
+
+namespace {
+
+// x^e mod p over GF(2), p of degree deg (bit deg set), deg >= 2.
+uint64_t gf2_powmod_x(uint64_t e, uint64_t p, int deg)
+{
+    auto mulmod = [&](uint64_t a, uint64_t b) {
+        uint64_t r = 0;
+        for (; b; b >>= 1) {
+            if (b & 1)
+                r ^= a;
+            a <<= 1;
+            if ((a >> deg) & 1)
+                a ^= p;
+        }
+        return r;
+    };
+    uint64_t r = 1, base = 2;
+    for (; e; e >>= 1) {
+        if (e & 1)
+            r = mulmod(r, base);
+        base = mulmod(base, base);
+    }
+    return r;
+}
+
+// Primitive: x has order 2^deg - 1 modulo p.
+bool gf2_primitive(uint64_t p, int deg)
+{
+    const uint64_t n = (1ull << deg) - 1;
+    if (gf2_powmod_x(n, p, deg) != 1)
+        return false;
+    uint64_t m = n;
+    for (uint64_t f = 2; f * f <= m; ++f) {
+        if (m % f)
+            continue;
+        while (m % f == 0)
+            m /= f;
+        if (gf2_powmod_x(n / f, p, deg) == 1)
+            return false;
+    }
+    return m == 1 || gf2_powmod_x(n / m, p, deg) != 1;
+}
+
+uint64_t splitmix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+std::vector<uint32_t> make_sobol_directions()
+{
+    std::vector<uint32_t> v((size_t)kSobolMaxDims * 32);
+    for (int k = 0; k < 32; ++k)
+        v[k] = 1u << (31 - k);   // dimension 0: van der Corput
+    int d = 1;
+    for (int deg = 1; deg <= 16 && d < kSobolMaxDims; ++deg) {
+        for (uint64_t p = (1ull << deg) | 1ull; p < (2ull << deg) && d < kSobolMaxDims; p += 2) {
+            if (deg > 1 && !gf2_primitive(p, deg))
+                continue;
+            uint32_t *vd = v.data() + (size_t)32 * d;
+            for (int k = 1; k <= 32; ++k) {
+                uint32_t x;
+                if (k <= deg) {
+                    // initial direction number m_k: odd, below 2^k
+                    const uint64_t h = splitmix64(((uint64_t)d << 32) | (uint64_t)k);
+                    const uint32_t m = 2u * (uint32_t)(h & ((1ull << (k - 1)) - 1)) + 1u;
+                    x = m << (32 - k);
+                } else {
+                    // v_k = v_{k-s} ^ (v_{k-s} >> s) ^ sum_i a_i v_{k-i},
+                    // p = x^s + a_1 x^(s-1) + ... + a_(s-1) x + 1
+                    x = vd[k - deg - 1] ^ (vd[k - deg - 1] >> deg);
+                    for (int i = 1; i < deg; ++i)
+                        if ((p >> (deg - i)) & 1)
+                            x ^= vd[k - i - 1];
+                }
+                vd[k - 1] = x;
+            }
+            ++d;
+        }
+    }
+    return v;
+}
+
+}  // namespace
+
+const uint32_t *sobol_directions()
+{
+    static std::vector<uint32_t> table;
+    static std::once_flag once;
+    std::call_once(once, [] { table = make_sobol_directions(); });
+    return table.data();
+}
+
+int qmc_dims(int32_t nwells, const oka_dist *dists)
+{
+    int d = 6;   // the MVN draw's normals
+    for (int s = 0; s < nwells + 3; ++s) {
+        d += dists[s].kind != OKA_DIST_CONST;
+        d += dists[s].kind == OKA_DIST_LOGNORMAL;
+        d += dists[s].p_negative > 0.0;
+    }
+    return d;
 }
 
 uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, int word)

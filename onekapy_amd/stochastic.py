@@ -150,10 +150,11 @@ def oka_dists(stochastic_wells, c_dist, p_dist, t_dist):
 
 
 def draw_realizations_device(stochastic_wells, c_dist, p_dist, t_dist, nrealizations, seed,
-                             first=0, antithetic=False):
+                             first=0, antithetic=False, qmc=False):
     """The same draws as draw_realizations, from the library's Philox4x32-10
     stream `seed` (realizations first .. first + n - 1).  With antithetic
-    (even count and first), realization 2k + 1 reflects 2k's draws."""
+    (even count and first), realization 2k + 1 reflects 2k's draws; with qmc,
+    realization j is Sobol' point j under the nested scramble keyed by `seed`."""
     R, nw = int(nrealizations), len(stochastic_wells)
     out = dict(q=np.zeros((R, nw)), K=np.zeros(R), n=np.zeros(R), b=np.zeros(R),
                z=np.zeros((R, 6)))
@@ -162,7 +163,7 @@ def draw_realizations_device(stochastic_wells, c_dist, p_dist, t_dist, nrealizat
     ctx.check(ctx.lib.oka_draw_realizations(
         ctx.h, nw, C.addressof(dists), R, int(seed) & (2 ** 64 - 1), int(first),
         _lib.ptr(out["q"]), _lib.ptr(out["K"]), _lib.ptr(out["n"]), _lib.ptr(out["b"]),
-        _lib.ptr(out["z"]), _lib.ANTITHETIC if antithetic else 0))
+        _lib.ptr(out["z"]), (_lib.ANTITHETIC if antithetic else 0) | (_lib.QMC if qmc else 0)))
     return out
 
 

@@ -149,11 +149,12 @@ class SurrogateMLP:
             acc["d2"].ctypes.data if mp else None, 0))
         return acc
 
-    def surrogate_sums(self, site, n, seed, first, std, antithetic=False):
+    def surrogate_sums(self, site, n, seed, first, std, antithetic=False, qmc=False):
         """Sums of s and s^2 over n fresh realizations drawn, fitted and
         featurized on the device (oka_cv_surrogate_sums).  With antithetic
         the realizations come in reflected pairs and "s2" sums the squared
-        pair means (n / 2 of them)."""
+        pair means (n / 2 of them); with qmc they are Sobol' points
+        first .. first + n - 1 under the nested scramble keyed by `seed`."""
         from .stochastic import oka_dists
         acc = {"s": np.zeros(self.n_out), "s2": np.zeros(self.n_out)}
         wxy = site.wells_xy()
@@ -165,7 +166,8 @@ class SurrogateMLP:
         self.ctx.check(self.ctx.lib.oka_cv_surrogate_sums(
             self.ctx.h, self.h, s, C.addressof(dists), int(n), int(seed) & (2 ** 64 - 1),
             int(first), mean.ctypes.data, sd.ctypes.data, acc["s"].ctypes.data,
-            acc["s2"].ctypes.data, _lib.ANTITHETIC if antithetic else 0))
+            acc["s2"].ctypes.data,
+            (_lib.ANTITHETIC if antithetic else 0) | (_lib.QMC if qmc else 0)))
         if not np.all(np.isfinite(acc["s"])):
             raise ArithmeticError("the fit failed for some surrogate draws")
         return acc
@@ -288,15 +290,24 @@ class ControlVariateResult:
         self.__dict__.update(kw)
 
 
+def _replicate_seed(base, fold, k):
+    """Seed of fold `fold`'s k-th QMC replicate (an independent scramble)."""
+    return int(np.random.SeedSequence([base, fold, k]).generate_state(1, np.uint64)[0])
+
+
 def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batch=512, lr=1e-3,
                     seed=0, precision=_lib.FP32, pilot=2000, window=None, n_hidden=512,
-                    antithetic=False):
+                    antithetic=False, qmc_replicates=0):
     """Cross-fitted control-variate estimate of the window cells' capture probability.
 
     Each fold's network takes train_steps Adam steps of `batch` samples (whole
     shuffled epochs, rounded up).  With antithetic, the surrogate term draws
     its realizations in antithetic pairs and its variance is that of the pair
-    means over the number of pairs.
+    means over the number of pairs.  With qmc_replicates = R > 0 instead, each
+    fold's surrogate realizations are R independently scrambled copies of the
+    first M_f / R Sobol' points (randomized quasi-Monte Carlo); the surrogate
+    mean is the mean of the R replicate means and its variance their sample
+    variance over R.
 
     Returns a ControlVariateResult with p_cv, its estimated variance var_cv,
     the plain estimate p_plain from the same traced realizations with its
@@ -304,6 +315,8 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     The folds run concurrently: times["folds"] is their wall time, "train"
     and "surrogate" the longest fold's time in each of its two phases.
     """
+    if antithetic and qmc_replicates:
+        raise ValueError("antithetic pairs and QMC replicates exclude each other")
     t = {}
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
@@ -351,7 +364,15 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
         loss = nets[f].train(X, marks, tscale, epochs, batch, lr, seed=seed + 100 + f, rows=tr)
         t2 = time.perf_counter()
         mf = int(surr_split[f + 1] - surr_split[f])
-        a_s = nets[f].surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std, antithetic)
+        if qmc_replicates:
+            npts = mf // qmc_replicates
+            mf = npts * qmc_replicates
+            reps = [nets[f].surrogate_sums(site, npts, _replicate_seed(surr_seed, f, k), 0, std,
+                                           qmc=True)["s"] / npts for k in range(qmc_replicates)]
+            a_s = {"reps": np.array(reps)}
+        else:
+            a_s = nets[f].surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std,
+                                         antithetic)
         a_d = nets[f].accumulate(X[lo:hi], marks[lo:hi], tscale)
         out[f] = (loss, a_s, a_d, mf, hi - lo, t2 - t1, time.perf_counter() - t2)
 
@@ -383,11 +404,18 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
         loss, a_s, a_d, mf, nf, _, _ = out[f]
         losses.append(loss)
         w = nf / n_traced
-        mean_s = a_s["s"] / mf
-        # the variance of mean_s: that of one surrogate value (or of one
-        # antithetic pair's mean) over the number of values (or pairs)
-        ms = mf // 2 if antithetic else mf
-        var_s = np.maximum(a_s["s2"] / ms - mean_s ** 2, 0.0)
+        if qmc_replicates:
+            # replicate means are independent and unbiased: their mean and
+            # the variance of that mean
+            mean_s = a_s["reps"].mean(0)
+            var_s = a_s["reps"].var(0, ddof=1)
+            ms = qmc_replicates
+        else:
+            mean_s = a_s["s"] / mf
+            # the variance of mean_s: that of one surrogate value (or of one
+            # antithetic pair's mean) over the number of values (or pairs)
+            ms = mf // 2 if antithetic else mf
+            var_s = np.maximum(a_s["s2"] / ms - mean_s ** 2, 0.0)
         mean_d = a_d["d"] / nf
         var_d = np.maximum(a_d["d2"] / nf - mean_d ** 2, 0.0)
         sum_s += w * mean_s

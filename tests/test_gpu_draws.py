@@ -65,6 +65,48 @@ def test_antithetic_pairs_reflect_the_even_draw(gpu_ctx):
         draw_realizations_device(*args, first=first + 1, antithetic=True)
 
 
+def test_qmc_draws_are_the_scrambled_sobol_points(gpu_ctx):
+    """OKA_QMC: realization j's uniforms are the scrambled Sobol' point j (the
+    checker's), dimension by dimension in the documented order."""
+    from oracle import sobol
+    mu = math.log(800.0)
+    wells = [(0.0, 0.0, 0.25, (300.0, 450.0, 700.0)), (1.0, 0.0, 0.25, Signed(Lognormal(mu, 0.6), 0.3)),
+             (2.0, 0.0, 0.25, 700.0)]
+    seed, first, n = 0x5EED_0F_50B0, 1000, 512
+    d = draw_realizations_device(wells, (2.0, 5.0), 0.25, (20.0, 30.0), n, seed, first, qmc=True)
+    # dims: K 0, thickness 1 (porosity is constant), normals 2-7, discharges of
+    # wells 0 and 1: 8, 9 (well 2 is constant), then well 1's angle 10, sign 11
+    v = sobol.direction_numbers(12)
+    for i in range(0, n, 7):
+        j = first + i
+        u = [sobol.uniform(v[dd], j, seed, dd) for dd in range(12)]
+        assert d["K"][i] == 2.0 + 3.0 * u[0]
+        assert d["b"][i] == 20.0 + 10.0 * u[1] and d["n"][i] == 0.25
+        for p in range(3):
+            rad = math.sqrt(-2.0 * math.log(u[2 + 2 * p]))
+            ang = 2.0 * math.pi * u[3 + 2 * p]
+            assert abs(d["z"][i, 2 * p] - rad * math.cos(ang)) < 1e-12
+            assert abs(d["z"][i, 2 * p + 1] - rad * math.sin(ang)) < 1e-12
+        fc = 150.0 / 400.0
+        tri = (300.0 + math.sqrt(u[8] * 400.0 * 150.0) if u[8] < fc
+               else 700.0 - math.sqrt((1.0 - u[8]) * 400.0 * 250.0))
+        assert abs(d["q"][i, 0] - tri) < 1e-9
+        g = math.sqrt(-2.0 * math.log(u[9])) * math.cos(2.0 * math.pi * u[10])
+        ln = math.exp(mu + 0.6 * g) * (-1.0 if u[11] < 0.3 else 1.0)
+        assert abs(d["q"][i, 1] - ln) < 1e-9 * abs(ln)
+        assert d["q"][i, 2] == 700.0
+    # 1024 points from 0: K's uniforms fill each interval of width 1/1024 once
+    d = draw_realizations_device(wells, (2.0, 5.0), 0.25, (20.0, 30.0), 1024, seed, 0, qmc=True)
+    bins = np.floor((d["K"] - 2.0) / 3.0 * 1024).astype(int)
+    assert np.array_equal(np.sort(bins), np.arange(1024))
+    with pytest.raises(_lib.OkaError):
+        draw_realizations_device(wells, (2.0, 5.0), 0.25, (20.0, 30.0), 4, seed, 0, antithetic=True,
+                                 qmc=True)
+    with pytest.raises(_lib.OkaError):
+        draw_realizations_device(wells, (2.0, 5.0), 0.25, (20.0, 30.0), 4, seed, 2 ** 32 - 2,
+                                 qmc=True)
+
+
 def test_distribution_moments(gpu_ctx):
     n = 400000
     wells = [(0.0, 0.0, 0.25, (300.0, 500.0, 700.0)),

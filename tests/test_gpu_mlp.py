@@ -137,11 +137,12 @@ def cfg0_judge(gpu_ctx):
     return s, counts, R
 
 
-@pytest.mark.parametrize("antithetic", [False, True])
-def test_control_variate_is_unbiased_and_reduces_variance(cfg0_judge, antithetic):
+@pytest.mark.parametrize("mode", ["plain", "antithetic", "qmc"])
+def test_control_variate_is_unbiased_and_reduces_variance(cfg0_judge, mode):
     s, counts, R = cfg0_judge
     res = control_variate(s, n_traced=16384, n_surrogate=131072, folds=2, train_steps=1500,
-                          batch=512, lr=1e-3, seed=7, n_hidden=256, antithetic=antithetic)
+                          batch=512, lr=1e-3, seed=7, n_hidden=256,
+                          antithetic=mode == "antithetic", qmc_replicates=16 if mode == "qmc" else 0)
     # the judge: the large plain run over the same window
     p_ref = cell_probabilities(counts, R, res.window)
     var_ref = p_ref * (1 - p_ref) / R
@@ -153,8 +154,8 @@ def test_control_variate_is_unbiased_and_reduces_variance(cfg0_judge, antithetic
     assert gain > 3.0, gain
 
 
-@pytest.mark.parametrize("antithetic", [False, True])
-def test_surrogate_sums_compose_draws_fit_features_and_network(gpu_ctx, antithetic):
+@pytest.mark.parametrize("mode", ["plain", "antithetic", "qmc"])
+def test_surrogate_sums_compose_draws_fit_features_and_network(gpu_ctx, mode):
     """oka_cv_surrogate_sums against its parts called one by one: the device
     draws, the fit + MVN draw, the standardized features and the network's
     probabilities, summed on the host (squared pair means when antithetic)."""
@@ -163,14 +164,15 @@ def test_surrogate_sums_compose_draws_fit_features_and_network(gpu_ctx, antithet
     s = sites.config0()
     net = SurrogateMLP(len(s.wells) + 9, 128, 128, max_batch=256, seed=3)
     seed, first, n = 0xBEEF, 4096, 1000   # four network passes, the last one ragged
+    antithetic, qmc = mode == "antithetic", mode == "qmc"
     d = draw_realizations_device(s.wells, s.c_dist, s.p_dist, s.t_dist, n, seed, first,
-                                 antithetic)
+                                 antithetic, qmc)
     _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
                               d["q"], d["z"], moments=False)
     feats = features(d, coef)
     std = Standardizer(feats)
     prob = net.predict(std(feats)).astype(np.float64)
-    acc = net.surrogate_sums(s, n, seed, first, std, antithetic)
+    acc = net.surrogate_sums(s, n, seed, first, std, antithetic, qmc)
     # FP32 partial sums per lane, FP64 across workgroups
     np.testing.assert_allclose(acc["s"], prob.sum(0), rtol=1e-5, atol=1e-6)
     if antithetic:
