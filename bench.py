@@ -79,8 +79,8 @@ def parse():
     ap.add_argument("--cv-config-qmc", type=int, default=32,
                     help="QMC replicates of the estimator at the workload's own sizes (scrambled "
                          "Sobol' points; 0: antithetic pairs like the tuned one)")
-    ap.add_argument("--cv-judge", type=int, default=1000000,
-                    help="realizations of the FP64 plain run that judges the estimates")
+    ap.add_argument("--cv-reference", type=int, default=1000000,
+                    help="realizations of the FP64 plain run the estimates are compared with")
     ap.add_argument("--cv-steps", type=int, default=1200)
     ap.add_argument("--cv-batch", type=int, default=2048)
     ap.add_argument("--cv-lr", type=float, default=5e-3)
@@ -89,11 +89,11 @@ def parse():
 
 def control_variate_leg(site, args, fp32_rate):
     """Learned control variate on the same site: N traced realizations (FP32)
-    + M surrogate evaluations, cross-fitted over 2 folds, judged against an
+    + M surrogate evaluations, cross-fitted over 2 folds, compared with an
     FP64 plain run.  Effective realizations/s = the plain-run size with the
     same maximum (over window cells) standard error, divided by the
     pipeline's wall time (draws, fits, pilot, tracing, training, surrogate).
-    Two estimators share the judge: the tuned one (N = --cv-traced, M =
+    Two estimators share that reference run: the tuned one (N = --cv-traced, M =
     --cv-surrogate, surrogate realizations in antithetic pairs) and the
     workload's own sizes (100,000 traced realizations to train on, 10^6
     surrogate evaluations as --cv-config-qmc scrambled Sobol' replicates)."""
@@ -121,16 +121,16 @@ def control_variate_leg(site, args, fp32_rate):
     configured = run(100000, 1000000, args.cv_antithetic and not args.cv_config_qmc,
                      args.cv_config_qmc)
     rng = np.random.default_rng(321)
-    d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, args.cv_judge, rng)
+    d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, args.cv_reference, rng)
     _, _, coef = fit_and_draw(site.wells_xy(), site.xt, site.yt, site.base, site.observations,
                               d["K"], d["b"], d["q"], d["z"])
-    print(f"bench: FP64 judge run of {args.cv_judge} realizations", file=sys.stderr, flush=True)
+    print(f"bench: FP64 reference run of {args.cv_reference} realizations", file=sys.stderr, flush=True)
     tj = time.perf_counter()
     counts, _ = trace_counts(site, d, coef, _lib.FP64_EXACT)
     tj = time.perf_counter() - tj
 
     def summary(res, wall, n_traced, n_surrogate):
-        p_ref = cell_probabilities(counts, args.cv_judge, res.window)
+        p_ref = cell_probabilities(counts, args.cv_reference, res.window)
         err_cv = np.abs(res.p_cv - p_ref)
         err_plain = np.abs(res.p_plain - p_ref)
         n_eff = n_traced * res.var_plain.max() / res.var_cv.max()
@@ -157,7 +157,7 @@ def control_variate_leg(site, args, fp32_rate):
     out["surrogate_sampling"] = sampling(args.cv_antithetic, 0)
     out.update({"folds": 2, "train_steps_per_fold": args.cv_steps, "batch": args.cv_batch,
                 "lr": args.cv_lr, "network": "34->512->512->4096 GELU, BF16 MFMA",
-                "judge": {"realizations": args.cv_judge, "tracer": "fp64", "seconds": tj}})
+                "reference_run": {"realizations": args.cv_reference, "tracer": "fp64", "seconds": tj}})
     out["as_configured"] = summary(*configured, 100000, 1000000)
     out["as_configured"]["surrogate_sampling"] = sampling(
         args.cv_antithetic and not args.cv_config_qmc, args.cv_config_qmc)

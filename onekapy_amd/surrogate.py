@@ -16,7 +16,7 @@ surrogate, and whose variance Var(y - s)/N + Var(s)/M is far below the plain
 Var(y)/N when s is a good predictor.  The N traced realizations are used both
 to train and to correct by K-fold cross-fitting (each fold is corrected by a
 network that never saw it), so no traced realization is spent on training
-alone.  The plain tracer remains the fallback and the judge of the error.
+alone.  The plain tracer remains the fallback and the reference for the error.
 
 The network (n_in -> 512 -> 512 -> 4096, GELU, BF16 weights, FP32
 accumulation) is trained and evaluated by liboka's hand-written BF16 MFMA

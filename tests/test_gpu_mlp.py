@@ -124,7 +124,7 @@ def test_training_fits_a_smooth_target(gpu_ctx):
 
 
 @pytest.fixture(scope="module")
-def cfg0_judge(gpu_ctx):
+def cfg0_plain_run(gpu_ctx):
     """A large plain run on configuration 0's site (FP32 tracer): node counts
     over R realizations."""
     s = sites.config0()
@@ -138,12 +138,12 @@ def cfg0_judge(gpu_ctx):
 
 
 @pytest.mark.parametrize("mode", ["plain", "antithetic", "qmc"])
-def test_control_variate_is_unbiased_and_reduces_variance(cfg0_judge, mode):
-    s, counts, R = cfg0_judge
+def test_control_variate_is_unbiased_and_reduces_variance(cfg0_plain_run, mode):
+    s, counts, R = cfg0_plain_run
     res = control_variate(s, n_traced=16384, n_surrogate=131072, folds=2, train_steps=1500,
                           batch=512, lr=1e-3, seed=7, n_hidden=256,
                           antithetic=mode == "antithetic", qmc_replicates=16 if mode == "qmc" else 0)
-    # the judge: the large plain run over the same window
+    # the reference: the large plain run over the same window
     p_ref = cell_probabilities(counts, R, res.window)
     var_ref = p_ref * (1 - p_ref) / R
     z = (res.p_cv - p_ref) / np.sqrt(res.var_cv + var_ref + 1e-12)
