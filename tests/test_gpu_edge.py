@@ -141,6 +141,48 @@ def test_config1_subset_bit_exact_and_invariants(gpu_ctx):
     assert res.counts[it, jt] == s.nrealizations
 
 
+def test_config1_full_size_properties(gpu_ctx):
+    """The headline workload at its full size (10^4 realizations x 1000
+    particles, 400^2 nodes), through properties that hold at any size: FP32
+    within 0.01 of FP64-exact at every node; node counts additive over
+    realization subsets (two halves, both tracers); the pool schedule equal to
+    the workgroup one; counts bounded by R, summing to the visited total and
+    equal to R at the target well."""
+    from onekapy_amd import fit_and_draw
+    s = sites.config1()
+    R = s.nrealizations
+    d = sites.draw(s, seed=11)
+    _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
+                              d["q"], d["z"])
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+
+    def run(sl, **kw):
+        real = Realizations(d["K"][sl], d["n"][sl], d["b"][sl], d["q"][sl], coef[sl])
+        pf = s.fixed_field()
+        return chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
+                                    s.tol, s.maxstep, s.umbra, **kw)
+
+    halves = (slice(0, R // 2), slice(R // 2, R))
+    out = {}
+    for prec in (_lib.FP64_EXACT, _lib.FP32):
+        full = run(slice(None), precision=prec)
+        assert full.stats.escaped == 0
+        assert list(full.stats.status_count) == [R * s.npaths, 0, 0, 0]
+        parts = [run(h, precision=prec).counts for h in halves]
+        assert np.array_equal(full.counts, parts[0] + parts[1])
+        assert full.counts.max() <= R and full.counts.sum() == full.stats.visited
+        pf = s.fixed_field()
+        jt = int(round((s.xt - pf.xmin) / pf.deltax))
+        it = int(round((s.yt - pf.ymin) / pf.deltay))
+        assert full.counts[it, jt] == R
+        out[prec] = full
+    pool = run(slice(None), precision=_lib.FP32, schedule=_lib.SCHED_POOL)
+    assert np.array_equal(pool.counts, out[_lib.FP32].counts)
+    err = np.abs(out[_lib.FP32].counts - out[_lib.FP64_EXACT].counts) / R
+    assert err.max() <= 0.01
+    assert err.sum() <= 0.01 * (out[_lib.FP64_EXACT].counts > 0).sum()
+
+
 def test_window_marks_match_oracle_per_realization(gpu_ctx):
     from onekapy_amd.surrogate import cell_probabilities, trace_marks, _window
     s, d = small_site(nreal=5, npaths=48)
