@@ -39,6 +39,29 @@ def test_batch_matches_per_site_runs(gpu_ctx):
     b.close()
 
 
+def test_config2_full_size_batch_matches_per_site_runs(gpu_ctx):
+    """The 64-site uneven workload at its full size (about 1.5 * 10^5
+    realizations x 256 particles) in one graph replay: every site's counts and
+    step statistics equal a run of that site alone, and the counts stay within
+    the site's realization count."""
+    ss = sites.config2()
+    draws = [sites.draw(s, seed=100 + i) for i, s in enumerate(ss)]
+    b = SiteBatch(ss, draws)
+    b.run()
+    total = 0
+    for i, (s, d) in enumerate(zip(ss, draws)):
+        print(f"site {i}: {s.name}", flush=True)
+        counts, coef, st = b.results(i)
+        ref_counts, ref_st = trace_counts(s, d, coef, _lib.FP32)
+        assert np.array_equal(counts, ref_counts), i
+        assert (st.attempted, st.accepted, st.visited) == (
+            ref_st.attempted, ref_st.accepted, ref_st.visited), i
+        assert counts.max() <= len(d["K"]) and counts.sum() == st.visited
+        total += len(d["K"])
+    assert total > 100000
+    b.close()
+
+
 def test_batch_replays_and_retunes_identically(gpu_ctx):
     """Every replay of the captured graph (fills, fits, pool launch) equals the
     eager per-site runs, with eager fits and traces on the device between the
