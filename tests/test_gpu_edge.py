@@ -183,6 +183,44 @@ def test_config1_full_size_properties(gpu_ctx):
     assert err.sum() <= 0.01 * (out[_lib.FP64_EXACT].counts > 0).sum()
 
 
+def test_config4_shape_properties(gpu_ctx):
+    """The large regional stress case at its real shape (200 wells, 10 %
+    injection, 200 observations, 2000^2 nodes at 10 m, 256 particles, 30
+    years) on 1,000 of its realizations: FP32 within 0.01 of FP64-exact at
+    every node, the pool schedule equal to the workgroup one, counts additive
+    over realization halves and bounded by R."""
+    from onekapy_amd import fit_and_draw
+    R = 1000
+    s = sites.config4(R)
+    d = sites.draw(s, seed=4)
+    _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
+                              d["q"], d["z"])
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+
+    def run(sl, **kw):
+        real = Realizations(d["K"][sl], d["n"][sl], d["b"][sl], d["q"][sl], coef[sl])
+        pf = s.fixed_field()
+        return chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
+                                    s.tol, s.maxstep, s.umbra, **kw)
+
+    print("fp32 workgroup", flush=True)
+    wg = run(slice(None), precision=_lib.FP32)
+    print("fp32 pool", flush=True)
+    pool = run(slice(None), precision=_lib.FP32, schedule=_lib.SCHED_POOL)
+    assert np.array_equal(pool.counts, wg.counts)
+    assert list(pool.stats.status_count) == list(wg.stats.status_count)
+    print("fp32 halves", flush=True)
+    halves = [run(h, precision=_lib.FP32, schedule=_lib.SCHED_POOL).counts
+              for h in (slice(0, R // 2), slice(R // 2, R))]
+    assert np.array_equal(wg.counts, halves[0] + halves[1])
+    assert wg.counts.max() <= R and wg.counts.sum() == wg.stats.visited
+    print("fp64", flush=True)
+    f64 = run(slice(None), precision=_lib.FP64_EXACT)
+    err = np.abs(wg.counts - f64.counts) / R
+    assert err.max() <= 0.01
+    assert err.sum() <= 0.01 * (f64.counts > 0).sum()
+
+
 def test_window_marks_match_oracle_per_realization(gpu_ctx):
     from onekapy_amd.surrogate import cell_probabilities, trace_marks, _window
     s, d = small_site(nreal=5, npaths=48)
