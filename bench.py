@@ -76,9 +76,10 @@ def parse():
     ap.add_argument("--cv-surrogate", type=int, default=10000000)
     ap.add_argument("--cv-antithetic", action=argparse.BooleanOptionalAction, default=True,
                     help="draw the tuned estimator's surrogate realizations in antithetic pairs")
-    ap.add_argument("--cv-config-qmc", type=int, default=32,
+    ap.add_argument("--cv-config-qmc", type=int, default=64,
                     help="QMC replicates of the estimator at the workload's own sizes (scrambled "
-                         "Sobol' points; 0: antithetic pairs like the tuned one)")
+                         "Sobol' points, a power of two per replicate, so the count rises to "
+                         "fill the budget; 0: antithetic pairs like the tuned one)")
     ap.add_argument("--cv-reference", type=int, default=1000000,
                     help="realizations of the FP64 plain run the estimates are compared with")
     ap.add_argument("--cv-steps", type=int, default=1200)
@@ -95,11 +96,12 @@ def control_variate_leg(site, args, fp32_rate):
     pipeline's wall time (draws, fits, pilot, tracing, training, surrogate).
     Two estimators share that reference run: the tuned one (N = --cv-traced, M =
     --cv-surrogate, surrogate realizations in antithetic pairs) and the
-    workload's own sizes (100,000 traced realizations to train on, 10^6
-    surrogate evaluations as --cv-config-qmc scrambled Sobol' replicates)."""
+    workload's own sizes (100,000 traced realizations to train on, at most
+    10^6 surrogate evaluations as scrambled Sobol' replicates, --cv-config-qmc
+    of them before rounding each to a power of two)."""
     from onekapy_amd import _lib
     from onekapy_amd.stochastic import draw_realizations, fit_and_draw
-    from onekapy_amd.surrogate import cell_probabilities, control_variate, trace_counts
+    from onekapy_amd.surrogate import cell_probabilities, control_variate, qmc_split, trace_counts
 
     def run(n_traced, n_surrogate, antithetic, qmc):
         print(f"bench: control variate: {n_traced} traced + {n_surrogate} surrogate",
@@ -107,14 +109,16 @@ def control_variate_leg(site, args, fp32_rate):
         t0 = time.perf_counter()
         res = control_variate(site, n_traced, n_surrogate, folds=2, train_steps=args.cv_steps,
                               batch=args.cv_batch, lr=args.cv_lr, seed=123,
-                              precision=_lib.FP32, antithetic=antithetic, qmc_replicates=qmc)
+                              precision=_lib.FP32, antithetic=antithetic, qmc_replicates=qmc,
+                              qmc_pow2=True)
         wall = time.perf_counter() - t0
         print(f"bench: control variate {wall:.2f} s {res.times}", file=sys.stderr, flush=True)
         return res, wall
 
-    def sampling(antithetic, qmc):
+    def sampling(antithetic, qmc, n_surrogate):
         if qmc:
-            return f"scrambled Sobol' points, {qmc} replicates per fold"
+            npts, nrep = qmc_split(n_surrogate // 2, qmc)
+            return f"scrambled Sobol' points, {nrep} replicates of {npts} per fold"
         return "antithetic pairs" if antithetic else "independent draws"
 
     tuned = run(args.cv_traced, args.cv_surrogate, args.cv_antithetic, 0)
@@ -154,13 +158,15 @@ def control_variate_leg(site, args, fp32_rate):
         }
 
     out = summary(*tuned, args.cv_traced, args.cv_surrogate)
-    out["surrogate_sampling"] = sampling(args.cv_antithetic, 0)
+    out["surrogate_sampling"] = sampling(args.cv_antithetic, 0, args.cv_surrogate)
+    out["surrogate_evals_used"] = int(tuned[0].n_surrogate_used)
     out.update({"folds": 2, "train_steps_per_fold": args.cv_steps, "batch": args.cv_batch,
                 "lr": args.cv_lr, "network": "34->512->512->4096 GELU, BF16 MFMA",
                 "reference_run": {"realizations": args.cv_reference, "tracer": "fp64", "seconds": tj}})
     out["as_configured"] = summary(*configured, 100000, 1000000)
     out["as_configured"]["surrogate_sampling"] = sampling(
-        args.cv_antithetic and not args.cv_config_qmc, args.cv_config_qmc)
+        args.cv_antithetic and not args.cv_config_qmc, args.cv_config_qmc, 1000000)
+    out["as_configured"]["surrogate_evals_used"] = int(configured[0].n_surrogate_used)
     return out
 
 

@@ -290,6 +290,19 @@ class ControlVariateResult:
         self.__dict__.update(kw)
 
 
+def qmc_split(m, replicates, pow2=True):
+    """(points per replicate, replicates) for m QMC evaluations: m // replicates
+    points, or with pow2 the largest power of two not above that (a Sobol' net's
+    balanced size) and as many replicates as then fit in m."""
+    npts = m // replicates
+    if npts < 1:
+        raise ValueError(f"{m} QMC evaluations cannot fill {replicates} replicates")
+    if pow2:
+        npts = 1 << (npts.bit_length() - 1)
+        return npts, m // npts
+    return npts, replicates
+
+
 def _replicate_seed(base, fold, k):
     """Seed of fold `fold`'s k-th QMC replicate (an independent scramble)."""
     return int(np.random.SeedSequence([base, fold, k]).generate_state(1, np.uint64)[0])
@@ -297,7 +310,7 @@ def _replicate_seed(base, fold, k):
 
 def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batch=512, lr=1e-3,
                     seed=0, precision=_lib.FP32, pilot=2000, window=None, n_hidden=512,
-                    antithetic=False, qmc_replicates=0):
+                    antithetic=False, qmc_replicates=0, qmc_pow2=False):
     """Cross-fitted control-variate estimate of the window cells' capture probability.
 
     Each fold's network takes train_steps Adam steps of `batch` samples (whole
@@ -305,9 +318,10 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     its realizations in antithetic pairs and its variance is that of the pair
     means over the number of pairs.  With qmc_replicates = R > 0 instead, each
     fold's surrogate realizations are R independently scrambled copies of the
-    first M_f / R Sobol' points (randomized quasi-Monte Carlo); the surrogate
-    mean is the mean of the R replicate means and its variance their sample
-    variance over R.
+    first M_f / R Sobol' points (randomized quasi-Monte Carlo; with qmc_pow2
+    the points per copy are rounded down to a power of two and R raised to
+    fill M_f, see qmc_split); the surrogate mean is the mean of the replicate
+    means and its variance their sample variance over the replicate count.
 
     Returns a ControlVariateResult with p_cv, its estimated variance var_cv,
     the plain estimate p_plain from the same traced realizations with its
@@ -365,10 +379,10 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
         t2 = time.perf_counter()
         mf = int(surr_split[f + 1] - surr_split[f])
         if qmc_replicates:
-            npts = mf // qmc_replicates
-            mf = npts * qmc_replicates
+            npts, nrep = qmc_split(mf, qmc_replicates, qmc_pow2)
+            mf = npts * nrep
             reps = [nets[f].surrogate_sums(site, npts, _replicate_seed(surr_seed, f, k), 0, std,
-                                           qmc=True)["s"] / npts for k in range(qmc_replicates)]
+                                           qmc=True)["s"] / npts for k in range(nrep)]
             a_s = {"reps": np.array(reps)}
         else:
             a_s = nets[f].surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std,
@@ -409,7 +423,7 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
             # the variance of that mean
             mean_s = a_s["reps"].mean(0)
             var_s = a_s["reps"].var(0, ddof=1)
-            ms = qmc_replicates
+            ms = a_s["reps"].shape[0]
         else:
             mean_s = a_s["s"] / mf
             # the variance of mean_s: that of one surrogate value (or of one
@@ -427,5 +441,6 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
     t["plain"] = time.perf_counter() - t0
     return ControlVariateResult(p_cv=sum_s + sum_d, var_cv=var_cv, p_plain=p_plain,
                                 var_plain=var_plain, window=window, times=t, folds=folds,
-                                n_traced=n_traced, n_surrogate=n_surrogate, losses=losses,
+                                n_traced=n_traced, n_surrogate=n_surrogate,
+                                n_surrogate_used=sum(o[3] for o in out), losses=losses,
                                 trace_stats=st)

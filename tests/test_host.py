@@ -184,3 +184,19 @@ def test_sites_are_well_posed():
     for k in a:
         assert np.array_equal(a[k], b[k])
 
+
+
+def test_qmc_split_keeps_within_the_budget():
+    from onekapy_amd.surrogate import qmc_split
+    assert qmc_split(500000, 32, pow2=False) == (15625, 32)
+    assert qmc_split(500000, 64) == (4096, 122)
+    assert qmc_split(500000, 16) == (16384, 30)
+    for m in (1, 7, 4096, 10 ** 6 + 3):
+        for r in (1, 3, 64):
+            if m < r:
+                with pytest.raises(ValueError):
+                    qmc_split(m, r)
+                continue
+            npts, nrep = qmc_split(m, r)
+            assert npts & (npts - 1) == 0 and nrep >= r and npts * nrep <= m
+            assert npts * (nrep + 1) > m          # no whole replicate left out
