@@ -121,6 +121,14 @@ def control_variate_leg(site, args, fp32_rate):
             return f"scrambled Sobol' points, {nrep} replicates of {npts} per fold"
         return "antithetic pairs" if antithetic else "independent draws"
 
+    # untimed warm-up of the code paths the timed runs take (first launches,
+    # first use of buffers), as the tracer legs' warmup steps do
+    print("bench: control variate warm-up", file=sys.stderr, flush=True)
+    for anti, qmc in ((args.cv_antithetic, 0), (False, args.cv_config_qmc)):
+        if anti or qmc or (not args.cv_antithetic and not qmc):
+            control_variate(site, 8192, 262144, folds=2, train_steps=64, batch=args.cv_batch,
+                            lr=args.cv_lr, seed=7, precision=_lib.FP32, antithetic=anti,
+                            qmc_replicates=qmc, qmc_pow2=True)
     tuned = run(args.cv_traced, args.cv_surrogate, args.cv_antithetic, 0)
     configured = run(100000, 1000000, args.cv_antithetic and not args.cv_config_qmc,
                      args.cv_config_qmc)

@@ -424,6 +424,19 @@ int oka_cv_surrogate_sums(oka_ctx *ctx, oka_mlp *mlp, const oka_site *site, cons
                           int64_t n, uint64_t seed, int64_t first, const double *feat_mean,
                           const double *feat_std, double *sums, double *sums_sq, uint32_t flags);
 
+/* nrep independent randomized-QMC replicates of the surrogate term in one
+ * call: replicate k is oka_cv_surrogate_sums with OKA_QMC over Sobol' points
+ * 0 .. npts - 1 under the scramble of seeds[k] (host memory), and adds its
+ * column sums to row k of sums (nrep x n_out, follows flags).  The site is
+ * staged once and the passes are enqueued back to back on the context's
+ * stream with one synchronization; the replicate means and their sample
+ * variance give the surrogate mean and its variance.  OKA_ANTITHETIC is an
+ * error here. */
+int oka_cv_surrogate_sums_qmc(oka_ctx *ctx, oka_mlp *mlp, const oka_site *site,
+                              const oka_dist *dists, int64_t npts, int32_t nrep,
+                              const uint64_t *seeds, const double *feat_mean,
+                              const double *feat_std, double *sums, uint32_t flags);
+
 /* ---- exact scalar helpers shared by host and device code (for tests) ----- */
 double oka_pow_fifth(double x);   /* correctly rounded pow(x, 0.2) (capturezone.py:247) */
 double oka_log(double x);         /* correctly rounded log(x) (model.py:264)            */

@@ -42,7 +42,8 @@ EXPORTED = (
     "oka_rasterize", "oka_pow_fifth", "oka_log", "oka_trace_marks", "oka_mlp_create",
     "oka_mlp_destroy", "oka_mlp_param_count", "oka_mlp_set_params", "oka_mlp_get_params",
     "oka_mlp_grad", "oka_mlp_train", "oka_mlp_predict", "oka_mlp_accumulate",
-    "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_mark_moments", "oka_batch_create",
+    "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_cv_surrogate_sums_qmc",
+    "oka_mark_moments", "oka_batch_create",
     "oka_batch_tune", "oka_batch_run", "oka_batch_results", "oka_batch_destroy",
     "oka_count_histogram",
 )
@@ -199,6 +200,9 @@ def load():
     L.oka_cv_surrogate_sums.restype = C.c_int
     L.oka_cv_surrogate_sums.argtypes = [_vp, _vp, C.POINTER(Site_), _vp, _i64, C.c_uint64, _i64,
                                         _vp, _vp, _vp, _vp, _u32]
+    L.oka_cv_surrogate_sums_qmc.restype = C.c_int
+    L.oka_cv_surrogate_sums_qmc.argtypes = [_vp, _vp, C.POINTER(Site_), _vp, _i64, _i32, _vp,
+                                            _vp, _vp, _vp, _u32]
     L.oka_batch_create.restype = C.c_int
     L.oka_batch_create.argtypes = [_vp, _i32, _vp, C.POINTER(TraceOpts_), C.POINTER(_vp)]
     L.oka_batch_tune.restype = C.c_int

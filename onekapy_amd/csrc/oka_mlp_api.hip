@@ -732,12 +732,18 @@ int oka_mark_moments(oka_ctx *c, int64_t n, int32_t ncell, const uint8_t *marks,
     return sync_unless_async(c, flags);
 }
 
-int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist *dists,
-                          int64_t n, uint64_t seed, int64_t first, const double *feat_mean,
-                          const double *feat_std, double *sums, double *sums_sq, uint32_t flags)
+// The surrogate sums of nrep runs over realizations first .. first + n - 1,
+// run k with seeds[k] into row k of sums / sums_sq (nrep x n_out): the site
+// is staged once and every run is enqueued on the context's stream.
+static int cv_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist *dists,
+                   int64_t n, const uint64_t *seeds, int32_t nrep, int64_t first,
+                   const double *feat_mean, const double *feat_std, double *sums,
+                   double *sums_sq, uint32_t flags)
 {
-    if (!c || !M || !site || !feat_mean || !feat_std || !sums)
+    if (!c || !M || !site || !feat_mean || !feat_std || !sums || !seeds)
         return OKA_ERR_ARG;
+    if (nrep < 0)
+        return fail(c, OKA_ERR_ARG, "negative replicate count");
     TRY(check_dists(c, site->nwells, dists));
     const int nw = site->nwells, nobs = site->nobs, nin = nw + 9;
     if (M->nin != nin)
@@ -784,14 +790,14 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
     // accumulators
     void *acc_base = nullptr;
     double *d_sums = sums, *d_sq = sums_sq;
+    const size_t acc_bytes = sizeof(double) * (size_t)nrep * M->nout;
     if (!(flags & OKA_PTR_DEVICE)) {
-        TRY(ensure(c, kSlotMlpOut2, sizeof(double) * 2 * M->nout, &acc_base));
+        TRY(ensure(c, kSlotMlpOut2, 2 * acc_bytes, &acc_base));
         d_sums = (double *)acc_base;
-        d_sq = sums_sq ? d_sums + M->nout : nullptr;
-        HIPCHK(c, hipMemcpyAsync(d_sums, sums, sizeof(double) * M->nout, hipMemcpyHostToDevice, s));
+        d_sq = sums_sq ? d_sums + (size_t)nrep * M->nout : nullptr;
+        HIPCHK(c, hipMemcpyAsync(d_sums, sums, acc_bytes, hipMemcpyHostToDevice, s));
         if (sums_sq)
-            HIPCHK(c, hipMemcpyAsync(d_sq, sums_sq, sizeof(double) * M->nout,
-                                     hipMemcpyHostToDevice, s));
+            HIPCHK(c, hipMemcpyAsync(d_sq, sums_sq, acc_bytes, hipMemcpyHostToDevice, s));
     }
     FitArgs fa{};
     fa.nwells = nw;
@@ -810,58 +816,79 @@ int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const ok
     fa.status = status;
     fa.q_well_stride = chunk;
     HIPCHK(c, launch_fit_logs(fa, d_logr2, s));
-    for (int64_t r0 = 0; r0 < n; r0 += chunk) {
-        const int valid = (int)std::min<int64_t>(chunk, n - r0);
-        DrawArgs da{};
-        da.nwells = nw;
-        da.dists = (const oka_dist *)dd;
-        da.n = valid;
-        da.first = first + r0;
-        da.seed = seed;
-        da.q = q;
-        da.K = K;
-        da.por = por;
-        da.thick = thick;
-        da.z = z;
-        da.antithetic = pairs;
-        da.sobol = sobol;
-        da.q_well_stride = chunk;   // well-major: the fit's loads are contiguous
-        HIPCHK(c, launch_draw(da, s));
-        fa.nreal = valid;
-        HIPCHK(c, launch_fit_draw(fa, s));
-        FeatureArgs fe{};
-        fe.nwells = nw;
-        fe.n = valid;
-        fe.q = q;
-        fe.K = K;
-        fe.por = por;
-        fe.thick = thick;
-        fe.coef = coef;
-        fe.mean = d_mean;
-        fe.std = d_std;
-        fe.x = x;
-        fe.q_well_stride = chunk;
-        HIPCHK(c, launch_features(fe, s));
-        for (int r1 = 0; r1 < valid; r1 += M->max_batch) {
-            const int nv = std::min(M->max_batch, valid - r1);
-            const int Bp = (nv + 63) / 64 * 64;
-            TRY(hidden_forward(M, Bp, nv, x, nullptr, r1, false));
-            GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
-            g.bias = M->p + M->ob3;
-            g.mvalid = nv;
-            g.pairs = pairs;
-            g.sums = d_sums;
-            g.sums_sq = d_sq;
-            HIPCHK(c, launch_gemm(kEpiAccum, g, s));
+    for (int32_t k = 0; k < nrep; ++k) {
+        for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+            const uint64_t seed = seeds[k];
+            const int valid = (int)std::min<int64_t>(chunk, n - r0);
+            DrawArgs da{};
+            da.nwells = nw;
+            da.dists = (const oka_dist *)dd;
+            da.n = valid;
+            da.first = first + r0;
+            da.seed = seed;
+            da.q = q;
+            da.K = K;
+            da.por = por;
+            da.thick = thick;
+            da.z = z;
+            da.antithetic = pairs;
+            da.sobol = sobol;
+            da.q_well_stride = chunk;   // well-major: the fit's loads are contiguous
+            HIPCHK(c, launch_draw(da, s));
+            fa.nreal = valid;
+            HIPCHK(c, launch_fit_draw(fa, s));
+            FeatureArgs fe{};
+            fe.nwells = nw;
+            fe.n = valid;
+            fe.q = q;
+            fe.K = K;
+            fe.por = por;
+            fe.thick = thick;
+            fe.coef = coef;
+            fe.mean = d_mean;
+            fe.std = d_std;
+            fe.x = x;
+            fe.q_well_stride = chunk;
+            HIPCHK(c, launch_features(fe, s));
+            for (int r1 = 0; r1 < valid; r1 += M->max_batch) {
+                const int nv = std::min(M->max_batch, valid - r1);
+                const int Bp = (nv + 63) / 64 * 64;
+                TRY(hidden_forward(M, Bp, nv, x, nullptr, r1, false));
+                GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
+                g.bias = M->p + M->ob3;
+                g.mvalid = nv;
+                g.pairs = pairs;
+                g.sums = d_sums + (size_t)k * M->nout;
+                g.sums_sq = d_sq ? d_sq + (size_t)k * M->nout : nullptr;
+                HIPCHK(c, launch_gemm(kEpiAccum, g, s));
+            }
         }
     }
     if (!(flags & OKA_PTR_DEVICE)) {
-        HIPCHK(c, hipMemcpyAsync(sums, d_sums, sizeof(double) * M->nout, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipMemcpyAsync(sums, d_sums, acc_bytes, hipMemcpyDeviceToHost, s));
         if (sums_sq)
-            HIPCHK(c, hipMemcpyAsync(sums_sq, d_sq, sizeof(double) * M->nout,
-                                     hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipMemcpyAsync(sums_sq, d_sq, acc_bytes, hipMemcpyDeviceToHost, s));
     }
     return sync_unless_async(c, flags);
+}
+
+int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist *dists,
+                          int64_t n, uint64_t seed, int64_t first, const double *feat_mean,
+                          const double *feat_std, double *sums, double *sums_sq, uint32_t flags)
+{
+    return cv_sums(c, M, site, dists, n, &seed, 1, first, feat_mean, feat_std, sums, sums_sq,
+                   flags);
+}
+
+int oka_cv_surrogate_sums_qmc(oka_ctx *c, oka_mlp *M, const oka_site *site,
+                              const oka_dist *dists, int64_t npts, int32_t nrep,
+                              const uint64_t *seeds, const double *feat_mean,
+                              const double *feat_std, double *sums, uint32_t flags)
+{
+    if (c && (flags & OKA_ANTITHETIC))
+        return fail(c, OKA_ERR_ARG, "OKA_QMC and OKA_ANTITHETIC exclude each other");
+    return cv_sums(c, M, site, dists, npts, seeds, nrep, 0, feat_mean, feat_std, sums, nullptr,
+                   flags | OKA_QMC);
 }
 
 }  // extern "C"

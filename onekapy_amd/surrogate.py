@@ -173,6 +173,26 @@ class SurrogateMLP:
         return acc
 
 
+def _surrogate_sums_qmc(net, site, npts, seeds, std):
+    """Row k: the surrogate sums of Sobol' points 0 .. npts - 1 under the
+    scramble of seeds[k] (oka_cv_surrogate_sums_qmc, one synchronization)."""
+    from .stochastic import oka_dists
+    seeds = np.ascontiguousarray([int(x) & (2 ** 64 - 1) for x in seeds], dtype=np.uint64)
+    out = np.zeros((len(seeds), net.n_out))
+    wxy = site.wells_xy()
+    obs = site.obs_array()
+    s = _lib.make_site(wxy, site.xt, site.yt, site.base, obs)
+    dists = oka_dists(site.wells, site.c_dist, site.p_dist, site.t_dist)
+    mean = np.ascontiguousarray(std.mean, dtype=np.float64)
+    sd = np.ascontiguousarray(std.std, dtype=np.float64)
+    net.ctx.check(net.ctx.lib.oka_cv_surrogate_sums_qmc(
+        net.ctx.h, net.h, s, C.addressof(dists), int(npts), len(seeds), seeds.ctypes.data,
+        mean.ctypes.data, sd.ctypes.data, out.ctypes.data, 0))
+    if not np.all(np.isfinite(out)):
+        raise ArithmeticError("the fit failed for some surrogate draws")
+    return out
+
+
 def features(d, coef):
     """Per-realization parameters: discharges, K, n, b, A..F (nwells + 9 columns)."""
     return np.concatenate([d["q"], d["K"][:, None], d["n"][:, None], d["b"][:, None],
@@ -381,9 +401,8 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
         if qmc_replicates:
             npts, nrep = qmc_split(mf, qmc_replicates, qmc_pow2)
             mf = npts * nrep
-            reps = [nets[f].surrogate_sums(site, npts, _replicate_seed(surr_seed, f, k), 0, std,
-                                           qmc=True)["s"] / npts for k in range(nrep)]
-            a_s = {"reps": np.array(reps)}
+            seeds = [_replicate_seed(surr_seed, f, k) for k in range(nrep)]
+            a_s = {"reps": _surrogate_sums_qmc(nets[f], site, npts, seeds, std) / npts}
         else:
             a_s = nets[f].surrogate_sums(site, mf, surr_seed, int(surr_split[f]), std,
                                          antithetic)
