@@ -124,14 +124,14 @@ def control_variate_leg(site, args, fp32_rate):
     # untimed warm-up of the code paths the timed runs take (first launches,
     # first use of buffers), as the tracer legs' warmup steps do
     print("bench: control variate warm-up", file=sys.stderr, flush=True)
-    for anti, qmc in ((args.cv_antithetic, 0), (False, args.cv_config_qmc)):
-        if anti or qmc or (not args.cv_antithetic and not qmc):
-            control_variate(site, 8192, 262144, folds=2, train_steps=64, batch=args.cv_batch,
-                            lr=args.cv_lr, seed=7, precision=_lib.FP32, antithetic=anti,
-                            qmc_replicates=qmc, qmc_pow2=True)
-    tuned = run(args.cv_traced, args.cv_surrogate, args.cv_antithetic, 0)
-    configured = run(100000, 1000000, args.cv_antithetic and not args.cv_config_qmc,
-                     args.cv_config_qmc)
+    tuned_mode = (args.cv_antithetic, 0)
+    config_mode = (args.cv_antithetic and not args.cv_config_qmc, args.cv_config_qmc)
+    for anti, qmc in sorted({tuned_mode, config_mode}):
+        control_variate(site, 8192, 262144, folds=2, train_steps=64, batch=args.cv_batch,
+                        lr=args.cv_lr, seed=7, precision=_lib.FP32, antithetic=anti,
+                        qmc_replicates=qmc, qmc_pow2=True)
+    tuned = run(args.cv_traced, args.cv_surrogate, *tuned_mode)
+    configured = run(100000, 1000000, *config_mode)
     rng = np.random.default_rng(321)
     d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, args.cv_reference, rng)
     _, _, coef = fit_and_draw(site.wells_xy(), site.xt, site.yt, site.base, site.observations,
