@@ -73,9 +73,12 @@ def parse():
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--only-cv", action="store_true", help="run only the control-variate leg")
     ap.add_argument("--cv-traced", type=int, default=80000)
-    ap.add_argument("--cv-surrogate", type=int, default=10000000)
+    ap.add_argument("--cv-surrogate", type=int, default=8000000)
+    ap.add_argument("--cv-qmc", type=int, default=128,
+                    help="QMC replicates of the tuned estimator (scrambled Sobol' points, a "
+                         "power of two per replicate; 0: Philox draws)")
     ap.add_argument("--cv-antithetic", action=argparse.BooleanOptionalAction, default=True,
-                    help="draw the tuned estimator's surrogate realizations in antithetic pairs")
+                    help="with Philox draws, draw the surrogate realizations in antithetic pairs")
     ap.add_argument("--cv-config-qmc", type=int, default=64,
                     help="QMC replicates of the estimator at the workload's own sizes (scrambled "
                          "Sobol' points, a power of two per replicate, so the count rises to "
@@ -95,10 +98,10 @@ def control_variate_leg(site, args, fp32_rate):
     same maximum (over window cells) standard error, divided by the
     pipeline's wall time (draws, fits, pilot, tracing, training, surrogate).
     Two estimators share that reference run: the tuned one (N = --cv-traced, M =
-    --cv-surrogate, surrogate realizations in antithetic pairs) and the
-    workload's own sizes (100,000 traced realizations to train on, at most
-    10^6 surrogate evaluations as scrambled Sobol' replicates, --cv-config-qmc
-    of them before rounding each to a power of two)."""
+    --cv-surrogate as --cv-qmc scrambled Sobol' replicates) and the workload's
+    own sizes (100,000 traced realizations to train on, at most 10^6 surrogate
+    evaluations as --cv-config-qmc replicates); replicate counts are raised
+    where the points per replicate are rounded down to a power of two."""
     from onekapy_amd import _lib
     from onekapy_amd.stochastic import draw_realizations, fit_and_draw
     from onekapy_amd.surrogate import cell_probabilities, control_variate, qmc_split, trace_counts
@@ -124,7 +127,7 @@ def control_variate_leg(site, args, fp32_rate):
     # untimed warm-up of the code paths the timed runs take (first launches,
     # first use of buffers), as the tracer legs' warmup steps do
     print("bench: control variate warm-up", file=sys.stderr, flush=True)
-    tuned_mode = (args.cv_antithetic, 0)
+    tuned_mode = (args.cv_antithetic and not args.cv_qmc, args.cv_qmc)
     config_mode = (args.cv_antithetic and not args.cv_config_qmc, args.cv_config_qmc)
     for anti, qmc in sorted({tuned_mode, config_mode}):
         control_variate(site, 8192, 262144, folds=2, train_steps=64, batch=args.cv_batch,
@@ -166,14 +169,13 @@ def control_variate_leg(site, args, fp32_rate):
         }
 
     out = summary(*tuned, args.cv_traced, args.cv_surrogate)
-    out["surrogate_sampling"] = sampling(args.cv_antithetic, 0, args.cv_surrogate)
+    out["surrogate_sampling"] = sampling(*tuned_mode, args.cv_surrogate)
     out["surrogate_evals_used"] = int(tuned[0].n_surrogate_used)
     out.update({"folds": 2, "train_steps_per_fold": args.cv_steps, "batch": args.cv_batch,
                 "lr": args.cv_lr, "network": "34->512->512->4096 GELU, BF16 MFMA",
                 "reference_run": {"realizations": args.cv_reference, "tracer": "fp64", "seconds": tj}})
     out["as_configured"] = summary(*configured, 100000, 1000000)
-    out["as_configured"]["surrogate_sampling"] = sampling(
-        args.cv_antithetic and not args.cv_config_qmc, args.cv_config_qmc, 1000000)
+    out["as_configured"]["surrogate_sampling"] = sampling(*config_mode, 1000000)
     out["as_configured"]["surrogate_evals_used"] = int(configured[0].n_surrogate_used)
     return out
 

@@ -357,6 +357,9 @@ struct DrawArgs {
     int32_t antithetic;         // odd realizations reflect their even partner's draws
     int64_t q_well_stride;      // q layout, as FitArgs::q_well_stride
     const uint32_t *sobol;      // QMC: device copy of sobol_directions() (>= qmc_dims x 32)
+    const uint32_t *qmc_keys;   // QMC: scramble keys, qmc_dims per replicate (qmc_key())
+    int32_t qmc_dims;
+    int64_t qmc_npts;           // QMC: points per replicate (0: a single replicate)
 };
 
 struct FeatureArgs {
@@ -382,6 +385,10 @@ uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub,
 constexpr int kSobolMaxDims = 1024;
 const uint32_t *sobol_directions();
 int qmc_dims(int32_t nwells, const oka_dist *dists);
+// The scramble key of dimension d under `seed`: word 0 of Philox4x32-10 at
+// counter (d, 2^32 - 1, kQmcKeySlot, 2), a counter the Philox draws never use.
+constexpr uint32_t kQmcKeySlot = 0x50B0150u;
+uint32_t qmc_key(uint64_t seed, int d);
 
 // Persistent FP32 tracer over a table of sites (oka_pool.hip).
 // A queue item is a chunk of `chunk` particles of one realization.  A

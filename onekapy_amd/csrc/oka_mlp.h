@@ -24,6 +24,9 @@ struct GemmArgs {
     int M, N, K, lda, ldb;
     int mvalid;              // rows >= mvalid are padding (kEpiAccum, kEpiProbs)
     int pairs;               // kEpiAccum: sums_sq takes ((s_2k + s_2k+1) / 2)^2 per row pair
+    // kEpiAccum with sum_rows > 0: row m counts into sums row (sum_row0 + m) / sum_rows
+    // (sums is then [replicates x N]; sum_row0 and sum_rows multiples of 1024)
+    int64_t sum_row0, sum_rows;
     int splits;              // K ranges (kEpiPartialF32), else 1
     const float *bias;       // [N]
     float *Cf;               // FP32 output, row stride ldc

@@ -392,9 +392,11 @@ k_gemm_accum(GemmArgs g, int rt)
     __syncthreads();
     if (threadIdx.x < S::BN) {
         const int n = ct * S::BN + threadIdx.x;
-        atomicAdd(&g.sums[n], (double)sm.col[0][threadIdx.x]);
+        // the workgroup's rows (at most 8 x 128) lie in one row of the sums
+        const int64_t sr = g.sum_rows ? (g.sum_row0 + (int64_t)rg * rt * S::BM) / g.sum_rows : 0;
+        atomicAdd(&g.sums[sr * g.N + n], (double)sm.col[0][threadIdx.x]);
         if (g.sums_sq)
-            atomicAdd(&g.sums_sq[n], (double)sm.col[1][threadIdx.x]);
+            atomicAdd(&g.sums_sq[sr * g.N + n], (double)sm.col[1][threadIdx.x]);
         if (g.marks && g.diffs)
             atomicAdd(&g.diffs[n], (double)sm.col[2][threadIdx.x]);
         if (g.marks && g.diffs_sq)

@@ -642,15 +642,28 @@ static int check_qmc(oka_ctx *c, int32_t nwells, const oka_dist *dists, int64_t 
     return OKA_OK;
 }
 
-// The direction numbers a QMC draw of these distributions uses, on the
-// device (the host table lives for the whole process).
-static int stage_sobol(oka_ctx *c, int32_t nwells, const oka_dist *dists, const uint32_t **out)
+// The direction numbers a QMC draw of these distributions uses and the
+// scramble keys of nrep replicates (seeds), on the device.  The copies are
+// synchronous, after the context's stream has drained, so no pending work
+// still reads the table (the direction numbers live for the whole process).
+static int stage_qmc(oka_ctx *c, int32_t nwells, const oka_dist *dists, const uint64_t *seeds,
+                     int32_t nrep, DrawArgs *a)
 {
-    const size_t bytes = sizeof(uint32_t) * 32 * (size_t)qmc_dims(nwells, dists);
+    const int dims = qmc_dims(nwells, dists);
+    const size_t bytes_v = sizeof(uint32_t) * 32 * (size_t)dims;
+    std::vector<uint32_t> keys((size_t)nrep * dims);
+    for (int32_t k = 0; k < nrep; ++k)
+        for (int d = 0; d < dims; ++d)
+            keys[(size_t)k * dims + d] = qmc_key(seeds[k], d);
     void *p;
-    TRY(ensure(c, kSlotQmc, bytes, &p));
-    HIPCHK(c, hipMemcpyAsync(p, sobol_directions(), bytes, hipMemcpyHostToDevice, c->stream));
-    *out = (const uint32_t *)p;
+    TRY(ensure(c, kSlotQmc, bytes_v + sizeof(uint32_t) * keys.size(), &p));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(p, sobol_directions(), bytes_v, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy((char *)p + bytes_v, keys.data(), sizeof(uint32_t) * keys.size(),
+                        hipMemcpyHostToDevice));
+    a->sobol = (const uint32_t *)p;
+    a->qmc_keys = (const uint32_t *)((char *)p + bytes_v);
+    a->qmc_dims = dims;
     return OKA_OK;
 }
 
@@ -678,7 +691,7 @@ int oka_draw_realizations(oka_ctx *c, int32_t nwells, const oka_dist *dists, int
     a.seed = seed;
     a.antithetic = (flags & OKA_ANTITHETIC) != 0;
     if (flags & OKA_QMC)
-        TRY(stage_sobol(c, nwells, dists, &a.sobol));
+        TRY(stage_qmc(c, nwells, dists, &seed, 1, &a));
     const void *dd;
     TRY(stage_in(c, kSlotCvDists, dists, sizeof(oka_dist) * (nwells + 3), 0, &dd));
     a.dists = (const oka_dist *)dd;
@@ -760,9 +773,17 @@ static int cv_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist 
     TRY(check_qmc(c, nw, dists, n, first, flags));
     HIPCHK(c, hipSetDevice(c->device));
     hipStream_t s = c->stream;
-    const uint32_t *sobol = nullptr;
+    DrawArgs qmc{};   // the QMC tables (unused without OKA_QMC)
     if (flags & OKA_QMC)
-        TRY(stage_sobol(c, nw, dists, &sobol));
+        TRY(stage_qmc(c, nw, dists, seeds, nrep, &qmc));
+    // QMC replicates whose point count is a multiple of 1024 run as one
+    // sequence of nrep * n realizations (replicate = index / n), so every
+    // launch has full chunks; the network passes (multiples of 1024 rows)
+    // then never straddle two replicates' sums
+    const bool batched = (flags & OKA_QMC) && nrep > 1 && n % 1024 == 0 && first == 0 &&
+                         M->max_batch % 1024 == 0;
+    const int64_t total = batched ? n * nrep : n;
+    const int32_t runs = batched ? 1 : nrep;
     // site, distributions, standardization (host memory)
     const size_t site_doubles = (size_t)2 * nw + 4 * nobs + (size_t)nobs * nw + 2 * nin;
     void *sb;
@@ -816,10 +837,10 @@ static int cv_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist 
     fa.status = status;
     fa.q_well_stride = chunk;
     HIPCHK(c, launch_fit_logs(fa, d_logr2, s));
-    for (int32_t k = 0; k < nrep; ++k) {
-        for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+    for (int32_t k = 0; k < runs; ++k) {
+        for (int64_t r0 = 0; r0 < total; r0 += chunk) {
             const uint64_t seed = seeds[k];
-            const int valid = (int)std::min<int64_t>(chunk, n - r0);
+            const int valid = (int)std::min<int64_t>(chunk, total - r0);
             DrawArgs da{};
             da.nwells = nw;
             da.dists = (const oka_dist *)dd;
@@ -832,7 +853,11 @@ static int cv_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist 
             da.thick = thick;
             da.z = z;
             da.antithetic = pairs;
-            da.sobol = sobol;
+            da.sobol = qmc.sobol;
+            da.qmc_dims = qmc.qmc_dims;
+            da.qmc_keys = qmc.qmc_keys ? qmc.qmc_keys + (batched ? 0 : (size_t)k * qmc.qmc_dims)
+                                       : nullptr;
+            da.qmc_npts = batched ? n : 0;
             da.q_well_stride = chunk;   // well-major: the fit's loads are contiguous
             HIPCHK(c, launch_draw(da, s));
             fa.nreal = valid;
@@ -860,6 +885,10 @@ static int cv_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist 
                 g.pairs = pairs;
                 g.sums = d_sums + (size_t)k * M->nout;
                 g.sums_sq = d_sq ? d_sq + (size_t)k * M->nout : nullptr;
+                if (batched) {   // rows of replicate (r0 + r1 + m) / n
+                    g.sum_row0 = r0 + r1;
+                    g.sum_rows = n;
+                }
                 HIPCHK(c, launch_gemm(kEpiAccum, g, s));
             }
         }

@@ -123,21 +123,15 @@ __host__ __device__ inline uint32_t nested_scramble(uint32_t x, uint32_t key)
     return __builtin_bitreverse32(x);
 }
 
-// Uniform in (0, 1) of dimension d of Sobol' point j under the scramble key
-// of `seed` for that dimension (Philox counter (d, 2^32 - 1, kQmcKeySlot, 2),
-// a counter the Philox draws never use).
-constexpr uint32_t kQmcKeySlot = 0x50B0150u;
-
-__device__ inline double qmc_u(const DrawArgs &a, uint32_t j, int d)
+// Uniform in (0, 1) of dimension d of Sobol' point j: the XOR of the
+// direction numbers of j's set bits, scrambled with key (qmc_key()).
+__device__ inline double qmc_u(const DrawArgs &a, uint32_t j, int d, const uint32_t *keys)
 {
     const uint32_t *v = a.sobol + 32 * d;
     uint32_t x = 0;
-    for (int k = 0; k < 32 && (j >> k) != 0u; ++k)
-        if ((j >> k) & 1u)
-            x ^= v[k];
-    const uint32_t key = philox4x32_10(U4{(uint32_t)d, 0xFFFFFFFFu, kQmcKeySlot, 2u},
-                                       (uint32_t)a.seed, (uint32_t)(a.seed >> 32)).x;
-    return ((double)nested_scramble(x, key) + 0.5) * 0x1p-32;
+    for (uint32_t b = j; b != 0u; b &= b - 1u)
+        x ^= v[__builtin_ctz(b)];
+    return ((double)nested_scramble(x, keys[d]) + 0.5) * 0x1p-32;
 }
 
 // Realization j of the QMC stream.  Dimensions, most influential first: the
@@ -147,7 +141,12 @@ __device__ inline double qmc_u(const DrawArgs &a, uint32_t j, int d)
 // thickness, then the wells).  A constant takes no dimension.
 __device__ inline void draw_qmc(const DrawArgs &a, int64_t i)
 {
-    const uint32_t j = (uint32_t)(a.first + i);
+    // realization first + i: point j of replicate rep (one replicate unless
+    // qmc_npts is set), scrambled with that replicate's keys
+    const int64_t gi = a.first + i;
+    const int64_t rep = a.qmc_npts ? gi / a.qmc_npts : 0;
+    const uint32_t j = (uint32_t)(a.qmc_npts ? gi % a.qmc_npts : gi);
+    const uint32_t *keys = a.qmc_keys + rep * a.qmc_dims;
     const int nw = a.nwells;
     int ncore = 0, nq = 0;
     for (int s = 0; s < 3; ++s)
@@ -159,18 +158,18 @@ __device__ inline void draw_qmc(const DrawArgs &a, int64_t i)
         const oka_dist &d = a.dists[slot];
         double u = 0.5, w = 0.5, us = 1.0;
         if (d.kind != OKA_DIST_CONST)
-            u = qmc_u(a, j, prim++);
+            u = qmc_u(a, j, prim++, keys);
         if (d.kind == OKA_DIST_LOGNORMAL)
-            w = qmc_u(a, j, sec++);
+            w = qmc_u(a, j, sec++, keys);
         if (d.p_negative > 0.0)
-            us = qmc_u(a, j, sec++);
+            us = qmc_u(a, j, sec++, keys);
         return dist_value(d, u, w, us, false);
     };
     a.K[i] = value(nw);
     a.por[i] = value(nw + 1);
     a.thick[i] = value(nw + 2);
     for (int p = 0; p < 3; ++p) {
-        const double u1 = qmc_u(a, j, prim), u2 = qmc_u(a, j, prim + 1);
+        const double u1 = qmc_u(a, j, prim, keys), u2 = qmc_u(a, j, prim + 1, keys);
         prim += 2;
         const double rad = sqrt(-2.0 * log(u1));
         double sn, cs;
@@ -280,6 +279,8 @@ hipError_t launch_draw(const DrawArgs &a, hipStream_t s)
 {
     if (a.n <= 0)
         return hipSuccess;
+    if (a.sobol && (a.qmc_dims <= 0 || !a.qmc_keys))
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_draw, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
@@ -397,6 +398,13 @@ int qmc_dims(int32_t nwells, const oka_dist *dists)
         d += dists[s].p_negative > 0.0;
     }
     return d;
+}
+
+uint32_t qmc_key(uint64_t seed, int d)
+{
+    const U4 o = philox4x32_10(U4{(uint32_t)d, 0xFFFFFFFFu, kQmcKeySlot, 2u}, (uint32_t)seed,
+                               (uint32_t)(seed >> 32));
+    return o.x;
 }
 
 uint32_t host_philox_word(uint64_t seed, int64_t r, uint32_t slot, uint32_t sub, int word)

@@ -185,22 +185,26 @@ def test_surrogate_sums_compose_draws_fit_features_and_network(gpu_ctx, mode):
         np.testing.assert_allclose(acc["s2"], (prob * prob).sum(0), rtol=1e-5, atol=1e-6)
 
 
-def test_qmc_replicates_in_one_call_match_separate_calls(gpu_ctx):
+@pytest.mark.parametrize("npts,max_batch", [(1000, 256), (2048, 1024), (3072, 2048)],
+                         ids=["per_replicate", "batched", "batched_straddling_passes"])
+def test_qmc_replicates_in_one_call_match_separate_calls(gpu_ctx, npts, max_batch):
     """oka_cv_surrogate_sums_qmc: row k equals oka_cv_surrogate_sums with
     OKA_QMC and seed k, up to the order of the workgroups' FP32 LDS and FP64
-    atomic additions."""
+    atomic additions.  Point counts that are multiples of 1024 run all
+    replicates as one sequence, whose network passes cover several replicates
+    (3 x 3072 points in passes of 2048 rows)."""
     from onekapy_amd.stochastic import draw_realizations_device
     from onekapy_amd.surrogate import Standardizer, _surrogate_sums_qmc, features
     s = sites.config0()
-    net = SurrogateMLP(len(s.wells) + 9, 128, 128, max_batch=256, seed=5)
+    net = SurrogateMLP(len(s.wells) + 9, 128, 128, max_batch=max_batch, seed=5)
     d = draw_realizations_device(s.wells, s.c_dist, s.p_dist, s.t_dist, 512, 3)
     _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
                               d["q"], d["z"], moments=False)
     std = Standardizer(features(d, coef))
     seeds = [11, 2 ** 63 + 5, 77]
-    rows = _surrogate_sums_qmc(net, s, 1000, seeds, std)
+    rows = _surrogate_sums_qmc(net, s, npts, seeds, std)
     for k, sd in enumerate(seeds):
-        one = net.surrogate_sums(s, 1000, sd, 0, std, qmc=True)["s"]
+        one = net.surrogate_sums(s, npts, sd, 0, std, qmc=True)["s"]
         np.testing.assert_allclose(rows[k], one, rtol=1e-6, atol=1e-9)
     assert not np.allclose(rows[0], rows[1])
 
