@@ -142,7 +142,8 @@ def test_control_variate_is_unbiased_and_reduces_variance(cfg0_plain_run, mode):
     s, counts, R = cfg0_plain_run
     res = control_variate(s, n_traced=16384, n_surrogate=131072, folds=2, train_steps=1500,
                           batch=512, lr=1e-3, seed=7, n_hidden=256,
-                          antithetic=mode == "antithetic", qmc_replicates=16 if mode == "qmc" else 0)
+                          antithetic=mode == "antithetic", qmc_replicates=16 if mode == "qmc" else 0,
+                          qmc_pow2=True)
     # the reference: the large plain run over the same window
     p_ref = cell_probabilities(counts, R, res.window)
     var_ref = p_ref * (1 - p_ref) / R
