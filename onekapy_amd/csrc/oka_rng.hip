@@ -134,11 +134,11 @@ __device__ inline double qmc_u(const DrawArgs &a, uint32_t j, int d, const uint3
     return ((double)nested_scramble(x, keys[d]) + 0.5) * 0x1p-32;
 }
 
-// Realization j of the QMC stream.  Dimensions, most influential first: the
-// inverse-CDF variates of K, porosity and thickness, the 6 uniforms of the
-// MVN draw's normals, those of the discharges (wells in order), then the
-// lognormals' angle variates and the sign variates (slots K, porosity,
-// thickness, then the wells).  A constant takes no dimension.
+// Realization first + i of the QMC draws.  Dimensions, most influential
+// first: the inverse-CDF variates of K, porosity and thickness, the 6
+// uniforms of the MVN draw's normals, those of the discharges (wells in
+// order), then the lognormals' angle variates and the sign variates (slots
+// K, porosity, thickness, then the wells).  A constant takes no dimension.
 __device__ inline void draw_qmc(const DrawArgs &a, int64_t i)
 {
     // realization first + i: point j of replicate rep (one replicate unless
