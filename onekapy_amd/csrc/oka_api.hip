@@ -275,8 +275,8 @@ int check_opts(oka_ctx *c, const oka_trace_opts *o)
         return fail(c, OKA_ERR_ARG, "start_xy is NULL");
     if (o->precision != OKA_FP64_EXACT && o->precision != OKA_FP32)
         return fail(c, OKA_ERR_ARG, "unknown precision %d", o->precision);
-    if (o->precision == OKA_FP32 && !o->confined)
-        return fail(c, OKA_ERR_ARG, "FP32 tracing supports the confined velocity only");
+    if (o->schedule == OKA_SCHED_POOL && !o->confined)
+        return fail(c, OKA_ERR_ARG, "the pool schedule traces the confined velocity only");
     if (o->schedule < OKA_SCHED_AUTO || o->schedule > OKA_SCHED_POOL)
         return fail(c, OKA_ERR_ARG, "unknown schedule %d", o->schedule);
     if (o->schedule == OKA_SCHED_POOL && o->precision != OKA_FP32)

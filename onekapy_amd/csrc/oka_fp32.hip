@@ -1,5 +1,6 @@
-// oka_fp32.hip -- FP32 particle tracing for gfx950 (confined velocity): one
-// 256-lane workgroup per realization, grid-stride over realizations.  The step,
+// oka_fp32.hip -- FP32 particle tracing for gfx950 (confined or unconfined
+// velocity): one 256-lane workgroup per realization, grid-stride over
+// realizations.  The step,
 // field and rasterizer are in oka_fp32_dev.h.  Results are not bit-identical
 // to the FP64 path; the difference in capture probability is reported against
 // the FP64-exact mode.
@@ -18,7 +19,7 @@ using namespace fp32;
 // At most 128 VGPRs, so that 4 waves share a SIMD: the tracer is bound by
 // VALU issue, and with 3 waves (the compiler's own choice, 129 VGPRs) the
 // SIMDs idle about 12 % of the time on LDS and scalar waits.
-template <int kKind>
+template <int kKind, bool kConfined>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_trace_fp32(TraceArgs a)
 {
@@ -28,7 +29,7 @@ k_trace_fp32(TraceArgs a)
     using Bits = typename std::conditional<kKind == kBmTiled, TileBits, FlatBits>::type;
     __shared__ int s_next;
     __shared__ unsigned long long s_stat[kStatCount];
-    __shared__ float s_reg[5];
+    __shared__ float s_reg[11];
 
     const int tid = threadIdx.x;
     const int P = a.npaths;
@@ -68,11 +69,15 @@ k_trace_fp32(TraceArgs a)
         if (tid < kStatCount)
             s_stat[tid] = 0;
         const double bn = a.b[r] * a.n[r];
+        // h = K b^2 / 2, the potential at which the head reaches the top of
+        // the aquifer (model.py:344)
+        const double h = kConfined ? 1.0 : 0.5 * a.K[r] * (a.b[r] * a.b[r]);
         for (int i = tid; i < a.nwells; i += blockDim.x) {
             const double qw = a.q[(int64_t)r * a.nwells + i];
             s_w[i] = make_float4((float)(a.wells_xy[2 * i] - a.xo),
                                  (float)(a.wells_xy[2 * i + 1] - a.yo),
-                                 (float)(qw * kInv2Pi / bn), 0.0f);
+                                 (float)(qw * kInv2Pi / bn),
+                                 kConfined ? 0.0f : (float)(qw * (kInv4Pi * kLn2) / h));
         }
         if (tid == 0) {
             const double *c = a.coef + 6 * (int64_t)r;
@@ -81,6 +86,8 @@ k_trace_fp32(TraceArgs a)
             s_reg[2] = (float)(c[2] / bn);
             s_reg[3] = (float)(c[3] / bn);
             s_reg[4] = (float)(c[4] / bn);
+            for (int j = 0; j < 6; ++j)
+                s_reg[5 + j] = (float)(c[j] / h);
             s_next = blockDim.x;
         }
         if constexpr (kKind != kBmTiled) {   // a tiled bitmap is left empty by its flush
@@ -98,6 +105,14 @@ k_trace_fp32(TraceArgs a)
         f.c = s_reg[2];
         f.d = s_reg[3];
         f.e = s_reg[4];
+        if constexpr (!kConfined) {
+            f.pa = s_reg[5];
+            f.pb = s_reg[6];
+            f.pc = s_reg[7];
+            f.pd = s_reg[8];
+            f.pe = s_reg[9];
+            f.pf = s_reg[10];
+        }
         // per-trace statistics go to LDS when the trace ends (few registers)
         unsigned long long n_cand = 0;   // candidate nodes of the lane's trace
 
@@ -107,6 +122,7 @@ k_trace_fp32(TraceArgs a)
         bool have = p < P;
         bool live = false, fresh = false;
         bool a_done = false;   // the disk around the trace's current point is marked
+        bool dry = false;      // an evaluation of the trace raised AquiferError
         for (;;) {
             bool ended = false;
             const int64_t k = (int64_t)r * P + p;
@@ -119,22 +135,30 @@ k_trace_fp32(TraceArgs a)
                        (float)(a.start_xy[2 * p + 1] - a.yo));
                 live = fresh = true;
                 a_done = false;
+                dry = false;
             }
             // one field evaluation for every lane: k1 of a fresh trace, else
             // the first stage of an attempted step
             double dtd;
             float dt, px, py, kx, ky;
             first_point(I, s, fresh, dtd, dt, px, py);
-            feval32(f, px, py, kx, ky);
+            feval32<kConfined>(f, px, py, kx, ky, dry);
             if (fresh) {
                 fresh = false;
-                if (!take_k1(I, s, kx, ky)) {
+                if (dry) {   // k1 raised: the trace is its start vertex (capturezone.py:218, 249)
+                    s.status = OKA_TRACE_TRUNCATED;
+                    live = false;
+                    ended = true;
+                } else if (!take_k1(I, s, kx, ky)) {
                     live = false;
                     ended = true;
                 }
             } else {
                 float ox, oy;
-                const int res = finish32(LaneEval{f}, I, s, dtd, dt, kx, ky, ox, oy);
+                const int res = finish32(LaneEval<kConfined>{f, dry}, I, s, dtd, dt, kx, ky, ox, oy);
+                // a stage raised: the step is not accepted and the trace ends
+                if (!kConfined && dry)
+                    s.status = OKA_TRACE_TRUNCATED;
                 if ((res & 1) && a.rasterize)
                     n_cand += (unsigned long long)raster32(bits, ep, gdx, gdy, igdx, igdy, umbra,
                                                            u2, ox, oy, s.x, s.y, !a_done, a_done);
@@ -211,12 +235,20 @@ hipError_t launch_trace_fp32(const TraceArgs &a, int grid_blocks, int kind, hipS
     dim3 grid(grid_blocks), block(kTraceBlock);
     if (!a.rasterize || kind == kBmLdsFlat) {
         lds += a.rasterize ? (size_t)a.nwords * 4 : 0;
-        hipLaunchKernelGGL(k_trace_fp32<kBmLdsFlat>, grid, block, lds, s, a);
+        if (a.confined)
+            hipLaunchKernelGGL((k_trace_fp32<kBmLdsFlat, true>), grid, block, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_trace_fp32<kBmLdsFlat, false>), grid, block, lds, s, a);
     } else if (kind == kBmTiled) {
         lds += (size_t)tile_lds_words(a.ntiles, a.tile_slots) * 4;
-        hipLaunchKernelGGL(k_trace_fp32<kBmTiled>, grid, block, lds, s, a);
+        if (a.confined)
+            hipLaunchKernelGGL((k_trace_fp32<kBmTiled, true>), grid, block, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_trace_fp32<kBmTiled, false>), grid, block, lds, s, a);
+    } else if (a.confined) {
+        hipLaunchKernelGGL((k_trace_fp32<kBmHbmFlat, true>), grid, block, lds, s, a);
     } else {
-        hipLaunchKernelGGL(k_trace_fp32<kBmHbmFlat>, grid, block, lds, s, a);
+        hipLaunchKernelGGL((k_trace_fp32<kBmHbmFlat, false>), grid, block, lds, s, a);
     }
     return hipGetLastError();
 }

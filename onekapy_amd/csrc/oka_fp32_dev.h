@@ -1,8 +1,9 @@
 // oka_fp32_dev.h -- FP32 tracing device code shared by the trace kernels:
 // the Dormand-Prince 5(4) step with the reference's step control
 // (oneka/capturezone.py:198-253), the confined velocity field (model.py:
-// 299-315, 423-427) and the segment rasterization (probabilityfield.py:
-// 292-310, 407-427), computed in FP32:
+// 299-315, 423-427), the unconfined one (model.py:207-266, 318-389) and the
+// segment rasterization (probabilityfield.py:292-310, 407-427), computed in
+// FP32:
 //   - coordinates are relative to the regional origin (the target well), so a
 //     float resolves millimetres anywhere on a 2000 x 2000 node grid;
 //   - particle positions are Kahan-compensated sums of their increments;
@@ -31,11 +32,14 @@ constexpr float B1 = 35.0f / 384.0f, B3 = 500.0f / 1113.0f, B4 = 125.0f / 192.0f
 constexpr float E1 = 71.0f / 57600.0f, E2 = -1.0f / 40.0f, E3 = -71.0f / 16695.0f,
                 E4 = 71.0f / 1920.0f, E5 = -17253.0f / 339200.0f, E6 = 22.0f / 525.0f;
 constexpr double kInv2Pi = 0.15915494309189535;
+constexpr double kInv4Pi = 0.07957747154594767;
+constexpr double kLn2 = 0.6931471805599453;
 
 struct Field32 {
-    const float4 *w;   // (xw - xo, yw - yo, q / (2 pi b n), 0)
+    const float4 *w;   // (xw - xo, yw - yo, q / (2 pi b n), q ln 2 / (4 pi h)), h = K b^2 / 2
     int nw;
     float a2, b2, c, d, e;  // regional: k_reg = ((a2 dx + c dy) + d, (b2 dy + c dx) + e)
+    float pa, pb, pc, pd, pe, pf;   // regional potential over h (unconfined only)
 };
 
 __device__ __forceinline__ void well_term(const float4 &w, float x, float y, float &vx, float &vy)
@@ -48,13 +52,38 @@ __device__ __forceinline__ void well_term(const float4 &w, float x, float y, flo
     vy = fmaf(s, ddy, vy);
 }
 
-// Wells are summed in index order; their LDS reads are issued 8 at a time so
-// that one trace alone (a long trace after its realization's others ended)
-// does not wait a full LDS round trip per well.
-__device__ __forceinline__ void feval32(const Field32 &f, float x, float y, float &kx, float &ky)
+// The confined well term plus the well's potential over h, q ln(r^2) /
+// (4 pi h), as w.w log2(r^2) with the hardware base-2 logarithm.
+__device__ __forceinline__ void well_term_pot(const float4 &w, float x, float y, float &vx,
+                                              float &vy, float &ph)
+{
+    const float ddx = x - w.x;
+    const float ddy = y - w.y;
+    const float r2 = fmaf(ddx, ddx, ddy * ddy);
+    const float s = w.z * __builtin_amdgcn_rcpf(r2);
+    vx = fmaf(s, ddx, vx);
+    vy = fmaf(s, ddy, vy);
+    ph = fmaf(w.w, __builtin_amdgcn_logf(r2), ph);
+}
+
+// k = -V at (x, y).  Wells are summed in index order; their LDS reads are
+// issued 8 at a time so that one trace alone (a long trace after its
+// realization's others ended) does not wait a full LDS round trip per well.
+//
+// Unconfined (kConfined false), with phi = potential / h (model.py:207-266):
+//   - phi <= 0: the reference raises AquiferError (model.py:342-343); `dry`
+//     is set and k is NaN, which ends the attempted step unaccepted;
+//   - phi < 1: head = sqrt(2 potential / K) = b sqrt(phi) < b, so V = Q / (head
+//     n) is the confined velocity times 1 / sqrt(phi) (model.py:344-345, 384-386);
+//   - otherwise head = (potential + h) / (K b) >= b and V = Q / (b n), the
+//     confined velocity (model.py:346-348, 381-383).
+template <bool kConfined = true>
+__device__ __forceinline__ void feval32(const Field32 &f, float x, float y, float &kx, float &ky,
+                                        bool &dry)
 {
     float vx = fmaf(f.a2, x, fmaf(f.c, y, f.d));
     float vy = fmaf(f.b2, y, fmaf(f.c, x, f.e));
+    float ph = 0.f;
     int i = 0;
     for (; i + 8 <= f.nw; i += 8) {
         float4 w[8];
@@ -62,21 +91,50 @@ __device__ __forceinline__ void feval32(const Field32 &f, float x, float y, floa
         for (int k = 0; k < 8; ++k)
             w[k] = f.w[i + k];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            well_term(w[k], x, y, vx, vy);
+        for (int k = 0; k < 8; ++k) {
+            if constexpr (kConfined)
+                well_term(w[k], x, y, vx, vy);
+            else
+                well_term_pot(w[k], x, y, vx, vy, ph);
+        }
     }
-    for (; i < f.nw; ++i)
-        well_term(f.w[i], x, y, vx, vy);
+    for (; i < f.nw; ++i) {
+        if constexpr (kConfined)
+            well_term(f.w[i], x, y, vx, vy);
+        else
+            well_term_pot(f.w[i], x, y, vx, vy, ph);
+    }
+    if constexpr (!kConfined) {
+        ph += fmaf(fmaf(f.pa, x, fmaf(f.pc, y, f.pd)), x, fmaf(fmaf(f.pb, y, f.pe), y, f.pf));
+        if (ph <= 0.f) {
+            dry = true;
+            vx = vy = __builtin_nanf("");
+        } else if (ph < 1.f) {
+            const float g = __builtin_amdgcn_rsqf(ph);
+            vx *= g;
+            vy *= g;
+        }
+    }
     kx = vx;
     ky = vy;
 }
 
-// The lane's own field evaluation, as finish32's evaluator.
+template <bool kConfined = true>
+__device__ __forceinline__ void feval32(const Field32 &f, float x, float y, float &kx, float &ky)
+{
+    bool dry = false;
+    feval32<kConfined>(f, x, y, kx, ky, dry);
+}
+
+// The lane's own field evaluation, as finish32's evaluator; `dry` collects the
+// unconfined field's AquiferError.
+template <bool kConfined = true>
 struct LaneEval {
     const Field32 &f;
+    bool &dry;
     __device__ __forceinline__ void operator()(float x, float y, float &kx, float &ky) const
     {
-        feval32(f, x, y, kx, ky);
+        feval32<kConfined>(f, x, y, kx, ky, dry);
     }
 };
 

@@ -585,7 +585,8 @@ k_trace_pool32(PoolArgs P)
                 }
             } else {
                 float ox, oy;
-                const int res = finish32(LaneEval{f}, I, tr, dtd, dt, kx, ky, ox, oy);
+                bool dry = false;   // set by the unconfined field only
+                const int res = finish32(LaneEval<>{f, dry}, I, tr, dtd, dt, kx, ky, ox, oy);
                 if ((res & 1) && h.raster) {
                     PoolBits bits = slot_bits(P, L, slot, h.tiles_x);
                     bits.last_t = last_t;

@@ -224,3 +224,48 @@ def test_fp32_error_cfg0(gpu_ctx):
     p64 = out[_lib.FP64_EXACT] / s.nrealizations
     p32 = out[_lib.FP32] / s.nrealizations
     assert np.abs(p32 - p64).max() <= 0.01
+
+
+@pytest.mark.parametrize("shift", [0.0, 0.7, 1e6])
+def test_fp32_unconfined_error_cfg0(gpu_ctx, shift):
+    """The unconfined velocity (model.py:318-389) on cfg 0, whose potentials
+    straddle K b^2 / 2 (the head is below the aquifer top in some realizations
+    and above it in others).  F is lowered by shift * K b^2 / 2: at 0.7 a fifth
+    of the realizations are dry at their start points (AquiferError, status
+    truncated), at 10^6 every one is.  The FP64-exact mode is bit-exact against
+    the oracle; the FP32 mode is within 0.01 of it in capture probability and
+    has the same status counts to 1 % of the traces."""
+    s = sites.config0()
+    d = sites.draw(s, seed=11)
+    _, _, coef = oracle_fit_draw(s, d)
+    coef[:, 5] -= shift * 0.5 * d["K"] * d["b"] ** 2
+    real = _site_real(s, d, coef)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    g = oracle_grid(s.fixed_field())
+    out, status = {}, {}
+    for prec in (_lib.FP64_EXACT, _lib.FP32):
+        pf = s.fixed_field()
+        res = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
+                                   s.tol, s.maxstep, s.umbra, confined=False, precision=prec)
+        assert res.counts.shape == (g.nrows, g.ncols)   # no trace left the fixed grid
+        out[prec], status[prec] = res.counts, res.trace_status
+    oc, ost = oracle_chronicle(s, d, coef, s.fixed_field(), confined=False)
+    assert np.array_equal(out[_lib.FP64_EXACT], oc)
+    n64 = np.bincount(status[_lib.FP64_EXACT], minlength=4)
+    assert np.array_equal(n64, ost[2:6])
+    n32 = np.bincount(status[_lib.FP32], minlength=4)
+    assert np.abs(n32 - n64).max() <= 0.01 * len(starts) * s.nrealizations
+    p64 = out[_lib.FP64_EXACT] / s.nrealizations
+    p32 = out[_lib.FP32] / s.nrealizations
+    assert np.abs(p32 - p64).max() <= 0.01
+    if shift == 0.0:
+        # the unconfined field is not the confined one on this site
+        pf = s.fixed_field()
+        conf = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
+                                    s.tol, s.maxstep, s.umbra, precision=_lib.FP32).counts
+        assert np.abs(conf / s.nrealizations - p32).max() > 0.05
+    elif shift == 0.7:
+        assert 0 < n64[_lib.TRACE_TRUNCATED] < len(starts) * s.nrealizations
+    else:
+        assert n32[_lib.TRACE_TRUNCATED] == n64[_lib.TRACE_TRUNCATED] == len(starts) * s.nrealizations
+        assert out[_lib.FP32].sum() == 0 and oc.sum() == 0
