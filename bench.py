@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (to scale HBM counters)")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
+    ap.add_argument("--unconfined", action="store_true",
+                    help="trace the unconfined velocity (compute_velocity) in the fp64/fp32 modes; "
+                         "the roofline keeps the confined flop count")
     ap.add_argument("--only-cv", action="store_true", help="run only the control-variate leg")
     ap.add_argument("--cv-traced", type=int, default=80000)
     ap.add_argument("--cv-surrogate", type=int, default=8000000)
@@ -212,7 +215,7 @@ def measured_traffic(kernel_prefix, nreal):
 class DeviceRun:
     """Inputs of one site resident on the GPU, and the launches of one step."""
 
-    def __init__(self, site, d, torch):
+    def __init__(self, site, d, torch, confined=True):
         from onekapy_amd import _lib
         from onekapy_amd.capturezone import start_points
         self.torch = torch
@@ -222,6 +225,7 @@ class DeviceRun:
         dev = torch.device("cuda:0")
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
         self.site = site
+        self.confined = confined
         self.R = len(d["K"])
         self.K, self.n, self.b = t(d["K"]), t(d["n"]), t(d["b"])
         self.q, self.z = t(d["q"]), t(d["z"])
@@ -252,7 +256,7 @@ class DeviceRun:
                                    L.PTR_DEVICE | L.ASYNC))
         s = self.site
         opts = L.make_trace_opts(s.npaths, s.duration, s.tol, s.maxstep, s.umbra,
-                                 self.starts.data_ptr(), precision, True, **sched)
+                                 self.starts.data_ptr(), precision, self.confined, **sched)
         stats = L.TraceStats_()
         c.check(c.lib.oka_trace_chronicle(c.h, self.site_s, self.R, self.K.data_ptr(),
                                           self.n.data_ptr(), self.b.data_ptr(), self.q.data_ptr(),
@@ -536,7 +540,7 @@ def main():
         print(json.dumps({"control_variate": cv}))
         return
     d = sites.draw(site, seed=1)
-    run = DeviceRun(site, d, torch)
+    run = DeviceRun(site, d, torch, confined=not args.unconfined)
     nw = len(site.wells)
     modes = {}
     wanted = [m.strip() for m in args.modes.split(",") if m.strip()]
@@ -605,7 +609,8 @@ def main():
                    "wells": nw, "observations": len(site.observations),
                    "duration_d": site.duration, "grid": [site.nrows, site.ncols],
                    "spacing_m": site.spacing, "umbra_m": site.umbra, "tol_m": site.tol,
-                   "maxstep_m": site.maxstep, "headline_mode": head},
+                   "maxstep_m": site.maxstep, "headline_mode": head,
+                   "velocity": "unconfined" if args.unconfined else "confined"},
         "particle_steps_per_s": steps_per_s,
         "error_vs_fp64": {"max": err_max, "l1": err_l1, "nodes": int(site.nrows * site.ncols)},
         "modes": {k: mode_summary(k) for k in modes},
