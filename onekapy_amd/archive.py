@@ -1,5 +1,5 @@
 """Archive an Oneka run as a bzip2 pickle, in the reference's format
-(oneka/archive.py:59-94): a dict of the run's inputs plus the
+(oneka/archive.py:46-89): a dict of the run's inputs plus the
 ProbabilityField, with the field stored under the reference's class path
 ``oneka.probabilityfield.ProbabilityField`` and the reference's attributes,
 so that archives move both ways between the reference and this package.
@@ -14,8 +14,6 @@ import copyreg
 import io
 import os
 import pickle
-import sys
-import types
 from datetime import datetime
 
 import numpy as np
@@ -31,30 +29,33 @@ KEYS = ("projectname", "runtime", "target", "npaths", "duration", "nrealizations
         "umbra", "smooth", "confined", "tol", "maxstep", "pfield")
 
 
-def _reference_field_class():
-    """A class that pickles as oneka.probabilityfield.ProbabilityField.  When the
-    reference package itself is importable its class is used; otherwise a
-    stand-in holding only the attribute state is registered under that path."""
-    try:
-        mod = __import__(REF_MODULE, fromlist=[REF_CLASS])
-        return getattr(mod, REF_CLASS)
-    except ImportError:
-        pass
-    mod = sys.modules.get(REF_MODULE)
-    if mod is None or not hasattr(mod, REF_CLASS):
-        pkg = sys.modules.setdefault("oneka", types.ModuleType("oneka"))
-        mod = types.ModuleType(REF_MODULE)
-        stand_in = type(REF_CLASS, (), {"__module__": REF_MODULE,
-                                        "__doc__": "Attribute state of a ProbabilityField."})
-        setattr(mod, REF_CLASS, stand_in)
-        sys.modules[REF_MODULE] = mod
-        pkg.probabilityfield = mod
-    return getattr(mod, REF_CLASS)
+class _FieldRecord:
+    """Attribute state of a ProbabilityField.  Archives name it by the
+    reference's class path, oneka.probabilityfield.ProbabilityField."""
+
+
+class _ArchivePickler(pickle._Pickler):
+    """A pickler that writes the class _FieldRecord as the global
+    oneka.probabilityfield.ProbabilityField without looking that module up, so
+    writing an archive never imports (or registers) a module named oneka.  The
+    class is intercepted in save() itself, ahead of the type dispatch table,
+    which other libraries in the process may have extended."""
+
+    def save(self, obj, save_persistent_id=True):
+        if obj is not _FieldRecord or id(obj) in self.memo:
+            return super().save(obj, save_persistent_id)
+        self.framer.commit_frame()
+        if self.proto >= 4:
+            super().save(REF_MODULE)
+            super().save(REF_CLASS)
+            self.write(pickle.STACK_GLOBAL)
+        else:
+            self.write(pickle.GLOBAL + (REF_MODULE + "\n" + REF_CLASS + "\n").encode("ascii"))
+        self.memoize(obj)
 
 
 def _as_reference_field(pf):
-    cls = _reference_field_class()
-    out = cls.__new__(cls)
+    out = _FieldRecord()
     for k in FIELD_STATE:
         setattr(out, k, getattr(pf, k))
     return out
@@ -74,8 +75,10 @@ def dump_oneka(projectname, runtime, target, npaths, duration, nrealizations, ba
     d = os.path.dirname(filename)
     if d:
         os.makedirs(d, exist_ok=True)
+    buf = io.BytesIO()
+    _ArchivePickler(buf, protocol=pickle.DEFAULT_PROTOCOL).dump(oneka_dict)
     with bz2.BZ2File(filename, "w") as fp:
-        pickle.dump(oneka_dict, fp)
+        fp.write(buf.getvalue())
     return filename
 
 

@@ -6,6 +6,9 @@ import json
 import os
 import pickle
 import pickletools
+import subprocess
+import sys
+import textwrap
 
 import numpy as np
 import pytest
@@ -104,3 +107,45 @@ def test_archive_loader_refuses_foreign_globals(tmp_path):
         pickle.dump({"pfield": Evil()}, fp)
     with pytest.raises(pickle.UnpicklingError):
         load_oneka(str(path))
+
+
+def test_archive_writer_never_imports_a_module_named_oneka(tmp_path):
+    """An importable package named oneka (standing in for the reference) is
+    neither imported nor shadowed by dump_oneka, while the archive still names
+    oneka.probabilityfield.ProbabilityField, and the plain pickle loader
+    (what oneka/archive.py:85-89 uses) resolves it to that package's class."""
+    pkg = tmp_path / "site" / "oneka"
+    pkg.mkdir(parents=True)
+    marker = tmp_path / "imported"
+    (pkg / "__init__.py").write_text(f"open({str(marker)!r}, 'a').write('pkg')\n")
+    (pkg / "probabilityfield.py").write_text("class ProbabilityField:\n    pass\n")
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = textwrap.dedent(f"""
+        import bz2, os, pickle, sys
+        sys.path[:0] = [{repo!r}, {str(tmp_path / "site")!r}]
+        from onekapy_amd import ProbabilityField
+        from onekapy_amd.archive import dump_oneka, load_oneka
+        pf = ProbabilityField(10.0, 10.0, 0.0, 0.0)
+        pf.expand(-55.0, 42.0, -31.0, 77.0)
+        pf.pgrid[2, 3] = 4.0
+        pf.total_weight = 4.0
+        path = dump_oneka("p", "t", 0, 10, 365.25, 4, 0.0, 1.0, 0.25, 20.0, [], [], 50.0, 10.0,
+                          20.0, 0.0, True, 1.0, 10.0, pf, filename={str(tmp_path / "a.bz2")!r})
+        assert not any(m == "oneka" or m.startswith("oneka.") for m in sys.modules), sys.modules
+        assert not os.path.exists({str(marker)!r})
+        back = load_oneka(path)["pfield"]
+        assert isinstance(back, ProbabilityField) and back.pgrid[2, 3] == 4.0
+        assert not os.path.exists({str(marker)!r})
+        with bz2.BZ2File(path) as fp:
+            ref = pickle.load(fp)["pfield"]
+        import oneka.probabilityfield as opf
+        assert type(ref) is opf.ProbabilityField
+        assert ref.nrows == pf.nrows and ref.total_weight == 4.0 and ref.pgrid[2, 3] == 4.0
+        print("ok")
+    """)
+    out = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "ok"
+    assert marker.read_text() == "pkg"   # imported only by the plain loader at the end
+    assert ("oneka.probabilityfield", "ProbabilityField") in _globals(str(tmp_path / "a.bz2"))
