@@ -108,3 +108,66 @@ def test_autotuner_picks_a_fitting_shape_and_caches_it(gpu_ctx, tmp_path):
         assert np.array_equal(b.results(i)[0], ref.results(i)[0])
     b.close()
     ref.close()
+
+
+# configuration-2 sites with few wells and short traces (so the oracle runs
+# them in seconds), and a mix with up to 135 wells and 36-year traces
+ORACLE_SITES = (44, 33)
+MIXED_SITES = (2, 7, 12, 33, 44, 53)
+
+
+def _cfg2_subset(idx, nreal):
+    ss = sites.config2()
+    out, draws = [], []
+    for i in idx:
+        s = ss[i]
+        d = sites.draw(s, seed=100 + i)
+        n = min(nreal, len(d["K"]))
+        out.append(s)
+        draws.append({k: v[:n] for k, v in d.items()})
+        s.nrealizations = n
+    return out, draws
+
+
+def test_config2_batch_within_bound_of_fp64_exact(gpu_ctx):
+    """Six configuration-2 sites (1 to 135 wells, 4.9 to 35.7 years, 144 to
+    670^2 nodes), first 300 realizations each, in one batch replay (FP32 pool
+    tracer): every site's capture probabilities are within 0.01 of the FP64-exact
+    tracer's on the same realizations, at every node."""
+    ss, draws = _cfg2_subset(MIXED_SITES, 300)
+    b = SiteBatch(ss, draws)
+    b.run()
+    for i, (s, d) in enumerate(zip(ss, draws)):
+        counts, coef, st = b.results(i)
+        ref, ref_st = trace_counts(s, d, coef, _lib.FP64_EXACT)
+        R = len(d["K"])
+        err = np.abs(counts - ref) / R
+        print(f"{s.name}: R {R}, max |p32 - p64| {err.max():.4f}, L1 {err.sum():.3f}, "
+              f"steps {st.attempted} vs {ref_st.attempted}", flush=True)
+        assert ref.sum() > 0
+        assert err.max() <= 0.01
+        assert err.sum() <= 0.01 * (ref > 0).sum()
+    b.close()
+
+
+def test_config2_sites_fp64_exact_match_oracle(gpu_ctx):
+    """Two configuration-2 sites (6 and 8 wells), first 8 realizations: the
+    batch's fits and MVN draws equal the oracle's, and the FP64-exact tracer on
+    those coefficients equals the oracle bit for bit (counts, steps, statuses)."""
+    from helpers import oracle_chronicle, oracle_fit_draw
+    ss, draws = _cfg2_subset(ORACLE_SITES, 8)
+    b = SiteBatch(ss, draws)
+    b.run()
+    for i, (s, d) in enumerate(zip(ss, draws)):
+        print(f"{s.name}: oracle", flush=True)
+        _, coef, _ = b.results(i)
+        _, _, ocoef = oracle_fit_draw(s, d)
+        assert np.array_equal(coef, ocoef)
+        counts, st = trace_counts(s, d, coef, _lib.FP64_EXACT)
+        ocounts, ost = oracle_chronicle(s, d, coef, s.fixed_field())
+        assert np.array_equal(counts, ocounts)
+        assert (st.attempted, st.accepted, st.visited, st.escaped) == (ost[0], ost[1], ost[7],
+                                                                       ost[6])
+        assert list(st.status_count) == list(ost[2:6])
+        assert counts.sum() > 0
+    b.close()

@@ -1,0 +1,90 @@
+"""The large regional stress case (200 wells, 10 % of them injecting, 200
+observations, 2000^2 nodes at 10 m, 256 particles, 30-year backtraces) under
+the oracle: the FP64-exact GPU path equals the oracle bit for bit on whole
+realizations -- fit and MVN draw, node counts (tiled and flat HBM bitmaps),
+attempted and accepted steps, visited nodes, and trace statuses including
+step-capped traces -- and the FP32 pool tracer agrees with it trace by trace."""
+import numpy as np
+import pytest
+
+from onekapy_amd import Realizations, _lib, chronicle_into_field, fit_and_draw, sites, start_points
+from helpers import oracle_chronicle, oracle_fit_draw
+
+pytestmark = pytest.mark.gpu
+
+# realizations 0..11 of the seed-4 draw of 96: with a 5,000-step cap, 3 and 5
+# hold 154 and 77 capped traces (traces spiralling into injection wells)
+NREAL_DRAW, SEED, FIRST = 96, 4, 12
+CAP = 5000
+
+
+def _subset():
+    s = sites.config4(NREAL_DRAW)
+    d = sites.draw(s, seed=SEED)
+    d = {k: v[:FIRST] for k, v in d.items()}
+    s.nrealizations = FIRST
+    return s, d
+
+
+def _run(s, d, coef, **kw):
+    real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    return chronicle_into_field(s.fixed_field(), s.wells_xy(), s.xt, s.yt, real, starts,
+                                s.duration, s.tol, s.maxstep, s.umbra, max_steps=CAP, **kw)
+
+
+@pytest.fixture(scope="module")
+def cfg4_oracle():
+    s, d = _subset()
+    _, _, coef = oracle_fit_draw(s, d)
+    import oracle as O
+    pf = s.fixed_field()
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    opts = O.trace_opts(s.duration, s.tol, s.maxstep, True, O.MATH_CR, CAP)
+    g = O.make_grid(pf.xmin, pf.ymin, pf.xmax, pf.ymax, pf.deltax, pf.deltay, pf.nrows, pf.ncols)
+    counts, st = O.chronicle_fixed(s.wells_xy(), s.xt, s.yt, s.base, d["K"], d["n"], d["b"],
+                                   d["q"], coef, starts, s.umbra, opts, g)
+    return s, d, coef, counts, st
+
+
+def test_config4_fit_and_draw_bit_exact(gpu_ctx, cfg4_oracle):
+    s, d, coef, _, _ = cfg4_oracle
+    _, _, gcoef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
+                               d["q"], d["z"])
+    assert np.array_equal(gcoef, coef)
+
+
+@pytest.mark.parametrize("mode", [_lib.BITMAP_AUTO, _lib.BITMAP_HBM_FLAT], ids=["tiled", "hbm_flat"])
+def test_config4_fp64_exact_matches_oracle(gpu_ctx, cfg4_oracle, mode):
+    s, d, coef, ocounts, ost = cfg4_oracle
+    print(f"cfg4 fp64-exact, bitmap mode {mode}", flush=True)
+    res = _run(s, d, coef, precision=_lib.FP64_EXACT, bitmap_mode=mode)
+    assert res.passes == 1 and ost[6] == 0
+    assert np.array_equal(res.counts, ocounts)
+    st = res.stats
+    assert (st.attempted, st.accepted, st.visited) == (ost[0], ost[1], ost[7])
+    assert list(st.status_count) == list(ost[2:6])
+    assert st.status_count[_lib.TRACE_STEP_CAP] > 100          # traces that hit the cap
+    assert np.array_equal(np.bincount(res.trace_status, minlength=4), ost[2:6])
+
+
+def test_config4_fp32_pool_agrees_with_fp64_exact(gpu_ctx, cfg4_oracle):
+    """On the same 12 realizations the FP32 pool tracer ends the same traces the
+    same way (to 1 % of the traces: a trace near the cap may need a few more or
+    fewer steps in FP32) and stays within the capture-probability error bound
+    on average over the visited nodes."""
+    s, d, coef, ocounts, ost = cfg4_oracle
+    f64 = _run(s, d, coef, precision=_lib.FP64_EXACT)
+    pool = _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL)
+    wg = _run(s, d, coef, precision=_lib.FP32)
+    assert np.array_equal(pool.counts, wg.counts)
+    assert np.array_equal(pool.trace_status, wg.trace_status)
+    mism = int((pool.trace_status != f64.trace_status).sum())
+    print(f"per-trace status mismatches FP32 vs FP64: {mism} of {len(f64.trace_status)}",
+          flush=True)
+    assert mism <= 0.01 * len(f64.trace_status)
+    R = len(d["K"])
+    err = np.abs(pool.counts - f64.counts) / R
+    print(f"max |p32 - p64| {err.max():.4f}, L1 {err.sum():.4f}, visited nodes "
+          f"{(f64.counts > 0).sum()}", flush=True)
+    assert err.sum() <= 0.01 * (f64.counts > 0).sum()

@@ -155,6 +155,47 @@ def test_control_variate_is_unbiased_and_reduces_variance(cfg0_plain_run, mode):
     assert gain > 3.0, gain
 
 
+@pytest.fixture(scope="module")
+def cfg1_fp64_plain_run(gpu_ctx):
+    """A large plain FP64-exact run on configuration 1's site (25 wells, 1000
+    particles, 20 years, 400^2 nodes): node counts over R realizations."""
+    s = sites.config1()
+    rng = np.random.default_rng(1234)
+    R = 100000
+    d = draw_realizations(s.wells, s.c_dist, s.p_dist, s.t_dist, R, rng)
+    _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
+                              d["q"], d["z"], moments=False)
+    counts, _ = trace_counts(s, d, coef, _lib.FP64_EXACT)
+    return s, counts, R
+
+
+def test_control_variate_config3_network_on_config1_site(cfg1_fp64_plain_run):
+    """The learned control variate at configuration 3's network shape (34 ->
+    512 -> 512 -> 64 x 64 cell logits) on configuration 1's site, at reduced
+    sizes (32,768 traced realizations, 2^19 scrambled Sobol' surrogate
+    evaluations): unbiased against a 10^5-realization plain FP64-exact run
+    (per-cell z-scores) and with less variance than the plain estimate from
+    the same traced realizations."""
+    s, counts, R = cfg1_fp64_plain_run
+    res = control_variate(s, n_traced=32768, n_surrogate=1 << 19, folds=2, train_steps=1200,
+                          batch=2048, lr=5e-3, seed=31, n_hidden=512,
+                          precision=_lib.FP64_EXACT, qmc_replicates=64, qmc_pow2=True)
+    assert (res.window.nrows, res.window.ncols) == (64, 64)
+    p_ref = cell_probabilities(counts, R, res.window)
+    var_ref = p_ref * (1 - p_ref) / R
+    z = (res.p_cv - p_ref) / np.sqrt(res.var_cv + var_ref + 1e-12)
+    live = res.var_cv + var_ref > 1e-9
+    gain = res.var_plain.sum() / res.var_cv.sum()
+    worst = res.var_plain.max() / res.var_cv[res.var_plain.argmax()]
+    print(f"cfg3 network on cfg1: max |z| {np.abs(z[live]).max():.2f}, rms z "
+          f"{np.sqrt(np.mean(z[live] ** 2)):.2f}, gain {gain:.1f}, at the worst plain cell "
+          f"{worst:.1f}, max |p_cv - p_ref| {np.abs(res.p_cv - p_ref).max():.4f}", flush=True)
+    assert live.sum() > 500
+    assert np.abs(z[live]).max() < 6.0
+    assert np.sqrt(np.mean(z[live] ** 2)) < 1.5
+    assert gain > 5.0, gain
+
+
 @pytest.mark.parametrize("mode", ["plain", "antithetic", "qmc"])
 def test_surrogate_sums_compose_draws_fit_features_and_network(gpu_ctx, mode):
     """oka_cv_surrogate_sums against its parts called one by one: the device
