@@ -227,6 +227,37 @@ int oka_trace_chronicle(oka_ctx *ctx, const oka_site *site, int32_t nreal, const
                         int32_t *counts, double *bbox, int32_t *trace_status,
                         int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags);
 
+/* ---- trace + chronicle without allocating ---------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc -- this is oka_trace_chronicle with the same arguments and
+ * results, whose scratch memory (statistics accumulators, the whole-grid
+ * epoch, and the HBM bitmaps of grids too large for LDS) is a device array of
+ * OKA_NOALLOC_SCRATCH_BYTES inside the library, which each GPU holds from the
+ * library's first context on it until the process ends.  It calls no
+ * allocator.  Restrictions:
+ *   - flags must include OKA_PTR_DEVICE: every array argument (site->wells_xy,
+ *     opts->start_xy and epochs included) is device memory;
+ *   - the schedule is the workgroup one (OKA_SCHED_POOL is OKA_ERR_ARG);
+ *   - HBM bitmaps (tiled grids' overflow tiles: ntiles x 128 bytes, flat HBM
+ *     bitmaps: nrows x ncols / 8 bytes, per workgroup in flight) share the
+ *     scratch, so large grids run fewer workgroups at once, and a grid whose
+ *     bitmap exceeds the scratch alone is OKA_ERR_CAPACITY.
+ * Every call on one GPU uses that GPU's one scratch array, which keeps the
+ * last call's contents (each call clears what must start at zero): calls that
+ * overlap on one GPU (several contexts, streams or host threads) race and
+ * must be serialized by the caller.  With stats (host memory) the call is
+ * synchronous; without it, OKA_ASYNC leaves it enqueued on the context's
+ * stream. */
+#define OKA_NOALLOC_SCRATCH_BYTES (64u << 20)
+int oka_trace_chronicle_noalloc(oka_ctx *ctx, const oka_site *site, int32_t nreal,
+                                const double *K, const double *n, const double *b,
+                                const double *q, const double *coef,
+                                const oka_trace_opts *opts, const oka_grid *grid,
+                                int32_t nepochs, const oka_epoch *epochs, int32_t rasterize,
+                                int32_t *counts, double *bbox, int32_t *trace_status,
+                                int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags);
+
 /* ---- per-realization marks pooled over a window: the training targets and
  *      the correction term of the learned control variate --------------------
  * No reference counterpart: the reference registers each realization into

@@ -45,7 +45,7 @@ EXPORTED = (
     "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_cv_surrogate_sums_qmc",
     "oka_mark_moments", "oka_batch_create",
     "oka_batch_tune", "oka_batch_run", "oka_batch_results", "oka_batch_destroy",
-    "oka_count_histogram",
+    "oka_count_histogram", "oka_trace_chronicle_noalloc",
 )
 
 DIST_CONST, DIST_UNIFORM, DIST_TRIANGULAR, DIST_LOGNORMAL = 0, 1, 2, 3
@@ -163,6 +163,8 @@ def load():
     L.oka_trace_chronicle.argtypes = [_vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp,
                                       C.POINTER(TraceOpts_), C.POINTER(Grid_), _i32, _vp, _i32,
                                       _vp, _vp, _vp, _vp, C.POINTER(TraceStats_), _u32]
+    L.oka_trace_chronicle_noalloc.restype = C.c_int
+    L.oka_trace_chronicle_noalloc.argtypes = L.oka_trace_chronicle.argtypes
     L.oka_grid_history.restype = C.c_int
     L.oka_grid_history.argtypes = [C.POINTER(Grid_), _i64, _vp, C.POINTER(Grid_), _vp, _i64,
                                    C.POINTER(_i64)]

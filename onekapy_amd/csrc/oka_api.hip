@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,6 +15,50 @@
 #include "oka_internal.h"
 
 using namespace oka;
+
+// Scratch memory of oka_trace_chronicle_noalloc: the launch's statistics
+// accumulators, the whole-grid epoch and the HBM bitmap region of grids too
+// large for LDS.  Each GPU the library runs on holds its own copy until the
+// process ends.  In host code `scratch` is only a stand-in for the symbol:
+// kernels get the device address that hipGetSymbolAddress returns for the
+// current device (oka_ctx::scratch).
+static __device__ __align__(256) uint32_t scratch[OKA_NOALLOC_SCRATCH_BYTES / 4];
+
+namespace {
+
+constexpr size_t kScrStats = 0;        // kStatCount x u64
+constexpr size_t kScrEpoch = 128;      // one oka_epoch
+constexpr size_t kScrBitmap = 256;     // the rest: bitmap words
+static_assert(sizeof(unsigned long long) * oka::kStatCount <= kScrEpoch, "statistics fit");
+static_assert(kScrEpoch + sizeof(oka_epoch) <= kScrBitmap, "the epoch fits");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kMaxDevices];
+void *g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+// The device address of `scratch` on `device`, which must be current; looked
+// up once per device.
+void *scratch_address(int device, hipError_t *err)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    if (!g_scratch_done[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+        g_scratch_err[device] = e;
+        g_scratch_done[device] = true;
+    }
+    *err = g_scratch_err[device];
+    return g_scratch_ptr[device];
+}
+
+}  // namespace
 
 namespace oka {
 
@@ -127,6 +172,13 @@ __global__ void k_count_histogram(const int32_t *counts, int64_t n, int32_t max_
         zeros += __shfl_xor(zeros, off);
     if ((threadIdx.x & 63) == 0 && zeros)
         atomicAdd(&hist[0], zeros);
+}
+
+// Writes one epoch into device memory (a kernel, so that callers may capture it).
+__global__ void k_store_epoch(oka_epoch e, oka_epoch *dst)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *dst = e;
 }
 
 __global__ void k_bits_to_u8(const uint32_t *bits, uint8_t *out, int64_t n)
@@ -359,6 +411,7 @@ int oka_ctx_create(oka_ctx **out, int device)
                               device) != hipSuccess)
         lds_cu = 0;
     c->lds_per_cu = lds_cu > 0 ? (size_t)lds_cu : prop.sharedMemPerBlock;
+    c->scratch = scratch_address(device, &c->scratch_err);
     *out = c;
     return OKA_OK;
 }
@@ -885,6 +938,125 @@ int oka_trace_chronicle(oka_ctx *c, const oka_site *site, int32_t nreal, const d
     return chronicle_impl(c, site, nreal, K, n, b, q, coef, o, g, nepochs, epochs, rasterize,
                           counts, bbox, trace_status, trace_steps, stats, flags, nullptr,
                           nullptr);
+}
+
+int oka_trace_chronicle_noalloc(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+                                const double *n, const double *b, const double *q,
+                                const double *coef, const oka_trace_opts *o, const oka_grid *g,
+                                int32_t nepochs, const oka_epoch *epochs, int32_t rasterize,
+                                int32_t *counts, double *bbox, int32_t *trace_status,
+                                int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags)
+{
+    if (!c)
+        return OKA_ERR_ARG;
+    if (!(flags & OKA_PTR_DEVICE))
+        return fail(c, OKA_ERR_ARG, "oka_trace_chronicle_noalloc takes device arrays only "
+                                    "(flags need OKA_PTR_DEVICE)");
+    if (!c->scratch)
+        return fail(c, OKA_ERR_DEVICE, "no scratch address on device %d: %s", c->device,
+                    hipGetErrorString(c->scratch_err));
+    if (!g)
+        return fail(c, OKA_ERR_ARG, "grid is NULL");
+    if (rasterize && (g->nrows <= 0 || g->ncols <= 0 || !counts))
+        return fail(c, OKA_ERR_ARG, "rasterization needs a non-empty grid and counts");
+    if (rasterize && (!(g->dx > 0.0) || !(g->dy > 0.0)))
+        return fail(c, OKA_ERR_RANGE, "grid spacing must be > 0");
+    if (nepochs < 0 || (nepochs > 0 && !epochs))
+        return fail(c, OKA_ERR_ARG, "bad epochs");
+    if (o && o->schedule == OKA_SCHED_POOL)
+        return fail(c, OKA_ERR_ARG, "oka_trace_chronicle_noalloc runs the workgroup schedule only");
+    TraceArgs a{};
+    TRY(trace_common(c, site, nreal, K, n, b, q, coef, o, flags, a));
+    if (stats)
+        memset(stats, 0, sizeof(*stats));
+    const int64_t ntr = (int64_t)nreal * o->npaths;
+    if (ntr == 0)
+        return OKA_OK;
+    a.gxmin = g->xmin;
+    a.gymin = g->ymin;
+    a.gxmax = g->xmax;
+    a.gymax = g->ymax;
+    a.gdx = g->dx;
+    a.gdy = g->dy;
+    a.gnrows = g->nrows;
+    a.gncols = g->ncols;
+    a.rasterize = rasterize ? 1 : 0;
+    const int64_t nnodes = (int64_t)g->nrows * g->ncols;
+    a.nwords = rasterize ? (int32_t)((nnodes + 31) / 32) : 0;
+    unsigned char *scr = static_cast<unsigned char *>(c->scratch);
+    a.stats = reinterpret_cast<unsigned long long *>(scr + kScrStats);
+    TRY(fill_u32(c, a.stats, 2 * kStatCount));   // 64-bit slots
+    if (nepochs == 0) {
+        const oka_epoch whole{0, g->xmin, g->ymin, g->nrows, g->ncols, 0, 0};
+        oka_epoch *dst = reinterpret_cast<oka_epoch *>(scr + kScrEpoch);
+        hipLaunchKernelGGL(k_store_epoch, dim3(1), dim3(64), 0, c->stream, whole, dst);
+        HIPCHK(c, hipGetLastError());
+        a.nepochs = 1;
+        a.epochs = dst;
+    } else {
+        a.nepochs = nepochs;
+        a.epochs = epochs;
+    }
+    a.counts = rasterize ? counts : nullptr;
+    a.bbox = bbox;
+    a.trace_status = trace_status;
+    a.trace_steps = trace_steps;
+
+    const bool fp64 = o->precision == OKA_FP64_EXACT;
+    const size_t base_lds = fp64 ? trace_fp64_wells_lds(a.nwells) : trace_fp32_wells_lds(a.nwells);
+    if (base_lds > c->max_lds)
+        return fail(c, OKA_ERR_ARG, "too many wells for LDS (%d)", a.nwells);
+    int kind = kBmLdsFlat;
+    int blocks = nreal;
+    if (rasterize) {
+        TRY(choose_bitmap(c, o, base_lds, a, &kind));
+        if (kind != kBmLdsFlat) {
+            // per-workgroup HBM bitmaps in the scratch region: as many
+            // workgroups in flight as the LDS lets run and the region holds
+            const size_t lds = base_lds + (kind == kBmTiled
+                                               ? (size_t)tile_lds_words(a.ntiles, a.tile_slots) * 4
+                                               : 0);
+            const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, c->lds_per_cu / lds));
+            const int64_t per_block = kind == kBmTiled ? (int64_t)a.ntiles * 32 : (int64_t)a.nwords;
+            const int64_t room = (int64_t)((OKA_NOALLOC_SCRATCH_BYTES - kScrBitmap) / 4) / per_block;
+            if (room < 1)
+                return fail(c, OKA_ERR_CAPACITY,
+                            "a %d x %d grid's bitmap (%lld words) exceeds the no-allocation "
+                            "scratch (%u bytes)", g->nrows, g->ncols, (long long)per_block,
+                            (unsigned)OKA_NOALLOC_SCRATCH_BYTES);
+            blocks = (int)std::min<int64_t>(std::min<int64_t>(nreal, (int64_t)c->num_cu * per_cu),
+                                            room);
+            a.gbitmap = reinterpret_cast<uint32_t *>(scr + kScrBitmap);
+            if (kind == kBmTiled)   // overflow tiles start empty; each flush leaves them so
+                TRY(fill_u32(c, a.gbitmap, per_block * blocks));
+        }
+    }
+    if (stats)
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (fp64)
+        HIPCHK(c, launch_trace_fp64(a, blocks, kind, c->stream));
+    else
+        HIPCHK(c, launch_trace_fp32(a, blocks, kind, c->stream));
+    if (!stats) {
+        if (!(flags & OKA_ASYNC))
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        return OKA_OK;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long h[kStatCount];
+    HIPCHK(c, hipMemcpy(h, a.stats, sizeof(h), hipMemcpyDeviceToHost));
+    stats->attempted = (int64_t)h[kStatAttempted];
+    stats->accepted = (int64_t)h[kStatAccepted];
+    for (int j = 0; j < 4; ++j)
+        stats->status_count[j] = (int64_t)h[kStatStatus0 + j];
+    stats->escaped = (int64_t)h[kStatEscaped];
+    stats->visited = (int64_t)h[kStatVisited];
+    stats->candidates = (int64_t)h[kStatCandidates];
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    stats->kernel_ms = ms;
+    return OKA_OK;
 }
 
 int oka_trace_marks(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,

@@ -42,6 +42,11 @@ struct oka_ctx {
     int warp_size = 64;
     std::string err;
     oka::DevBuf buf[oka::kSlotCount];
+    // device address of this device's copy of the no-allocation scratch
+    // array (oka_trace_chronicle_noalloc), looked up at create; null if the
+    // lookup failed, with its HIP error in scratch_err
+    void *scratch = nullptr;
+    hipError_t scratch_err = hipSuccess;
 };
 
 namespace oka {
