@@ -439,6 +439,26 @@ int oka_mark_moments(oka_ctx *ctx, int64_t n, int32_t ncell, const uint8_t *mark
 int oka_count_histogram(oka_ctx *ctx, int64_t n, const int32_t *counts, int32_t max_count,
                         int64_t *hist, uint32_t flags);
 
+/* ---- the probability field after the run -----------------------------------
+ * Pr = pgrid / total_weight on an nrows x ncols node lattice
+ * (visualize.py:230), and when radius > 0 its Gaussian smoothing as
+ * scipy.ndimage.gaussian_filter(Pr, sigma, mode='constant', cval=0.0) computes
+ * it for the probability map (visualize.py:232-233): a correlation along axis
+ * 0 then axis 1 with the 2 radius + 1 symmetric weights (scipy's normalized
+ * kernel, radius = int(4 sigma + 0.5), computed by the caller), zeros outside
+ * the lattice, in scipy's summation order, FP64.  out is nrows x ncols.
+ * pgrid and out follow flags; weights are host memory. */
+int oka_field_smooth(oka_ctx *ctx, int32_t nrows, int32_t ncols, const double *pgrid,
+                     double total_weight, int32_t radius, const double *weights, double *out,
+                     uint32_t flags);
+
+/* The impact curve's probabilities (visualize.py:382): pr = the n values of
+ * pgrid / total_weight sorted in descending order (a radix sort on the device;
+ * for fields of whole counts oka_count_histogram gives the same array).
+ * pgrid and pr follow flags. */
+int oka_field_sorted_probabilities(oka_ctx *ctx, int64_t n, const double *pgrid,
+                                   double total_weight, double *pr, uint32_t flags);
+
 /* The surrogate term of the control variate for n fresh realizations of the
  * stream `seed` (starting at realization `first`), computed entirely on the
  * device in chunks: draws, fit + MVN draw (as oka_fit_draw), standardized

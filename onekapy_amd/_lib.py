@@ -45,7 +45,8 @@ EXPORTED = (
     "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_cv_surrogate_sums_qmc",
     "oka_mark_moments", "oka_batch_create",
     "oka_batch_tune", "oka_batch_run", "oka_batch_results", "oka_batch_destroy",
-    "oka_count_histogram", "oka_trace_chronicle_noalloc",
+    "oka_count_histogram", "oka_trace_chronicle_noalloc", "oka_field_smooth",
+    "oka_field_sorted_probabilities",
 )
 
 DIST_CONST, DIST_UNIFORM, DIST_TRIANGULAR, DIST_LOGNORMAL = 0, 1, 2, 3
@@ -217,6 +218,10 @@ def load():
     L.oka_batch_destroy.argtypes = [_vp]
     L.oka_count_histogram.restype = C.c_int
     L.oka_count_histogram.argtypes = [_vp, _i64, _vp, _i32, _vp, _u32]
+    L.oka_field_smooth.restype = C.c_int
+    L.oka_field_smooth.argtypes = [_vp, _i32, _i32, _vp, _d, _i32, _vp, _vp, _u32]
+    L.oka_field_sorted_probabilities.restype = C.c_int
+    L.oka_field_sorted_probabilities.argtypes = [_vp, _i64, _vp, _d, _vp, _u32]
     L.oka_mark_moments.restype = C.c_int
     L.oka_mark_moments.argtypes = [_vp, _i64, _i32, _vp, _vp, _vp, _u32]
     L.oka_pow_fifth.restype = _d

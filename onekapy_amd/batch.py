@@ -1,10 +1,12 @@
 """Many sites in one persistent, graph-captured GPU run (oka_batch_*).
 
 Each site is a ``create_stochastic_capturezone`` problem (oneka/stochastic.py:
-76-270) on a fixed, pre-expanded ProbabilityField.  The library stages every
-site once, captures one run -- per site the fit and MVN draw of every
-realization, then all particles of all sites in one launch of the pool tracer
-(FP32, work stealing over live particles) -- as a HIP graph, and replays it.
+76-270) on a fixed, pre-expanded ProbabilityField, tracing its own velocity
+field (``site.confined``).  The library stages every site once, captures one
+run -- per site the fit and MVN draw of every realization, then all particles
+of all sites in one launch of the pool tracer (work stealing over live
+particles, FP32 or FP64-exact for the whole batch) -- as a HIP graph, and
+replays it.
 """
 import ctypes as C
 
@@ -25,9 +27,10 @@ def _knob_opts(knobs):
 
 class SiteBatch:
     """sites: objects with the attributes of ``sites.Site``; draws: per site a
-    dict(q, K, n, b, z) of its realization parameters."""
+    dict(q, K, n, b, z) of its realization parameters; precision: the batch's
+    tracer arithmetic (_lib.FP32 or _lib.FP64_EXACT)."""
 
-    def __init__(self, sites, draws, knobs=None, device=0):
+    def __init__(self, sites, draws, knobs=None, device=0, precision=_lib.FP32):
         self.ctx = _lib.context(device)
         self.sites = list(sites)
         self.knobs = dict(knobs or {})
@@ -47,7 +50,7 @@ class SiteBatch:
             b.K, b.n, b.b, b.q = (_lib.ptr(a) for a in arrays)
             b.z = _lib.ptr(z)
             b.opts = _lib.make_trace_opts(s.npaths, s.duration, s.tol, s.maxstep, s.umbra,
-                                          _lib.ptr(starts), _lib.FP32, s.confined)
+                                          _lib.ptr(starts), precision, s.confined)
             b.grid = _lib.make_grid(pf.xmin, pf.ymin, pf.xmax, pf.ymax, pf.deltax, pf.deltay,
                                     pf.nrows, pf.ncols)
             keep.append((wxy, obs, st, starts, arrays, z))

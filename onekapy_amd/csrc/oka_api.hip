@@ -204,14 +204,18 @@ int fill_u32(oka_ctx *c, void *p, int64_t nwords) { return fill_u32v(c, p, 0u, n
 
 
 int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opts *o,
-               PoolShape *out)
+               int precision, PoolShape *out)
 {
     PoolShape &sh = *out;
+    if (precision != OKA_FP32 && precision != OKA_FP64_EXACT)
+        return fail(c, OKA_ERR_ARG, "unknown precision %d", precision);
+    sh.precision = precision;
     sh.chunk = o->pool_chunk > 0 ? o->pool_chunk : 64;
     sh.nsites = nsites;
     // first[]: queue items, then realizations, per site (2 x (nsites + 1))
     sh.first.assign(2 * ((size_t)nsites + 1), 0);
     int64_t total = 0, nreal = 0, chunked = 0;
+    int nconf = 0;
     sh.max_nw = 1;
     sh.max_tiles = 1;
     for (int i = 0; i < nsites; ++i) {
@@ -225,9 +229,9 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
         sh.max_nw = std::max(sh.max_nw, (int)sites[i].nwells);
         if (sites[i].rasterize)
             sh.max_tiles = std::max(sh.max_tiles, (int)sites[i].ntiles);
-        if (!sites[i].confined)
-            return fail(c, OKA_ERR_ARG, "the pool schedule traces the confined velocity only");
+        nconf += sites[i].confined ? 1 : 0;
     }
+    sh.conf_mode = nconf == nsites ? 0 : (nconf == 0 ? 1 : 2);
     sh.first[nsites] = total;
     sh.first[2 * (size_t)nsites + 1] = nreal;
     sh.total = total;
@@ -235,23 +239,26 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
         return fail(c, OKA_ERR_ARG, "grid too large for the pool schedule (%d tiles)",
                     sh.max_tiles);
     sh.slots = o->pool_slots > 0 ? std::min(o->pool_slots, 16) : 2;
+    // the FP64 kernel's registers allow 2 waves per SIMD, the FP32 one's 4
+    const int max_wpc = precision == OKA_FP64_EXACT ? 8 : 16;
     if (o->pool_waves_per_cu > 0) {
         sh.wpc = std::min(o->pool_waves_per_cu, 32);
     } else {
-        // default: the most waves per CU (of 16, 12, 8, 6, 4) whose LDS share
-        // still holds the slots' wells and maps and 8 pool tiles per slot
+        // default: the most waves per CU (of 16, 12, 8, 6, 4, no more than
+        // the registers let run) whose LDS share still holds the slots'
+        // wells and maps and 8 pool tiles per slot
         const int choice[] = {16, 12, 8, 6, 4};
         sh.wpc = 4;
         for (int w : choice) {
-            if (pool_lds_bytes(sh.slots, sh.max_nw, 8) <=
-                std::min(c->lds_per_cu / (size_t)w, c->max_lds)) {
+            if (w <= max_wpc && pool_lds_bytes(precision, sh.slots, sh.max_nw, 8) <=
+                                    std::min(c->lds_per_cu / (size_t)w, c->max_lds)) {
                 sh.wpc = w;
                 break;
             }
         }
     }
     const size_t budget = std::min(c->lds_per_cu / sh.wpc, c->max_lds);
-    const size_t fixed = pool_lds_bytes(sh.slots, sh.max_nw, 0);
+    const size_t fixed = pool_lds_bytes(precision, sh.slots, sh.max_nw, 0);
     if (fixed > budget)
         return fail(c, OKA_ERR_ARG, "%d wells x %d slots do not fit in LDS at %d waves per CU",
                     sh.max_nw, sh.slots, sh.wpc);
@@ -309,7 +316,7 @@ int pool_launch(oka_ctx *c, const PoolShape &sh, const PoolArgs &p)
         TRY(fill_u32(c, p.rec_done, sh.rec_words - 2 * sh.nrec));
     }
     if (sh.total > 0)
-        HIPCHK(c, launch_trace_pool32(p, sh.waves, c->stream));
+        HIPCHK(c, launch_trace_pool(p, sh.waves, sh.precision, sh.conf_mode, c->stream));
     return OKA_OK;
 }
 
@@ -327,12 +334,8 @@ int check_opts(oka_ctx *c, const oka_trace_opts *o)
         return fail(c, OKA_ERR_ARG, "start_xy is NULL");
     if (o->precision != OKA_FP64_EXACT && o->precision != OKA_FP32)
         return fail(c, OKA_ERR_ARG, "unknown precision %d", o->precision);
-    if (o->schedule == OKA_SCHED_POOL && !o->confined)
-        return fail(c, OKA_ERR_ARG, "the pool schedule traces the confined velocity only");
     if (o->schedule < OKA_SCHED_AUTO || o->schedule > OKA_SCHED_POOL)
         return fail(c, OKA_ERR_ARG, "unknown schedule %d", o->schedule);
-    if (o->schedule == OKA_SCHED_POOL && o->precision != OKA_FP32)
-        return fail(c, OKA_ERR_ARG, "the pool schedule traces in FP32");
     return OKA_OK;
 }
 
@@ -713,7 +716,7 @@ static int chronicle_finish(oka_ctx *c, const TraceArgs &a, int32_t rasterize, i
 int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opts *o, bool timed)
 {
     PoolShape sh;
-    TRY(pool_shape(c, sites, nsites, o, &sh));
+    TRY(pool_shape(c, sites, nsites, o, o->precision, &sh));
     void *p_sites, *p_first, *p_queue, *p_hbm, *p_rec = nullptr;
     TRY(ensure(c, kSlotPoolSites, sizeof(TraceArgs) * nsites, &p_sites));
     TRY(ensure(c, kSlotPoolFirst, sizeof(int64_t) * sh.first.size(), &p_first));

@@ -20,6 +20,7 @@ struct oka_batch {
     unsigned long long *stats = nullptr;   // [nsites][kStatCount]
     std::vector<void *> allocs;
     oka_trace_opts tune{};
+    int precision = OKA_FP32;   // every site's opts.precision
     PoolShape shape;
     void *d_sites = nullptr, *d_first = nullptr, *d_queue = nullptr;
     void *d_fits = nullptr, *d_fit_first = nullptr;   // fit table, its two prefix arrays
@@ -69,8 +70,8 @@ int stage_site(oka_batch *B, int s, const oka_batch_site &bs)
         return fail(c, OKA_ERR_ARG, "site %d: missing realization inputs", s);
     if (o.npaths > 0 && !o.start_xy)
         return fail(c, OKA_ERR_ARG, "site %d: start_xy is NULL", s);
-    if (!o.confined)
-        return fail(c, OKA_ERR_ARG, "site %d: batches trace the confined velocity", s);
+    if (o.precision != B->precision)
+        return fail(c, OKA_ERR_ARG, "site %d: every site of a batch traces in one precision", s);
     if (g.nrows <= 0 || g.ncols <= 0 || !(g.dx > 0.0) || !(g.dy > 0.0))
         return fail(c, OKA_ERR_ARG, "site %d: bad grid", s);
     const int nw = site->nwells, nr = bs.nreal;
@@ -103,7 +104,7 @@ int stage_site(oka_batch *B, int s, const oka_batch_site &bs)
     TRY(balloc(B, sizeof(int32_t) * std::max(nr, 1), &p));
     f.status = (int32_t *)p;
     a.npaths = o.npaths;
-    a.confined = 1;
+    a.confined = o.confined ? 1 : 0;
     a.duration = o.duration;
     a.tol = o.tol;
     a.maxstep = o.maxstep;
@@ -139,7 +140,7 @@ int capture(oka_batch *B)
         (void)hipGraphExecDestroy(B->exec);
         B->exec = nullptr;
     }
-    TRY(pool_shape(c, B->args.data(), B->nsites, &B->tune, &B->shape));
+    TRY(pool_shape(c, B->args.data(), B->nsites, &B->tune, B->precision, &B->shape));
     const PoolShape &sh = B->shape;
     const size_t first_bytes = sizeof(int64_t) * sh.first.size();
     if (!B->d_first || B->first_cap < first_bytes) {
@@ -203,6 +204,7 @@ int oka_batch_create(oka_ctx *c, int32_t nsites, const oka_batch_site *sites,
     B->nnodes.assign(nsites, 0);
     if (tune)
         B->tune = *tune;
+    B->precision = sites[0].opts.precision;
     void *p;
     int rc = balloc(B, sizeof(unsigned long long) * kStatCount * nsites, &p);
     B->stats = (unsigned long long *)p;

@@ -62,13 +62,14 @@ int fill_u32v(oka_ctx *c, void *p, uint32_t value, int64_t nwords);
 
 // Shape of a persistent pool launch (oka_pool.hip) over a table of sites.
 struct PoolShape {
+    int precision = OKA_FP32, conf_mode = 0;   // the arithmetic and velocity fields traced
     int nsites = 0, chunk = 64, slots = 4, wpc = 8, waves = 0, pool_tiles = 0;
     int max_nw = 1, max_tiles = 1, dirty_words = 1;
     int64_t total = 0, nrec = 0, hbm_words = 0, rec_words = 0;
     std::vector<int64_t> first;   // items, then realizations, per site: 2 x (nsites + 1)
 };
 int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opts *o,
-               PoolShape *out);
+               int precision, PoolShape *out);
 // Device buffers: sites [nsites] TraceArgs, first [2 (nsites + 1)] int64
 // (uploaded by the caller), queue (8 bytes), hbm [hbm_words], rec [rec_words].
 PoolArgs pool_args(const PoolShape &sh, void *sites, void *first, void *queue, void *hbm,

@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) k_fit_logs(FitArgs a, double *logr2)
     const int i = (int)(t / a.nwells), w = (int)(t % a.nwells);
     const double dx = a.obs[4 * i] - a.wells_xy[2 * w];
     const double dy = a.obs[4 * i + 1] - a.wells_xy[2 * w + 1];
-    logr2[t] = cr_log(dx * dx + dy * dy);
+    logr2[t] = log_rn(dx * dx + dy * dy);
 }
 
 // One weighted row (model.py:545-565).  Returns 0 or OKA_ERR_RANGE.
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(256) k_fit_logs_multi(const FitArgs *sites, in
     const int i = (int)(k / a.nwells), w = (int)(k % a.nwells);
     const double dx = a.obs[4 * i] - a.wells_xy[2 * w];
     const double dy = a.obs[4 * i + 1] - a.wells_xy[2 * w + 1];
-    ((double *)a.logr2)[k] = cr_log(dx * dx + dy * dy);
+    ((double *)a.logr2)[k] = log_rn(dx * dx + dy * dy);
 }
 
 // Fit and MVN draw of every realization of every site of a table.

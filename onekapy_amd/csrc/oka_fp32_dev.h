@@ -126,6 +126,102 @@ __device__ __forceinline__ void feval32(const Field32 &f, float x, float y, floa
     feval32<kConfined>(f, x, y, kx, ky, dry);
 }
 
+// Far wells as local expansions (the FP32 pool on sites of many wells).  The
+// plane is cut into square boxes of side L (target-relative coordinates,
+// box (i, j) has its lower-left corner at (x0 + i L, y0 + j L)); a well is
+// near a box when it lies in one of the 3 x 3 boxes around it, else far.
+// With F(z) = sum_w s_w / (z - z_w) (z = x + i y, s_w the well's folded
+// strength), a well term of k = -V is s_w (dx, dy) / r^2 = conj(s_w / (z - z_w)),
+// so k = conj(F) plus the regional part.  For z in box b with centre c, u =
+// (z - c) / L (|u| <= 1/sqrt(2)) and v_w = (z_w - c) / L (|v_w| >= 3/2 for a
+// far well):
+//   s_w / (z - z_w) = -(s_w / (L v_w)) sum_k (u / v_w)^k,
+// so the far wells sum to B_b(u) = sum_{k < order} beta_k u^k with
+//   beta_k = -(1/L) sum_{far w} s_w / v_w^(k+1),
+// truncated where |u / v_w|^order <= (sqrt(2) / 3)^order (6e-6 at order 16).
+// The near wells are summed directly, as in feval32.
+struct FarField {
+    int nbx, nby, order;
+    float x0, y0, len, ilen;        // box grid origin and side, 1 / side
+    const int *near_first;          // [nbx nby + 1]: CSR offsets of the near lists
+    const int *near;                // near-well indices per box
+    const float2 *beta;             // [nbx nby][order] of this slot
+};
+
+__device__ __forceinline__ bool far_near(const FarField &F, int b, float wx, float wy)
+{
+    const int bx = b % F.nbx, by = b / F.nbx;
+    const float wbx = floorf((wx - F.x0) * F.ilen), wby = floorf((wy - F.y0) * F.ilen);
+    return fabsf(wbx - (float)bx) <= 1.0f && fabsf(wby - (float)by) <= 1.0f;
+}
+
+// beta of boxes lane, lane + stride, ... from the wells w[0, nw): each lane
+// keeps one box's coefficients in registers while it walks the wells.
+template <int kMaxOrder>
+__device__ __forceinline__ void far_coefficients(const FarField &F, const float4 *w, int nw,
+                                                 float2 *beta, int lane, int stride)
+{
+    const int nb = F.nbx * F.nby;
+    for (int b = lane; b < nb; b += stride) {
+        const float cx = F.x0 + ((float)(b % F.nbx) + 0.5f) * F.len;
+        const float cy = F.y0 + ((float)(b / F.nbx) + 0.5f) * F.len;
+        float ar[kMaxOrder], ai[kMaxOrder];
+#pragma unroll
+        for (int k = 0; k < kMaxOrder; ++k)
+            ar[k] = ai[k] = 0.0f;
+        for (int i = 0; i < nw; ++i) {
+            const float4 wl = w[i];
+            if (far_near(F, b, wl.x, wl.y))
+                continue;
+            const float vx = (wl.x - cx) * F.ilen, vy = (wl.y - cy) * F.ilen;
+            const float iv = 1.0f / fmaf(vx, vx, vy * vy);
+            const float qx = vx * iv, qy = -vy * iv;       // 1 / v_w
+            float tx = wl.z * qx, ty = wl.z * qy;          // s_w / v_w^(k+1)
+#pragma unroll
+            for (int k = 0; k < kMaxOrder; ++k) {
+                if (k < F.order) {
+                    ar[k] -= tx;
+                    ai[k] -= ty;
+                }
+                const float nx = fmaf(tx, qx, -ty * qy);
+                ty = fmaf(tx, qy, ty * qx);
+                tx = nx;
+            }
+        }
+        for (int k = 0; k < F.order; ++k)
+            beta[(size_t)b * F.order + k] = make_float2(ar[k] * F.ilen, ai[k] * F.ilen);
+    }
+}
+
+// k = -V at (x, y) with the far wells of its box taken from the expansion;
+// false (k untouched) when (x, y) lies outside the boxes.
+__device__ __forceinline__ bool feval32_far(const Field32 &f, const FarField &F, float x, float y,
+                                            float &kx, float &ky)
+{
+    const float fx = (x - F.x0) * F.ilen, fy = (y - F.y0) * F.ilen;
+    const float bxf = floorf(fx), byf = floorf(fy);
+    if (!(bxf >= 0.0f && bxf < (float)F.nbx && byf >= 0.0f && byf < (float)F.nby))
+        return false;
+    const int b = (int)byf * F.nbx + (int)bxf;
+    const float ux = fx - bxf - 0.5f, uy = fy - byf - 0.5f;
+    const float2 *a = F.beta + (size_t)b * F.order;
+    float gr = a[F.order - 1].x, gi = a[F.order - 1].y;
+    for (int k = F.order - 2; k >= 0; --k) {
+        const float2 ak = a[k];
+        const float nr = fmaf(gr, ux, fmaf(-gi, uy, ak.x));
+        gi = fmaf(gr, uy, fmaf(gi, ux, ak.y));
+        gr = nr;
+    }
+    float vx = fmaf(f.a2, x, fmaf(f.c, y, f.d)) + gr;
+    float vy = fmaf(f.b2, y, fmaf(f.c, x, f.e)) - gi;
+    const int j1 = F.near_first[b + 1];
+    for (int j = F.near_first[b]; j < j1; ++j)
+        well_term(f.w[F.near[j]], x, y, vx, vy);
+    kx = vx;
+    ky = vy;
+    return true;
+}
+
 // The lane's own field evaluation, as finish32's evaluator; `dry` collects the
 // unconfined field's AquiferError.
 template <bool kConfined = true>

@@ -390,7 +390,8 @@ int qmc_dims(int32_t nwells, const oka_dist *dists);
 constexpr uint32_t kQmcKeySlot = 0x50B0150u;
 uint32_t qmc_key(uint64_t seed, int d);
 
-// Persistent FP32 tracer over a table of sites (oka_pool.hip).
+// Persistent tracer over a table of sites (oka_pool_impl.h; FP32 in
+// oka_pool.hip, FP64-exact in oka_pool64.hip).
 // A queue item is a chunk of `chunk` particles of one realization.  A
 // realization of several chunks is gathered in a realization record in HBM
 // (its visited-node tiles and dirty-tile bits), which the wave that merges
@@ -413,10 +414,16 @@ struct PoolArgs {
     uint32_t *rec_bits;         // [nrec][max_ntiles][32]; zero before the launch
     uint32_t *rec_dirty;        // [nrec][dirty_words]; zero before the launch
 };
-size_t pool_lds_bytes(int slots, int max_nwells, int pool_tiles);
+// LDS bytes of one wave of the pool in `precision` (OKA_FP32 or
+// OKA_FP64_EXACT); conf_mode: 0 every site confined, 1 every site
+// unconfined, 2 a choice per site.
+size_t pool_lds_bytes(int precision, int slots, int max_nwells, int pool_tiles);
+size_t pool64_lds_bytes(int slots, int max_nwells, int pool_tiles);
 int pool_max_tiles();
 int pool_max_pool_tiles();
-hipError_t launch_trace_pool32(const PoolArgs &p, int waves, hipStream_t s);
+hipError_t launch_trace_pool(const PoolArgs &p, int waves, int precision, int conf_mode,
+                             hipStream_t s);
+hipError_t launch_trace_pool64(const PoolArgs &p, int waves, int conf_mode, hipStream_t s);
 
 // oka_fp64.hip
 hipError_t launch_trace_fp64(const TraceArgs &a, int grid_blocks, int kind,

@@ -1,674 +1,219 @@
-// oka_pool.hip -- persistent FP32 tracer with work stealing over live particles.
+// oka_pool.hip -- the persistent pool tracer (oka_pool_impl.h) in FP32.
 //
-// One wave (64 lanes) per workgroup, a persistent grid.  Each wave keeps up to
-// `slots` realizations in flight: when a lane's trace ends it takes the next
-// particle of the newest open realization, and when that realization has no
-// particles left the wave opens another one from a global queue shared by all
-// waves (possibly of another site).  A realization is flushed as soon as its
-// last particle ends, so a few long traces (particles circling an injection
-// well, a 50-year backtrace beside a 1-year one) no longer idle the lanes that
-// finished their own work.
-//
-// Per realization slot, the visited-node bitmap is kept as 32 x 32-node tiles:
-// a small LDS hash maps a tile to one of the slot's LDS pool tiles, or to the
-// slot's HBM region when the pool is full; tiles that find no hash entry live
-// in the HBM region unmapped, and the flush then scans that region.  All
-// coordination is within one wave; barriers separate the phases of an
-// iteration so that LDS writes by one lane are seen by the others.
-//
-// The step, field and rasterizer are those of the workgroup tracer
-// (oka_fp32_dev.h: oneka/capturezone.py:198-253, model.py:299-315, 423-427,
-// probabilityfield.py:292-310, 407-427).
+// The step, field and rasterizer are those of the FP32 workgroup tracer
+// (oka_fp32_dev.h: oneka/capturezone.py:198-253, model.py:207-266, 299-315,
+// 318-389, 423-427, probabilityfield.py:292-310, 407-427), with each slot's
+// constants folded exactly as that tracer folds them per realization.  The
+// dispatch between the FP32 and FP64-exact pools is here too.
 #include <math.h>
 
 #include "oka_fp32_dev.h"
 #include "oka_internal.h"
+#include "oka_pool_impl.h"
 
 namespace oka {
 namespace {
 
 using namespace fp32;
+using namespace pool;
 
-constexpr int kWave = 64;
-constexpr int kHashBits = 7;
-constexpr int kHash = 1 << kHashBits;                 // map entries per slot
-constexpr int kValBits = 11;
-constexpr uint32_t kValMask = (1u << kValBits) - 1u;  // pool tile, or kValHbm
-constexpr uint32_t kValHbm = kValMask;
-constexpr uint32_t kValNone = 0xffffffffu;              // no map entry (find)
-constexpr uint32_t kKeyEmpty = 0xffffffffu;
+struct PoolFP32 {
+    using Well = float4;   // (xw - xo, yw - yo, q / (2 pi b n), q ln 2 / (4 pi h))
+    struct Consts {
+        float a2, b2, c, d, e;              // regional velocity over -(b n)
+        float pa, pb, pc, pd, pe, pf;       // regional potential over h (unconfined)
+        float tol, maxstep, gdx, gdy, igdx, igdy, umbra, u2;
+        float gx0, gx1, gy0, gy1;           // grid bounds relative to the origin
+        double dur, absdur, sg, xo, yo;
+        long long cap;
+    };
+    struct Lane {
+        Trace32 tr;
+        Epoch32 ep;
+        unsigned long long cand;   // candidate nodes of the trace
+        uint32_t last_t, last_v;   // the lane's tile cache (PoolBits::lookup)
+        bool fresh;                // k1 not yet evaluated
+        bool a_done;               // the disk around the trace's current point is marked
+    };
+    // At most 128 VGPRs (one 64-bit value spills, in the flush's scan of an
+    // overfull map), so that 16 waves fit a CU when the LDS allows them.
+    static constexpr int kWavesPerEU = 4;
 
-struct __align__(16) SlotHdr {   // a multiple of 16 bytes: the float4 wells follow
-    unsigned long long att, acc, esc, cand, vis, st[4];
-    int64_t r;          // realization within its site
-    int64_t g;          // realization id over all sites
-    int state;          // 0 free, 1 open, 2 waiting for its realization record
-    int site;
-    int next, np, rem;  // next particle to hand out, end of the chunk, not yet ended
-    int nchunks, rec;   // chunks of the realization; its record when nchunks > 1
-    int pool_next;      // pool tiles handed out (may exceed the pool)
-    int hash_full;      // a tile found no map entry
-    float a2, b2, c, d, e;
-    // the site's integrator and raster constants, read by the lanes when
-    // they need them rather than kept in registers per lane
-    double dur, absdur, sg;
-    long long cap;
-    float tol, maxstep, gdx, gdy, igdx, igdy, umbra, u2;
-    int nw, raster, tiles_x;
-};
-
-__device__ __forceinline__ uint32_t tile_hash(uint32_t t)
-{
-    return (t * 2654435761u) >> (32 - kHashBits);
-}
-
-// One slot's tiled bitmap, as seen by one lane.
-struct PoolBits {
-    uint32_t *keys;     // LDS [kHash]: (tile << kValBits) | value, or kKeyEmpty
-    uint32_t *pool;     // LDS [pool_tiles][32]
-    int *pool_next, *hash_full;
-    int pool_tiles;
-    uint32_t *hbm;      // [ntiles][32]
-    int tiles_x;
-    // the last tile this lane found mapped and its value (a mapping lasts
-    // until the flush); kept by the lane between steps
-    uint32_t last_t, last_v;
-
-    __device__ __forceinline__ uint32_t *tile_words(uint32_t t, uint32_t v) const
+    __device__ static bool stage_wells(float4 *w, const TraceArgs &A, int64_t r)
     {
-        return v == kValHbm ? hbm + (int64_t)t * 32 : pool + v * 32;
-    }
-    // Tile t's value (pool tile or kValHbm), or kValNone when unmapped.
-    __device__ __forceinline__ uint32_t find(uint32_t t) const
-    {
-        uint32_t h = tile_hash(t);
-        for (int k = 0; k < kHash; ++k) {
-            const uint32_t e = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (e == kKeyEmpty)
-                return kValNone;
-            if ((e >> kValBits) == t)
-                return e & kValMask;
-            h = (h + 1) & (kHash - 1);
+        const double bn = A.b[r] * A.n[r];
+        // h = K b^2 / 2, the potential at which the head reaches the top of
+        // the aquifer (model.py:344)
+        const double h = A.confined ? 1.0 : 0.5 * A.K[r] * (A.b[r] * A.b[r]);
+        for (int i = threadIdx.x; i < A.nwells; i += kWave) {
+            const double qw = A.q[r * A.nwells + i];
+            w[i] = make_float4((float)(A.wells_xy[2 * i] - A.xo),
+                               (float)(A.wells_xy[2 * i + 1] - A.yo), (float)(qw * kInv2Pi / bn),
+                               A.confined ? 0.0f : (float)(qw * (kInv4Pi * kLn2) / h));
         }
-        return kValNone;
+        return true;
     }
-    // Maps tile t (or finds the mapping another lane made first); kValHbm
-    // when it lives in the HBM region, unmapped if the map is full.
-    __device__ __forceinline__ uint32_t claim(uint32_t t)
+
+    __device__ static void stage_consts(Consts &k, const TraceArgs &A, int64_t r, bool)
     {
-        const int v0 = atomicAdd(pool_next, 1);
-        const uint32_t v = v0 < pool_tiles ? (uint32_t)v0 : kValHbm;
-        const uint32_t want = (t << kValBits) | v;
-        uint32_t h = tile_hash(t);
-        for (int k = 0; k < kHash; ++k) {
-            const uint32_t prev = atomicCAS(&keys[h], kKeyEmpty, want);
-            if (prev == kKeyEmpty)
-                return v;
-            if ((prev >> kValBits) == t)   // another lane mapped it first
-                return prev & kValMask;
-            h = (h + 1) & (kHash - 1);
-        }
-        atomicOr(hash_full, 1);   // unmapped: the flush scans the HBM region
-        return kValHbm;
-    }
-    __device__ __forceinline__ uint32_t lookup(uint32_t t)
-    {
-        if (t == last_t)
-            return last_v;
-        const uint32_t v = find(t);
-        if (v != kValNone) {
-            last_t = t;
-            last_v = v;
-        }
-        return v;
-    }
-    __device__ __forceinline__ bool test(int row, int col)
-    {
-        const uint32_t t = (uint32_t)((row >> 5) * tiles_x + (col >> 5));
-        uint32_t v = lookup(t);
-        if (v == kValNone) {
-            if (!__hip_atomic_load(hash_full, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-                return false;
-            v = kValHbm;
-        }
-        return (tile_words(t, v)[row & 31] >> (col & 31)) & 1u;
-    }
-    __device__ __forceinline__ void set(int row, int col)
-    {
-        const uint32_t t = (uint32_t)((row >> 5) * tiles_x + (col >> 5));
-        uint32_t v = lookup(t);
-        if (v == kValNone) {
-            v = claim(t);
-            last_t = t;
-            last_v = v;
-        }
-        atomicOr(&tile_words(t, v)[row & 31], 1u << (col & 31));
-    }
-    __device__ __forceinline__ int stride() const { return 0; }
-    // Marks columns c0..c1 (inclusive) of one row, one tile word at a time.
-    __device__ __forceinline__ void mark(int row, int /*base*/, int c0, int c1)
-    {
-        const int tr = (row >> 5) * tiles_x, r = row & 31;
-        for (int c = c0; c <= c1;) {
-            const int b = c & 31;
-            const int n = min(c1 - c + 1, 32 - b);
-            const uint32_t m = span_mask(n, b);
-            const uint32_t t = (uint32_t)(tr + (c >> 5));
-            uint32_t v = lookup(t);
-            if (v == kValNone) {
-                v = claim(t);
-                last_t = t;
-                last_v = v;
-            }
-            uint32_t *w = tile_words(t, v) + r;
-            if ((*w & m) != m)
-                atomicOr(w, m);
-            c += n;
-        }
-    }
-};
-
-struct PoolLds {
-    SlotHdr *hdr;       // [slots]
-    float4 *wells;      // [slots][max_nwells]
-    uint32_t *keys;     // [slots][kHash]
-    uint32_t *pool;     // [slots][pool_tiles][32]
-};
-
-__device__ __forceinline__ PoolLds carve(const PoolArgs &P, void *lds)
-{
-    PoolLds L;
-    L.hdr = reinterpret_cast<SlotHdr *>(lds);
-    L.wells = reinterpret_cast<float4 *>(L.hdr + P.slots);
-    L.keys = reinterpret_cast<uint32_t *>(L.wells + (size_t)P.slots * P.max_nwells);
-    L.pool = L.keys + (size_t)P.slots * kHash;
-    return L;
-}
-
-__device__ __forceinline__ PoolBits slot_bits(const PoolArgs &P, const PoolLds &L, int s, int tiles_x)
-{
-    PoolBits b;
-    b.keys = L.keys + (size_t)s * kHash;
-    b.pool = L.pool + (size_t)s * P.pool_tiles * 32;
-    b.pool_next = &L.hdr[s].pool_next;
-    b.hash_full = &L.hdr[s].hash_full;
-    b.pool_tiles = P.pool_tiles;
-    b.hbm = P.hbm + ((int64_t)blockIdx.x * P.slots + s) * P.max_ntiles * 32;
-    b.tiles_x = tiles_x;
-    b.last_t = kKeyEmpty;
-    b.last_v = kValNone;
-    return b;
-}
-
-// Slot s's field and integrator, read from its header.
-__device__ __forceinline__ Field32 slot_field(const PoolArgs &P, const PoolLds &L, int s)
-{
-    const SlotHdr &h = L.hdr[s];
-    Field32 f;
-    f.w = L.wells + (size_t)s * P.max_nwells;
-    f.nw = h.nw;
-    f.a2 = h.a2;
-    f.b2 = h.b2;
-    f.c = h.c;
-    f.d = h.d;
-    f.e = h.e;
-    return f;
-}
-
-__device__ __forceinline__ Integ32 slot_integ(const SlotHdr &h)
-{
-    Integ32 I;
-    I.dur = h.dur;
-    I.absdur = h.absdur;
-    I.sg = h.sg;
-    I.tol = h.tol;
-    I.maxstep = h.maxstep;
-    I.cap = h.cap;
-    return I;
-}
-
-// Site and realization of queue item `item` (site_first: prefix sums of nreal).
-__device__ __forceinline__ int find_site(const PoolArgs &P, int64_t item)
-{
-    int lo = 0, hi = P.nsites - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (P.site_first[mid] <= item)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
-// Claims realization record h.rec for h.g (lane 0; the result is broadcast).
-__device__ __forceinline__ bool claim_record(const PoolArgs &P, const SlotHdr &h)
-{
-    int ok = 0;
-    if (threadIdx.x == 0) {
-        const unsigned long long prev = atomicCAS((unsigned long long *)&P.rec_owner[h.rec],
-                                                  ~0ull, (unsigned long long)h.g);
-        ok = prev == ~0ull || prev == (unsigned long long)h.g;
-    }
-    return __shfl(ok, 0) != 0;
-}
-
-// Opens slot s for queue item `item`: wells, regional coefficients, header.
-// A chunk of a realization of several chunks also claims the realization's
-// record (state 2 while an older realization still holds it).  Wave-uniform;
-// ends with a barrier.
-__device__ void open_slot(const PoolArgs &P, const PoolLds &L, int s, int64_t item)
-{
-    const int site = find_site(P, item);
-    const TraceArgs &A = P.sites[site];
-    const int nch = (A.npaths + P.chunk - 1) / P.chunk;
-    const int64_t local = item - P.site_first[site];
-    const int64_t r = nch > 0 ? local / nch : 0;
-    const int ch = nch > 0 ? (int)(local - r * nch) : 0;
-    const double bn = A.b[r] * A.n[r];
-    float4 *w = L.wells + (size_t)s * P.max_nwells;
-    for (int i = threadIdx.x; i < A.nwells; i += kWave) {
-        const double qw = A.q[r * A.nwells + i];
-        w[i] = make_float4((float)(A.wells_xy[2 * i] - A.xo), (float)(A.wells_xy[2 * i + 1] - A.yo),
-                           (float)(qw * 0.15915494309189535 / bn), 0.0f);
-    }
-    if (threadIdx.x == 0) {
-        SlotHdr &h = L.hdr[s];
+        const double bn = A.b[r] * A.n[r];
+        const double h = A.confined ? 1.0 : 0.5 * A.K[r] * (A.b[r] * A.b[r]);
         const double *c = A.coef + 6 * r;
-        h.a2 = (float)(2.0 * c[0] / bn);
-        h.b2 = (float)(2.0 * c[1] / bn);
-        h.c = (float)(c[2] / bn);
-        h.d = (float)(c[3] / bn);
-        h.e = (float)(c[4] / bn);
-        h.att = h.acc = h.esc = h.cand = h.vis = 0;
-        for (int j = 0; j < 4; ++j)
-            h.st[j] = 0;
-        h.r = r;
-        h.g = P.real_first[site] + r;
-        h.site = site;
-        h.next = ch * P.chunk;
-        h.np = min(A.npaths, h.next + P.chunk);
-        h.rem = h.np - h.next;
-        h.nchunks = nch;
-        h.rec = nch > 1 ? (int)(h.g % P.nrec) : -1;
-        h.pool_next = 0;
-        h.hash_full = 0;
-        h.dur = A.duration;
-        h.absdur = fabs(A.duration);
-        h.sg = A.duration > 0.0 ? 1.0
+        k.a2 = (float)(2.0 * c[0] / bn);
+        k.b2 = (float)(2.0 * c[1] / bn);
+        k.c = (float)(c[2] / bn);
+        k.d = (float)(c[3] / bn);
+        k.e = (float)(c[4] / bn);
+        k.pa = (float)(c[0] / h);
+        k.pb = (float)(c[1] / h);
+        k.pc = (float)(c[2] / h);
+        k.pd = (float)(c[3] / h);
+        k.pe = (float)(c[4] / h);
+        k.pf = (float)(c[5] / h);
+        k.dur = A.duration;
+        k.absdur = fabs(A.duration);
+        k.sg = A.duration > 0.0 ? 1.0
                                 : (A.duration < 0.0 ? -1.0 : (A.duration == 0.0 ? 0.0 : (double)NAN));
-        h.cap = A.max_steps;
-        h.tol = (float)A.tol;
-        h.maxstep = (float)A.maxstep;
-        h.gdx = (float)A.gdx;
-        h.gdy = (float)A.gdy;
-        h.igdx = (float)(1.0 / A.gdx);
-        h.igdy = (float)(1.0 / A.gdy);
-        h.umbra = (float)A.umbra;
-        h.u2 = h.umbra * h.umbra;
-        h.nw = A.nwells;
-        h.raster = A.rasterize != 0;
-        h.tiles_x = A.tiles_x;
+        k.cap = A.max_steps;
+        k.tol = (float)A.tol;
+        k.maxstep = (float)A.maxstep;
+        k.gdx = (float)A.gdx;
+        k.gdy = (float)A.gdy;
+        k.igdx = (float)(1.0 / A.gdx);
+        k.igdy = (float)(1.0 / A.gdy);
+        k.umbra = (float)A.umbra;
+        k.u2 = k.umbra * k.umbra;
+        k.gx0 = (float)(A.gxmin - A.xo);
+        k.gx1 = (float)(A.gxmax - A.xo);
+        k.gy0 = (float)(A.gymin - A.yo);
+        k.gy1 = (float)(A.gymax - A.yo);
+        k.xo = A.xo;
+        k.yo = A.yo;
     }
-    __syncthreads();
-    const bool ready = L.hdr[s].nchunks <= 1 || claim_record(P, L.hdr[s]);
-    if (threadIdx.x == 0)
-        L.hdr[s].state = ready ? 1 : 2;
-    __syncthreads();
-}
 
-// Flushes slot s into its site's counts and statistics and frees it.
-// Wave-uniform; ends with a barrier.
-__device__ void flush_slot(const PoolArgs &P, const PoolLds &L, int s)
-{
-    SlotHdr &h = L.hdr[s];
-    const TraceArgs &A = P.sites[h.site];
-    const PoolBits b = slot_bits(P, L, s, A.tiles_x);
-    unsigned long long vis = 0;
-    if (A.rasterize) {
-        for (int k = threadIdx.x; k < kHash * 32; k += kWave) {
-            const uint32_t e = b.keys[k >> 5];
-            if (e == kKeyEmpty)
-                continue;
-            const uint32_t t = e >> kValBits;
-            uint32_t *w = b.tile_words(t, e & kValMask) + (k & 31);
-            const uint32_t bits = *w;
-            if (!bits)
-                continue;
-            *w = 0u;
-            vis += count_bits(A, (int)(t / A.tiles_x) * 32 + (k & 31), (int)(t % A.tiles_x) * 32,
-                              bits, nullptr);
-        }
-        if (h.hash_full) {   // unmapped tiles: scan the whole HBM region
-            for (int k = threadIdx.x; k < A.ntiles * 32; k += kWave) {
-                const uint32_t bits = b.hbm[k];
-                if (!bits)
-                    continue;
-                b.hbm[k] = 0u;
-                const int t = k >> 5;
-                vis += count_bits(A, (t / A.tiles_x) * 32 + (k & 31), (t % A.tiles_x) * 32, bits,
-                                  nullptr);
-            }
-        }
+    __device__ static Integ32 integ(const Consts &k)
+    {
+        Integ32 I;
+        I.dur = k.dur;
+        I.absdur = k.absdur;
+        I.sg = k.sg;
+        I.tol = k.tol;
+        I.maxstep = k.maxstep;
+        I.cap = k.cap;
+        return I;
     }
-    for (int off = 32; off > 0; off >>= 1)
-        vis += __shfl_xor(vis, off);
-    __syncthreads();
-    for (int k = threadIdx.x; k < kHash; k += kWave)
-        b.keys[k] = kKeyEmpty;
-    if (threadIdx.x == 0) {
-        if (A.stats) {
-            atomicAdd(&A.stats[kStatAttempted], h.att);
-            atomicAdd(&A.stats[kStatAccepted], h.acc);
-            atomicAdd(&A.stats[kStatEscaped], h.esc);
-            atomicAdd(&A.stats[kStatCandidates], h.cand);
-            atomicAdd(&A.stats[kStatVisited], vis);
-            for (int j = 0; j < 4; ++j)
-                atomicAdd(&A.stats[kStatStatus0 + j], h.st[j]);
+
+    __device__ static Field32 field(const Consts &k, const float4 *w, int nw, bool confined)
+    {
+        Field32 f;
+        f.w = w;
+        f.nw = nw;
+        f.a2 = k.a2;
+        f.b2 = k.b2;
+        f.c = k.c;
+        f.d = k.d;
+        f.e = k.e;
+        if (!confined) {
+            f.pa = k.pa;
+            f.pb = k.pb;
+            f.pc = k.pc;
+            f.pd = k.pd;
+            f.pe = k.pe;
+            f.pf = k.pf;
         }
-        h.pool_next = 0;
-        h.hash_full = 0;
-        h.state = 0;
+        return f;
     }
-    __syncthreads();
-}
 
-// A finished chunk of a realization of several chunks: ORs the chunk's marked
-// words into the realization record and sets its dirty-tile bits; the wave
-// that merges the last chunk flushes the record into the counts and frees
-// it.  Wave-uniform; ends with a barrier.
-__device__ void merge_slot(const PoolArgs &P, const PoolLds &L, int s)
-{
-    SlotHdr &h = L.hdr[s];
-    const TraceArgs &A = P.sites[h.site];
-    const PoolBits b = slot_bits(P, L, s, A.tiles_x);
-    uint32_t *rb = P.rec_bits + (int64_t)h.rec * P.max_ntiles * 32;
-    uint32_t *rd = P.rec_dirty + (int64_t)h.rec * P.dirty_words;
-    if (A.rasterize) {
-        for (int k = threadIdx.x; k < kHash * 32; k += kWave) {
-            const uint32_t e = b.keys[k >> 5];
-            if (e == kKeyEmpty)
-                continue;
-            const uint32_t t = e >> kValBits;
-            uint32_t *w = b.tile_words(t, e & kValMask) + (k & 31);
-            const uint32_t bits = *w;
-            if (!bits)
-                continue;
-            *w = 0u;
-            atomicOr(&rb[(int64_t)t * 32 + (k & 31)], bits);
-            atomicOr(&rd[t >> 5], 1u << (t & 31));
-        }
-        if (h.hash_full) {
-            for (int k = threadIdx.x; k < A.ntiles * 32; k += kWave) {
-                const uint32_t bits = b.hbm[k];
-                if (!bits)
-                    continue;
-                b.hbm[k] = 0u;
-                atomicOr(&rb[k], bits);
-                atomicOr(&rd[k >> 10], 1u << ((k >> 5) & 31));
-            }
-        }
+    __device__ static void open(Lane &ln, const Consts &k, const TraceArgs &A, int64_t r, int p,
+                                bool raster)
+    {
+        if (raster)
+            ln.ep = find_epoch32(A, r * A.npaths + p);
+        ln.cand = 0;
+        ln.last_t = kKeyEmpty;
+        ln.last_v = kValNone;
+        open32(ln.tr, integ(k), (float)(A.start_xy[2 * p] - A.xo),
+               (float)(A.start_xy[2 * p + 1] - A.yo));
+        ln.fresh = true;
+        ln.a_done = false;
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < kHash; k += kWave)
-        b.keys[k] = kKeyEmpty;
-    if (threadIdx.x == 0) {
-        if (A.stats) {
-            atomicAdd(&A.stats[kStatAttempted], h.att);
-            atomicAdd(&A.stats[kStatAccepted], h.acc);
-            atomicAdd(&A.stats[kStatEscaped], h.esc);
-            atomicAdd(&A.stats[kStatCandidates], h.cand);
-            for (int j = 0; j < 4; ++j)
-                atomicAdd(&A.stats[kStatStatus0 + j], h.st[j]);
+
+    // One field evaluation per live lane: k1 of a fresh trace (which costs
+    // the wave no evaluation of its own), else the first stage of an
+    // attempted step and then the rest of the step.
+    template <bool kConfined>
+    __device__ static int advance(Lane &ln, const Consts &k, const float4 *w, int nw,
+                                  PoolBits &bits, bool raster)
+    {
+        const Field32 f = field(k, w, nw, kConfined);
+        const Integ32 I = integ(k);
+        double dtd;
+        float dt, px, py, kx, ky;
+        bool dry = false;   // an evaluation raised AquiferError (unconfined only)
+        first_point(I, ln.tr, ln.fresh, dtd, dt, px, py);
+        feval32<kConfined>(f, px, py, kx, ky, dry);
+        if (ln.fresh) {
+            ln.fresh = false;
+            if (dry) {   // k1 raised: the trace is its start vertex (capturezone.py:218, 249)
+                ln.tr.status = OKA_TRACE_TRUNCATED;
+                return 2;
+            }
+            return take_k1(I, ln.tr, kx, ky) ? 0 : 2;
         }
-        h.pool_next = 0;
-        h.hash_full = 0;
+        float ox, oy;
+        const int res = finish32(LaneEval<kConfined>{f, dry}, I, ln.tr, dtd, dt, kx, ky, ox, oy);
+        // a stage raised: the step is not accepted and the trace ends
+        if (!kConfined && dry)
+            ln.tr.status = OKA_TRACE_TRUNCATED;
+        if ((res & 1) && raster) {
+            bits.last_t = ln.last_t;
+            bits.last_v = ln.last_v;
+            ln.cand += (unsigned long long)raster32(bits, ln.ep, k.gdx, k.gdy, k.igdx, k.igdy,
+                                                    k.umbra, k.u2, ox, oy, ln.tr.x, ln.tr.y,
+                                                    !ln.a_done, ln.a_done);
+            ln.last_t = bits.last_t;
+            ln.last_v = bits.last_v;
+        }
+        return res;
     }
-    // publish the chunk's marks before counting it
-    __threadfence();
-    int last = 0;
-    if (threadIdx.x == 0)
-        last = atomicAdd(&P.rec_done[h.rec], 1) == h.nchunks - 1;
-    last = __shfl(last, 0);
-    if (last) {
-        __threadfence();
-        unsigned long long vis = 0;
-        if (A.rasterize) {
-            const int dwords = (A.ntiles + 31) / 32;
-            for (int dw = 0; dw < dwords; ++dw) {
-                uint32_t dbits = __hip_atomic_load(&rd[dw], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                while (dbits) {
-                    const int t = dw * 32 + __ffs(dbits) - 1;
-                    dbits &= dbits - 1u;
-                    if (threadIdx.x < 32) {
-                        uint32_t *w = &rb[(int64_t)t * 32 + threadIdx.x];
-                        const uint32_t bits = __hip_atomic_load(w, __ATOMIC_RELAXED,
-                                                                __HIP_MEMORY_SCOPE_AGENT);
-                        if (bits) {
-                            __hip_atomic_store(w, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            vis += count_bits(A, (t / A.tiles_x) * 32 + (int)threadIdx.x,
-                                              (t % A.tiles_x) * 32, bits, nullptr);
-                        }
-                    }
-                }
-                if (threadIdx.x == 0)
-                    __hip_atomic_store(&rd[dw], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
-        for (int off = 32; off > 0; off >>= 1)
-            vis += __shfl_xor(vis, off);
-        if (threadIdx.x == 0) {
-            if (A.stats)
-                atomicAdd(&A.stats[kStatVisited], vis);
-            __hip_atomic_store(&P.rec_done[h.rec], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        // the record is clean before another realization may claim it
-        __threadfence();
-        if (threadIdx.x == 0)
-            atomicExch((unsigned long long *)&P.rec_owner[h.rec], ~0ull);
+
+    __device__ static void summary(const Lane &ln, const TraceArgs &A, long long &att,
+                                   long long &acc, unsigned long long &cand, int &status,
+                                   double bb[4], bool &escaped)
+    {
+        const Trace32 &t = ln.tr;
+        att = t.att;
+        acc = t.acc;
+        cand = ln.cand;
+        status = t.status;
+        bb[0] = A.xo + (double)t.bx0;
+        bb[1] = A.xo + (double)t.bx1;
+        bb[2] = A.yo + (double)t.by0;
+        bb[3] = A.yo + (double)t.by1;
+        const float gx0 = (float)(A.gxmin - A.xo), gx1 = (float)(A.gxmax - A.xo);
+        const float gy0 = (float)(A.gymin - A.yo), gy1 = (float)(A.gymax - A.yo);
+        escaped = !(t.bx0 > gx0 && t.bx1 < gx1 && t.by0 > gy0 && t.by1 < gy1);
     }
-    if (threadIdx.x == 0)
-        h.state = 0;
-    __syncthreads();
-}
-
-// At most 128 VGPRs (one 64-bit value spills, in the flush's scan of an
-// overfull map), so that 16 waves fit a CU when the LDS allows them.
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(4)))
-k_trace_pool32(PoolArgs P)
-{
-    extern __shared__ __align__(16) unsigned char pool_lds[];
-    const PoolLds L = carve(P, pool_lds);
-    const int lane = threadIdx.x;
-    for (int s = 0; s < P.slots; ++s) {
-        if (lane == 0) {
-            L.hdr[s].state = 0;
-            L.hdr[s].pool_next = 0;
-            L.hdr[s].hash_full = 0;
-        }
-        for (int k = lane; k < kHash; k += kWave)
-            L.keys[(size_t)s * kHash + k] = kKeyEmpty;
-    }
-    for (int k = lane; k < P.slots * P.pool_tiles * 32; k += kWave)
-        L.pool[k] = 0u;
-    __syncthreads();
-
-    // the lane's trace; its slot's constants stay in the slot header
-    bool live = false, fresh = false;
-    bool a_done = false;   // the disk around the trace's current point is marked
-    int slot = 0, pidx = 0;
-    unsigned long long cand = 0;
-    Trace32 tr;
-    Epoch32 ep;
-    uint32_t last_t = kKeyEmpty, last_v = kValNone;   // the lane's tile cache
-    // wave-uniform
-    bool queue_done = false;
-    int intake = -1;
-
-    for (;;) {
-        // A. idle lanes take particles from the intake slot, opening slots as needed
-        bool ended = false;
-        unsigned long long need = __ballot(!live);
-        while (need != 0ull) {
-            if (intake >= 0 && L.hdr[intake].state == 2) {   // its record is still busy
-                const bool ok = claim_record(P, L.hdr[intake]);
-                if (threadIdx.x == 0 && ok)
-                    L.hdr[intake].state = 1;
-                __syncthreads();
-                if (!ok)
-                    break;
-            }
-            if (intake >= 0 && L.hdr[intake].next < L.hdr[intake].np) {
-                const int next = L.hdr[intake].next;
-                const int avail = L.hdr[intake].np - next;
-                const int rank = __popcll(need & ((1ull << lane) - 1ull));
-                const int took = min(__popcll(need), avail);
-                if (((need >> lane) & 1ull) && rank < avail) {
-                    slot = intake;
-                    pidx = next + rank;
-                    const SlotHdr &h = L.hdr[slot];
-                    const TraceArgs &A = P.sites[h.site];
-                    if (h.raster)
-                        ep = find_epoch32(A, h.r * A.npaths + pidx);
-                    cand = 0;
-                    last_t = kKeyEmpty;
-                    last_v = kValNone;
-                    open32(tr, slot_integ(h), (float)(A.start_xy[2 * pidx] - A.xo),
-                           (float)(A.start_xy[2 * pidx + 1] - A.yo));
-                    live = fresh = true;
-                    a_done = false;
-                }
-                __syncthreads();
-                if (lane == 0)
-                    L.hdr[intake].next = next + took;
-                __syncthreads();
-                need = __ballot(!live && !ended);
-                continue;
-            }
-            if (queue_done)
-                break;
-            int fs = -1;
-            for (int s = 0; s < P.slots; ++s)
-                if (fs < 0 && L.hdr[s].state == 0)
-                    fs = s;
-            if (fs < 0)
-                break;
-            unsigned long long item = 0ull;
-            if (lane == 0)
-                item = atomicAdd(P.queue, 1ull);
-            item = __shfl(item, 0);
-            if (item >= (unsigned long long)P.total) {
-                queue_done = true;
-                break;
-            }
-            open_slot(P, L, fs, (int64_t)item);
-            intake = fs;
-        }
-
-        // B. one field evaluation per live lane (k1 of a fresh trace, else the
-        // first stage of an attempted step), then the rest of the step
-        if (live) {
-            const SlotHdr &h = L.hdr[slot];
-            const Field32 f = slot_field(P, L, slot);
-            const Integ32 I = slot_integ(h);
-            double dtd;
-            float dt, px, py, kx, ky;
-            first_point(I, tr, fresh, dtd, dt, px, py);
-            feval32(f, px, py, kx, ky);
-            if (fresh) {
-                fresh = false;
-                if (!take_k1(I, tr, kx, ky)) {
-                    live = false;
-                    ended = true;
-                }
-            } else {
-                float ox, oy;
-                bool dry = false;   // set by the unconfined field only
-                const int res = finish32(LaneEval<>{f, dry}, I, tr, dtd, dt, kx, ky, ox, oy);
-                if ((res & 1) && h.raster) {
-                    PoolBits bits = slot_bits(P, L, slot, h.tiles_x);
-                    bits.last_t = last_t;
-                    bits.last_v = last_v;
-                    cand += (unsigned long long)raster32(bits, ep, h.gdx, h.gdy, h.igdx, h.igdy,
-                                                         h.umbra, h.u2, ox, oy, tr.x, tr.y,
-                                                         !a_done, a_done);
-                    last_t = bits.last_t;
-                    last_v = bits.last_v;
-                }
-                if (res & 2) {
-                    live = false;
-                    ended = true;
-                }
-            }
-        }
-        __syncthreads();
-
-        // C. ended traces: statistics, per-trace outputs, the slot's count down
-        if (ended) {
-            SlotHdr &h = L.hdr[slot];
-            const TraceArgs &A = P.sites[h.site];
-            atomicAdd(&h.att, (unsigned long long)tr.att);
-            atomicAdd(&h.acc, (unsigned long long)tr.acc);
-            atomicAdd(&h.cand, cand);
-            atomicAdd(&h.st[tr.status & 3], 1ull);
-            const float gx0 = (float)(A.gxmin - A.xo), gx1 = (float)(A.gxmax - A.xo);
-            const float gy0 = (float)(A.gymin - A.yo), gy1 = (float)(A.gymax - A.yo);
-            if (!(tr.bx0 > gx0 && tr.bx1 < gx1 && tr.by0 > gy0 && tr.by1 < gy1))
-                atomicAdd(&h.esc, 1ull);
-            const int64_t k = h.r * A.npaths + pidx;
-            if (A.bbox) {
-                A.bbox[4 * k] = A.xo + (double)tr.bx0;
-                A.bbox[4 * k + 1] = A.xo + (double)tr.bx1;
-                A.bbox[4 * k + 2] = A.yo + (double)tr.by0;
-                A.bbox[4 * k + 3] = A.yo + (double)tr.by1;
-            }
-            if (A.trace_status)
-                A.trace_status[k] = tr.status;
-            if (A.trace_steps)
-                A.trace_steps[k] = tr.att;
-            atomicSub(&h.rem, 1);
-        }
-        __syncthreads();
-
-        // D. flush finished realizations
-        bool open = false;
-        for (int s = 0; s < P.slots; ++s) {
-            const SlotHdr &h = L.hdr[s];
-            if (h.state == 1 && h.next == h.np && h.rem == 0) {
-                if (h.nchunks > 1)
-                    merge_slot(P, L, s);
-                else
-                    flush_slot(P, L, s);
-                if (intake == s)
-                    intake = -1;
-            }
-            open = open || L.hdr[s].state != 0;
-        }
-        if (queue_done && !open && __ballot(live) == 0ull)
-            break;
-    }
-}
-
-static_assert(sizeof(SlotHdr) % 16 == 0, "slot headers keep the wells 16-byte aligned");
+};
 
 }  // namespace
 
-size_t pool_lds_bytes(int slots, int max_nwells, int pool_tiles)
+size_t pool_lds_bytes(int precision, int slots, int max_nwells, int pool_tiles)
 {
-    return (size_t)slots * (sizeof(SlotHdr) + (size_t)max_nwells * sizeof(float4) +
-                            (size_t)kHash * 4 + (size_t)pool_tiles * 128);
+    if (precision == OKA_FP64_EXACT)
+        return pool64_lds_bytes(slots, max_nwells, pool_tiles);
+    return pool::lds_bytes<PoolFP32>(slots, max_nwells, pool_tiles);
 }
 
-int pool_max_tiles() { return (int)((1u << (32 - kValBits)) - 1u); }
-int pool_max_pool_tiles() { return (int)kValHbm; }
+int pool_max_tiles() { return (int)((1u << (32 - pool::kValBits)) - 1u); }
+int pool_max_pool_tiles() { return (int)pool::kValHbm; }
 
-hipError_t launch_trace_pool32(const PoolArgs &p, int waves, hipStream_t s)
+hipError_t launch_trace_pool(const PoolArgs &p, int waves, int precision, int conf_mode,
+                             hipStream_t s)
 {
-    const size_t lds = pool_lds_bytes(p.slots, p.max_nwells, p.pool_tiles);
-    hipLaunchKernelGGL(k_trace_pool32, dim3(waves), dim3(kWave), lds, s, p);
-    return hipGetLastError();
+    if (precision == OKA_FP64_EXACT)
+        return launch_trace_pool64(p, waves, conf_mode, s);
+    return pool::launch<PoolFP32>(p, waves, conf_mode, s);
 }
 
 }  // namespace oka

@@ -171,3 +171,45 @@ def test_config2_sites_fp64_exact_match_oracle(gpu_ctx):
         assert list(st.status_count) == list(ost[2:6])
         assert counts.sum() > 0
     b.close()
+
+
+def test_fp64_batch_matches_oracle(gpu_ctx):
+    """A batch in FP64-exact (the pool tracer in the workgroup tracer's
+    arithmetic): every site's counts and statistics equal the oracle's on the
+    batch's own fits, which equal the oracle's fits."""
+    from helpers import oracle_chronicle, oracle_fit_draw
+    ss, draws = _cfg2_subset((44, 33, 7), 16)
+    b = SiteBatch(ss, draws, precision=_lib.FP64_EXACT)
+    b.run()
+    for i, (s, d) in enumerate(zip(ss, draws)):
+        print(f"{s.name}: fp64 batch vs oracle", flush=True)
+        counts, coef, st = b.results(i)
+        _, _, ocoef = oracle_fit_draw(s, d)
+        assert np.array_equal(coef, ocoef)
+        ocounts, ost = oracle_chronicle(s, d, coef, s.fixed_field())
+        assert np.array_equal(counts, ocounts)
+        assert (st.attempted, st.accepted, st.visited, st.escaped) == (ost[0], ost[1], ost[7],
+                                                                       ost[6])
+        assert list(st.status_count) == list(ost[2:6])
+    b.close()
+
+
+@pytest.mark.parametrize("precision", [_lib.FP32, _lib.FP64_EXACT], ids=["fp32", "fp64"])
+def test_batch_mixes_confined_and_unconfined_sites(gpu_ctx, precision):
+    """Sites tracing the unconfined velocity beside confined ones in one batch
+    (the pool's per-slot choice of field): each site's counts and statistics
+    equal that site traced alone with its own field."""
+    ss, draws = _cfg2_subset((44, 33, 7, 12), 24)
+    ss[1].confined = False
+    ss[3].confined = False
+    b = SiteBatch(ss, draws, precision=precision)
+    b.run()
+    for i, (s, d) in enumerate(zip(ss, draws)):
+        print(f"{s.name}: confined {s.confined}", flush=True)
+        counts, coef, st = b.results(i)
+        ref, ref_st = trace_counts(s, d, coef, precision)
+        assert np.array_equal(counts, ref)
+        assert (st.attempted, st.accepted, st.visited) == (ref_st.attempted, ref_st.accepted,
+                                                            ref_st.visited)
+        assert list(st.status_count) == list(ref_st.status_count)
+    b.close()

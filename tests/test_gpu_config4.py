@@ -54,11 +54,16 @@ def test_config4_fit_and_draw_bit_exact(gpu_ctx, cfg4_oracle):
     assert np.array_equal(gcoef, coef)
 
 
-@pytest.mark.parametrize("mode", [_lib.BITMAP_AUTO, _lib.BITMAP_HBM_FLAT], ids=["tiled", "hbm_flat"])
-def test_config4_fp64_exact_matches_oracle(gpu_ctx, cfg4_oracle, mode):
+CFG4_RUNS = [("tiled", dict(bitmap_mode=_lib.BITMAP_AUTO)),
+             ("hbm_flat", dict(bitmap_mode=_lib.BITMAP_HBM_FLAT)),
+             ("pool", dict(schedule=_lib.SCHED_POOL))]
+
+
+@pytest.mark.parametrize("name,kw", CFG4_RUNS, ids=[c[0] for c in CFG4_RUNS])
+def test_config4_fp64_exact_matches_oracle(gpu_ctx, cfg4_oracle, name, kw):
     s, d, coef, ocounts, ost = cfg4_oracle
-    print(f"cfg4 fp64-exact, bitmap mode {mode}", flush=True)
-    res = _run(s, d, coef, precision=_lib.FP64_EXACT, bitmap_mode=mode)
+    print(f"cfg4 fp64-exact, {name}", flush=True)
+    res = _run(s, d, coef, precision=_lib.FP64_EXACT, **kw)
     assert res.passes == 1 and ost[6] == 0
     assert np.array_equal(res.counts, ocounts)
     st = res.stats

@@ -46,6 +46,45 @@ def test_trace_status_cases_match_oracle(gpu_ctx, case):
     assert np.array_equal(verts[0, :nverts[0]], ov)
 
 
+DIV_CASES = {
+    # the start point shares its x (then its y) with a well: zero numerators
+    "same_x_as_well": ([(0.0, 0.0, 0.25, 500.0), (30.0, 0.0, 0.25, 200.0),
+                        (-40.0, 25.0, 0.25, -150.0)], (30.0, 7.5)),
+    "same_y_as_well": ([(0.0, 0.0, 0.25, 500.0), (30.0, 0.0, 0.25, 200.0),
+                        (-40.0, 25.0, 0.25, -150.0)], (-12.0, 25.0)),
+    # coordinates beyond 2^60: every evaluation takes the scaled divisions
+    "beyond_2p60": ([(1.5e18, 0.0, 0.25, 500.0), (1.5e18 + 4096.0, 2048.0, 0.25, 200.0)],
+                    (1.5e18 + 1024.0, 512.0)),
+    # a discharge below 2^-100 and a well coordinate below 2^-200: the
+    # realization takes the scaled divisions
+    "tiny_strength": ([(0.0, 0.0, 0.25, 500.0), (30.0, 0.0, 0.25, 1e-40)], (1.25, 0.5)),
+    "tiny_coordinate": ([(0.0, 0.0, 0.25, 500.0), (1e-70, 40.0, 0.25, 200.0)], (1.25, 0.5)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(DIV_CASES))
+@pytest.mark.parametrize("confined", [True, False], ids=["confined", "unconfined"])
+def test_fp64_division_paths_bit_exact_vs_oracle(gpu_ctx, case, confined):
+    """The FP64 tracer divides with the short sequence where no operand can
+    need div_scale's scaling and with the full one elsewhere; both give the
+    oracle's vertices bit for bit, including zero numerators (a point on a
+    well's grid line) and points, wells and discharges outside the short
+    sequence's range."""
+    wells, start = DIV_CASES[case]
+    coef = (1e-5, -2e-5, 0.0, 1e-2, -5e-3, 2000.0)
+    xo, yo = wells[0][0], wells[0][1]
+    w = np.array(wells, dtype=float)
+    real = Realizations([10.0], [0.25], [20.0], w[:, 3].reshape(1, -1), np.array(coef, float))
+    om = O.OracleModel(0.0, 10.0, 0.25, 20.0, w, xo, yo, coef)
+    verts, nverts, status = trace_vertices(np.ascontiguousarray(w[:, :2]), xo, yo, real,
+                                           np.array([start]), 400.0, 1.0, 10.0, confined, 0)
+    ov, res = om.backtrace(start[0], start[1], 400.0, 1.0, 10.0, confined=confined)
+    assert status[0] == res.status
+    assert nverts[0] == res.nverts > 3
+    assert np.array_equal(verts[0, :nverts[0]], ov)
+    assert np.array_equal(np.signbit(verts[0, :nverts[0]]), np.signbit(ov))
+
+
 def test_empty_batches_are_noops(gpu_ctx):
     s, d = small_site(nreal=2, npaths=4)
     _, _, coef = oracle_fit_draw(s, d)
@@ -299,8 +338,9 @@ def test_pool_schedule_rejects_what_does_not_fit(gpu_ctx):
     with pytest.raises(_lib.OkaError, match="do not fit in LDS"):
         chronicle_into_field(*args, precision=_lib.FP32, schedule=_lib.SCHED_POOL,
                              pool_slots=8, pool_waves_per_cu=32)
-    with pytest.raises(_lib.OkaError, match="FP32"):
-        chronicle_into_field(*args, precision=_lib.FP64_EXACT, schedule=_lib.SCHED_POOL)
+    with pytest.raises(_lib.OkaError, match="do not fit in LDS"):
+        chronicle_into_field(*args, precision=_lib.FP64_EXACT, schedule=_lib.SCHED_POOL,
+                             pool_slots=16, pool_waves_per_cu=32)
 
 
 def test_pool_schedule_many_tiles_and_uneven_lifetimes(gpu_ctx):

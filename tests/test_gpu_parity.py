@@ -226,46 +226,103 @@ def test_fp32_error_cfg0(gpu_ctx):
     assert np.abs(p32 - p64).max() <= 0.01
 
 
-@pytest.mark.parametrize("shift", [0.0, 0.7, 1e6])
-def test_fp32_unconfined_error_cfg0(gpu_ctx, shift):
+def _dry_at_start(status, P):
+    """Realizations all of whose traces ended truncated (AquiferError)."""
+    return set(np.nonzero((status.reshape(-1, P) == _lib.TRACE_TRUNCATED).all(axis=1))[0])
+
+
+UNCONF_CASES = [(0.0, _lib.BITMAP_AUTO, 0), (0.7, _lib.BITMAP_AUTO, 0), (1e6, _lib.BITMAP_AUTO, 0),
+                (0.7, _lib.BITMAP_TILED, 1), (0.7, _lib.BITMAP_TILED, 3),
+                (0.7, _lib.BITMAP_HBM_FLAT, 0)]
+
+
+@pytest.mark.parametrize("shift,mode,slots", UNCONF_CASES,
+                         ids=[f"{c[0]:g}-mode{c[1]}-slots{c[2]}" for c in UNCONF_CASES])
+def test_fp32_unconfined_error_cfg0(gpu_ctx, shift, mode, slots):
     """The unconfined velocity (model.py:318-389) on cfg 0, whose potentials
     straddle K b^2 / 2 (the head is below the aquifer top in some realizations
     and above it in others).  F is lowered by shift * K b^2 / 2: at 0.7 a fifth
     of the realizations are dry at their start points (AquiferError, status
     truncated), at 10^6 every one is.  The FP64-exact mode is bit-exact against
-    the oracle; the FP32 mode is within 0.01 of it in capture probability and
-    has the same status counts to 1 % of the traces."""
+    the oracle; the FP32 mode is within 0.01 of it in capture probability, ends
+    the same traces the same way (per trace, to 0.1 %) and finds the same
+    realizations dry at their start points.  The bitmap placements (flat LDS,
+    tiled with 1 or 3 LDS slots, flat HBM) give the same FP32 counts."""
+    print(f"unconfined shift {shift} bitmap {mode} slots {slots}", flush=True)
     s = sites.config0()
     d = sites.draw(s, seed=11)
     _, _, coef = oracle_fit_draw(s, d)
     coef[:, 5] -= shift * 0.5 * d["K"] * d["b"] ** 2
     real = _site_real(s, d, coef)
     starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    P, ntr = len(starts), len(starts) * s.nrealizations
     g = oracle_grid(s.fixed_field())
     out, status = {}, {}
     for prec in (_lib.FP64_EXACT, _lib.FP32):
         pf = s.fixed_field()
         res = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
-                                   s.tol, s.maxstep, s.umbra, confined=False, precision=prec)
+                                   s.tol, s.maxstep, s.umbra, confined=False, precision=prec,
+                                   bitmap_mode=mode, tile_slots=slots)
         assert res.counts.shape == (g.nrows, g.ncols)   # no trace left the fixed grid
         out[prec], status[prec] = res.counts, res.trace_status
+    if mode != _lib.BITMAP_AUTO:
+        pf = s.fixed_field()
+        flat = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
+                                    s.tol, s.maxstep, s.umbra, confined=False,
+                                    precision=_lib.FP32, bitmap_mode=_lib.BITMAP_LDS_FLAT)
+        assert np.array_equal(out[_lib.FP32], flat.counts)
+        assert np.array_equal(status[_lib.FP32], flat.trace_status)
     oc, ost = oracle_chronicle(s, d, coef, s.fixed_field(), confined=False)
     assert np.array_equal(out[_lib.FP64_EXACT], oc)
     n64 = np.bincount(status[_lib.FP64_EXACT], minlength=4)
     assert np.array_equal(n64, ost[2:6])
     n32 = np.bincount(status[_lib.FP32], minlength=4)
-    assert np.abs(n32 - n64).max() <= 0.01 * len(starts) * s.nrealizations
+    assert (status[_lib.FP32] != status[_lib.FP64_EXACT]).sum() <= 0.001 * ntr
+    assert _dry_at_start(status[_lib.FP32], P) == _dry_at_start(status[_lib.FP64_EXACT], P)
     p64 = out[_lib.FP64_EXACT] / s.nrealizations
     p32 = out[_lib.FP32] / s.nrealizations
     assert np.abs(p32 - p64).max() <= 0.01
     if shift == 0.0:
+        assert not _dry_at_start(status[_lib.FP64_EXACT], P)
         # the unconfined field is not the confined one on this site
         pf = s.fixed_field()
         conf = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration,
                                     s.tol, s.maxstep, s.umbra, precision=_lib.FP32).counts
         assert np.abs(conf / s.nrealizations - p32).max() > 0.05
     elif shift == 0.7:
-        assert 0 < n64[_lib.TRACE_TRUNCATED] < len(starts) * s.nrealizations
+        assert 0 < n64[_lib.TRACE_TRUNCATED] < ntr
+        assert 0 < len(_dry_at_start(status[_lib.FP64_EXACT], P)) < s.nrealizations
     else:
-        assert n32[_lib.TRACE_TRUNCATED] == n64[_lib.TRACE_TRUNCATED] == len(starts) * s.nrealizations
+        assert n32[_lib.TRACE_TRUNCATED] == n64[_lib.TRACE_TRUNCATED] == ntr
         assert out[_lib.FP32].sum() == 0 and oc.sum() == 0
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64_EXACT, _lib.FP32], ids=["fp64", "fp32"])
+def test_unconfined_window_marks_match_counts(gpu_ctx, precision):
+    """Window marks of the unconfined field (F lowered as above, a fifth of the
+    realizations dry): per realization they equal the cell sums of that
+    realization's own node counts, and summed over realizations the counts of
+    the chronicle launch."""
+    from onekapy_amd.surrogate import trace_marks, _window
+    s = sites.config0()
+    s.nrealizations = 24
+    s.confined = False
+    d = sites.draw(s, seed=11)
+    _, _, coef = oracle_fit_draw(s, d)
+    coef[:, 5] -= 0.7 * 0.5 * d["K"] * d["b"] ** 2
+    win = _window(20, 22, 3, 20)          # 20 x 20 cells of 3 x 3 nodes
+    marks, st = trace_marks(s, d, coef, win, precision)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    total = np.zeros(win.nrows * win.ncols, np.int64)
+    for r in range(s.nrealizations):
+        pf = s.fixed_field()
+        real = Realizations(d["K"][r:r + 1], d["n"][r:r + 1], d["b"][r:r + 1], d["q"][r:r + 1],
+                            coef[r:r + 1])
+        c = chronicle_into_field(pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration, s.tol,
+                                 s.maxstep, s.umbra, confined=False, precision=precision).counts
+        want = np.zeros((win.nrows * 3, win.ncols * 3), np.int64)
+        want[:, :] = c[20:20 + win.nrows * 3, 22:22 + win.ncols * 3]
+        want = want.reshape(win.nrows, 3, win.ncols, 3).sum(axis=(1, 3)).ravel()
+        assert np.array_equal(marks[r].astype(np.int64), want), r
+        total += want
+    assert total.sum() > 0 and marks.sum() == total.sum()

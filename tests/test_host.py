@@ -58,6 +58,27 @@ def test_host_exact_math_matches_quad_precision():
     assert L.oka_log(0.0) == -math.inf and math.isnan(L.oka_log(-1.0))
 
 
+def test_host_log_fast_path_matches_quad_precision():
+    """oka_log is the table-driven log with its rounding test (log_rn): against
+    the oracle's binary128 logarithm on the table's hard cases -- x within
+    2^-6 and 2^-30 of 1 from both sides (no cancellation), every table row's
+    boundaries, the upper-half rows' shift, and squared distances of the
+    tracers' range -- plus powers of two and subnormals."""
+    L = _lib.load()
+    rng = np.random.default_rng(29)
+    rows = 1.0 + np.arange(129) / 128.0
+    xs = np.concatenate([
+        1.0 + rng.uniform(-2 ** -7, 2 ** -7, 4000), 1.0 + rng.uniform(-2 ** -30, 2 ** -30, 2000),
+        np.nextafter(rows, 0.0), rows[:-1], np.nextafter(rows[:-1], 3.0),
+        np.ldexp(rng.uniform(1.0, 2.0, 6000), rng.integers(-300, 300, 6000)),
+        rng.uniform(1e-4, 1e8, 6000), 2.0 ** np.arange(-1074, 1024, 7).astype(float),
+        [5e-324, 2.2250738585072014e-308, 1.0, 1.7976931348623157e308]])
+    for x in xs:
+        x = float(x)
+        assert L.oka_log(x) == O.log(x, O.MATH_CR), x.hex()
+    assert L.oka_log(math.inf) == math.inf
+
+
 def _oracle_sequence(init, bboxes):
     """Sequential expands (probabilityfield.py:175-261) on the oracle field,
     returning the geometry after each one."""
