@@ -10,11 +10,14 @@ Workload (default): 25 wells, 40 piezometers, 10^4 realizations x 1000
 particles, 20-year backtrace, 400 x 400 nodes at 20 m, umbra 40 m, tol 1 m,
 maxstep 10 m.  Synthetic, seeded.
 
-The headline ``value`` is realizations/s of the FP32 tracer when its capture
-probabilities are within 0.01 (max over nodes) of the FP64-exact tracer on the
-same realizations; otherwise it is the FP64-exact rate.  Both rates, their
-particle-steps/s, the error, the dominant kernel's roofline fraction and a CPU
-baseline are in the JSON line.
+The headline ``value`` is realizations/s of the FP64-exact tracer (the
+reference's own arithmetic, bit-exact against the oracle).  The FP32 tracer's
+rate is reported beside it (``fp32_gated``) with its capture-probability error
+against the FP64-exact tracer on the same realizations (max over nodes, gate
+0.01), as are the learned control variate on the same site, the 64-site uneven
+batch (configuration 2) and the 200-well stress site on a realization subset
+(configuration 4), each with its own error sample against FP64-exact.  The
+dominant kernel's roofline fraction and the CPU baselines are in the line too.
 """
 import argparse
 import json
@@ -71,6 +74,11 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (to scale HBM counters)")
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the configuration-2 and configuration-4 legs of the default run")
+    ap.add_argument("--extra-knobs", default="2,16,64",
+                    help="pool launch shape SLOTS,WAVES_PER_CU,CHUNK of the embedded "
+                         "configuration-2 and -4 legs")
     ap.add_argument("--unconfined", action="store_true",
                     help="trace the unconfined velocity (compute_velocity) in the fp64/fp32 modes; "
                          "the roofline keeps the confined flop count")
@@ -282,65 +290,118 @@ def time_mode(run, precision, steps, warmup, torch, **sched):
                 stats=st, counts=counts)
 
 
-def cpu_baselines(site, d, coef, budget_s):
-    """Reference-speed Python restatement on 1 core, and the C oracle on all
-    cores, each on a bounded sample of the same workload."""
-    import oracle as O
+def pyref_realizations(site, d, reals, budget_s, stride, confined):
+    """The numpy-scalar restatement of the reference (oracle/pyref.py) on
+    realizations `reals` of draws d: per realization the fit (lstsq + inv),
+    the MVN draw, then every stride-th particle traced and rasterized, until
+    budget_s of wall time is spent.  Returns (traces, attempted steps,
+    seconds)."""
     from oracle import pyref
+    from onekapy_amd.sites import fixed_bounds
+    t0 = time.perf_counter()
+    traces = steps = 0
+    wxy = site.wells_xy()
+    for r in reals:
+        m = pyref.PyModel(site.base, d["K"][r], d["n"][r], d["b"][r],
+                          [(wxy[i, 0], wxy[i, 1], 0.25, d["q"][r][i]) for i in range(len(wxy))])
+        ev, cov = m.fit(site.obs_array(), site.xt, site.yt)
+        m.coef = np.random.default_rng(r).multivariate_normal(ev.reshape(6), cov)   # stochastic.py:241
+        pf = pyref.PyField(site.spacing, site.spacing, site.xt, site.yt)
+        pf.expand(*fixed_bounds(site.xt, site.yt, site.spacing, site.nrows, site.ncols))
+        vel = m.velocity_confined if confined else m.velocity
+
+        def feval(xy):
+            vx, vy = vel(xy[0], xy[1])
+            return np.array([-vx, -vy])
+
+        for xs, ys in pyref.start_points(site.xt, site.yt, site.rt, site.npaths)[::stride]:
+            v, att, _acc, _st = pyref.backtrace(xs, ys, site.duration, site.tol, site.maxstep,
+                                                feval)
+            pf.rasterize([a[0] for a in v], [a[1] for a in v], site.umbra)
+            traces += 1
+            steps += att
+            if time.perf_counter() - t0 > budget_s:
+                return traces, steps, time.perf_counter() - t0
+        pf.register(1.0)
+    return traces, steps, time.perf_counter() - t0
+
+
+def _pyref_worker(job):
+    return pyref_realizations(*job)
+
+
+def cpu_baselines(site, d, budget_s, confined=True):
+    """Reference-speed CPU baselines on bounded samples of the workload, each
+    realization fitted, drawn and traced on the CPU: the numpy-scalar
+    restatement in one process (how the reference runs) and in one process per
+    host core over disjoint realizations, and the C oracle with OpenMP.  Run
+    before the process touches the GPU."""
+    import multiprocessing as mp
+    import oracle as O
     from onekapy_amd.capturezone import start_points
     out = {}
-    # (1) numpy-scalar restatement, one process: the reference's own speed.
-    wxy = site.wells_xy()
-    m = pyref.PyModel(site.base, d["K"][0], d["n"][0], d["b"][0],
-                      [(wxy[i, 0], wxy[i, 1], 0.25, d["q"][0][i]) for i in range(len(wxy))],
-                      site.xt, site.yt, coef[0])
-    starts = start_points(site.xt, site.yt, site.rt, site.npaths)
-    from onekapy_amd.sites import fixed_bounds
-    pf = pyref.PyField(site.spacing, site.spacing, site.xt, site.yt)
-    pf.expand(*fixed_bounds(site.xt, site.yt, site.spacing, site.nrows, site.ncols))
-    t0 = time.perf_counter()
-    steps = 0
-    np_done = 0
-    for xs, ys in starts:
-        def feval(xy):
-            vx, vy = m.velocity_confined(xy[0], xy[1])
-            return np.array([-vx, -vy])
-        v, att, _acc, _st = pyref.backtrace(xs, ys, site.duration, site.tol, site.maxstep, feval)
-        pf.rasterize([a[0] for a in v], [a[1] for a in v], site.umbra)
-        steps += att
-        np_done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    el = time.perf_counter() - t0
-    frac = np_done / site.npaths
-    out["python"] = {"value": frac / el, "unit": "realizations/s", "cores": 1, "kind": "port",
+    P = site.npaths
+    # (1) one process: every 4th particle of the first 4 realizations
+    tr, steps, el = pyref_realizations(site, d, range(4), budget_s, 4, confined)
+    out["python"] = {"value": tr / P / el, "unit": "realizations/s", "cores": 1, "kind": "port",
                      "particle_steps_per_s": steps / el,
-                     "sample": f"realization 0, first {np_done} of {site.npaths} particles "
-                               f"({steps} attempted steps, {el:.1f} s), numpy-scalar "
-                               f"restatement with the reference's loop structure"}
-    # (2) C oracle, FP64, OpenMP over realizations.
+                     "sample": f"{tr} traces (every 4th particle of realizations 0-3, each "
+                               f"fitted and drawn; {steps} attempted steps, {el:.1f} s), "
+                               f"numpy-scalar restatement with the reference's loop structure, "
+                               f"one process"}
+    # (2) one process per core (at most 16, the box's share), realization i in process i
+    nproc = max(1, min(16, os.cpu_count() or 1))
+    jobs = [(site, {k: v[:i + 1] for k, v in d.items()}, [i], budget_s, 1, confined)
+            for i in range(nproc)]
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(nproc) as pool:
+        res = pool.map(_pyref_worker, jobs)
+    wall = time.perf_counter() - t0
+    tr = sum(r[0] for r in res)
+    steps = sum(r[1] for r in res)
+    el = max(r[2] for r in res)
+    out["python_processes"] = {
+        "value": tr / P / el, "unit": "realizations/s", "cores": nproc, "kind": "port",
+        "particle_steps_per_s": steps / el,
+        "sample": f"{nproc} processes, realization i in process i ({tr} traces, {steps} "
+                  f"attempted steps, {el:.1f} s per process, {wall:.1f} s with process start)"}
+    # (3) C oracle, FP64, OpenMP over realizations
     threads = O.max_threads()
-    g = O.make_grid(pf.xmin, pf.ymin, pf.xmax, pf.ymax, pf.dx, pf.dy, pf.nrows, pf.ncols)
-    opts = O.trace_opts(site.duration, site.tol, site.maxstep, True, O.MATH_GLIBC)
+    pf = site.fixed_field()
+    starts = start_points(site.xt, site.yt, site.rt, P)
+    g = O.make_grid(pf.xmin, pf.ymin, pf.xmax, pf.ymax, pf.deltax, pf.deltay, pf.nrows, pf.ncols)
+    opts = O.trace_opts(site.duration, site.tol, site.maxstep, confined, O.MATH_GLIBC)
+    wxy = site.wells_xy()
+
+    def oracle_coef(n):
+        coef = np.zeros((n, 6))
+        for r in range(n):
+            om = O.OracleModel(site.base, d["K"][r], d["n"][r], d["b"][r],
+                               np.c_[wxy, np.full(len(wxy), 0.25), d["q"][r]])
+            ev, cov = om.fit(site.obs_array(), site.xt, site.yt, O.MATH_GLIBC)
+            coef[r] = O.mvn(ev, cov, d["z"][r])
+        return coef
+
     nr = max(threads, 2)
     while True:
         t0 = time.perf_counter()
+        coef = oracle_coef(nr)
         _, st = O.chronicle_fixed(wxy, site.xt, site.yt, site.base, d["K"][:nr], d["n"][:nr],
-                                  d["b"][:nr], d["q"][:nr], coef[:nr], starts, site.umbra, opts,
-                                  g, nthreads=threads)
+                                  d["b"][:nr], d["q"][:nr], coef, starts, site.umbra, opts, g,
+                                  nthreads=threads)
         el = time.perf_counter() - t0
         if el > 0.3 * budget_s or nr >= len(d["K"]):
             break
         nr = min(len(d["K"]), int(nr * max(2.0, 0.5 * budget_s / max(el, 1e-3))))
     out["c_oracle"] = {"value": nr / el, "unit": "realizations/s", "cores": threads,
                        "kind": "port", "particle_steps_per_s": st[0] / el,
-                       "sample": f"first {nr} realizations x {site.npaths} particles "
-                                 f"({int(st[0])} attempted steps, {el:.1f} s), C FP64 oracle, "
-                                 f"OpenMP"}
+                       "sample": f"first {nr} realizations x {P} particles, fits and draws "
+                                 f"included ({int(st[0])} attempted steps, {el:.1f} s), C FP64 "
+                                 f"oracle, OpenMP"}
     return out
 
 
-def oracle_sample_rate(site, d, coef, budget_s, max_real=None):
+def oracle_sample_rate(site, d, coef, budget_s, max_real=None, confined=True):
     """The C FP64 oracle (OpenMP, all host threads) on the first realizations
     of a site, growing the sample until it takes a sizeable part of budget_s."""
     import oracle as O
@@ -348,7 +409,7 @@ def oracle_sample_rate(site, d, coef, budget_s, max_real=None):
     pf = site.fixed_field()
     starts = start_points(site.xt, site.yt, site.rt, site.npaths)
     g = O.make_grid(pf.xmin, pf.ymin, pf.xmax, pf.ymax, pf.deltax, pf.deltay, pf.nrows, pf.ncols)
-    opts = O.trace_opts(site.duration, site.tol, site.maxstep, True, O.MATH_GLIBC)
+    opts = O.trace_opts(site.duration, site.tol, site.maxstep, confined, O.MATH_GLIBC)
     threads = O.max_threads()
     cap = len(d["K"]) if max_real is None else min(max_real, len(d["K"]))
     nr = min(cap, max(1, threads // 4))
@@ -418,6 +479,24 @@ def bench_cfg2(args):
         errs.append(np.abs(c32 - c64) / n)
     err_max = float(max(e.max() for e in errs))
     err_l1 = float(sum(e.sum() for e in errs))
+    b.close()
+    # the same 64 sites in FP64-exact: the pool in the workgroup tracer's arithmetic
+    log("cfg2: FP64-exact batch, one warm-up and one timed replay")
+    b64 = SiteBatch(ss, draws, precision=_lib.FP64_EXACT)
+    b64.run()
+    t0 = time.perf_counter()
+    ms64 = b64.run()
+    dt64 = time.perf_counter() - t0
+    res64 = [b64.results(i) for i in range(len(ss))]
+    flops64 = sum(canonical_flops(r[2], len(s.wells)) for r, s in zip(res64, ss))
+    ach64 = flops64 / (ms64 * 1e-3) / 1e12
+    fp64 = {"value": R / dt64, "unit": "realizations/s", "dtype": "f64", "seconds": dt64,
+            "particle_steps_per_s": sum(r[2].attempted for r in res64) / dt64,
+            "roofline": {"bound": "valu", "achieved": ach64, "peak": PEAK_TFLOPS["fp64"],
+                         "unit": "TFLOP/s", "frac": ach64 / PEAK_TFLOPS["fp64"],
+                         "kernel": "pool tracer (fp64-exact), whole batch",
+                         "flops_per_launch": flops64}}
+    b64.close()
     cpu = None
     if not args.no_cpu:
         log("cfg2: CPU baseline (C oracle) on site 0")
@@ -445,6 +524,7 @@ def bench_cfg2(args):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "kernel": "pool tracer (fp32), whole batch", "flops_per_launch": flops},
+        "fp64_exact": fp64,
         "cpu_baseline": cpu,
     }
 
@@ -480,8 +560,10 @@ def bench_cfg4(args, torch):
     n = min(1000, R)
     coef = run.coef.cpu().numpy()
     dd = {k: v[:n] for k, v in d.items()}
-    log(f"cfg4: FP64-exact subset of {n} realizations")
-    c64, st64 = trace_counts(site, dd, coef[:n], _lib.FP64_EXACT)
+    log(f"cfg4: FP64-exact subset of {n} realizations (pool schedule)")
+    t64 = time.perf_counter()
+    c64, st64 = trace_counts(site, dd, coef[:n], _lib.FP64_EXACT, _lib.SCHED_POOL)
+    t64 = time.perf_counter() - t64
     c32, _ = trace_counts(site, dd, coef[:n], _lib.FP32)
     err = np.abs(c32 - c64) / n
     cpu = None
@@ -505,13 +587,42 @@ def bench_cfg4(args, torch):
         "particle_steps_per_s": st.attempted * args.steps / m["seconds"],
         "status_count": [int(v) for v in st.status_count],
         "error_vs_fp64": {"max": float(err.max()), "l1": float(err.sum()),
-                          "sample": f"first {n} realizations, FP64-exact tracer"},
+                          "sample": f"first {n} realizations, FP64-exact tracer on the pool "
+                                    f"schedule ({t64:.1f} s)"},
         "autotune": [{"knobs": k, "ms": ms} for k, ms in trials],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "kernel": "pool tracer (fp32)", "flops_per_launch": flops},
         "cpu_baseline": cpu,
     }
+
+
+def _embedded(result, keys=("value", "unit", "ms_per_step", "particle_steps_per_s", "dtype",
+                             "error_vs_fp64", "roofline", "cpu_baseline", "config", "status_count",
+                             "steps", "warmup", "fp64_exact")):
+    return {k: result[k] for k in keys if k in result}
+
+
+def cfg2_leg(knobs, args):
+    """Configuration 2 inside the default run: one warm-up and one timed replay
+    of the 64-site batch with a fixed pool shape, plus its FP64-exact error
+    sample and the C oracle on site 0."""
+    a = argparse.Namespace(**vars(args))
+    a.steps, a.warmup, a.cfg2_scale = 1, 1, 1.0
+    a.knobs = ",".join(str(knobs[k]) for k in ("pool_slots", "pool_waves_per_cu", "pool_chunk"))
+    a.cpu_seconds = min(args.cpu_seconds, 8.0)
+    return _embedded(bench_cfg2(a))
+
+
+def cfg4_leg(knobs, args, torch):
+    """Configuration 4 inside the default run: a stated 20,000-realization subset
+    of the 10^6, one warm-up and one timed step with a fixed pool shape, plus
+    its FP64-exact error sample and the C oracle on its first realizations."""
+    a = argparse.Namespace(**vars(args))
+    a.steps, a.warmup, a.cfg4_nreal = 1, 1, 20000
+    a.knobs = ",".join(str(knobs[k]) for k in ("pool_slots", "pool_waves_per_cu", "pool_chunk"))
+    a.cpu_seconds = min(args.cpu_seconds, 8.0)
+    return _embedded(bench_cfg4(a, torch))
 
 
 def main():
@@ -540,7 +651,13 @@ def main():
         print(json.dumps({"control_variate": cv}))
         return
     d = sites.draw(site, seed=1)
-    run = DeviceRun(site, d, torch, confined=not args.unconfined)
+    confined = not args.unconfined
+    cpu = None
+    if not args.no_cpu:
+        # before the process touches the GPU: the multi-process leg starts workers
+        print("bench: CPU baselines", file=sys.stderr, flush=True)
+        cpu = cpu_baselines(site, d, args.cpu_seconds, confined)
+    run = DeviceRun(site, d, torch, confined=confined)
     nw = len(site.wells)
     modes = {}
     wanted = [m.strip() for m in args.modes.split(",") if m.strip()]
@@ -554,15 +671,13 @@ def main():
         print(f"bench: {name} {m['ms_per_step']:.1f} ms/step, trace kernel {m['kernel_ms']:.1f} ms, "
               f"{m['stats'].attempted} steps", file=sys.stderr, flush=True)
     R = run.R
+    err_max = err_l1 = None
     if "fp64" in modes and "fp32" in modes:
         p64 = modes["fp64"]["counts"] / R
         p32 = modes["fp32"]["counts"] / R
         err_max = float(np.abs(p32 - p64).max())
         err_l1 = float(np.abs(p32 - p64).sum())
-        head = "fp32" if err_max <= 0.01 else "fp64"
-    else:
-        err_max = err_l1 = None
-        head = next(iter(modes))
+    head = "fp64" if "fp64" in modes else next(iter(modes))
     hm = modes[head]
     value = R * args.steps / hm["seconds"]
     steps_per_s = hm["stats"].attempted * args.steps / hm["seconds"]
@@ -570,15 +685,16 @@ def main():
     achieved = flops / (hm["kernel_ms"] * 1e-3) / 1e12
     peak = PEAK_TFLOPS[head]
 
-    cpu = None
-    if not args.no_cpu:
-        coef = run.coef.cpu().numpy()
-        print("bench: CPU baselines", file=sys.stderr, flush=True)
-        cpu = cpu_baselines(site, d, coef, args.cpu_seconds)
-
     cv = None
-    if not args.no_cv and "fp32" in modes:
+    if not args.no_cv and "fp32" in modes and confined:
         cv = control_variate_leg(site, args, R * args.steps / modes["fp32"]["seconds"])
+
+    extra = {}
+    if not args.no_extra and args.config == "cfg1":
+        knobs = dict(zip(("pool_slots", "pool_waves_per_cu", "pool_chunk"),
+                         (int(v) for v in args.extra_knobs.split(","))))
+        extra["config2"] = cfg2_leg(knobs, args)
+        extra["config4"] = cfg4_leg(knobs, args, torch)
 
     def mode_summary(name):
         m = modes[name]
@@ -591,6 +707,20 @@ def main():
                 "escaped_traces": int(m["stats"].escaped),
                 "status_count": [int(v) for v in m["stats"].status_count],
                 "kernel_tflops": kernel_flops(m["stats"], nw) / (m["kernel_ms"] * 1e-3) / 1e12}
+
+    fp32_gated = None
+    if "fp32" in modes and err_max is not None:
+        m32 = modes["fp32"]
+        fp32_gated = {"value": R * args.steps / m32["seconds"], "unit": "realizations/s",
+                      "dtype": "f32", "gate": "max |p32 - p64| over nodes <= 0.01",
+                      "passed": err_max <= 0.01, "error_vs_fp64": {"max": err_max, "l1": err_l1},
+                      "particle_steps_per_s": m32["stats"].attempted * args.steps / m32["seconds"],
+                      "roofline": {"bound": "valu",
+                                   "achieved": mode_summary("fp32")["kernel_tflops"],
+                                   "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                                   "frac": mode_summary("fp32")["kernel_tflops"]
+                                   / PEAK_TFLOPS["fp32"],
+                                   "traffic": measured_traffic("k_trace_fp32", R)}}
 
     line = {
         "metric": "realizations/sec and particle-steps/sec at max capture-prob error <=0.01 vs FP64",
@@ -610,18 +740,26 @@ def main():
                    "duration_d": site.duration, "grid": [site.nrows, site.ncols],
                    "spacing_m": site.spacing, "umbra_m": site.umbra, "tol_m": site.tol,
                    "maxstep_m": site.maxstep, "headline_mode": head,
-                   "velocity": "unconfined" if args.unconfined else "confined"},
+                   "velocity": "confined" if confined else "unconfined"},
         "particle_steps_per_s": steps_per_s,
-        "error_vs_fp64": {"max": err_max, "l1": err_l1, "nodes": int(site.nrows * site.ncols)},
+        "error_vs_fp64": {"max": 0.0 if head == "fp64" else err_max,
+                          "l1": 0.0 if head == "fp64" else err_l1,
+                          "nodes": int(site.nrows * site.ncols),
+                          "note": "the FP64-exact tracer is the reference arithmetic itself"
+                                  if head == "fp64" else "FP32 against FP64-exact"},
         "modes": {k: mode_summary(k) for k in modes},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
                      "traffic": measured_traffic(f"k_trace_{head}", R),
                      "kernel": f"trace+chronicle ({head})",
                      "flops_per_launch": flops},
-        "cpu_baseline": None if cpu is None else dict(cpu["python"], c_oracle=cpu["c_oracle"]),
+        "cpu_baseline": None if cpu is None else dict(cpu["python"],
+                                                       processes=cpu["python_processes"],
+                                                       c_oracle=cpu["c_oracle"]),
+        "fp32_gated": fp32_gated,
         "control_variate": cv,
     }
+    line.update(extra)
     print(json.dumps(line))
 
 

@@ -8,7 +8,7 @@ reference-equivalent CPU baseline.  The tests use it to cross-check the C oracle
 bit for bit: two independent restatements of the same reference lines.
 
 Reference lines restated (OnekaPy tree):
-  oneka/model.py:258-266, 299-315, 423-427, 545-565, 598-604
+  oneka/model.py:226-237, 258-266, 299-315, 341-350, 377-389, 423-427, 545-565, 598-604
   oneka/capturezone.py:109-123, 198-253
   oneka/probabilityfield.py:136-151, 196-261, 292-310, 334-339, 357-359, 407-427
   oneka/stochastic.py:220-265
@@ -70,6 +70,31 @@ class PyModel:
     def velocity_confined(self, x, y):
         qx, qy = self.discharge(x, y)
         return qx / (self.b * self.n), qy / (self.b * self.n)
+
+    def potential(self, x, y):
+        """model.py:226-237: the regional quadratic plus the wells."""
+        c = self.coef
+        dx = x - self.xo
+        dy = y - self.yo
+        phi = c[0] * dx ** 2 + c[1] * dy ** 2 + c[2] * dx * dy + c[3] * dx + c[4] * dy + c[5]
+        return phi + self.wells_potential(x, y)
+
+    def velocity(self, x, y):
+        """model.py:341-350 and 377-389: the unconfined velocity; raises
+        ArithmeticError where the reference raises AquiferError."""
+        qx, qy = self.discharge(x, y)
+        phi = self.potential(x, y)
+        if phi <= 0:
+            raise ArithmeticError("potential <= 0")
+        elif phi < 0.5 * self.K * self.b ** 2:
+            head = np.sqrt(2.0 * phi / self.K)
+        else:
+            head = (phi + 0.5 * self.K * self.b ** 2) / (self.K * self.b)
+        if head <= 0:
+            raise ArithmeticError("head <= 0")
+        elif head > self.b:
+            return qx / (self.b * self.n), qy / (self.b * self.n)
+        return qx / (head * self.n), qy / (head * self.n)
 
     def fit(self, obs, xo, yo):
         """model.py:497-606: weighted rows, then the reference's lstsq and inv."""
