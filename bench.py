@@ -66,6 +66,9 @@ def parse():
                     help="cfg4: realizations of the autotuning runs (runs much smaller than "
                          "the timed one are bound by their single longest trace and mislead "
                          "the tuner)")
+    ap.add_argument("--far-order", type=int, default=0,
+                    help="oka_trace_opts.far_order of the FP32 pool legs (cfg2, cfg4): 0 lets "
+                         "the library choose, < 0 sums every well directly")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="approximate CPU time budget of each CPU baseline")
@@ -447,13 +450,13 @@ def bench_cfg2(args):
     if best is None:
         small = sites.config2(scale=0.05)
         sdraws = [sites.draw(s, seed=100 + i) for i, s in enumerate(small)]
-        tb = SiteBatch(small, sdraws)
+        tb = SiteBatch(small, sdraws, far_order=args.far_order)
         log("cfg2: autotuning the pool launch on a 5 % copy")
         best, trials = tune_batch(tb, log=log)
         tb.close()
     ss = sites.config2(scale=args.cfg2_scale)
     draws = [sites.draw(s, seed=100 + i) for i, s in enumerate(ss)]
-    b = SiteBatch(ss, draws, knobs=best)
+    b = SiteBatch(ss, draws, knobs=best, far_order=args.far_order)
     R = sum(b.nreal)
     log(f"cfg2: 64 sites, {R} realizations x 256 particles: {args.warmup} warmup + "
         f"{args.steps} timed replays")
@@ -467,7 +470,8 @@ def bench_cfg2(args):
     flops = sum(canonical_flops(r[2], len(s.wells)) for r, s in zip(res, ss))
     kernel_ms = float(np.mean(kms))
     achieved = flops / (kernel_ms * 1e-3) / 1e12
-    # FP32 (batch) against the FP64-exact tracer on the first 400 realizations of 6
+    # FP32 (the batch's arithmetic: the pool tracer with the same far-well
+    # choice) against the FP64-exact tracer on the first 400 realizations of 6
     # sites (one node flipped in one realization is then 0.0025)
     errs = []
     for i in range(6):
@@ -475,7 +479,7 @@ def bench_cfg2(args):
         dd = {k: v[:n] for k, v in draws[i].items()}
         coef = res[i][1][:n]
         c64, _ = trace_counts(ss[i], dd, coef, _lib.FP64_EXACT)
-        c32, _ = trace_counts(ss[i], dd, coef, _lib.FP32)
+        c32, _ = trace_counts(ss[i], dd, coef, _lib.FP32, _lib.SCHED_POOL, args.far_order)
         errs.append(np.abs(c32 - c64) / n)
     err_max = float(max(e.max() for e in errs))
     err_l1 = float(sum(e.sum() for e in errs))
@@ -518,8 +522,8 @@ def bench_cfg2(args):
                    "particles": 256, "scale": args.cfg2_scale, "autotuned": best},
         "particle_steps_per_s": att * args.steps / dt,
         "error_vs_fp64": {"max": err_max, "l1": err_l1,
-                          "sample": "first 400 realizations (or all) of sites 0-5, "
-                                    "FP64-exact tracer"},
+                          "sample": "first 400 realizations (or all) of sites 0-5, FP32 "
+                                    "pool tracer as timed against the FP64-exact tracer"},
         "autotune": [{"knobs": k, "ms": ms} for k, ms in trials],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
@@ -541,7 +545,7 @@ def bench_cfg4(args, torch):
     best, trials = fixed_knobs(args)
     if best is None:
         small = sites.config4(args.cfg4_tune_nreal)
-        tb = SiteBatch([small], [sites.draw(small, seed=40)])
+        tb = SiteBatch([small], [sites.draw(small, seed=40)], far_order=args.far_order)
         log(f"cfg4: autotuning the pool launch on {args.cfg4_tune_nreal} realizations")
         # one sweep over the knobs that matter here (each replay takes seconds)
         search = (("pool_waves_per_cu", (8, 12, 16)), ("pool_slots", (2, 3)),
@@ -551,7 +555,7 @@ def bench_cfg4(args, torch):
     run = DeviceRun(site, d, torch)
     log(f"cfg4: {args.cfg4_nreal} realizations: {args.warmup} warmup + {args.steps} timed steps")
     m = time_mode(run, _lib.FP32, args.steps, args.warmup, torch, schedule=_lib.SCHED_POOL,
-                  **best)
+                  far_order=args.far_order, **best)
     R = run.R
     st = m["stats"]
     flops = canonical_flops(st, len(site.wells))
@@ -564,7 +568,7 @@ def bench_cfg4(args, torch):
     t64 = time.perf_counter()
     c64, st64 = trace_counts(site, dd, coef[:n], _lib.FP64_EXACT, _lib.SCHED_POOL)
     t64 = time.perf_counter() - t64
-    c32, _ = trace_counts(site, dd, coef[:n], _lib.FP32)
+    c32, _ = trace_counts(site, dd, coef[:n], _lib.FP32, _lib.SCHED_POOL, args.far_order)
     err = np.abs(c32 - c64) / n
     cpu = None
     if not args.no_cpu:
@@ -587,8 +591,9 @@ def bench_cfg4(args, torch):
         "particle_steps_per_s": st.attempted * args.steps / m["seconds"],
         "status_count": [int(v) for v in st.status_count],
         "error_vs_fp64": {"max": float(err.max()), "l1": float(err.sum()),
-                          "sample": f"first {n} realizations, FP64-exact tracer on the pool "
-                                    f"schedule ({t64:.1f} s)"},
+                          "sample": f"first {n} realizations, FP32 pool tracer as timed "
+                                    f"against the FP64-exact tracer on the pool schedule "
+                                    f"({t64:.1f} s)"},
         "autotune": [{"knobs": k, "ms": ms} for k, ms in trials],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,

@@ -149,7 +149,10 @@ typedef struct {
     int32_t pool_waves_per_cu; /* OKA_SCHED_POOL: resident waves per CU (<= 0: the most
                                   of 16, 12, 8, 6, 4 whose LDS share fits the slots) */
     int32_t pool_chunk;      /* OKA_SCHED_POOL: particles per work item (<= 0: 64)   */
-    int32_t reserved;
+    int32_t far_order;       /* OKA_SCHED_POOL, FP32, confined: wells beyond the 3 x 3
+                                boxes around a point summed as a local expansion of
+                                this order (1..16); 0: order 16 from 64 wells on,
+                                where the plan pays; < 0: every well summed directly */
 } oka_trace_opts;
 
 /* How particles are scheduled on the GPU.  WORKGROUP: one 256-lane workgroup

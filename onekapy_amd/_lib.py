@@ -83,7 +83,7 @@ class TraceOpts_(C.Structure):
                 ("duration", _d), ("tol", _d), ("maxstep", _d), ("umbra", _d),
                 ("max_steps", _i64), ("start_xy", _vp), ("bitmap_mode", _i32),
                 ("tile_slots", _i32), ("pool_slots", _i32), ("pool_waves_per_cu", _i32),
-                ("pool_chunk", _i32), ("reserved", _i32)]
+                ("pool_chunk", _i32), ("far_order", _i32)]
 
 
 class DeviceInfo_(C.Structure):
@@ -321,7 +321,8 @@ SCHED_AUTO, SCHED_WORKGROUP, SCHED_POOL = 0, 1, 2
 
 def make_trace_opts(npaths, duration, tol, maxstep, umbra, start_xy_ptr, precision=FP64_EXACT,
                     confined=True, max_steps=0, bitmap_mode=BITMAP_AUTO, tile_slots=0,
-                    schedule=SCHED_AUTO, pool_slots=0, pool_waves_per_cu=0, pool_chunk=0):
+                    schedule=SCHED_AUTO, pool_slots=0, pool_waves_per_cu=0, pool_chunk=0,
+                    far_order=0):
     o = TraceOpts_()
     o.npaths = int(npaths)
     o.precision = int(precision)
@@ -335,6 +336,7 @@ def make_trace_opts(npaths, duration, tol, maxstep, umbra, start_xy_ptr, precisi
     o.pool_slots = int(pool_slots)
     o.pool_waves_per_cu = int(pool_waves_per_cu)
     o.pool_chunk = int(pool_chunk)
+    o.far_order = int(far_order)
     return o
 
 

@@ -28,9 +28,11 @@ def _knob_opts(knobs):
 class SiteBatch:
     """sites: objects with the attributes of ``sites.Site``; draws: per site a
     dict(q, K, n, b, z) of its realization parameters; precision: the batch's
-    tracer arithmetic (_lib.FP32 or _lib.FP64_EXACT)."""
+    tracer arithmetic (_lib.FP32 or _lib.FP64_EXACT); far_order: every site's
+    oka_trace_opts.far_order (FP32, confined sites: distant wells as local
+    expansions; 0 lets the library choose, < 0 sums every well directly)."""
 
-    def __init__(self, sites, draws, knobs=None, device=0, precision=_lib.FP32):
+    def __init__(self, sites, draws, knobs=None, device=0, precision=_lib.FP32, far_order=0):
         self.ctx = _lib.context(device)
         self.sites = list(sites)
         self.knobs = dict(knobs or {})
@@ -50,7 +52,8 @@ class SiteBatch:
             b.K, b.n, b.b, b.q = (_lib.ptr(a) for a in arrays)
             b.z = _lib.ptr(z)
             b.opts = _lib.make_trace_opts(s.npaths, s.duration, s.tol, s.maxstep, s.umbra,
-                                          _lib.ptr(starts), precision, s.confined)
+                                          _lib.ptr(starts), precision, s.confined,
+                                          far_order=far_order)
             b.grid = _lib.make_grid(pf.xmin, pf.ymin, pf.xmax, pf.ymax, pf.deltax, pf.deltay,
                                     pf.nrows, pf.ncols)
             keep.append((wxy, obs, st, starts, arrays, z))

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -230,6 +231,9 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
         if (sites[i].rasterize)
             sh.max_tiles = std::max(sh.max_tiles, (int)sites[i].ntiles);
         nconf += sites[i].confined ? 1 : 0;
+        if (sites[i].far_order > 0)
+            sh.far_stride = std::max<int64_t>(
+                sh.far_stride, (int64_t)sites[i].far_nbx * sites[i].far_nby * kFarTerms);
     }
     sh.conf_mode = nconf == nsites ? 0 : (nconf == 0 ? 1 : 2);
     sh.first[nsites] = total;
@@ -280,10 +284,122 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
     return OKA_OK;
 }
 
+int far_requested(const TraceArgs &a, int precision, int far_order)
+{
+    if (precision != OKA_FP32 || !a.confined || a.nwells < 2 || far_order < 0)
+        return 0;
+    if (far_order > 0)
+        return std::min(far_order, kFarTerms);
+    return a.nwells >= kFarAutoWells ? kFarTerms : 0;
+}
+
+// Boxes of side L over the wells and an eighth of their extent around them
+// (in the FP32 tracers' coordinates, relative to the target), sized so that
+// the 3 x 3 boxes around a point hold about kFarNear wells, with at most
+// kFarMaxSide boxes a side.
+// Every well of a box's 3 x 3 neighbourhood is summed directly; the rest are
+// at least 1.5 L from the box centre while the point is within L / sqrt(2),
+// so the expansion's terms fall as (0.47)^k.  In the automatic mode a plan
+// whose near lists average more than half the wells is not used.
+void far_plan(const TraceArgs &a, const double *wxy, int precision, int far_order,
+              FarPlan *out)
+{
+    FarPlan &P = *out;
+    P = FarPlan();
+    const int order = far_requested(a, precision, far_order);
+    const int nw = a.nwells;
+    if (order == 0 || !wxy)
+        return;
+    std::vector<double> wx(nw), wy(nw);
+    double x0 = HUGE_VAL, x1 = -HUGE_VAL, y0 = HUGE_VAL, y1 = -HUGE_VAL;
+    for (int i = 0; i < nw; ++i) {
+        wx[i] = (double)(float)(wxy[2 * i] - a.xo);
+        wy[i] = (double)(float)(wxy[2 * i + 1] - a.yo);
+        if (!std::isfinite(wx[i]) || !std::isfinite(wy[i]))
+            return;
+        x0 = std::min(x0, wx[i]);
+        x1 = std::max(x1, wx[i]);
+        y0 = std::min(y0, wy[i]);
+        y1 = std::max(y1, wy[i]);
+    }
+    // an eighth of the extent beyond the wells on each side; the plan does not
+    // depend on the grid, so a chronicle's passes trace the same paths
+    const double mx = 0.125 * (x1 - x0), my = 0.125 * (y1 - y0);
+    x0 -= mx;
+    x1 += mx;
+    y0 -= my;
+    y1 += my;
+    const double W = x1 - x0, H = y1 - y0;
+    double L = std::sqrt((double)kFarNear * W * H / (9.0 * nw));
+    const double side = kFarMaxSide - 0.5;
+    L = std::max({L, W / side, H / side, 1e-3 * std::max(W, H)});
+    if (!(L > 0.0) || !std::isfinite(L))
+        return;
+    const int nbx = (int)std::floor(W / L) + 1, nby = (int)std::floor(H / L) + 1;
+    const int nb = nbx * nby;
+    // centre the boxes on the bounds: every well lies at least a few
+    // rounding steps inside
+    const float fx0 = (float)(x0 - 0.5 * (nbx * L - W));
+    const float fy0 = (float)(y0 - 0.5 * (nby * L - H));
+    const float fl = (float)L;
+    P.wbox.resize(2 * (size_t)nw);
+    for (int i = 0; i < nw; ++i) {
+        const int bx = (int)std::floor((wx[i] - fx0) / fl);
+        const int by = (int)std::floor((wy[i] - fy0) / fl);
+        P.wbox[2 * i] = std::min(std::max(bx, 0), nbx - 1);
+        P.wbox[2 * i + 1] = std::min(std::max(by, 0), nby - 1);
+    }
+    // near rows of a fixed length (a multiple of 4, read 4 words at a time):
+    // the count, the indices in index order, then -1
+    std::vector<std::vector<int32_t>> lists(nb);
+    int64_t nnear = 0;
+    int most = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int bx = b % nbx, by = b / nbx;
+        for (int i = 0; i < nw; ++i)
+            if (std::abs(P.wbox[2 * i] - bx) <= 1 && std::abs(P.wbox[2 * i + 1] - by) <= 1)
+                lists[b].push_back(i);
+        nnear += (int64_t)lists[b].size();
+        most = std::max(most, (int)lists[b].size());
+    }
+    P.near_stride = (most + 1 + 3) / 4 * 4;
+    P.near.assign((size_t)nb * P.near_stride, -1);
+    for (int b = 0; b < nb; ++b) {
+        int32_t *row = P.near.data() + (size_t)b * P.near_stride;
+        row[0] = (int32_t)lists[b].size();
+        std::copy(lists[b].begin(), lists[b].end(), row + 1);
+    }
+    if (far_order == 0 && 2 * nnear > (int64_t)nb * nw) {
+        P = FarPlan();
+        return;
+    }
+    P.order = order;
+    P.nbx = nbx;
+    P.nby = nby;
+    P.x0 = fx0;
+    P.y0 = fy0;
+    P.len = fl;
+}
+
+void far_args(const FarPlan &P, const int32_t *near, const int32_t *wbox, TraceArgs &a)
+{
+    a.far_order = P.order;
+    a.far_nbx = P.nbx;
+    a.far_nby = P.nby;
+    a.far_x0 = P.x0;
+    a.far_y0 = P.y0;
+    a.far_len = P.len;
+    a.far_near_stride = P.near_stride;
+    a.far_near = near;
+    a.far_wbox = wbox;
+}
+
 PoolArgs pool_args(const PoolShape &sh, void *sites, void *first, void *queue, void *hbm,
-                   void *rec)
+                   void *rec, void *far_beta)
 {
     PoolArgs p{};
+    p.far_beta = (float2 *)far_beta;
+    p.far_stride = sh.far_stride;
     p.sites = (const TraceArgs *)sites;
     p.nsites = sh.nsites;
     p.site_first = (const int64_t *)first;
@@ -336,6 +452,8 @@ int check_opts(oka_ctx *c, const oka_trace_opts *o)
         return fail(c, OKA_ERR_ARG, "unknown precision %d", o->precision);
     if (o->schedule < OKA_SCHED_AUTO || o->schedule > OKA_SCHED_POOL)
         return fail(c, OKA_ERR_ARG, "unknown schedule %d", o->schedule);
+    if (o->far_order > kFarTerms)
+        return fail(c, OKA_ERR_ARG, "far_order %d above %d", o->far_order, kFarTerms);
     return OKA_OK;
 }
 
@@ -710,6 +828,36 @@ static int chronicle_finish(oka_ctx *c, const TraceArgs &a, int32_t rasterize, i
                      int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags,
                      const oka_window *win, uint8_t *marks);
 
+// The far-well plan of a single-site pool launch, in the context's slots.
+int stage_far(oka_ctx *c, const oka_site *site, const oka_trace_opts *o, uint32_t flags,
+              TraceArgs &a)
+{
+    if (far_requested(a, o->precision, o->far_order) == 0)
+        return OKA_OK;
+    std::vector<double> wh;
+    const double *wxy = site->wells_xy;
+    if (flags & OKA_PTR_DEVICE) {
+        wh.resize(2 * (size_t)a.nwells);
+        HIPCHK(c, hipMemcpy(wh.data(), wxy, sizeof(double) * wh.size(), hipMemcpyDeviceToHost));
+        wxy = wh.data();
+    }
+    FarPlan P;
+    far_plan(a, wxy, o->precision, o->far_order, &P);
+    if (P.order == 0)
+        return OKA_OK;
+    void *d[2];
+    const std::vector<int32_t> *v[2] = {&P.near, &P.wbox};
+    const Slot slot[2] = {kSlotFarNear, kSlotFarWbox};
+    for (int i = 0; i < 2; ++i) {
+        TRY(ensure(c, slot[i], sizeof(int32_t) * std::max<size_t>(v[i]->size(), 1), &d[i]));
+        if (!v[i]->empty())
+            HIPCHK(c, hipMemcpy(d[i], v[i]->data(), sizeof(int32_t) * v[i]->size(),
+                                hipMemcpyHostToDevice));
+    }
+    far_args(P, (const int32_t *)d[0], (const int32_t *)d[1], a);
+    return OKA_OK;
+}
+
 // Launches the persistent pool tracer over `nsites` sites (host copies of
 // their argument blocks, with device pointers inside), with its buffers in the
 // context's slots; times it with the context events when `timed`.
@@ -717,7 +865,10 @@ int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opt
 {
     PoolShape sh;
     TRY(pool_shape(c, sites, nsites, o, o->precision, &sh));
-    void *p_sites, *p_first, *p_queue, *p_hbm, *p_rec = nullptr;
+    void *p_sites, *p_first, *p_queue, *p_hbm, *p_rec = nullptr, *p_beta = nullptr;
+    if (sh.far_stride > 0)
+        TRY(ensure(c, kSlotFarBeta, sizeof(float2) * sh.waves * sh.slots * sh.far_stride,
+                   &p_beta));
     TRY(ensure(c, kSlotPoolSites, sizeof(TraceArgs) * nsites, &p_sites));
     TRY(ensure(c, kSlotPoolFirst, sizeof(int64_t) * sh.first.size(), &p_first));
     TRY(ensure(c, kSlotPoolQueue, sizeof(unsigned long long), &p_queue));
@@ -728,7 +879,7 @@ int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opt
                              c->stream));
     HIPCHK(c, hipMemcpyAsync(p_first, sh.first.data(), sizeof(int64_t) * sh.first.size(),
                              hipMemcpyHostToDevice, c->stream));
-    const PoolArgs p = pool_args(sh, p_sites, p_first, p_queue, p_hbm, p_rec);
+    const PoolArgs p = pool_args(sh, p_sites, p_first, p_queue, p_hbm, p_rec, p_beta);
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     TRY(pool_launch(c, sh, p));
@@ -855,6 +1006,7 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
             return fail(c, OKA_ERR_ARG, "window marks need the workgroup schedule");
         a.tiles_x = (a.gncols + 31) / 32;
         a.ntiles = a.tiles_x * ((a.gnrows + 31) / 32);
+        TRY(stage_far(c, site, o, flags, a));
         TRY(run_pool(c, &a, 1, o, timed));
         return chronicle_finish(c, a, rasterize, nnodes, ntr, counts, bbox, trace_status,
                                 trace_steps, stats, flags, win, marks);

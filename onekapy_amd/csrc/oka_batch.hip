@@ -25,8 +25,8 @@ struct oka_batch {
     void *d_sites = nullptr, *d_first = nullptr, *d_queue = nullptr;
     void *d_fits = nullptr, *d_fit_first = nullptr;   // fit table, its two prefix arrays
     int64_t fit_logs = 0, fit_reals = 0;
-    void *d_hbm = nullptr, *d_rec = nullptr;
-    size_t hbm_cap = 0, rec_cap = 0, first_cap = 0;
+    void *d_hbm = nullptr, *d_rec = nullptr, *d_beta = nullptr;
+    size_t hbm_cap = 0, rec_cap = 0, first_cap = 0, beta_cap = 0;
     hipGraphExec_t exec = nullptr;
 };
 
@@ -70,6 +70,9 @@ int stage_site(oka_batch *B, int s, const oka_batch_site &bs)
         return fail(c, OKA_ERR_ARG, "site %d: missing realization inputs", s);
     if (o.npaths > 0 && !o.start_xy)
         return fail(c, OKA_ERR_ARG, "site %d: start_xy is NULL", s);
+    if (o.far_order > kFarTerms)
+        return fail(c, OKA_ERR_ARG, "site %d: far_order %d above %d", s, o.far_order,
+                    kFarTerms);
     if (o.precision != B->precision)
         return fail(c, OKA_ERR_ARG, "site %d: every site of a batch traces in one precision", s);
     if (g.nrows <= 0 || g.ncols <= 0 || !(g.dx > 0.0) || !(g.dy > 0.0))
@@ -128,6 +131,14 @@ int stage_site(oka_batch *B, int s, const oka_batch_site &bs)
     TRY(balloc(B, sizeof(int32_t) * B->nnodes[s], &p));
     a.counts = (int32_t *)p;
     a.stats = B->stats + (size_t)s * kStatCount;
+    FarPlan P;
+    far_plan(a, site->wells_xy, o.precision, o.far_order, &P);
+    if (P.order > 0) {
+        const int32_t *d[2];
+        TRY(bcopy(B, P.near.data(), P.near.size(), &d[0]));
+        TRY(bcopy(B, P.wbox.data(), P.wbox.size(), &d[1]));
+        far_args(P, d[0], d[1], a);
+    }
     return OKA_OK;
 }
 
@@ -156,7 +167,13 @@ int capture(oka_batch *B)
         TRY(balloc(B, (size_t)sh.rec_words * 4, &B->d_rec));
         B->rec_cap = (size_t)sh.rec_words * 4;
     }
-    const PoolArgs p = pool_args(sh, B->d_sites, B->d_first, B->d_queue, B->d_hbm, B->d_rec);
+    const size_t beta_bytes = sizeof(float2) * sh.waves * sh.slots * sh.far_stride;
+    if (beta_bytes > 0 && (!B->d_beta || B->beta_cap < beta_bytes)) {
+        TRY(balloc(B, beta_bytes, &B->d_beta));
+        B->beta_cap = beta_bytes;
+    }
+    const PoolArgs p =
+        pool_args(sh, B->d_sites, B->d_first, B->d_queue, B->d_hbm, B->d_rec, B->d_beta);
     hipGraph_t graph = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = fill_u32(c, B->stats, 2 * (int64_t)kStatCount * B->nsites);

@@ -139,62 +139,100 @@ __device__ __forceinline__ void feval32(const Field32 &f, float x, float y, floa
 // so the far wells sum to B_b(u) = sum_{k < order} beta_k u^k with
 //   beta_k = -(1/L) sum_{far w} s_w / v_w^(k+1),
 // truncated where |u / v_w|^order <= (sqrt(2) / 3)^order (6e-6 at order 16).
-// The near wells are summed directly, as in feval32.
+// Every box keeps kFarTerms coefficients (zero from the order up), so the
+// evaluation is one fixed-length Horner sum.  The near wells are summed
+// directly, as in feval32.
+// Loads through global-memory pointers (address space 1): global loads,
+// counted apart from LDS reads, of clang vector types.
+typedef float VecF4 __attribute__((ext_vector_type(4)));
+typedef int VecI4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) VecF4 *GlobalF4;
+typedef const __attribute__((address_space(1))) VecI4 *GlobalI4;
+
 struct FarField {
-    int nbx, nby, order;
+    int nbx, nby, order, near_stride;
     float x0, y0, len, ilen;        // box grid origin and side, 1 / side
-    const int *near_first;          // [nbx nby + 1]: CSR offsets of the near lists
-    const int *near;                // near-well indices per box
-    const float2 *beta;             // [nbx nby][order] of this slot
+    const int *near;                // [nbx nby][near_stride] (a multiple of 4):
+                                    // per box the count of near wells, their
+                                    // indices, then -1
+    const int *wbox;                // [nw][2]: each well's box column and row
+    float2 *beta;                   // [nbx nby][kFarTerms] of this slot
 };
 
-__device__ __forceinline__ bool far_near(const FarField &F, int b, float wx, float wy)
-{
-    const int bx = b % F.nbx, by = b / F.nbx;
-    const float wbx = floorf((wx - F.x0) * F.ilen), wby = floorf((wy - F.y0) * F.ilen);
-    return fabsf(wbx - (float)bx) <= 1.0f && fabsf(wby - (float)by) <= 1.0f;
-}
-
 // beta of boxes lane, lane + stride, ... from the wells w[0, nw): each lane
-// keeps one box's coefficients in registers while it walks the wells.
-template <int kMaxOrder>
+// walks the wells once per kFarChunk coefficients of one box, keeping only
+// those in registers (the power s_w / v_w^(k0+1) that starts a chunk is
+// rebuilt by repeated multiplication).  Coefficients from the order up to
+// kFarTerms are zero.
+constexpr int kFarChunk = 4;
+static_assert(kFarTerms % kFarChunk == 0 && kFarTerms % 2 == 0, "whole chunks and float4 pairs");
+
 __device__ __forceinline__ void far_coefficients(const FarField &F, const float4 *w, int nw,
                                                  float2 *beta, int lane, int stride)
 {
     const int nb = F.nbx * F.nby;
     for (int b = lane; b < nb; b += stride) {
-        const float cx = F.x0 + ((float)(b % F.nbx) + 0.5f) * F.len;
-        const float cy = F.y0 + ((float)(b / F.nbx) + 0.5f) * F.len;
-        float ar[kMaxOrder], ai[kMaxOrder];
+        const int bx = b % F.nbx, by = b / F.nbx;
+        const float cx = F.x0 + ((float)bx + 0.5f) * F.len;
+        const float cy = F.y0 + ((float)by + 0.5f) * F.len;
+        for (int k0 = 0; k0 < kFarTerms; k0 += kFarChunk) {
+            float ar[kFarChunk], ai[kFarChunk];
 #pragma unroll
-        for (int k = 0; k < kMaxOrder; ++k)
-            ar[k] = ai[k] = 0.0f;
-        for (int i = 0; i < nw; ++i) {
-            const float4 wl = w[i];
-            if (far_near(F, b, wl.x, wl.y))
-                continue;
-            const float vx = (wl.x - cx) * F.ilen, vy = (wl.y - cy) * F.ilen;
-            const float iv = 1.0f / fmaf(vx, vx, vy * vy);
-            const float qx = vx * iv, qy = -vy * iv;       // 1 / v_w
-            float tx = wl.z * qx, ty = wl.z * qy;          // s_w / v_w^(k+1)
+            for (int k = 0; k < kFarChunk; ++k)
+                ar[k] = ai[k] = 0.0f;
+            for (int i = 0; k0 < F.order && i < nw; ++i) {
+                if (abs(F.wbox[2 * i] - bx) <= 1 && abs(F.wbox[2 * i + 1] - by) <= 1)
+                    continue;   // near: summed directly
+                const float4 wl = w[i];
+                const float vx = (wl.x - cx) * F.ilen, vy = (wl.y - cy) * F.ilen;
+                const float iv = 1.0f / fmaf(vx, vx, vy * vy);
+                const float qx = vx * iv, qy = -vy * iv;   // 1 / v_w
+                float tx = wl.z * qx, ty = wl.z * qy;      // s_w / v_w
+                for (int k = 0; k < k0; ++k) {
+                    const float nx = fmaf(tx, qx, -ty * qy);
+                    ty = fmaf(tx, qy, ty * qx);
+                    tx = nx;
+                }
 #pragma unroll
-            for (int k = 0; k < kMaxOrder; ++k) {
-                if (k < F.order) {
+                for (int k = 0; k < kFarChunk; ++k) {
                     ar[k] -= tx;
                     ai[k] -= ty;
+                    const float nx = fmaf(tx, qx, -ty * qy);
+                    ty = fmaf(tx, qy, ty * qx);
+                    tx = nx;
                 }
-                const float nx = fmaf(tx, qx, -ty * qy);
-                ty = fmaf(tx, qy, ty * qx);
-                tx = nx;
+            }
+#pragma unroll
+            for (int k = 0; k < kFarChunk; ++k) {
+                const bool on = k0 + k < F.order;
+                beta[(size_t)b * kFarTerms + k0 + k] =
+                    make_float2(on ? ar[k] * F.ilen : 0.0f, on ? ai[k] * F.ilen : 0.0f);
             }
         }
-        for (int k = 0; k < F.order; ++k)
-            beta[(size_t)b * F.order + k] = make_float2(ar[k] * F.ilen, ai[k] * F.ilen);
     }
 }
 
+// The well terms of four near wells (index -1: none).
+__device__ __forceinline__ void near_terms(const float4 *w, VecI4 n, float x, float y, float &vx,
+                                           float &vy)
+{
+    const float4 w0 = w[max(n.x, 0)], w1 = w[max(n.y, 0)];
+    const float4 w2 = w[max(n.z, 0)], w3 = w[max(n.w, 0)];
+    if (n.x >= 0)
+        well_term(w0, x, y, vx, vy);
+    if (n.y >= 0)
+        well_term(w1, x, y, vx, vy);
+    if (n.z >= 0)
+        well_term(w2, x, y, vx, vy);
+    if (n.w >= 0)
+        well_term(w3, x, y, vx, vy);
+}
+
 // k = -V at (x, y) with the far wells of its box taken from the expansion;
-// false (k untouched) when (x, y) lies outside the boxes.
+// false (k untouched) when (x, y) lies outside the boxes.  The box's
+// coefficients and the first four words of its near row (the count and three
+// indices) are loaded together, and each next four indices while the current
+// four are summed.
 __device__ __forceinline__ bool feval32_far(const Field32 &f, const FarField &F, float x, float y,
                                             float &kx, float &ky)
 {
@@ -203,24 +241,50 @@ __device__ __forceinline__ bool feval32_far(const Field32 &f, const FarField &F,
     if (!(bxf >= 0.0f && bxf < (float)F.nbx && byf >= 0.0f && byf < (float)F.nby))
         return false;
     const int b = (int)byf * F.nbx + (int)bxf;
+    const GlobalI4 row = (GlobalI4)(F.near + (size_t)b * F.near_stride);
+    VecI4 cur = row[0];
+    const GlobalF4 a = (GlobalF4)(F.beta + (size_t)b * kFarTerms);
+    VecF4 c[kFarTerms / 2];   // coefficients 2i (x, y) and 2i + 1 (z, w)
+#pragma unroll
+    for (int i = 0; i < kFarTerms / 2; ++i)
+        c[i] = a[i];
     const float ux = fx - bxf - 0.5f, uy = fy - byf - 0.5f;
-    const float2 *a = F.beta + (size_t)b * F.order;
-    float gr = a[F.order - 1].x, gi = a[F.order - 1].y;
-    for (int k = F.order - 2; k >= 0; --k) {
-        const float2 ak = a[k];
-        const float nr = fmaf(gr, ux, fmaf(-gi, uy, ak.x));
-        gi = fmaf(gr, uy, fmaf(gi, ux, ak.y));
+    // Horner in u from the highest coefficient
+    float gr = c[kFarTerms / 2 - 1].z, gi = c[kFarTerms / 2 - 1].w;
+#pragma unroll
+    for (int k = kFarTerms - 2; k >= 0; --k) {
+        const VecF4 ck = c[k >> 1];
+        const float ar = (k & 1) ? ck.z : ck.x, ai = (k & 1) ? ck.w : ck.y;
+        const float nr = fmaf(gr, ux, fmaf(-gi, uy, ar));
+        gi = fmaf(gr, uy, fmaf(gi, ux, ai));
         gr = nr;
     }
     float vx = fmaf(f.a2, x, fmaf(f.c, y, f.d)) + gr;
     float vy = fmaf(f.b2, y, fmaf(f.c, x, f.e)) - gi;
-    const int j1 = F.near_first[b + 1];
-    for (int j = F.near_first[b]; j < j1; ++j)
-        well_term(f.w[F.near[j]], x, y, vx, vy);
+    const VecI4 none = {-1, -1, -1, -1};
+    const int groups = (cur.x + 4) >> 2;   // groups of 4 words holding the count and indices
+    cur.x = -1;
+    for (int g = 0; g < groups; ++g) {
+        const VecI4 next = g + 1 < groups ? row[g + 1] : none;
+        near_terms(f.w, cur, x, y, vx, vy);
+        cur = next;
+    }
     kx = vx;
     ky = vy;
     return true;
 }
+
+// The confined field through the far-well expansions where the point lies
+// in a box, directly elsewhere, as finish32's evaluator.
+struct LaneEvalFar {
+    const Field32 &f;
+    const FarField &F;
+    __device__ __forceinline__ void operator()(float x, float y, float &kx, float &ky) const
+    {
+        if (!feval32_far(f, F, x, y, kx, ky))
+            feval32<true>(f, x, y, kx, ky);
+    }
+};
 
 // The lane's own field evaluation, as finish32's evaluator; `dry` collects the
 // unconfined field's AquiferError.

@@ -18,6 +18,17 @@ enum BitmapKind { kBmLdsFlat = 0, kBmTiled = 1, kBmHbmFlat = 2 };
 constexpr int kTileEmpty = -1, kTileOverflow = -2;
 constexpr int kMinTileSlots = 16;
 
+// Far-well expansions of the FP32 pool (fp32::FarField): the number of
+// coefficients each box keeps and evaluates (those above a plan's order are
+// zero), which is also the largest order and the one oka_trace_opts.far_order
+// = 0 chooses from kFarAutoWells wells on.
+constexpr int kFarTerms = 16;
+constexpr int kFarAutoWells = 64;
+// A plan's boxes hold about kFarNear wells in each 3 x 3 neighbourhood, with
+// at most kFarMaxSide boxes a side.
+constexpr int kFarNear = 2;
+constexpr int kFarMaxSide = 64;
+
 // Device-side statistics accumulators (unsigned 64-bit, atomically added).
 enum StatSlot {
     kStatAttempted = 0,
@@ -68,6 +79,14 @@ struct TraceArgs {
     // per-realization window marks (oka_trace_marks); wmarks == nullptr: off
     int32_t w_row0, w_col0, w_cell, w_nr, w_nc;
     uint8_t *wmarks;            // nreal x (w_nr * w_nc)
+    // FP32 pool: far wells as local expansions of this order (0: every well
+    // summed directly); boxes of side far_len from (far_x0, far_y0), in
+    // coordinates relative to (xo, yo) (fp32::FarField)
+    int32_t far_order, far_nbx, far_nby, far_near_stride;
+    float far_x0, far_y0, far_len;
+    const int32_t *far_near;        // [far_nbx * far_nby][far_near_stride]: per box
+                                    // the near-well count, their indices, then -1
+    const int32_t *far_wbox;        // [nwells][2]: each well's box column and row
 };
 
 // One marked node of a realization: its count (probabilityfield.py:357-359)
@@ -413,6 +432,8 @@ struct PoolArgs {
     int32_t *rec_done;          // [nrec]: chunks merged; zero before the launch
     uint32_t *rec_bits;         // [nrec][max_ntiles][32]; zero before the launch
     uint32_t *rec_dirty;        // [nrec][dirty_words]; zero before the launch
+    float2 *far_beta;           // [waves][slots][far_stride]: far-well coefficients
+    int64_t far_stride;         // float2 per slot (the largest nbx * nby * order)
 };
 // LDS bytes of one wave of the pool in `precision` (OKA_FP32 or
 // OKA_FP64_EXACT); conf_mode: 0 every site confined, 1 every site

@@ -26,6 +26,7 @@ struct PoolFP32 {
         float gx0, gx1, gy0, gy1;           // grid bounds relative to the origin
         double dur, absdur, sg, xo, yo;
         long long cap;
+        FarField far;                       // far.order == 0: every well summed directly
     };
     struct Lane {
         Trace32 tr;
@@ -54,7 +55,8 @@ struct PoolFP32 {
         return true;
     }
 
-    __device__ static void stage_consts(Consts &k, const TraceArgs &A, int64_t r, bool)
+    __device__ static void stage_consts(Consts &k, const PoolArgs &P, const TraceArgs &A,
+                                        int64_t r, int s, bool)
     {
         const double bn = A.b[r] * A.n[r];
         const double h = A.confined ? 1.0 : 0.5 * A.K[r] * (A.b[r] * A.b[r]);
@@ -89,6 +91,27 @@ struct PoolFP32 {
         k.gy1 = (float)(A.gymax - A.yo);
         k.xo = A.xo;
         k.yo = A.yo;
+        FarField &F = k.far;
+        F.order = A.confined && A.far_order > 0 ? min(A.far_order, kFarTerms) : 0;
+        F.nbx = A.far_nbx;
+        F.nby = A.far_nby;
+        F.x0 = A.far_x0;
+        F.y0 = A.far_y0;
+        F.len = A.far_len;
+        F.ilen = 1.0f / A.far_len;
+        F.near_stride = A.far_near_stride;
+        F.near = A.far_near;
+        F.wbox = A.far_wbox;
+        F.beta = P.far_beta + ((int64_t)blockIdx.x * P.slots + s) * P.far_stride;   // 128-byte rows
+    }
+
+    // The far-well coefficients of the slot's realization, one box per lane.
+    __device__ static bool stage_slot(const Consts &k, const float4 *w, int nw)
+    {
+        if (k.far.order <= 0)
+            return false;
+        far_coefficients(k.far, w, nw, k.far.beta, threadIdx.x, kWave);
+        return true;
     }
 
     __device__ static Integ32 integ(const Consts &k)
@@ -151,7 +174,11 @@ struct PoolFP32 {
         float dt, px, py, kx, ky;
         bool dry = false;   // an evaluation raised AquiferError (unconfined only)
         first_point(I, ln.tr, ln.fresh, dtd, dt, px, py);
-        feval32<kConfined>(f, px, py, kx, ky, dry);
+        const bool far = kConfined && k.far.order > 0;
+        if (far)
+            LaneEvalFar{f, k.far}(px, py, kx, ky);
+        else
+            feval32<kConfined>(f, px, py, kx, ky, dry);
         if (ln.fresh) {
             ln.fresh = false;
             if (dry) {   // k1 raised: the trace is its start vertex (capturezone.py:218, 249)
@@ -161,7 +188,9 @@ struct PoolFP32 {
             return take_k1(I, ln.tr, kx, ky) ? 0 : 2;
         }
         float ox, oy;
-        const int res = finish32(LaneEval<kConfined>{f, dry}, I, ln.tr, dtd, dt, kx, ky, ox, oy);
+        const int res = far ? finish32(LaneEvalFar{f, k.far}, I, ln.tr, dtd, dt, kx, ky, ox, oy)
+                            : finish32(LaneEval<kConfined>{f, dry}, I, ln.tr, dtd, dt, kx, ky, ox,
+                                       oy);
         // a stage raised: the step is not accepted and the trace ends
         if (!kConfined && dry)
             ln.tr.status = OKA_TRACE_TRUNCATED;

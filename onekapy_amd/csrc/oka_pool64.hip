@@ -55,7 +55,8 @@ struct PoolFP64 {
     }
 
     // make_field and make_integ of the workgroup tracer, per slot.
-    __device__ static void stage_consts(Consts &k, const TraceArgs &A, int64_t r, bool plain)
+    __device__ static void stage_consts(Consts &k, const PoolArgs &, const TraceArgs &A, int64_t r,
+                                        int, bool plain)
     {
         const double *c = A.coef + 6 * r;
         k.xo = A.xo;
@@ -86,6 +87,8 @@ struct PoolFP64 {
         k.u2 = A.umbra * A.umbra;
         k.plain = plain ? 1 : 0;
     }
+
+    __device__ static bool stage_slot(const Consts &, const Well64 *, int) { return false; }
 
     __device__ static Field64 field(const Consts &k, const Well64 *w, int nw)
     {

@@ -27,6 +27,8 @@
 //   stage_wells(...)    the lanes stage realization r's wells (returns the
 //                       wave-uniform flag that stage_consts receives);
 //   stage_consts(...)   lane 0 fills the slot's constants;
+//   stage_slot(...)     the lanes' work on the staged slot (after a barrier),
+//                       such as per-realization tables that the lanes then read;
 //   open(...)           a lane takes particle p of the slot's realization;
 //   advance<kConf>(...) one iteration of the lane's trace (one field
 //                       evaluation for a fresh trace, an attempted step
@@ -265,10 +267,11 @@ __device__ void open_slot(const PoolArgs &P, const PoolLds<Prec> &L, int s, int6
     const int64_t local = item - P.site_first[site];
     const int64_t r = nch > 0 ? local / nch : 0;
     const int ch = nch > 0 ? (int)(local - r * nch) : 0;
-    const bool flag = Prec::stage_wells(L.wells + (size_t)s * P.max_nwells, A, r);
+    typename Prec::Well *w = L.wells + (size_t)s * P.max_nwells;
+    const bool flag = Prec::stage_wells(w, A, r);
     if (threadIdx.x == 0) {
         SlotHdr<Prec> &h = L.hdr[s];
-        Prec::stage_consts(h.k, A, r, flag);
+        Prec::stage_consts(h.k, P, A, r, s, flag);
         h.att = h.acc = h.esc = h.cand = h.vis = 0;
         for (int j = 0; j < 4; ++j)
             h.st[j] = 0;
@@ -288,6 +291,8 @@ __device__ void open_slot(const PoolArgs &P, const PoolLds<Prec> &L, int s, int6
         h.confined = A.confined != 0;
     }
     __syncthreads();
+    if (Prec::stage_slot(L.hdr[s].k, w, A.nwells))
+        __syncthreads();
     const bool ready = L.hdr[s].nchunks <= 1 || claim_record(P, L.hdr[s]);
     if (threadIdx.x == 0)
         L.hdr[s].state = ready ? 1 : 2;

@@ -49,9 +49,10 @@ class Realizations:
 
 
 class ChronicleResult:
-    def __init__(self, counts, status, stats, passes):
+    def __init__(self, counts, status, stats, passes, bbox=None):
         self.counts = counts          # int32 [nrows, ncols] on the field's final grid
         self.trace_status = status    # int32 [R * P]
+        self.bbox = bbox              # float64 [R * P, 4]: xmin, xmax, ymin, ymax per trace
         self.stats = stats            # _lib.TraceStats_ of the rasterizing launch
         self.passes = passes          # 1 (fused pass was exact) or 2 (history re-trace)
 
@@ -68,7 +69,7 @@ def _launch(ctx, site, real, opts, grid, epochs, rasterize, counts, bbox, status
 def chronicle_into_field(pfield, wells_xy, xo, yo, real, starts, duration, tol, maxstep, umbra,
                          confined=True, precision=_lib.FP64_EXACT, max_steps=0,
                          bitmap_mode=_lib.BITMAP_AUTO, tile_slots=0, schedule=_lib.SCHED_AUTO,
-                         pool_slots=0, pool_waves_per_cu=0, pool_chunk=0):
+                         pool_slots=0, pool_waves_per_cu=0, pool_chunk=0, far_order=0):
     """Trace all realizations and rasterize them into ``pfield``'s lattice.
 
     Grows ``pfield`` exactly as the reference's per-trace ``rasterize`` calls
@@ -76,7 +77,10 @@ def chronicle_into_field(pfield, wells_xy, xo, yo, real, starts, duration, tol, 
     node (the caller registers them).  Nothing is registered here.
     bitmap_mode / tile_slots place the per-realization visited-node bitmap
     and schedule / pool_slots / pool_waves_per_cu / pool_chunk choose how particles are
-    scheduled (oka_trace_opts); every choice gives the same counts.
+    scheduled (oka_trace_opts); every choice gives the same counts.  far_order
+    (FP32 pool, confined) sums distant wells as local expansions: 0 lets the
+    library choose (on from 64 wells), < 0 sums every well directly, as the
+    workgroup schedule does.
     """
     wells_xy = _lib.f64(wells_xy).reshape(-1, 2)
     starts = _lib.f64(starts).reshape(-1, 2)
@@ -86,18 +90,19 @@ def chronicle_into_field(pfield, wells_xy, xo, yo, real, starts, duration, tol, 
     site = _lib.make_site(wells_xy, xo, yo)
     opts = _lib.make_trace_opts(P, duration, tol, maxstep, umbra, _lib.ptr(starts), precision,
                                 confined, max_steps, bitmap_mode, tile_slots, schedule,
-                                pool_slots, pool_waves_per_cu, pool_chunk)
+                                pool_slots, pool_waves_per_cu, pool_chunk, far_order)
     bbox = np.zeros((ntr, 4))
     status = np.zeros(ntr, dtype=np.int32)
     stats = _lib.TraceStats_()
     if ntr == 0:
-        return ChronicleResult(np.zeros((pfield.nrows, pfield.ncols), np.int32), status, stats, 0)
+        return ChronicleResult(np.zeros((pfield.nrows, pfield.ncols), np.int32), status, stats, 0,
+                               bbox)
     if not pfield.empty:
         g = pfield.grid()
         counts = np.zeros((pfield.nrows, pfield.ncols), dtype=np.int32)
         _launch(ctx, site, real, opts, g, None, True, counts, bbox, status, stats)
         if stats.escaped == 0:
-            return ChronicleResult(counts, status, stats, 1)
+            return ChronicleResult(counts, status, stats, 1, bbox)
     else:
         _launch(ctx, site, real, opts, pfield.grid(), None, False, None, bbox, status, stats)
     final, epochs = _lib.grid_history(pfield.grid(), bbox)
@@ -106,7 +111,7 @@ def chronicle_into_field(pfield, wells_xy, xo, yo, real, starts, duration, tol, 
     counts = np.zeros((pfield.nrows, pfield.ncols), dtype=np.int32)
     stats = _lib.TraceStats_()
     _launch(ctx, site, real, opts, pfield.grid(), epochs, True, counts, None, status, stats)
-    return ChronicleResult(counts, status, stats, 2)
+    return ChronicleResult(counts, status, stats, 2, bbox)
 
 
 def warn_statuses(status):

@@ -214,14 +214,15 @@ def _site_struct(site):
     return wxy, _lib.make_site(wxy, site.xt, site.yt)
 
 
-def trace_counts(site, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_AUTO):
+def trace_counts(site, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_AUTO, far_order=0):
     """Fine-grid counts of a batch of realizations on the site's fixed grid."""
     pf = site.fixed_field()
     g = pf.grid()
     wxy, s = _site_struct(site)
     starts = start_points(site.xt, site.yt, site.rt, site.npaths)
     o = _lib.make_trace_opts(site.npaths, site.duration, site.tol, site.maxstep, site.umbra,
-                             _lib.ptr(starts), precision, site.confined, schedule=schedule)
+                             _lib.ptr(starts), precision, site.confined, schedule=schedule,
+                             far_order=far_order)
     counts = np.zeros((pf.nrows, pf.ncols), np.int32)
     st = _lib.TraceStats_()
     K, n, b, q = (np.ascontiguousarray(d[k]) for k in ("K", "n", "b", "q"))

@@ -18,8 +18,10 @@ def _small_batch():
 
 
 def test_batch_matches_per_site_runs(gpu_ctx):
+    """Every site of a batch with far-well expansions off (far_order -1) equals
+    the workgroup tracer run on that site alone."""
     ss, draws = _small_batch()
-    b = SiteBatch(ss, draws)
+    b = SiteBatch(ss, draws, far_order=-1)
     b.run()
     for i, (s, d) in enumerate(zip(ss, draws)):
         print(f"site {i}: {s.name}", flush=True)
@@ -29,13 +31,32 @@ def test_batch_matches_per_site_runs(gpu_ctx):
         assert np.array_equal(coef, ref_coef)
         # the same tracer, one site at a time, on the site's fixed lattice (the
         # batch drops what falls outside it, as a pre-expanded field does)
-        ref_counts, ref_st = trace_counts(s, d, ref_coef, _lib.FP32)
+        ref_counts, ref_st = trace_counts(s, d, ref_coef, _lib.FP32, far_order=-1)
         assert np.array_equal(counts, ref_counts)
         assert (st.attempted, st.accepted, st.visited, st.escaped) == (
             ref_st.attempted, ref_st.accepted, ref_st.visited, ref_st.escaped)
         assert counts.max() <= len(d["K"])
     fields = b.fields()
     assert fields[0].total_weight == len(draws[0]["K"])
+    b.close()
+
+
+def test_batch_far_expansions_match_single_site_pool(gpu_ctx):
+    """With the library's choice of far-well expansions (sites of 64 wells or
+    more), each site of the batch equals the pool tracer run on that site alone
+    with the same choice: the batch stages the same box plan, and each slot's
+    coefficients depend only on its realization."""
+    ss, draws = _small_batch()
+    assert max(len(s.wells) for s in ss) >= 64
+    b = SiteBatch(ss, draws)
+    b.run()
+    for i, (s, d) in enumerate(zip(ss, draws)):
+        print(f"site {i}: {len(s.wells)} wells", flush=True)
+        counts, coef, st = b.results(i)
+        ref_counts, ref_st = trace_counts(s, d, coef, _lib.FP32, _lib.SCHED_POOL)
+        assert np.array_equal(counts, ref_counts), i
+        assert (st.attempted, st.accepted, st.visited) == (
+            ref_st.attempted, ref_st.accepted, ref_st.visited), i
     b.close()
 
 
@@ -52,7 +73,7 @@ def test_config2_full_size_batch_matches_per_site_runs(gpu_ctx):
     for i, (s, d) in enumerate(zip(ss, draws)):
         print(f"site {i}: {s.name}", flush=True)
         counts, coef, st = b.results(i)
-        ref_counts, ref_st = trace_counts(s, d, coef, _lib.FP32)
+        ref_counts, ref_st = trace_counts(s, d, coef, _lib.FP32, _lib.SCHED_POOL)
         assert np.array_equal(counts, ref_counts), i
         assert (st.attempted, st.accepted, st.visited) == (
             ref_st.attempted, ref_st.accepted, ref_st.visited), i
@@ -71,7 +92,7 @@ def test_batch_replays_and_retunes_identically(gpu_ctx):
     for s, d in zip(ss, draws):
         _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"],
                                   d["b"], d["q"], d["z"])
-        eager.append(trace_counts(s, d, coef, _lib.FP32)[0])
+        eager.append(trace_counts(s, d, coef, _lib.FP32, _lib.SCHED_POOL)[0])
     b = SiteBatch(ss, draws)
     for rep in range(4):                      # counts restart from zero on every replay
         print(f"replay {rep}", flush=True)
@@ -207,7 +228,7 @@ def test_batch_mixes_confined_and_unconfined_sites(gpu_ctx, precision):
     for i, (s, d) in enumerate(zip(ss, draws)):
         print(f"{s.name}: confined {s.confined}", flush=True)
         counts, coef, st = b.results(i)
-        ref, ref_st = trace_counts(s, d, coef, precision)
+        ref, ref_st = trace_counts(s, d, coef, precision, _lib.SCHED_POOL)
         assert np.array_equal(counts, ref)
         assert (st.attempted, st.accepted, st.visited) == (ref_st.attempted, ref_st.accepted,
                                                             ref_st.visited)

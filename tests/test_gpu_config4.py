@@ -80,7 +80,7 @@ def test_config4_fp32_pool_agrees_with_fp64_exact(gpu_ctx, cfg4_oracle):
     on average over the visited nodes."""
     s, d, coef, ocounts, ost = cfg4_oracle
     f64 = _run(s, d, coef, precision=_lib.FP64_EXACT)
-    pool = _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL)
+    pool = _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL, far_order=-1)
     wg = _run(s, d, coef, precision=_lib.FP32)
     assert np.array_equal(pool.counts, wg.counts)
     assert np.array_equal(pool.trace_status, wg.trace_status)
@@ -93,3 +93,38 @@ def test_config4_fp32_pool_agrees_with_fp64_exact(gpu_ctx, cfg4_oracle):
     print(f"max |p32 - p64| {err.max():.4f}, L1 {err.sum():.4f}, visited nodes "
           f"{(f64.counts > 0).sum()}", flush=True)
     assert err.sum() <= 0.01 * (f64.counts > 0).sum()
+
+
+@pytest.mark.parametrize("order", [0, 8, 16], ids=["auto", "order8", "order16"])
+def test_config4_far_expansions_agree_with_fp64_exact(gpu_ctx, cfg4_oracle, order):
+    """The FP32 pool tracer with the wells beyond the 3 x 3 boxes around each
+    point summed as local expansions (the automatic choice, order 16, and
+    order 8) ends the same traces the same way as FP64-exact to 1 % of the traces
+    and keeps the capture-probability error within the bound on average over
+    the visited nodes; its traces stay close to the direct FP32 sum's."""
+    s, d, coef, ocounts, ost = cfg4_oracle
+    f64 = _run(s, d, coef, precision=_lib.FP64_EXACT)
+    direct = _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL, far_order=-1)
+    far = _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL, far_order=order)
+    assert not np.array_equal(far.counts, direct.counts)   # the expansions were used
+    mism = int((far.trace_status != f64.trace_status).sum())
+    R = len(d["K"])
+    err = np.abs(far.counts - f64.counts) / R
+    rel = abs(far.stats.attempted - f64.stats.attempted) / f64.stats.attempted
+    print(f"order {order}: status mismatches {mism}, max |p - p64| {err.max():.4f}, "
+          f"L1 {err.sum():.4f}, visited {(f64.counts > 0).sum()}, steps {rel:.2e}", flush=True)
+    assert mism <= 0.01 * len(f64.trace_status)
+    assert err.sum() <= 0.01 * (f64.counts > 0).sum()
+    assert rel < 0.01
+    # bounding boxes of the traces that end normally in both
+    ok = (far.trace_status == 0) & (direct.trace_status == 0)
+    dev = np.abs(far.bbox[ok] - direct.bbox[ok]).max(axis=1)
+    print(f"bbox deviation from the direct sum: median {np.median(dev):.3g} m, "
+          f"95th percentile {np.percentile(dev, 95):.3g} m", flush=True)
+    assert np.median(dev) < 1.0
+
+
+def test_config4_far_order_above_limit_is_rejected(gpu_ctx, cfg4_oracle):
+    s, d, coef, _, _ = cfg4_oracle
+    with pytest.raises(_lib.OkaError, match="far_order"):
+        _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL, far_order=17)

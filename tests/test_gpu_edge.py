@@ -226,8 +226,9 @@ def test_config4_shape_properties(gpu_ctx):
     """The large regional stress case at its real shape (200 wells, 10 %
     injection, 200 observations, 2000^2 nodes at 10 m, 256 particles, 30
     years) on 1,000 of its realizations: FP32 within 0.01 of FP64-exact at
-    every node, the pool schedule equal to the workgroup one, counts additive
-    over realization halves and bounded by R."""
+    every node, with every well summed directly and with far-well expansions,
+    the pool schedule equal to the workgroup one, counts additive over
+    realization halves and bounded by R."""
     from onekapy_amd import fit_and_draw
     R = 1000
     s = sites.config4(R)
@@ -245,19 +246,23 @@ def test_config4_shape_properties(gpu_ctx):
     print("fp32 workgroup", flush=True)
     wg = run(slice(None), precision=_lib.FP32)
     print("fp32 pool", flush=True)
-    pool = run(slice(None), precision=_lib.FP32, schedule=_lib.SCHED_POOL)
+    pool = run(slice(None), precision=_lib.FP32, schedule=_lib.SCHED_POOL, far_order=-1)
     assert np.array_equal(pool.counts, wg.counts)
     assert list(pool.stats.status_count) == list(wg.stats.status_count)
-    print("fp32 halves", flush=True)
+    print("fp32 pool, far-well expansions: halves and whole", flush=True)
+    far = run(slice(None), precision=_lib.FP32, schedule=_lib.SCHED_POOL)
     halves = [run(h, precision=_lib.FP32, schedule=_lib.SCHED_POOL).counts
               for h in (slice(0, R // 2), slice(R // 2, R))]
-    assert np.array_equal(wg.counts, halves[0] + halves[1])
+    assert np.array_equal(far.counts, halves[0] + halves[1])
+    assert far.counts.max() <= R and far.counts.sum() == far.stats.visited
     assert wg.counts.max() <= R and wg.counts.sum() == wg.stats.visited
     print("fp64", flush=True)
     f64 = run(slice(None), precision=_lib.FP64_EXACT)
-    err = np.abs(wg.counts - f64.counts) / R
-    assert err.max() <= 0.01
-    assert err.sum() <= 0.01 * (f64.counts > 0).sum()
+    for name, res in (("direct", wg), ("far", far)):
+        err = np.abs(res.counts - f64.counts) / R
+        print(f"{name}: max |p32 - p64| {err.max():.4f}, L1 {err.sum():.3f}", flush=True)
+        assert err.max() <= 0.01
+        assert err.sum() <= 0.01 * (f64.counts > 0).sum()
 
 
 def test_window_marks_match_oracle_per_realization(gpu_ctx):
