@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define OKA_ABI_VERSION 4
+#define OKA_ABI_VERSION 5
 
 enum {
     OKA_OK = 0,

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(_lib.EXPORTED)
     for name in declared:
         assert hasattr(L, name), name
-    assert L.oka_abi_version() == 4
+    assert L.oka_abi_version() == 5
 
 
 def test_missing_library_fails_loudly(monkeypatch):
