@@ -86,17 +86,14 @@ def parse():
                     help="trace the unconfined velocity (compute_velocity) in the fp64/fp32 modes; "
                          "the roofline keeps the confined flop count")
     ap.add_argument("--only-cv", action="store_true", help="run only the control-variate leg")
-    ap.add_argument("--cv-traced", type=int, default=80000)
-    ap.add_argument("--cv-surrogate", type=int, default=8000000)
-    ap.add_argument("--cv-qmc", type=int, default=128,
-                    help="QMC replicates of the tuned estimator (scrambled Sobol' points, a "
-                         "power of two per replicate; 0: Philox draws)")
-    ap.add_argument("--cv-antithetic", action=argparse.BooleanOptionalAction, default=True,
-                    help="with Philox draws, draw the surrogate realizations in antithetic pairs")
-    ap.add_argument("--cv-config-qmc", type=int, default=64,
-                    help="QMC replicates of the estimator at the workload's own sizes (scrambled "
-                         "Sobol' points, a power of two per replicate, so the count rises to "
-                         "fill the budget; 0: antithetic pairs like the tuned one)")
+    ap.add_argument("--cv-train", type=int, default=100000,
+                    help="traced realizations the larger estimator's network trains on")
+    ap.add_argument("--cv-correct", type=int, default=200000,
+                    help="traced realizations that correct the larger estimator")
+    ap.add_argument("--cv-surrogate", type=int, default=2000000)
+    ap.add_argument("--cv-qmc", type=int, default=64,
+                    help="scrambled Sobol' replicates of the surrogate term (a power of two "
+                         "points each, so the count rises to fill the budget)")
     ap.add_argument("--cv-reference", type=int, default=1000000,
                     help="realizations of the FP64 plain run the estimates are compared with")
     ap.add_argument("--cv-steps", type=int, default=1200)
@@ -106,49 +103,40 @@ def parse():
 
 
 def control_variate_leg(site, args, fp32_rate):
-    """Learned control variate on the same site: N traced realizations (FP32)
-    + M surrogate evaluations, cross-fitted over 2 folds, compared with an
-    FP64 plain run.  Effective realizations/s = the plain-run size with the
-    same maximum (over window cells) standard error, divided by the
-    pipeline's wall time (draws, fits, pilot, tracing, training, surrogate).
-    Two estimators share that reference run: the tuned one (N = --cv-traced, M =
-    --cv-surrogate as --cv-qmc scrambled Sobol' replicates) and the workload's
-    own sizes (100,000 traced realizations to train on, at most 10^6 surrogate
-    evaluations as --cv-config-qmc replicates); replicate counts are raised
-    where the points per replicate are rounded down to a power of two."""
+    """Learned control variate on the same site at node resolution
+    (surrogate.control_variate_nodes): T traced realizations (FP32) train the
+    network, N further ones correct it node by node with coefficients on the
+    3 x 3 cells around each node, plus M surrogate evaluations; nodes outside
+    the window take the plain estimate of all T + N.  Both estimators are
+    compared node by node with one FP64-exact plain run: the workload's own
+    sizes (T = 100,000, as configured, N = 100,000, M = 10^6) and a larger one
+    with less error (--cv-train, --cv-correct, --cv-surrogate).  Effective realizations = the
+    plain-run size with the same largest node variance, max p (1 - p) /
+    max var; per second of the whole pipeline's wall time (draws, fits,
+    pilot, tracing, training, surrogate, correction)."""
     from onekapy_amd import _lib
     from onekapy_amd.stochastic import draw_realizations, fit_and_draw
-    from onekapy_amd.surrogate import cell_probabilities, control_variate, qmc_split, trace_counts
+    from onekapy_amd.surrogate import control_variate_nodes, trace_counts
 
-    def run(n_traced, n_surrogate, antithetic, qmc):
-        print(f"bench: control variate: {n_traced} traced + {n_surrogate} surrogate",
-              file=sys.stderr, flush=True)
+    def run(n_train, n_correct, n_surrogate):
+        print(f"bench: control variate: {n_train} traced to train, {n_correct} to correct, "
+              f"{n_surrogate} surrogate", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        res = control_variate(site, n_traced, n_surrogate, folds=2, train_steps=args.cv_steps,
-                              batch=args.cv_batch, lr=args.cv_lr, seed=123,
-                              precision=_lib.FP32, antithetic=antithetic, qmc_replicates=qmc,
-                              qmc_pow2=True)
+        res = control_variate_nodes(site, n_train, n_correct, n_surrogate,
+                                    train_steps=args.cv_steps, batch=args.cv_batch,
+                                    lr=args.cv_lr, seed=123, precision=_lib.FP32,
+                                    qmc_replicates=args.cv_qmc)
         wall = time.perf_counter() - t0
         print(f"bench: control variate {wall:.2f} s {res.times}", file=sys.stderr, flush=True)
         return res, wall
 
-    def sampling(antithetic, qmc, n_surrogate):
-        if qmc:
-            npts, nrep = qmc_split(n_surrogate // 2, qmc)
-            return f"scrambled Sobol' points, {nrep} replicates of {npts} per fold"
-        return "antithetic pairs" if antithetic else "independent draws"
-
     # untimed warm-up of the code paths the timed runs take (first launches,
     # first use of buffers), as the tracer legs' warmup steps do
     print("bench: control variate warm-up", file=sys.stderr, flush=True)
-    tuned_mode = (args.cv_antithetic and not args.cv_qmc, args.cv_qmc)
-    config_mode = (args.cv_antithetic and not args.cv_config_qmc, args.cv_config_qmc)
-    for anti, qmc in sorted({tuned_mode, config_mode}):
-        control_variate(site, 8192, 262144, folds=2, train_steps=64, batch=args.cv_batch,
-                        lr=args.cv_lr, seed=7, precision=_lib.FP32, antithetic=anti,
-                        qmc_replicates=qmc, qmc_pow2=True)
-    tuned = run(args.cv_traced, args.cv_surrogate, *tuned_mode)
-    configured = run(100000, 1000000, *config_mode)
+    control_variate_nodes(site, 8192, 8192, 1 << 17, train_steps=64, batch=args.cv_batch,
+                          lr=args.cv_lr, seed=7, precision=_lib.FP32, qmc_replicates=16)
+    tuned = run(args.cv_train, args.cv_correct, args.cv_surrogate)
+    configured = run(100000, 100000, 1000000)
     rng = np.random.default_rng(321)
     d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, args.cv_reference, rng)
     _, _, coef = fit_and_draw(site.wells_xy(), site.xt, site.yt, site.base, site.observations,
@@ -157,40 +145,50 @@ def control_variate_leg(site, args, fp32_rate):
     tj = time.perf_counter()
     counts, _ = trace_counts(site, d, coef, _lib.FP64_EXACT)
     tj = time.perf_counter() - tj
+    p_ref = counts / float(args.cv_reference)
+    var_ref = p_ref * (1 - p_ref) / args.cv_reference
 
-    def summary(res, wall, n_traced, n_surrogate):
-        p_ref = cell_probabilities(counts, args.cv_reference, res.window)
-        err_cv = np.abs(res.p_cv - p_ref)
+    def summary(res, wall):
+        n_tr = res.n_train + res.n_correct
+        err = np.abs(res.p - p_ref)
         err_plain = np.abs(res.p_plain - p_ref)
-        n_eff = n_traced * res.var_plain.max() / res.var_cv.max()
+        pc = np.clip(res.p, 0.0, 1.0)
+        n_eff = (pc * (1 - pc)).max() / res.var.max()
+        live = (p_ref > 0.02) & (p_ref < 0.98)
+        z = (res.p - p_ref)[live] / np.sqrt(res.var + var_ref)[live]
         eff_rate = n_eff / wall
+        w = res.window
         return {
-            "traced": n_traced, "surrogate_evals": n_surrogate,
-            "window": {"row0": res.window.row0, "col0": res.window.col0,
-                       "cell_nodes": res.window.cell,
-                       "cells": [res.window.nrows, res.window.ncols]},
-            "max_error_cv": float(err_cv.max()), "l1_error_cv": float(err_cv.sum()),
+            "traced_train": res.n_train, "traced_correct": res.n_correct,
+            "surrogate_evals": res.n_surrogate,
+            "surrogate_sampling": f"scrambled Sobol' points, {res.qmc[1]} replicates of "
+                                  f"{res.qmc[0]}",
+            "window": {"row0": w.row0, "col0": w.col0, "cell_nodes": w.cell,
+                       "cells": [w.nrows, w.ncols]},
+            "nodes": int(res.p.size),
+            "max_error": float(err.max()), "l1_error": float(err.sum()),
             "max_error_plain_same_traced": float(err_plain.max()),
             "l1_error_plain_same_traced": float(err_plain.sum()),
-            "max_std_cv": float(np.sqrt(res.var_cv.max())),
+            "rms_z_vs_reference": float(np.sqrt(np.mean(z ** 2))),
+            "max_std": float(np.sqrt(res.var.max())),
             "max_std_plain_same_traced": float(np.sqrt(res.var_plain.max())),
-            "variance_ratio_at_max": float(res.var_plain.max() / res.var_cv.max()),
-            "variance_ratio_sum": float(res.var_plain.sum() / res.var_cv.sum()),
+            "variance_ratio_at_max": float(res.var_plain.max() / res.var.max()),
+            "outside_window_mass": res.outside_mass, "total_mass": res.total_mass,
+            "outside_window_max": res.outside_max,
             "effective_realizations": float(n_eff),
+            "effective_per_traced": float(n_eff / n_tr),
             "wall_seconds": wall, "phase_seconds": res.times,
             "effective_realizations_per_s": float(eff_rate),
             "vs_fp32_plain_rate": float(eff_rate / fp32_rate) if fp32_rate else None,
         }
 
-    out = summary(*tuned, args.cv_traced, args.cv_surrogate)
-    out["surrogate_sampling"] = sampling(*tuned_mode, args.cv_surrogate)
-    out["surrogate_evals_used"] = int(tuned[0].n_surrogate_used)
-    out.update({"folds": 2, "train_steps_per_fold": args.cv_steps, "batch": args.cv_batch,
+    out = summary(*tuned)
+    out.update({"resolution": "nodes", "train_steps": args.cv_steps, "batch": args.cv_batch,
                 "lr": args.cv_lr, "network": "34->512->512->4096 GELU, BF16 MFMA",
-                "reference_run": {"realizations": args.cv_reference, "tracer": "fp64", "seconds": tj}})
-    out["as_configured"] = summary(*configured, 100000, 1000000)
-    out["as_configured"]["surrogate_sampling"] = sampling(*config_mode, 1000000)
-    out["as_configured"]["surrogate_evals_used"] = int(configured[0].n_surrogate_used)
+                "controls": "the 3 x 3 cells around each node, cross-fitted over 2 parts",
+                "reference_run": {"realizations": args.cv_reference, "tracer": "fp64",
+                                  "seconds": tj}})
+    out["as_configured"] = summary(*configured)
     return out
 
 

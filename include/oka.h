@@ -269,8 +269,10 @@ int oka_trace_chronicle_noalloc(oka_ctx *ctx, const oka_site *site, int32_t nrea
  * fine node (0, 0) is fine node (row0, col0) of `grid` (cells may lie partly
  * outside the grid).  For realization r, marks[r * nrows * ncols + c] is the
  * number of fine nodes of cell c that the realization visited (capped at 255).
- * Traces run on the fixed grid as in oka_trace_chronicle with nepochs == 0;
- * counts (may be NULL) accumulates as there. */
+ * node_masks (may be NULL; cells of at most 4 x 4 nodes) receives which of
+ * them: bit i * cell + j of node_masks[r * nrows * ncols + c] is fine node
+ * (i, j) of cell c.  Traces run on the fixed grid as in oka_trace_chronicle
+ * with nepochs == 0; counts (may be NULL) accumulates as there. */
 typedef struct {
     int32_t row0, col0;      /* fine-grid index of the window's first node     */
     int32_t cell;            /* fine nodes per cell side (1..15)               */
@@ -280,7 +282,8 @@ typedef struct {
 int oka_trace_marks(oka_ctx *ctx, const oka_site *site, int32_t nreal, const double *K,
                     const double *n, const double *b, const double *q, const double *coef,
                     const oka_trace_opts *opts, const oka_grid *grid, const oka_window *win,
-                    uint8_t *marks, int32_t *counts, oka_trace_stats *stats, uint32_t flags);
+                    uint8_t *marks, uint16_t *node_masks, int32_t *counts,
+                    oka_trace_stats *stats, uint32_t flags);
 
 /* ---- batches: many sites in one persistent, graph-captured run ------------
  * (the realization loop of stochastic.py:187-207 for many independent sites
@@ -425,6 +428,32 @@ typedef struct {
 int oka_draw_realizations(oka_ctx *ctx, int32_t nwells, const oka_dist *dists, int64_t n,
                           uint64_t seed, int64_t first, double *q, double *K, double *por,
                           double *thick, double *z, uint32_t flags);
+
+/* The correction term of the control variate at node resolution.  Traces the
+ * nreal correction realizations (as oka_trace_marks, window `win` of the
+ * network's n_out cells of cell x cell nodes, cell <= 4; counts, may be NULL,
+ * accumulates every node of the grid as in oka_trace_chronicle), evaluates
+ * the network on their standardized inputs x [nreal x n_in], and reduces, for
+ * nparts contiguous parts of the realizations (part p: rows
+ * [p nreal / nparts, (p + 1) nreal / nparts)), with s_a the network's output
+ * for cell a (the window's cells row-major, nrows x ncols) and y the visit of
+ * node k (= i * cell + j, fine node (i, j)) of cell a:
+ *   sum_s[p][a] = sum s_a,
+ *   sum_ss[p][a][h] = sum s_a s_b, b = a + offset h (0 if b is outside the
+ *     window), for the 13 offsets (dr, dc) = (0, 0..2), (1, -2..2), (2, -2..2),
+ *   sum_y[p][a][k] = sum y,
+ *   sum_ys[p][a][k][m] = sum y s_b, b = cell (r + m / 3 - 1, c + m % 3 - 1) of
+ *     cell a = (r, c) (0 if b is outside the window), m = 0..8
+ * (FP64, in a fixed order: the same inputs give the same sums) -- the terms
+ * of a per-node linear control on the 3 x 3 cells around the node's cell.
+ * The marks and node masks never leave the device.  Arrays follow flags;
+ * the four sums are host memory and are overwritten. */
+int oka_cv_node_sums(oka_ctx *ctx, oka_mlp *mlp, const oka_site *site, int32_t nreal,
+                     const double *K, const double *n, const double *b, const double *q,
+                     const double *coef, const float *x, const oka_trace_opts *opts,
+                     const oka_grid *grid, const oka_window *win, int32_t nparts, int32_t *counts,
+                     double *sum_s, double *sum_ss, double *sum_y, double *sum_ys,
+                     oka_trace_stats *stats, uint32_t flags);
 
 /* Exact moments of window marks [n x ncell] (uint8): per cell c,
  * sums[c] += sum over rows of marks[r, c] and, when sums_sq is set,

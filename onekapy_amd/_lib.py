@@ -39,7 +39,8 @@ EXPORTED = (
     "oka_abi_version", "oka_ctx_device_info", "oka_ctx_create", "oka_ctx_destroy", "oka_last_error",
     "oka_ctx_set_stream", "oka_ctx_stream", "oka_ctx_synchronize", "oka_model_eval",
     "oka_fit_draw", "oka_trace_chronicle", "oka_grid_history", "oka_trace_vertices",
-    "oka_rasterize", "oka_pow_fifth", "oka_log", "oka_trace_marks", "oka_mlp_create",
+    "oka_rasterize", "oka_pow_fifth", "oka_log", "oka_trace_marks", "oka_cv_node_sums",
+    "oka_mlp_create",
     "oka_mlp_destroy", "oka_mlp_param_count", "oka_mlp_set_params", "oka_mlp_get_params",
     "oka_mlp_grad", "oka_mlp_train", "oka_mlp_predict", "oka_mlp_accumulate",
     "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_cv_surrogate_sums_qmc",
@@ -177,7 +178,12 @@ def load():
     L.oka_trace_marks.restype = C.c_int
     L.oka_trace_marks.argtypes = [_vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp,
                                   C.POINTER(TraceOpts_), C.POINTER(Grid_), C.POINTER(Window_),
-                                  _vp, _vp, C.POINTER(TraceStats_), _u32]
+                                  _vp, _vp, _vp, C.POINTER(TraceStats_), _u32]
+    L.oka_cv_node_sums.restype = C.c_int
+    L.oka_cv_node_sums.argtypes = [_vp, _vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp,
+                                   _vp, C.POINTER(TraceOpts_), C.POINTER(Grid_),
+                                   C.POINTER(Window_), _i32, _vp, _vp, _vp, _vp, _vp,
+                                   C.POINTER(TraceStats_), _u32]
     L.oka_mlp_create.restype = C.c_int
     L.oka_mlp_create.argtypes = [_vp, _i32, _i32, _i32, _i32, C.POINTER(_vp)]
     L.oka_mlp_destroy.restype = None

@@ -826,7 +826,7 @@ static int trace_common(oka_ctx *c, const oka_site *site, int32_t nreal, const d
 static int chronicle_finish(oka_ctx *c, const TraceArgs &a, int32_t rasterize, int64_t nnodes,
                      int64_t ntr, int32_t *counts, double *bbox, int32_t *trace_status,
                      int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags,
-                     const oka_window *win, uint8_t *marks);
+                     const oka_window *win, uint8_t *marks, uint16_t *masks);
 
 // The far-well plan of a single-site pool launch, in the context's slots.
 int stage_far(oka_ctx *c, const oka_site *site, const oka_trace_opts *o, uint32_t flags,
@@ -928,7 +928,7 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
                           const oka_epoch *epochs, int32_t rasterize, int32_t *counts,
                           double *bbox, int32_t *trace_status, int64_t *trace_steps,
                           oka_trace_stats *stats, uint32_t flags, const oka_window *win,
-                          uint8_t *marks)
+                          uint8_t *marks, uint16_t *masks = nullptr)
 {
     if (!c)
         return OKA_ERR_ARG;
@@ -997,6 +997,11 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
         TRY(stage_out(c, kSlotMarks, marks, (size_t)nreal * win->nrows * win->ncols, flags, false,
                       &o2));
         a.wmarks = (uint8_t *)o2;
+        if (masks) {
+            TRY(stage_out(c, kSlotMasks, masks,
+                          sizeof(uint16_t) * nreal * win->nrows * win->ncols, flags, false, &o2));
+            a.wmasks = (uint16_t *)o2;
+        }
     }
 
     const bool timed = stats != nullptr;
@@ -1009,7 +1014,7 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
         TRY(stage_far(c, site, o, flags, a));
         TRY(run_pool(c, &a, 1, o, timed));
         return chronicle_finish(c, a, rasterize, nnodes, ntr, counts, bbox, trace_status,
-                                trace_steps, stats, flags, win, marks);
+                                trace_steps, stats, flags, win, marks, masks);
     }
 
     // launch shape: one realization per 256-lane workgroup, grid-stride
@@ -1045,14 +1050,14 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
     if (timed)
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     return chronicle_finish(c, a, rasterize, nnodes, ntr, counts, bbox, trace_status, trace_steps,
-                            stats, flags, win, marks);
+                            stats, flags, win, marks, masks);
 }
 
 // Copies a trace launch's outputs back (host buffers) and reads its statistics.
 int chronicle_finish(oka_ctx *c, const TraceArgs &a, int32_t rasterize, int64_t nnodes,
                      int64_t ntr, int32_t *counts, double *bbox, int32_t *trace_status,
                      int64_t *trace_steps, oka_trace_stats *stats, uint32_t flags,
-                     const oka_window *win, uint8_t *marks)
+                     const oka_window *win, uint8_t *marks, uint16_t *masks)
 {
     const int32_t nreal = a.nreal;
     TRY(copy_back(c, rasterize ? counts : nullptr, a.counts, sizeof(int32_t) * nnodes, flags));
@@ -1061,6 +1066,9 @@ int chronicle_finish(oka_ctx *c, const TraceArgs &a, int32_t rasterize, int64_t 
     TRY(copy_back(c, trace_steps, a.trace_steps, sizeof(int64_t) * ntr, flags));
     if (marks)
         TRY(copy_back(c, marks, a.wmarks, (size_t)nreal * win->nrows * win->ncols, flags));
+    if (masks)
+        TRY(copy_back(c, masks, a.wmasks, sizeof(uint16_t) * nreal * win->nrows * win->ncols,
+                      flags));
     if ((flags & OKA_ASYNC) && (flags & OKA_PTR_DEVICE) && !stats)
         return OKA_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1217,12 +1225,16 @@ int oka_trace_chronicle_noalloc(oka_ctx *c, const oka_site *site, int32_t nreal,
 int oka_trace_marks(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
                     const double *n, const double *b, const double *q, const double *coef,
                     const oka_trace_opts *o, const oka_grid *g, const oka_window *win,
-                    uint8_t *marks, int32_t *counts, oka_trace_stats *stats, uint32_t flags)
+                    uint8_t *marks, uint16_t *node_masks, int32_t *counts,
+                    oka_trace_stats *stats, uint32_t flags)
 {
     if (c && !marks)
         return fail(c, OKA_ERR_ARG, "marks is NULL");
+    if (c && node_masks && win && win->cell > 4)
+        return fail(c, OKA_ERR_ARG, "node masks need cells of at most 4 x 4 nodes (cell %d)",
+                    win->cell);
     return chronicle_impl(c, site, nreal, K, n, b, q, coef, o, g, 0, nullptr, 1, counts,
-                          nullptr, nullptr, nullptr, stats, flags, win, marks);
+                          nullptr, nullptr, nullptr, stats, flags, win, marks, node_masks);
 }
 
 int oka_trace_vertices(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,

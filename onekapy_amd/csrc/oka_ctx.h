@@ -21,7 +21,8 @@ enum Slot {
     kSlotXY, kSlotOut, kSlotEStatus, kSlotRgrid, kSlotMarks, kSlotMlpX, kSlotMlpMarks,
     kSlotMlpOrder, kSlotMlpOut, kSlotMlpOut2, kSlotMlpLoss, kSlotCvDists, kSlotCvSite,
     kSlotCvDraw, kSlotPoolSites, kSlotPoolFirst, kSlotPoolQueue, kSlotPoolHbm, kSlotPoolRec, kSlotQmc,
-    kSlotFarNear, kSlotFarWbox, kSlotFarBeta,
+    kSlotFarNear, kSlotFarWbox, kSlotFarBeta, kSlotMasks, kSlotNodeProbs, kSlotNodeMasks,
+    kSlotNodeMarks, kSlotNodeIn, kSlotNodeX, kSlotNodeCounts, kSlotNodePart, kSlotNodeAcc,
     kSlotCount
 };
 

@@ -79,6 +79,7 @@ struct TraceArgs {
     // per-realization window marks (oka_trace_marks); wmarks == nullptr: off
     int32_t w_row0, w_col0, w_cell, w_nr, w_nc;
     uint8_t *wmarks;            // nreal x (w_nr * w_nc)
+    uint16_t *wmasks;           // nreal x (w_nr * w_nc) node masks (w_cell <= 4), or nullptr
     // FP32 pool: far wells as local expansions of this order (0: every well
     // summed directly); boxes of side far_len from (far_x0, far_y0), in
     // coordinates relative to (xo, yo) (fp32::FarField)
@@ -90,7 +91,9 @@ struct TraceArgs {
 };
 
 // One marked node of a realization: its count (probabilityfield.py:357-359)
-// and, when a window is requested, its cell's LDS counter.
+// and, when a window is requested, its cell's LDS word: the count of the
+// cell's marked nodes, or with node masks (cells of at most 4 x 4 nodes) the
+// bit (row in cell) * cell + (column in cell).
 __device__ inline void count_node(const TraceArgs &a, int row, int col, unsigned int *s_win)
 {
     if (a.counts)
@@ -99,8 +102,13 @@ __device__ inline void count_node(const TraceArgs &a, int row, int col, unsigned
         const int ci = row - a.w_row0, cj = col - a.w_col0;
         if (ci >= 0 && cj >= 0) {
             const int r = ci / a.w_cell, c = cj / a.w_cell;
-            if (r < a.w_nr && c < a.w_nc)
-                atomicAdd(&s_win[r * a.w_nc + c], 1u);
+            if (r < a.w_nr && c < a.w_nc) {
+                if (a.wmasks)
+                    atomicOr(&s_win[r * a.w_nc + c],
+                             1u << ((ci - r * a.w_cell) * a.w_cell + (cj - c * a.w_cell)));
+                else
+                    atomicAdd(&s_win[r * a.w_nc + c], 1u);
+            }
         }
     }
 }
@@ -321,14 +329,21 @@ __device__ inline unsigned long long flush_tiles(const TraceArgs &a, TileBits &t
     return vis;
 }
 
-// Writes the window counters of realization r (capped at 255) and clears them.
+// Writes the window counters of realization r (capped at 255), or its node
+// masks and their counts, and clears them.
 __device__ inline void store_window(const TraceArgs &a, int r, unsigned int *s_win)
 {
     const int n = a.w_nr * a.w_nc;
     uint8_t *dst = a.wmarks + (int64_t)r * n;
+    uint16_t *msk = a.wmasks ? a.wmasks + (int64_t)r * n : nullptr;
     for (int c = threadIdx.x; c < n; c += blockDim.x) {
         const unsigned int v = s_win[c];
-        dst[c] = (uint8_t)(v < 255u ? v : 255u);
+        if (msk) {
+            msk[c] = (uint16_t)v;
+            dst[c] = (uint8_t)__popc(v);
+        } else {
+            dst[c] = (uint8_t)(v < 255u ? v : 255u);
+        }
         s_win[c] = 0u;
     }
 }

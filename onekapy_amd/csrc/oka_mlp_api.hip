@@ -901,6 +901,284 @@ static int cv_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist 
     return sync_unless_async(c, flags);
 }
 
+}  // extern "C"
+
+// ---- node-resolution correction sums ---------------------------------------
+
+namespace {
+
+// The node-resolution sums take each window node's controls from the 3 x 3
+// cells around its own (kNbr of them, neighbour m = cell (r + m / 3 - 1,
+// c + m % 3 - 1); a neighbour outside the window is absent and counts as 0),
+// and the products of two controls from pairs of cells kPairs half-offsets
+// apart (the offsets (dr, dc) of a 5 x 5 block with dr > 0, or dr = 0 and
+// dc >= 0; the other half by symmetry).
+constexpr int kNbr = 9;
+constexpr int kPairs = 13;
+
+__device__ __forceinline__ void pair_offset(int h, int &dr, int &dc)
+{
+    if (h < 3) {
+        dr = 0;
+        dc = h;
+    } else if (h < 8) {
+        dr = 1;
+        dc = h - 5;
+    } else {
+        dr = 2;
+        dc = h - 10;
+    }
+}
+
+// The window's cells over rows [lo, hi) of one chunk (probs [rows x nout],
+// nout = nr nc cells), in gridDim.y row ranges: thread (cell a, split sp)
+// writes part[(sp nout + a) (1 + kPairs) + f]: f = 0 the sum of s_a, 1 + h
+// the sum of s_a s_b for b = a + offset h (0 when b is outside the window).
+// FP64 accumulators in row order.
+__global__ void __launch_bounds__(256) k_node_cells(const float *probs, int nr, int nc, int rows,
+                                                    double *part)
+{
+    const int nout = nr * nc;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nout)
+        return;
+    const int sp = blockIdx.y, splits = gridDim.y;
+    const int lo = (int)((int64_t)rows * sp / splits);
+    const int hi = (int)((int64_t)rows * (sp + 1) / splits);
+    const int ar = a / nc, ac = a % nc;
+    int bi[kPairs];
+#pragma unroll
+    for (int h = 0; h < kPairs; ++h) {
+        int dr, dc;
+        pair_offset(h, dr, dc);
+        const int br = ar + dr, bc = ac + dc;
+        bi[h] = (br < nr && bc >= 0 && bc < nc) ? br * nc + bc : -1;
+    }
+    double s1 = 0.0, q[kPairs];
+#pragma unroll
+    for (int h = 0; h < kPairs; ++h)
+        q[h] = 0.0;
+    for (int r = lo; r < hi; ++r) {
+        const float *row = probs + (int64_t)r * nout;
+        const double sa = (double)row[a];
+        s1 += sa;
+#pragma unroll
+        for (int h = 0; h < kPairs; ++h)
+            if (bi[h] >= 0)
+                q[h] = fma(sa, (double)row[bi[h]], q[h]);
+    }
+    double *o = part + ((int64_t)sp * nout + a) * (1 + kPairs);
+    o[0] = s1;
+#pragma unroll
+    for (int h = 0; h < kPairs; ++h)
+        o[1 + h] = q[h];
+}
+
+// The window's nodes over the same rows: thread (cell a, node k, split sp)
+// writes part[((sp nout + a) nk + k) (1 + kNbr) + f]: f = 0 the rows with
+// node k of cell a visited (bit k of the cell's mask), 1 + m the sum over
+// them of neighbour m's s.
+__global__ void __launch_bounds__(256) k_node_nodes(const float *probs, const uint16_t *masks,
+                                                    int nr, int nc, int nk, int rows, double *part)
+{
+    const int nout = nr * nc;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nout * nk)
+        return;
+    const int a = (int)(t / nk), k = (int)(t % nk);
+    const int sp = blockIdx.y, splits = gridDim.y;
+    const int lo = (int)((int64_t)rows * sp / splits);
+    const int hi = (int)((int64_t)rows * (sp + 1) / splits);
+    const int ar = a / nc, ac = a % nc;
+    int ni[kNbr];
+#pragma unroll
+    for (int m = 0; m < kNbr; ++m) {
+        const int br = ar + m / 3 - 1, bc = ac + m % 3 - 1;
+        ni[m] = (br >= 0 && br < nr && bc >= 0 && bc < nc) ? br * nc + bc : -1;
+    }
+    double y = 0.0, ys[kNbr];
+#pragma unroll
+    for (int m = 0; m < kNbr; ++m)
+        ys[m] = 0.0;
+    for (int r = lo; r < hi; ++r) {
+        if (!((masks[(int64_t)r * nout + a] >> k) & 1u))
+            continue;
+        const float *row = probs + (int64_t)r * nout;
+        y += 1.0;
+#pragma unroll
+        for (int m = 0; m < kNbr; ++m)
+            if (ni[m] >= 0)
+                ys[m] += (double)row[ni[m]];
+    }
+    double *o = part + (((int64_t)sp * nout + a) * nk + k) * (1 + kNbr);
+    o[0] = y;
+#pragma unroll
+    for (int m = 0; m < kNbr; ++m)
+        o[1 + m] = ys[m];
+}
+
+// acc[i] += sum over splits (in order) of part[sp * n + i].
+__global__ void k_node_reduce(const double *part, int splits, int64_t n, double *acc)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double t = acc[i];
+    for (int sp = 0; sp < splits; ++sp)
+        t += part[(int64_t)sp * n + i];
+    acc[i] = t;
+}
+
+constexpr int kNodeSplits = 32;
+
+}  // namespace
+
+extern "C" {
+
+int oka_cv_node_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, int32_t nreal,
+                     const double *K, const double *n, const double *b, const double *q,
+                     const double *coef, const float *x, const oka_trace_opts *opts,
+                     const oka_grid *grid, const oka_window *win, int32_t nparts, int32_t *counts,
+                     double *sum_s, double *sum_ss, double *sum_y, double *sum_ys,
+                     oka_trace_stats *stats, uint32_t flags)
+{
+    if (!c || !M || M->ctx != c)
+        return c ? fail(c, OKA_ERR_ARG, "the network belongs to another context") : OKA_ERR_ARG;
+    if (!site || !opts || !grid || !win || !x || !sum_s || !sum_ss || !sum_y || !sum_ys ||
+        nreal < 0 || nparts < 1 || nparts > nreal)
+        return fail(c, OKA_ERR_ARG, "bad node-sum arguments");
+    if (win->cell < 1 || win->cell > 4 || (int64_t)win->nrows * win->ncols != M->nout)
+        return fail(c, OKA_ERR_ARG, "the window must have the network's %d cells of at most "
+                    "4 x 4 nodes", M->nout);
+    if (opts->npaths > 0 && !opts->start_xy)
+        return fail(c, OKA_ERR_ARG, "start_xy is NULL");
+    HIPCHK(c, hipSetDevice(c->device));
+    const hipStream_t s = c->stream;
+    const int nw = site->nwells, P = opts->npaths, nk = win->cell * win->cell;
+    const int fc = 1 + kPairs, fn = 1 + kNbr;      // per-cell and per-node fields
+    const int64_t nout = M->nout;
+    const int64_t ncf = nout * fc, nnf = nout * nk * fn;
+    // the inputs on the device, once (the per-chunk trace calls take device pointers)
+    oka_site sd = *site;
+    oka_trace_opts od = *opts;
+    const double *dK = K, *dn = n, *db = b, *dq = q, *dc = coef;
+    const float *dx = x;
+    if (!(flags & OKA_PTR_DEVICE)) {
+        const size_t parts[7] = {2 * (size_t)nw, 2 * (size_t)P, (size_t)nreal, (size_t)nreal,
+                                 (size_t)nreal, (size_t)nreal * nw, 6 * (size_t)nreal};
+        const double *src[7] = {site->wells_xy, opts->start_xy, K, n, b, q, coef};
+        size_t total = 0;
+        for (size_t v : parts)
+            total += v;
+        void *in;
+        TRY(ensure(c, kSlotNodeIn, sizeof(double) * std::max<size_t>(total, 1), &in));
+        const double *dst[7];
+        double *d = (double *)in;
+        for (int i = 0; i < 7; ++i) {
+            if (parts[i] && !src[i])
+                return fail(c, OKA_ERR_ARG, "missing realization inputs");
+            if (parts[i])
+                HIPCHK(c, hipMemcpyAsync(d, src[i], sizeof(double) * parts[i],
+                                         hipMemcpyHostToDevice, s));
+            dst[i] = d;
+            d += parts[i];
+        }
+        sd.wells_xy = dst[0];
+        od.start_xy = dst[1];
+        dK = dst[2];
+        dn = dst[3];
+        db = dst[4];
+        dq = dst[5];
+        dc = dst[6];
+        void *xv;
+        TRY(ensure(c, kSlotNodeX, sizeof(float) * std::max<int64_t>((int64_t)nreal * M->nin, 1),
+                   &xv));
+        HIPCHK(c, hipMemcpyAsync(xv, x, sizeof(float) * (size_t)nreal * M->nin,
+                                 hipMemcpyHostToDevice, s));
+        dx = (const float *)xv;
+    }
+    sd.obs = nullptr;
+    sd.nobs = 0;
+    void *dcounts = nullptr;
+    const int64_t nnodes = (int64_t)grid->nrows * grid->ncols;
+    if (counts)
+        TRY(stage_out(c, kSlotNodeCounts, counts, sizeof(int32_t) * nnodes, flags, true, &dcounts));
+    const int chunk = M->max_batch;
+    void *probs, *masks, *marks, *part, *acc;
+    TRY(ensure(c, kSlotNodeProbs, sizeof(float) * chunk * nout, &probs));
+    TRY(ensure(c, kSlotNodeMasks, sizeof(uint16_t) * chunk * nout, &masks));
+    TRY(ensure(c, kSlotNodeMarks, (size_t)chunk * nout, &marks));
+    TRY(ensure(c, kSlotNodePart, sizeof(double) * kNodeSplits * (ncf + nnf), &part));
+    TRY(ensure(c, kSlotNodeAcc, sizeof(double) * nparts * (ncf + nnf), &acc));
+    TRY(fill_u32(c, acc, 2 * (int64_t)nparts * (ncf + nnf)));
+    double *pc = (double *)part, *pn = pc + kNodeSplits * ncf;
+    oka_trace_stats tot{};
+    // each part's rows in chunks of the network's batch: trace (marks and node
+    // masks stay on the device), forward, then the part's node sums
+    for (int p = 0; p < nparts; ++p) {
+        const int64_t lo = (int64_t)nreal * p / nparts, hi = (int64_t)nreal * (p + 1) / nparts;
+        double *accp = (double *)acc + (int64_t)p * (ncf + nnf);
+        for (int64_t r0 = lo; r0 < hi; r0 += chunk) {
+            const int rows = (int)std::min<int64_t>(chunk, hi - r0);
+            oka_trace_stats st{};
+            TRY(oka_trace_marks(c, &sd, rows, dK + r0, dn + r0, db + r0, dq + r0 * nw, dc + 6 * r0,
+                                &od, grid, win, (uint8_t *)marks, (uint16_t *)masks,
+                                (int32_t *)dcounts, &st, OKA_PTR_DEVICE));
+            tot.attempted += st.attempted;
+            tot.accepted += st.accepted;
+            for (int j = 0; j < 4; ++j)
+                tot.status_count[j] += st.status_count[j];
+            tot.escaped += st.escaped;
+            tot.visited += st.visited;
+            tot.candidates += st.candidates;
+            tot.kernel_ms += st.kernel_ms;
+            const int Bp = (rows + 63) / 64 * 64;
+            TRY(hidden_forward(M, Bp, rows, dx, nullptr, r0, false));
+            GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
+            g.bias = M->p + M->ob3;
+            g.mvalid = rows;
+            g.Cf = (float *)probs;
+            g.ldc = M->nout;
+            HIPCHK(c, launch_gemm(kEpiProbs, g, s));
+            hipLaunchKernelGGL(k_node_cells, dim3((unsigned)((nout + 255) / 256), kNodeSplits),
+                               dim3(256), 0, s, (const float *)probs, win->nrows, win->ncols, rows,
+                               pc);
+            hipLaunchKernelGGL(k_node_nodes,
+                               dim3((unsigned)((nout * nk + 255) / 256), kNodeSplits), dim3(256),
+                               0, s, (const float *)probs, (const uint16_t *)masks, win->nrows,
+                               win->ncols, nk, rows, pn);
+            hipLaunchKernelGGL(k_node_reduce, dim3((unsigned)((ncf + 255) / 256)), dim3(256), 0, s,
+                               (const double *)pc, kNodeSplits, ncf, accp);
+            hipLaunchKernelGGL(k_node_reduce, dim3((unsigned)((nnf + 255) / 256)), dim3(256), 0, s,
+                               (const double *)pn, kNodeSplits, nnf, accp + ncf);
+            HIPCHK(c, hipGetLastError());
+        }
+    }
+    // unpack [part]([cell][f], [cell][node][f]) into the four outputs
+    std::vector<double> h((size_t)nparts * (ncf + nnf));
+    HIPCHK(c, hipMemcpyAsync(h.data(), acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    if (counts)
+        TRY(copy_back(c, counts, dcounts, sizeof(int32_t) * nnodes, flags));
+    HIPCHK(c, hipStreamSynchronize(s));
+    for (int p = 0; p < nparts; ++p) {
+        const double *hc = h.data() + (size_t)p * (ncf + nnf), *hn = hc + ncf;
+        for (int64_t a = 0; a < nout; ++a) {
+            sum_s[p * nout + a] = hc[a * fc];
+            for (int q = 0; q < kPairs; ++q)
+                sum_ss[(p * nout + a) * kPairs + q] = hc[a * fc + 1 + q];
+            for (int k = 0; k < nk; ++k) {
+                const double *v = hn + (a * nk + k) * fn;
+                sum_y[(p * nout + a) * nk + k] = v[0];
+                for (int m = 0; m < kNbr; ++m)
+                    sum_ys[((p * nout + a) * nk + k) * kNbr + m] = v[1 + m];
+            }
+        }
+    }
+    if (stats)
+        *stats = tot;
+    return OKA_OK;
+}
+
 int oka_cv_surrogate_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, const oka_dist *dists,
                           int64_t n, uint64_t seed, int64_t first, const double *feat_mean,
                           const double *feat_std, double *sums, double *sums_sq, uint32_t flags)

@@ -18,6 +18,15 @@ to train and to correct by K-fold cross-fitting (each fold is corrected by a
 network that never saw it), so no traced realization is spent on training
 alone.  The plain tracer remains the fallback and the reference for the error.
 
+At node resolution (control_variate_nodes, the ProbabilityField the
+reference returns) one network trained on a separate set of traced
+realizations serves every node of the window: node j's estimate is its plain
+mean over the correction realizations minus a linear combination of the
+surrogate outputs of the 3 x 3 cells around its own (each less its mean over
+fresh surrogate draws), with coefficients fitted on the other part of the
+correction set, so that the estimate stays unbiased.  Nodes outside the
+window keep the plain estimate of all traced realizations.
+
 The network (n_in -> 512 -> 512 -> 4096, GELU, BF16 weights, FP32
 accumulation) is trained and evaluated by liboka's hand-written BF16 MFMA
 kernels (oka_mlp_*).
@@ -255,8 +264,12 @@ def _window(row0, col0, cell, cells):
     return w
 
 
-def trace_marks(site, d, coef, window, precision=_lib.FP32):
-    """Per-realization visited-node counts per window cell (uint8 [R, cells])."""
+def trace_marks(site, d, coef, window, precision=_lib.FP32, masks=False, counts=None):
+    """Per-realization visited-node counts per window cell (uint8 [R, cells]);
+    with masks, also which of the cell's nodes (uint16 [R, cells], bit
+    i * cell + j for fine node (i, j) of the cell; cells of at most 4 x 4
+    nodes).  counts (int32 [nrows, ncols] of the site's fixed grid, or None)
+    accumulates every realization's visited nodes."""
     pf = site.fixed_field()
     g = pf.grid()
     wxy, s = _site_struct(site)
@@ -265,14 +278,18 @@ def trace_marks(site, d, coef, window, precision=_lib.FP32):
                              _lib.ptr(starts), precision, site.confined)
     R = len(d["K"])
     marks = np.zeros((R, window.nrows * window.ncols), np.uint8)
+    mk = np.zeros((R, window.nrows * window.ncols), np.uint16) if masks else None
+    if counts is not None and (counts.dtype != np.int32 or not counts.flags.c_contiguous or
+                               counts.shape != (pf.nrows, pf.ncols)):
+        raise ValueError("counts must be a C-contiguous int32 array of the grid's shape")
     st = _lib.TraceStats_()
     K, n, b, q = (np.ascontiguousarray(d[k]) for k in ("K", "n", "b", "q"))
     coef = np.ascontiguousarray(coef)
     ctx = _lib.context()
     ctx.check(ctx.lib.oka_trace_marks(ctx.h, s, R, _lib.ptr(K), _lib.ptr(n), _lib.ptr(b),
                                       _lib.ptr(q), _lib.ptr(coef), o, g, window, _lib.ptr(marks),
-                                      None, st, 0))
-    return marks, st
+                                      _lib.ptr(mk), _lib.ptr(counts), st, 0))
+    return (marks, mk, st) if masks else (marks, st)
 
 
 def cell_probabilities(counts, nreal, window):
@@ -464,3 +481,263 @@ def control_variate(site, n_traced, n_surrogate, folds=2, train_steps=2000, batc
                                 n_traced=n_traced, n_surrogate=n_surrogate,
                                 n_surrogate_used=sum(o[3] for o in out), losses=losses,
                                 trace_stats=st)
+
+
+def _node_image(a, window):
+    """[..., cells, cell^2] per-cell node values -> [..., nrows * cell, ncols * cell]
+    (node k = i * cell + j of cell (r, c) is window node (r * cell + i, c * cell + j))."""
+    c, nr, nc = window.cell, window.nrows, window.ncols
+    lead = a.shape[:-2]
+    a = a.reshape(lead + (nr, nc, c, c))
+    a = np.moveaxis(a, -2, -3)                      # [..., nr, c(i), nc, c(j)]
+    return a.reshape(lead + (nr * c, nc * c))
+
+
+def _window_slices(window, shape):
+    """(grid rows, grid cols, window rows, window cols) slices of the part of
+    the window that lies on a grid of `shape`."""
+    h, w = window.nrows * window.cell, window.ncols * window.cell
+    r0, c0 = window.row0, window.col0
+    gr0, gc0 = max(r0, 0), max(c0, 0)
+    gr1, gc1 = min(r0 + h, shape[0]), min(c0 + w, shape[1])
+    if gr1 <= gr0 or gc1 <= gc0:
+        return None
+    return (slice(gr0, gr1), slice(gc0, gc1), slice(gr0 - r0, gr1 - r0),
+            slice(gc0 - c0, gc1 - c0))
+
+
+# The per-node controls: the 3 x 3 cells around a node's own cell (neighbour
+# m = cell (r + m // 3 - 1, c + m % 3 - 1)), and the 13 half-offsets of a
+# 5 x 5 block whose pair sums give their products (oka_cv_node_sums).
+NBR = [(m // 3 - 1, m % 3 - 1) for m in range(9)]
+PAIRS = [(0, 0), (0, 1), (0, 2)] + [(1, dc) for dc in range(-2, 3)] + \
+        [(2, dc) for dc in range(-2, 3)]
+
+
+def neighbours(window):
+    """[cells, 9] index of each cell's neighbour m (-1 outside the window)."""
+    nr, nc = window.nrows, window.ncols
+    r, c = np.divmod(np.arange(nr * nc), nc)
+    out = np.full((nr * nc, 9), -1, np.int64)
+    for m, (dr, dc) in enumerate(NBR):
+        rr, cc = r + dr, c + dc
+        ok = (rr >= 0) & (rr < nr) & (cc >= 0) & (cc < nc)
+        out[ok, m] = rr[ok] * nc + cc[ok]
+    return out
+
+
+def _pair_products(ss, nb):
+    """E-sums of s_b s_b' for every pair (m, l) of each cell's neighbours, from
+    the half-offset pair sums ss [..., cells, 13]: [..., cells, 9, 9]."""
+    hidx = {d: h for h, d in enumerate(PAIRS)}
+    out = np.zeros(ss.shape[:-2] + (nb.shape[0], 9, 9))
+    for m, (r1, c1) in enumerate(NBR):
+        for l, (r2, c2) in enumerate(NBR):
+            d = (r2 - r1, c2 - c1)
+            if d in hidx:
+                base, h = nb[:, m], hidx[d]
+            else:
+                base, h = nb[:, l], hidx[(-d[0], -d[1])]
+            ok = (nb[:, m] >= 0) & (nb[:, l] >= 0)
+            out[..., ok, m, l] = ss[..., base[ok], h]
+    return out
+
+
+def node_sums(net, site, d, coef, X, window, parts=2, precision=_lib.FP32, counts=None):
+    """oka_cv_node_sums: per part, the sums of s per cell [parts, cells], of
+    s_a s_b over the 13 half-offsets [parts, cells, 13], of y per node and of
+    y s_b over the node's 9 neighbouring cells [parts, cells, cell^2(, 9)];
+    counts (int32 grid, or None) accumulates the realizations' visited
+    nodes."""
+    pf = site.fixed_field()
+    g = pf.grid()
+    wxy, s = _site_struct(site)
+    starts = start_points(site.xt, site.yt, site.rt, site.npaths)
+    o = _lib.make_trace_opts(site.npaths, site.duration, site.tol, site.maxstep, site.umbra,
+                             _lib.ptr(starts), precision, site.confined)
+    R = len(d["K"])
+    ncell, nk = window.nrows * window.ncols, window.cell * window.cell
+    out = {"s": np.zeros((parts, ncell)), "ss": np.zeros((parts, ncell, len(PAIRS))),
+           "y": np.zeros((parts, ncell, nk)), "ys": np.zeros((parts, ncell, nk, 9))}
+    K, n, b, q = (np.ascontiguousarray(d[k], dtype=np.float64) for k in ("K", "n", "b", "q"))
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    if counts is not None and (counts.dtype != np.int32 or not counts.flags.c_contiguous or
+                               counts.shape != (pf.nrows, pf.ncols)):
+        raise ValueError("counts must be a C-contiguous int32 array of the grid's shape")
+    st = _lib.TraceStats_()
+    net.ctx.check(net.ctx.lib.oka_cv_node_sums(
+        net.ctx.h, net.h, s, R, _lib.ptr(K), _lib.ptr(n), _lib.ptr(b), _lib.ptr(q),
+        _lib.ptr(coef), _lib.ptr(X), o, g, window, int(parts), _lib.ptr(counts),
+        _lib.ptr(out["s"]), _lib.ptr(out["ss"]), _lib.ptr(out["y"]), _lib.ptr(out["ys"]), st, 0))
+    return out, st
+
+
+def node_estimate(sums, n_rows, nb, mu_reps, ridge=1e-2, flat=1e-10, min_events=30):
+    """The cross-fitted node estimate from node_sums over parts of n_rows[p]
+    realizations.  Node j (node k of cell a) is corrected by the surrogate
+    outputs s of the cells around a (nb: neighbours(window)):
+        p_j = sum_p w_p (ybar_p - beta_(-p) . (sbar_p - mu)),   w_p = n_p / N,
+    beta_(-p) the ridge regression of y_j on the controls over the other
+    parts' realizations, in correlation form: each control divided by its
+    standard deviation, `ridge` added to the correlation matrix's diagonal,
+    and controls whose variance is below `flat` (a cell whose output hardly
+    moves) left out; a node visited (or missed) by fewer than min_events of
+    the other parts' realizations takes no control at all (too few events
+    to fit 9 coefficients).  Each part's coefficients depend only on the
+    other parts, so the estimate is unbiased whatever the coefficients.  mu_reps [R, cells] are independent unbiased
+    replicate means of the surrogate (their mean is mu).  The variance is
+        sum_p w_p^2 var_p(y - beta_(-p) . s) / n_p + var(beta_bar . mu),
+    beta_bar = sum_p w_p beta_(-p), the last term from the replicates.
+    Returns (p, var, beta_bar) as [cells, cell^2] (and [..., 9]) arrays."""
+    n_rows = np.asarray(n_rows, dtype=np.float64)
+    N = n_rows.sum()
+    P = len(n_rows)
+    absent = nb < 0
+    nbi = np.where(absent, 0, nb)
+    mu_reps = np.asarray(mu_reps, dtype=np.float64)
+    mu = mu_reps.mean(0)
+    tot = {k: v.sum(0) for k, v in sums.items()}
+
+    def moments(S, SS, Y, YS, n):
+        """Means, control covariance [cells, 9, 9] and cross-covariance
+        [cells, nk, 9] of one set of sums over n realizations."""
+        sb = np.where(absent, 0.0, (S / n)[nbi])                   # [cells, 9]
+        G = _pair_products(SS, nb) / n - sb[:, :, None] * sb[:, None, :]
+        yb = Y / n                                                   # [cells, nk]
+        C = YS / n - yb[:, :, None] * sb[:, None, :]
+        C = np.where(absent[:, None, :], 0.0, C)
+        return sb, G, yb, C
+
+    p = np.zeros(sums["y"].shape[1:])
+    var = np.zeros_like(p)
+    bbar = np.zeros(p.shape + (9,))
+    mu_nb = np.where(absent, 0.0, mu[nbi])
+    for h in range(P):
+        nh, no = n_rows[h], N - n_rows[h]
+        _, Go, _, Co = moments(tot["s"] - sums["s"][h], tot["ss"] - sums["ss"][h],
+                               tot["y"] - sums["y"][h], tot["ys"] - sums["ys"][h], no)
+        diag = np.einsum("cii->ci", Go)
+        off = absent | ~(diag > flat)                                # controls left out
+        sd = np.where(off, 1.0, np.sqrt(np.where(off, 1.0, diag)))
+        A = Go / (sd[:, :, None] * sd[:, None, :])
+        A = np.where(off[:, :, None] | off[:, None, :], 0.0, A)
+        A = A + np.eye(9)[None] * np.where(off, 1.0, ridge)[:, :, None]
+        rhs = np.where(off[:, None, :], 0.0, Co / sd[:, None, :])   # [cells, nk, 9]
+        beta = np.linalg.solve(A, np.swapaxes(rhs, 1, 2))            # [cells, 9, nk]
+        beta = np.swapaxes(beta, 1, 2) / sd[:, None, :]              # [cells, nk, 9]
+        ev = tot["y"] - sums["y"][h]                                 # the other parts' visits
+        beta = np.where((np.minimum(ev, no - ev) < min_events)[:, :, None], 0.0, beta)
+        sh, Gh, yh, Ch = moments(sums["s"][h], sums["ss"][h], sums["y"][h], sums["ys"][h], nh)
+        w = nh / N
+        p += w * (yh - np.matmul(beta, (sh - mu_nb)[:, :, None])[:, :, 0])
+        vd = (yh * (1.0 - yh) - 2.0 * (beta * Ch).sum(-1)
+              + (np.matmul(beta, Gh) * beta).sum(-1))
+        var += w * w * np.maximum(vd, 0.0) / nh
+        bbar += w * beta
+    # the surrogate term's variance from the replicates' spread
+    R = mu_reps.shape[0]
+    dev = np.where(absent[:, None, :], 0.0, np.swapaxes((mu_reps - mu)[:, nbi], 0, 1))
+    proj = np.matmul(dev, np.swapaxes(bbar, 1, 2))                 # [cells, R, nk]
+    var += (proj ** 2).sum(1) / (R - 1) / R
+    return p, var, bbar
+
+
+class NodeEstimate:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def control_variate_nodes(site, n_train, n_correct, n_surrogate, train_steps=1200, batch=2048,
+                          lr=5e-3, seed=0, precision=_lib.FP32, pilot=2000, window=None,
+                          n_hidden=512, qmc_replicates=64, qmc_pow2=True, parts=2):
+    """Control-variate estimate of every node's capture probability: the
+    ProbabilityField of create_stochastic_capturezone (stochastic.py:76-270)
+    on the site's fixed lattice.
+
+    n_train traced realizations train one surrogate network on their window
+    cells; n_correct further traced realizations and n_surrogate fresh
+    surrogate evaluations (qmc_replicates scrambled Sobol' replicates) form
+    the estimate of each window node (node_estimate: controls from the 3 x 3
+    cells around the node's cell, coefficients cross-fitted over `parts`
+    parts of the correction realizations).  Nodes outside the
+    window take the plain estimate over all n_train + n_correct traced
+    realizations, which are independent draws of the same distribution.
+
+    Returns a NodeEstimate: p (float64 [nrows, ncols]), var (its variance per
+    node: the estimator's in the window, p (1 - p) / n_traced outside), the
+    plain estimate p_plain and its variance from the same n_train + n_correct
+    realizations, the window, the probability mass outside the window
+    (outside_mass, and outside_max its largest node value), and per-phase
+    times in seconds."""
+    if n_train < batch or n_correct < 2 * parts:
+        raise ValueError("too few realizations")
+    t = {}
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    R = n_train + n_correct
+    d = draw_realizations(site.wells, site.c_dist, site.p_dist, site.t_dist, R, rng)
+    wxy = site.wells_xy()
+    _, _, coef = fit_and_draw(wxy, site.xt, site.yt, site.base, site.observations, d["K"],
+                              d["b"], d["q"], d["z"], moments=False)
+    dt = {k: v[:n_train] for k, v in d.items()}
+    dc = {k: v[n_train:] for k, v in d.items()}
+    t["fit_draw"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pf = site.fixed_field()
+    if window is None:
+        npil = min(pilot, n_train)
+        pc, _ = trace_counts(site, {k: v[:npil] for k, v in dt.items()}, coef[:npil], precision)
+        window = choose_window(pc)
+        if window.cell > 4:
+            raise ValueError(f"the capture zone needs cells of {window.cell} nodes; node "
+                             f"masks take at most 4")
+    t["pilot"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    counts = np.zeros((pf.nrows, pf.ncols), np.int32)
+    marks, st_train = trace_marks(site, dt, coef[:n_train], window, precision, counts=counts)
+    t["trace_train"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tscale = 1.0 / (window.cell * window.cell)
+    F = features(d, coef)
+    std = Standardizer(F[:n_train])
+    X = std(F)
+    ncell = window.nrows * window.ncols
+    net = SurrogateMLP(X.shape[1], n_hidden, ncell, seed=seed + 1)
+    epochs = max(1, int(math.ceil(train_steps / max(1, n_train // batch))))
+    losses = net.train(X[:n_train], marks, tscale, epochs, batch, lr, seed=seed + 100)
+    t["train"] = time.perf_counter() - t0
+    del marks
+    t0 = time.perf_counter()
+    npts, nrep = qmc_split(n_surrogate, qmc_replicates, qmc_pow2)
+    surr_seed = int(rng.integers(1, 2 ** 63))
+    reps = _surrogate_sums_qmc(net, site, npts, [_replicate_seed(surr_seed, 0, k)
+                                                 for k in range(nrep)], std) / npts
+    t["surrogate"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sums, st_corr = node_sums(net, site, dc, coef[n_train:], X[n_train:], window, parts,
+                              precision, counts)
+    t["trace_correct"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bounds = [n_correct * p // parts for p in range(parts + 1)]
+    n_rows = np.diff(bounds)
+    p_cells, var_cells, beta = node_estimate(sums, n_rows, neighbours(window), reps)
+    p_win, var_win = _node_image(p_cells, window), _node_image(var_cells, window)
+    p_plain = counts / float(R)
+    var_plain = p_plain * (1.0 - p_plain) / R
+    p = p_plain.copy()
+    var = var_plain.copy()
+    inside = np.zeros(p.shape, bool)
+    sl = _window_slices(window, p.shape)
+    if sl is not None:
+        gr, gc, wr, wc = sl
+        p[gr, gc] = p_win[wr, wc]
+        var[gr, gc] = var_win[wr, wc]
+        inside[gr, gc] = True
+    t["estimate"] = time.perf_counter() - t0
+    return NodeEstimate(p=p, var=var, p_plain=p_plain, var_plain=var_plain, window=window,
+                        inside=inside, outside_mass=float(p_plain[~inside].sum()),
+                        outside_max=float(p_plain[~inside].max(initial=0.0)),
+                        total_mass=float(p_plain.sum()), n_train=n_train, n_correct=n_correct,
+                        n_surrogate=npts * nrep, qmc=(npts, nrep), parts=parts, beta=beta,
+                        times=t, losses=losses, trace_stats=(st_train, st_corr))
