@@ -455,6 +455,25 @@ int oka_cv_node_sums(oka_ctx *ctx, oka_mlp *mlp, const oka_site *site, int32_t n
                      double *sum_s, double *sum_ss, double *sum_y, double *sum_ys,
                      oka_trace_stats *stats, uint32_t flags);
 
+/* The same sums in two calls, so that the correction set can be traced
+ * while the network trains (on another context): oka_cv_node_trace traces
+ * nreal realizations as oka_cv_node_sums does and keeps their node masks in
+ * device memory owned by *out (nreal x cells x 2 bytes; cells of at most
+ * 4 x 4 nodes); oka_cv_node_reduce runs a network on the set's standardized
+ * inputs x [nreal x n_in] and returns the sums of oka_cv_node_sums over
+ * nparts parts (any context of the set's device; the set must not be in use
+ * by another call).  oka_node_set_destroy frees the set. */
+typedef struct oka_node_set oka_node_set;
+int oka_cv_node_trace(oka_ctx *ctx, const oka_site *site, int32_t nreal, const double *K,
+                      const double *n, const double *b, const double *q, const double *coef,
+                      const oka_trace_opts *opts, const oka_grid *grid, const oka_window *win,
+                      int32_t *counts, oka_node_set **out, oka_trace_stats *stats,
+                      uint32_t flags);
+int oka_cv_node_reduce(oka_mlp *mlp, const oka_node_set *set, const float *x, int32_t nparts,
+                       double *sum_s, double *sum_ss, double *sum_y, double *sum_ys,
+                       uint32_t flags);
+void oka_node_set_destroy(oka_node_set *set);
+
 /* Exact moments of window marks [n x ncell] (uint8): per cell c,
  * sums[c] += sum over rows of marks[r, c] and, when sums_sq is set,
  * sums_sq[c] += sum of marks[r, c]^2 -- the plain Monte Carlo estimate of the

@@ -40,7 +40,7 @@ EXPORTED = (
     "oka_ctx_set_stream", "oka_ctx_stream", "oka_ctx_synchronize", "oka_model_eval",
     "oka_fit_draw", "oka_trace_chronicle", "oka_grid_history", "oka_trace_vertices",
     "oka_rasterize", "oka_pow_fifth", "oka_log", "oka_trace_marks", "oka_cv_node_sums",
-    "oka_mlp_create",
+    "oka_cv_node_trace", "oka_cv_node_reduce", "oka_node_set_destroy", "oka_mlp_create",
     "oka_mlp_destroy", "oka_mlp_param_count", "oka_mlp_set_params", "oka_mlp_get_params",
     "oka_mlp_grad", "oka_mlp_train", "oka_mlp_predict", "oka_mlp_accumulate",
     "oka_draw_realizations", "oka_cv_surrogate_sums", "oka_cv_surrogate_sums_qmc",
@@ -179,6 +179,14 @@ def load():
     L.oka_trace_marks.argtypes = [_vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp,
                                   C.POINTER(TraceOpts_), C.POINTER(Grid_), C.POINTER(Window_),
                                   _vp, _vp, _vp, C.POINTER(TraceStats_), _u32]
+    L.oka_cv_node_trace.restype = C.c_int
+    L.oka_cv_node_trace.argtypes = [_vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp,
+                                    C.POINTER(TraceOpts_), C.POINTER(Grid_), C.POINTER(Window_),
+                                    _vp, C.POINTER(_vp), C.POINTER(TraceStats_), _u32]
+    L.oka_cv_node_reduce.restype = C.c_int
+    L.oka_cv_node_reduce.argtypes = [_vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _u32]
+    L.oka_node_set_destroy.restype = None
+    L.oka_node_set_destroy.argtypes = [_vp]
     L.oka_cv_node_sums.restype = C.c_int
     L.oka_cv_node_sums.argtypes = [_vp, _vp, C.POINTER(Site_), _i32, _vp, _vp, _vp, _vp, _vp,
                                    _vp, C.POINTER(TraceOpts_), C.POINTER(Grid_),

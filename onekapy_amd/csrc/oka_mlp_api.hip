@@ -1031,135 +1031,139 @@ __global__ void k_node_reduce(const double *part, int splits, int64_t n, double 
 
 constexpr int kNodeSplits = 32;
 
-}  // namespace
+// Device copies of a correction set's trace inputs (or the caller's device
+// arrays as they are), with the site and options pointing at them.
+struct NodeInputs {
+    oka_site site;
+    oka_trace_opts opts;
+    const double *K, *n, *b, *q, *coef;
+};
 
-extern "C" {
-
-int oka_cv_node_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, int32_t nreal,
-                     const double *K, const double *n, const double *b, const double *q,
-                     const double *coef, const float *x, const oka_trace_opts *opts,
-                     const oka_grid *grid, const oka_window *win, int32_t nparts, int32_t *counts,
-                     double *sum_s, double *sum_ss, double *sum_y, double *sum_ys,
-                     oka_trace_stats *stats, uint32_t flags)
+int stage_node_inputs(oka_ctx *c, const oka_site *site, const oka_trace_opts *opts, int32_t nreal,
+                      const double *K, const double *n, const double *b, const double *q,
+                      const double *coef, uint32_t flags, NodeInputs *out)
 {
-    if (!c || !M || M->ctx != c)
-        return c ? fail(c, OKA_ERR_ARG, "the network belongs to another context") : OKA_ERR_ARG;
-    if (!site || !opts || !grid || !win || !x || !sum_s || !sum_ss || !sum_y || !sum_ys ||
-        nreal < 0 || nparts < 1 || nparts > nreal)
-        return fail(c, OKA_ERR_ARG, "bad node-sum arguments");
-    if (win->cell < 1 || win->cell > 4 || (int64_t)win->nrows * win->ncols != M->nout)
-        return fail(c, OKA_ERR_ARG, "the window must have the network's %d cells of at most "
-                    "4 x 4 nodes", M->nout);
+    NodeInputs &in = *out;
+    in.site = *site;
+    in.opts = *opts;
+    in.K = K;
+    in.n = n;
+    in.b = b;
+    in.q = q;
+    in.coef = coef;
+    in.site.obs = nullptr;
+    in.site.nobs = 0;
+    if (flags & OKA_PTR_DEVICE)
+        return OKA_OK;
+    const int nw = site->nwells, P = opts->npaths;
+    const size_t parts[7] = {2 * (size_t)nw, 2 * (size_t)P, (size_t)nreal, (size_t)nreal,
+                             (size_t)nreal, (size_t)nreal * nw, 6 * (size_t)nreal};
+    const double *src[7] = {site->wells_xy, opts->start_xy, K, n, b, q, coef};
+    size_t total = 0;
+    for (size_t v : parts)
+        total += v;
+    void *buf;
+    TRY(ensure(c, kSlotNodeIn, sizeof(double) * std::max<size_t>(total, 1), &buf));
+    const double *dst[7];
+    double *d = (double *)buf;
+    for (int i = 0; i < 7; ++i) {
+        if (parts[i] && !src[i])
+            return fail(c, OKA_ERR_ARG, "missing realization inputs");
+        if (parts[i])
+            HIPCHK(c, hipMemcpyAsync(d, src[i], sizeof(double) * parts[i], hipMemcpyHostToDevice,
+                                     c->stream));
+        dst[i] = d;
+        d += parts[i];
+    }
+    in.site.wells_xy = dst[0];
+    in.opts.start_xy = dst[1];
+    in.K = dst[2];
+    in.n = dst[3];
+    in.b = dst[4];
+    in.q = dst[5];
+    in.coef = dst[6];
+    return OKA_OK;
+}
+
+int check_node_args(oka_ctx *c, int64_t nout, const oka_site *site, const oka_trace_opts *opts,
+                    const oka_grid *grid, const oka_window *win, int32_t nreal)
+{
+    if (!site || !opts || !grid || !win || nreal < 0)
+        return fail(c, OKA_ERR_ARG, "bad node-trace arguments");
+    if (win->cell < 1 || win->cell > 4 || (nout > 0 && (int64_t)win->nrows * win->ncols != nout))
+        return fail(c, OKA_ERR_ARG, "the window must have the network's %lld cells of at most "
+                    "4 x 4 nodes", (long long)nout);
     if (opts->npaths > 0 && !opts->start_xy)
         return fail(c, OKA_ERR_ARG, "start_xy is NULL");
-    HIPCHK(c, hipSetDevice(c->device));
+    return OKA_OK;
+}
+
+void add_stats(oka_trace_stats &tot, const oka_trace_stats &st)
+{
+    tot.attempted += st.attempted;
+    tot.accepted += st.accepted;
+    for (int j = 0; j < 4; ++j)
+        tot.status_count[j] += st.status_count[j];
+    tot.escaped += st.escaped;
+    tot.visited += st.visited;
+    tot.candidates += st.candidates;
+    tot.kernel_ms += st.kernel_ms;
+}
+
+// Traces rows [r0, r0 + rows) of the staged set: node masks into `masks`
+// (rows x cells, device), cell marks into `marks` (scratch), counts added.
+int trace_node_rows(oka_ctx *c, const NodeInputs &in, int64_t r0, int rows, const oka_grid *grid,
+                    const oka_window *win, uint16_t *masks, uint8_t *marks, int32_t *dcounts,
+                    oka_trace_stats &tot)
+{
+    const int nw = in.site.nwells;
+    oka_trace_stats st{};
+    TRY(oka_trace_marks(c, &in.site, rows, in.K + r0, in.n + r0, in.b + r0, in.q + r0 * nw,
+                        in.coef + 6 * r0, &in.opts, grid, win, marks, masks, dcounts, &st,
+                        OKA_PTR_DEVICE));
+    add_stats(tot, st);
+    return OKA_OK;
+}
+
+// The network on rows [r0, r0 + rows) of dx and the node sums of those rows
+// (masks: their rows x cells node masks) added to one part's accumulators.
+int reduce_node_rows(oka_mlp *M, const float *dx, int64_t r0, int rows, const uint16_t *masks,
+                     const oka_window *win, float *probs, double *part, double *accp)
+{
+    oka_ctx *c = M->ctx;
     const hipStream_t s = c->stream;
-    const int nw = site->nwells, P = opts->npaths, nk = win->cell * win->cell;
-    const int fc = 1 + kPairs, fn = 1 + kNbr;      // per-cell and per-node fields
+    const int nk = win->cell * win->cell;
     const int64_t nout = M->nout;
+    const int64_t ncf = nout * (1 + kPairs), nnf = nout * nk * (1 + kNbr);
+    const int Bp = (rows + 63) / 64 * 64;
+    TRY(hidden_forward(M, Bp, rows, dx, nullptr, r0, false));
+    GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
+    g.bias = M->p + M->ob3;
+    g.mvalid = rows;
+    g.Cf = probs;
+    g.ldc = M->nout;
+    HIPCHK(c, launch_gemm(kEpiProbs, g, s));
+    double *pc = part, *pn = part + kNodeSplits * ncf;
+    hipLaunchKernelGGL(k_node_cells, dim3((unsigned)((nout + 255) / 256), kNodeSplits), dim3(256),
+                       0, s, (const float *)probs, win->nrows, win->ncols, rows, pc);
+    hipLaunchKernelGGL(k_node_nodes, dim3((unsigned)((nout * nk + 255) / 256), kNodeSplits),
+                       dim3(256), 0, s, (const float *)probs, masks, win->nrows, win->ncols, nk,
+                       rows, pn);
+    hipLaunchKernelGGL(k_node_reduce, dim3((unsigned)((ncf + 255) / 256)), dim3(256), 0, s,
+                       (const double *)pc, kNodeSplits, ncf, accp);
+    hipLaunchKernelGGL(k_node_reduce, dim3((unsigned)((nnf + 255) / 256)), dim3(256), 0, s,
+                       (const double *)pn, kNodeSplits, nnf, accp + ncf);
+    HIPCHK(c, hipGetLastError());
+    return OKA_OK;
+}
+
+// The part accumulators [part]([cell][1 + kPairs], [cell][node][1 + kNbr])
+// unpacked into the four host outputs.
+void unpack_node_sums(const std::vector<double> &h, int nparts, int64_t nout, int nk,
+                      double *sum_s, double *sum_ss, double *sum_y, double *sum_ys)
+{
+    const int fc = 1 + kPairs, fn = 1 + kNbr;
     const int64_t ncf = nout * fc, nnf = nout * nk * fn;
-    // the inputs on the device, once (the per-chunk trace calls take device pointers)
-    oka_site sd = *site;
-    oka_trace_opts od = *opts;
-    const double *dK = K, *dn = n, *db = b, *dq = q, *dc = coef;
-    const float *dx = x;
-    if (!(flags & OKA_PTR_DEVICE)) {
-        const size_t parts[7] = {2 * (size_t)nw, 2 * (size_t)P, (size_t)nreal, (size_t)nreal,
-                                 (size_t)nreal, (size_t)nreal * nw, 6 * (size_t)nreal};
-        const double *src[7] = {site->wells_xy, opts->start_xy, K, n, b, q, coef};
-        size_t total = 0;
-        for (size_t v : parts)
-            total += v;
-        void *in;
-        TRY(ensure(c, kSlotNodeIn, sizeof(double) * std::max<size_t>(total, 1), &in));
-        const double *dst[7];
-        double *d = (double *)in;
-        for (int i = 0; i < 7; ++i) {
-            if (parts[i] && !src[i])
-                return fail(c, OKA_ERR_ARG, "missing realization inputs");
-            if (parts[i])
-                HIPCHK(c, hipMemcpyAsync(d, src[i], sizeof(double) * parts[i],
-                                         hipMemcpyHostToDevice, s));
-            dst[i] = d;
-            d += parts[i];
-        }
-        sd.wells_xy = dst[0];
-        od.start_xy = dst[1];
-        dK = dst[2];
-        dn = dst[3];
-        db = dst[4];
-        dq = dst[5];
-        dc = dst[6];
-        void *xv;
-        TRY(ensure(c, kSlotNodeX, sizeof(float) * std::max<int64_t>((int64_t)nreal * M->nin, 1),
-                   &xv));
-        HIPCHK(c, hipMemcpyAsync(xv, x, sizeof(float) * (size_t)nreal * M->nin,
-                                 hipMemcpyHostToDevice, s));
-        dx = (const float *)xv;
-    }
-    sd.obs = nullptr;
-    sd.nobs = 0;
-    void *dcounts = nullptr;
-    const int64_t nnodes = (int64_t)grid->nrows * grid->ncols;
-    if (counts)
-        TRY(stage_out(c, kSlotNodeCounts, counts, sizeof(int32_t) * nnodes, flags, true, &dcounts));
-    const int chunk = M->max_batch;
-    void *probs, *masks, *marks, *part, *acc;
-    TRY(ensure(c, kSlotNodeProbs, sizeof(float) * chunk * nout, &probs));
-    TRY(ensure(c, kSlotNodeMasks, sizeof(uint16_t) * chunk * nout, &masks));
-    TRY(ensure(c, kSlotNodeMarks, (size_t)chunk * nout, &marks));
-    TRY(ensure(c, kSlotNodePart, sizeof(double) * kNodeSplits * (ncf + nnf), &part));
-    TRY(ensure(c, kSlotNodeAcc, sizeof(double) * nparts * (ncf + nnf), &acc));
-    TRY(fill_u32(c, acc, 2 * (int64_t)nparts * (ncf + nnf)));
-    double *pc = (double *)part, *pn = pc + kNodeSplits * ncf;
-    oka_trace_stats tot{};
-    // each part's rows in chunks of the network's batch: trace (marks and node
-    // masks stay on the device), forward, then the part's node sums
-    for (int p = 0; p < nparts; ++p) {
-        const int64_t lo = (int64_t)nreal * p / nparts, hi = (int64_t)nreal * (p + 1) / nparts;
-        double *accp = (double *)acc + (int64_t)p * (ncf + nnf);
-        for (int64_t r0 = lo; r0 < hi; r0 += chunk) {
-            const int rows = (int)std::min<int64_t>(chunk, hi - r0);
-            oka_trace_stats st{};
-            TRY(oka_trace_marks(c, &sd, rows, dK + r0, dn + r0, db + r0, dq + r0 * nw, dc + 6 * r0,
-                                &od, grid, win, (uint8_t *)marks, (uint16_t *)masks,
-                                (int32_t *)dcounts, &st, OKA_PTR_DEVICE));
-            tot.attempted += st.attempted;
-            tot.accepted += st.accepted;
-            for (int j = 0; j < 4; ++j)
-                tot.status_count[j] += st.status_count[j];
-            tot.escaped += st.escaped;
-            tot.visited += st.visited;
-            tot.candidates += st.candidates;
-            tot.kernel_ms += st.kernel_ms;
-            const int Bp = (rows + 63) / 64 * 64;
-            TRY(hidden_forward(M, Bp, rows, dx, nullptr, r0, false));
-            GemmArgs g = gemm(M->h2, M->nh, M->w3h, M->nh, Bp, M->nout, M->nh);
-            g.bias = M->p + M->ob3;
-            g.mvalid = rows;
-            g.Cf = (float *)probs;
-            g.ldc = M->nout;
-            HIPCHK(c, launch_gemm(kEpiProbs, g, s));
-            hipLaunchKernelGGL(k_node_cells, dim3((unsigned)((nout + 255) / 256), kNodeSplits),
-                               dim3(256), 0, s, (const float *)probs, win->nrows, win->ncols, rows,
-                               pc);
-            hipLaunchKernelGGL(k_node_nodes,
-                               dim3((unsigned)((nout * nk + 255) / 256), kNodeSplits), dim3(256),
-                               0, s, (const float *)probs, (const uint16_t *)masks, win->nrows,
-                               win->ncols, nk, rows, pn);
-            hipLaunchKernelGGL(k_node_reduce, dim3((unsigned)((ncf + 255) / 256)), dim3(256), 0, s,
-                               (const double *)pc, kNodeSplits, ncf, accp);
-            hipLaunchKernelGGL(k_node_reduce, dim3((unsigned)((nnf + 255) / 256)), dim3(256), 0, s,
-                               (const double *)pn, kNodeSplits, nnf, accp + ncf);
-            HIPCHK(c, hipGetLastError());
-        }
-    }
-    // unpack [part]([cell][f], [cell][node][f]) into the four outputs
-    std::vector<double> h((size_t)nparts * (ncf + nnf));
-    HIPCHK(c, hipMemcpyAsync(h.data(), acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
-    if (counts)
-        TRY(copy_back(c, counts, dcounts, sizeof(int32_t) * nnodes, flags));
-    HIPCHK(c, hipStreamSynchronize(s));
     for (int p = 0; p < nparts; ++p) {
         const double *hc = h.data() + (size_t)p * (ncf + nnf), *hn = hc + ncf;
         for (int64_t a = 0; a < nout; ++a) {
@@ -1174,8 +1178,209 @@ int oka_cv_node_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, int32_t nreal
             }
         }
     }
+}
+
+// Work buffers of the node reduction: probabilities and masks of one chunk,
+// the split partial sums and the part accumulators (zeroed).
+int node_buffers(oka_mlp *M, int nk, int nparts, float **probs, double **part, double **acc)
+{
+    oka_ctx *c = M->ctx;
+    const int64_t nout = M->nout;
+    const int64_t ncf = nout * (1 + kPairs), nnf = nout * nk * (1 + kNbr);
+    void *p1, *p2, *p3;
+    TRY(ensure(c, kSlotNodeProbs, sizeof(float) * M->max_batch * nout, &p1));
+    TRY(ensure(c, kSlotNodePart, sizeof(double) * kNodeSplits * (ncf + nnf), &p2));
+    TRY(ensure(c, kSlotNodeAcc, sizeof(double) * nparts * (ncf + nnf), &p3));
+    TRY(fill_u32(c, p3, 2 * (int64_t)nparts * (ncf + nnf)));
+    *probs = (float *)p1;
+    *part = (double *)p2;
+    *acc = (double *)p3;
+    return OKA_OK;
+}
+
+int stage_features(oka_mlp *M, const float *x, int64_t nreal, uint32_t flags, const float **dx)
+{
+    oka_ctx *c = M->ctx;
+    *dx = x;
+    if (flags & OKA_PTR_DEVICE)
+        return OKA_OK;
+    void *xv;
+    TRY(ensure(c, kSlotNodeX, sizeof(float) * std::max<int64_t>(nreal * M->nin, 1), &xv));
+    HIPCHK(c, hipMemcpyAsync(xv, x, sizeof(float) * (size_t)nreal * M->nin, hipMemcpyHostToDevice,
+                             c->stream));
+    *dx = (const float *)xv;
+    return OKA_OK;
+}
+
+}  // namespace
+
+struct oka_node_set {
+    oka_ctx *c = nullptr;
+    int64_t nreal = 0;
+    oka_window win{};
+    uint16_t *masks = nullptr;      // [nreal][cells] node masks on the device
+};
+
+extern "C" {
+
+int oka_cv_node_sums(oka_ctx *c, oka_mlp *M, const oka_site *site, int32_t nreal,
+                     const double *K, const double *n, const double *b, const double *q,
+                     const double *coef, const float *x, const oka_trace_opts *opts,
+                     const oka_grid *grid, const oka_window *win, int32_t nparts, int32_t *counts,
+                     double *sum_s, double *sum_ss, double *sum_y, double *sum_ys,
+                     oka_trace_stats *stats, uint32_t flags)
+{
+    if (!c || !M || M->ctx != c)
+        return c ? fail(c, OKA_ERR_ARG, "the network belongs to another context") : OKA_ERR_ARG;
+    TRY(check_node_args(c, M->nout, site, opts, grid, win, nreal));
+    if (!x || !sum_s || !sum_ss || !sum_y || !sum_ys || nparts < 1 || nparts > nreal)
+        return fail(c, OKA_ERR_ARG, "bad node-sum arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nk = win->cell * win->cell;
+    const int64_t nout = M->nout;
+    NodeInputs in;
+    TRY(stage_node_inputs(c, site, opts, nreal, K, n, b, q, coef, flags, &in));
+    const float *dx;
+    TRY(stage_features(M, x, nreal, flags, &dx));
+    void *dcounts = nullptr;
+    const int64_t nnodes = (int64_t)grid->nrows * grid->ncols;
+    if (counts)
+        TRY(stage_out(c, kSlotNodeCounts, counts, sizeof(int32_t) * nnodes, flags, true, &dcounts));
+    const int chunk = M->max_batch;
+    void *masks, *marks;
+    TRY(ensure(c, kSlotNodeMasks, sizeof(uint16_t) * chunk * nout, &masks));
+    TRY(ensure(c, kSlotNodeMarks, (size_t)chunk * nout, &marks));
+    float *probs;
+    double *part, *acc;
+    TRY(node_buffers(M, nk, nparts, &probs, &part, &acc));
+    const int64_t per_part = nout * (1 + kPairs) + nout * nk * (1 + kNbr);
+    oka_trace_stats tot{};
+    // each part's rows in chunks of the network's batch: trace (marks and node
+    // masks stay on the device), forward, then the part's node sums
+    for (int p = 0; p < nparts; ++p) {
+        const int64_t lo = (int64_t)nreal * p / nparts, hi = (int64_t)nreal * (p + 1) / nparts;
+        for (int64_t r0 = lo; r0 < hi; r0 += chunk) {
+            const int rows = (int)std::min<int64_t>(chunk, hi - r0);
+            TRY(trace_node_rows(c, in, r0, rows, grid, win, (uint16_t *)masks, (uint8_t *)marks,
+                                (int32_t *)dcounts, tot));
+            TRY(reduce_node_rows(M, dx, r0, rows, (const uint16_t *)masks, win, probs, part,
+                                 acc + p * per_part));
+        }
+    }
+    std::vector<double> h((size_t)nparts * per_part);
+    HIPCHK(c, hipMemcpyAsync(h.data(), acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    if (counts)
+        TRY(copy_back(c, counts, dcounts, sizeof(int32_t) * nnodes, flags));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unpack_node_sums(h, nparts, nout, nk, sum_s, sum_ss, sum_y, sum_ys);
     if (stats)
         *stats = tot;
+    return OKA_OK;
+}
+
+int oka_cv_node_trace(oka_ctx *c, const oka_site *site, int32_t nreal, const double *K,
+                      const double *n, const double *b, const double *q, const double *coef,
+                      const oka_trace_opts *opts, const oka_grid *grid, const oka_window *win,
+                      int32_t *counts, oka_node_set **out, oka_trace_stats *stats, uint32_t flags)
+{
+    if (!c || !out)
+        return c ? fail(c, OKA_ERR_ARG, "out is NULL") : OKA_ERR_ARG;
+    *out = nullptr;
+    TRY(check_node_args(c, 0, site, opts, grid, win, nreal));
+    if ((int64_t)win->nrows * win->ncols > 16384)
+        return fail(c, OKA_ERR_ARG, "at most 16384 window cells");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t ncell = (int64_t)win->nrows * win->ncols;
+    NodeInputs in;
+    TRY(stage_node_inputs(c, site, opts, nreal, K, n, b, q, coef, flags, &in));
+    void *dcounts = nullptr;
+    const int64_t nnodes = (int64_t)grid->nrows * grid->ncols;
+    if (counts)
+        TRY(stage_out(c, kSlotNodeCounts, counts, sizeof(int32_t) * nnodes, flags, true, &dcounts));
+    oka_node_set *S = new oka_node_set();
+    S->c = c;
+    S->nreal = nreal;
+    S->win = *win;
+    if (hipMalloc((void **)&S->masks, sizeof(uint16_t) * std::max<int64_t>(nreal * ncell, 1)) !=
+        hipSuccess) {
+        delete S;
+        return fail(c, OKA_ERR_NOMEM, "no device memory for %d x %lld node masks", nreal,
+                    (long long)ncell);
+    }
+    // chunks of realizations: the cell marks go to a chunk-sized scratch
+    const int chunk = 16384;
+    void *marks;
+    int rc = ensure(c, kSlotNodeMarks, (size_t)std::min<int64_t>(chunk, std::max(nreal, 1)) * ncell,
+                    &marks);
+    oka_trace_stats tot{};
+    for (int64_t r0 = 0; rc == OKA_OK && r0 < nreal; r0 += chunk) {
+        const int rows = (int)std::min<int64_t>(chunk, nreal - r0);
+        rc = trace_node_rows(c, in, r0, rows, grid, win, S->masks + r0 * ncell, (uint8_t *)marks,
+                             (int32_t *)dcounts, tot);
+    }
+    if (rc == OKA_OK && counts)
+        rc = copy_back(c, counts, dcounts, sizeof(int32_t) * nnodes, flags);
+    if (rc == OKA_OK && hipStreamSynchronize(c->stream) != hipSuccess)
+        rc = fail(c, OKA_ERR_DEVICE, "node trace failed");
+    if (rc != OKA_OK) {
+        oka_node_set_destroy(S);
+        return rc;
+    }
+    if (stats)
+        *stats = tot;
+    *out = S;
+    return OKA_OK;
+}
+
+void oka_node_set_destroy(oka_node_set *S)
+{
+    if (!S)
+        return;
+    if (S->masks) {
+        (void)hipSetDevice(S->c->device);
+        (void)hipFree(S->masks);
+    }
+    delete S;
+}
+
+int oka_cv_node_reduce(oka_mlp *M, const oka_node_set *S, const float *x, int32_t nparts,
+                       double *sum_s, double *sum_ss, double *sum_y, double *sum_ys,
+                       uint32_t flags)
+{
+    if (!M)
+        return OKA_ERR_ARG;
+    oka_ctx *c = M->ctx;
+    if (!S || !x || !sum_s || !sum_ss || !sum_y || !sum_ys || nparts < 1 || nparts > S->nreal)
+        return fail(c, OKA_ERR_ARG, "bad node-reduce arguments");
+    if (S->c->device != c->device)
+        return fail(c, OKA_ERR_ARG, "the node set lives on another device");
+    if ((int64_t)S->win.nrows * S->win.ncols != M->nout)
+        return fail(c, OKA_ERR_ARG, "the node set's window has %d cells, the network %d outputs",
+                    S->win.nrows * S->win.ncols, M->nout);
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nk = S->win.cell * S->win.cell;
+    const int64_t nout = M->nout, nreal = S->nreal;
+    const float *dx;
+    TRY(stage_features(M, x, nreal, flags, &dx));
+    float *probs;
+    double *part, *acc;
+    TRY(node_buffers(M, nk, nparts, &probs, &part, &acc));
+    const int64_t per_part = nout * (1 + kPairs) + nout * nk * (1 + kNbr);
+    const int chunk = M->max_batch;
+    for (int p = 0; p < nparts; ++p) {
+        const int64_t lo = nreal * p / nparts, hi = nreal * (p + 1) / nparts;
+        for (int64_t r0 = lo; r0 < hi; r0 += chunk) {
+            const int rows = (int)std::min<int64_t>(chunk, hi - r0);
+            TRY(reduce_node_rows(M, dx, r0, rows, S->masks + r0 * nout, &S->win, probs, part,
+                                 acc + p * per_part));
+        }
+    }
+    std::vector<double> h((size_t)nparts * per_part);
+    HIPCHK(c, hipMemcpyAsync(h.data(), acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unpack_node_sums(h, nparts, nout, nk, sum_s, sum_ss, sum_y, sum_ys);
     return OKA_OK;
 }
 

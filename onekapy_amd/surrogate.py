@@ -526,6 +526,12 @@ def neighbours(window):
     return out
 
 
+def _grid_shape(nb):
+    """(rows, columns) of the window's cells from its neighbour table."""
+    nc = int(nb[0, 7]) if nb[0, 7] >= 0 else nb.shape[0]   # neighbour (+1, 0) of cell 0
+    return nb.shape[0] // nc, nc
+
+
 def _pair_products(ss, nb):
     """E-sums of s_b s_b' for every pair (m, l) of each cell's neighbours, from
     the half-offset pair sums ss [..., cells, 13]: [..., cells, 9, 9]."""
@@ -571,6 +577,65 @@ def node_sums(net, site, d, coef, X, window, parts=2, precision=_lib.FP32, count
         _lib.ptr(coef), _lib.ptr(X), o, g, window, int(parts), _lib.ptr(counts),
         _lib.ptr(out["s"]), _lib.ptr(out["ss"]), _lib.ptr(out["y"]), _lib.ptr(out["ys"]), st, 0))
     return out, st
+
+
+class NodeSet:
+    """A traced correction set's node masks, kept on the device
+    (oka_cv_node_trace); close() frees them."""
+
+    def __init__(self, ctx, h, nreal, window):
+        self.ctx, self.h, self.nreal, self.window = ctx, h, nreal, window
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.oka_node_set_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def node_trace(site, d, coef, window, precision=_lib.FP32, counts=None, ctx=None):
+    """Traces the realizations d (coef: their A..F) and keeps their window
+    node masks on the device: (NodeSet, trace statistics).  counts (int32
+    grid, or None) accumulates their visited nodes."""
+    ctx = ctx or _lib.context()
+    pf = site.fixed_field()
+    g = pf.grid()
+    wxy, s = _site_struct(site)
+    starts = start_points(site.xt, site.yt, site.rt, site.npaths)
+    o = _lib.make_trace_opts(site.npaths, site.duration, site.tol, site.maxstep, site.umbra,
+                             _lib.ptr(starts), precision, site.confined)
+    K, n, b, q = (np.ascontiguousarray(d[k], dtype=np.float64) for k in ("K", "n", "b", "q"))
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    if counts is not None and (counts.dtype != np.int32 or not counts.flags.c_contiguous or
+                               counts.shape != (pf.nrows, pf.ncols)):
+        raise ValueError("counts must be a C-contiguous int32 array of the grid's shape")
+    st = _lib.TraceStats_()
+    h = C.c_void_p()
+    ctx.check(ctx.lib.oka_cv_node_trace(ctx.h, s, len(K), _lib.ptr(K), _lib.ptr(n), _lib.ptr(b),
+                                        _lib.ptr(q), _lib.ptr(coef), o, g, window,
+                                        _lib.ptr(counts), C.byref(h), st, 0))
+    return NodeSet(ctx, h, len(K), window), st
+
+
+def node_reduce(net, nodeset, X, parts=2):
+    """The node sums of a NodeSet under the network `net` (oka_cv_node_reduce),
+    as node_sums returns them."""
+    w = nodeset.window
+    ncell, nk = w.nrows * w.ncols, w.cell * w.cell
+    out = {"s": np.zeros((parts, ncell)), "ss": np.zeros((parts, ncell, len(PAIRS))),
+           "y": np.zeros((parts, ncell, nk)), "ys": np.zeros((parts, ncell, nk, 9))}
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    if X.shape[0] != nodeset.nreal:
+        raise ValueError("one input row per traced realization")
+    net.ctx.check(net.ctx.lib.oka_cv_node_reduce(
+        net.h, nodeset.h, _lib.ptr(X), int(parts), _lib.ptr(out["s"]), _lib.ptr(out["ss"]),
+        _lib.ptr(out["y"]), _lib.ptr(out["ys"]), 0))
+    return out
 
 
 def node_estimate(sums, n_rows, nb, mu_reps, ridge=1e-2, flat=1e-10, min_events=30):
@@ -635,11 +700,19 @@ def node_estimate(sums, n_rows, nb, mu_reps, ridge=1e-2, flat=1e-10, min_events=
               + (np.matmul(beta, Gh) * beta).sum(-1))
         var += w * w * np.maximum(vd, 0.0) / nh
         bbar += w * beta
-    # the surrogate term's variance from the replicates' spread
+    # the surrogate term's variance from the replicates' spread: their
+    # covariance over each cell's neighbours, through the same pair offsets
     R = mu_reps.shape[0]
-    dev = np.where(absent[:, None, :], 0.0, np.swapaxes((mu_reps - mu)[:, nbi], 0, 1))
-    proj = np.matmul(dev, np.swapaxes(bbar, 1, 2))                 # [cells, R, nk]
-    var += (proj ** 2).sum(1) / (R - 1) / R
+    dev = mu_reps - mu                                               # [R, cells]
+    nr, nc = _grid_shape(nb)
+    dq = np.zeros((nb.shape[0], len(PAIRS)))
+    dv = dev.reshape(R, nr, nc)
+    for h, (dr, dc) in enumerate(PAIRS):
+        c0, c1 = max(0, -dc), nc - max(0, dc)
+        prod = dv[:, :nr - dr, c0:c1] * dv[:, dr:, c0 + dc:c1 + dc]
+        dq.reshape(nr, nc, -1)[:nr - dr, c0:c1, h] = prod.sum(0)
+    cov_mu = _pair_products(dq, nb) / ((R - 1) * R)                 # [cells, 9, 9]
+    var += (np.matmul(bbar, cov_mu) * bbar).sum(-1)
     return p, var, bbar
 
 
@@ -650,7 +723,7 @@ class NodeEstimate:
 
 def control_variate_nodes(site, n_train, n_correct, n_surrogate, train_steps=1200, batch=2048,
                           lr=5e-3, seed=0, precision=_lib.FP32, pilot=2000, window=None,
-                          n_hidden=512, qmc_replicates=64, qmc_pow2=True, parts=2):
+                          n_hidden=512, qmc_replicates=64, qmc_pow2=True, parts=2, overlap=True):
     """Control-variate estimate of every node's capture probability: the
     ProbabilityField of create_stochastic_capturezone (stochastic.py:76-270)
     on the site's fixed lattice.
@@ -663,6 +736,9 @@ def control_variate_nodes(site, n_train, n_correct, n_surrogate, train_steps=120
     parts of the correction realizations).  Nodes outside the
     window take the plain estimate over all n_train + n_correct traced
     realizations, which are independent draws of the same distribution.
+    With overlap, the network trains and sums the surrogate term on a
+    second context while the correction set is traced; the node sums follow
+    (oka_cv_node_trace, oka_cv_node_reduce).
 
     Returns a NodeEstimate: p (float64 [nrows, ncols]), var (its variance per
     node: the estimator's in the window, p (1 - p) / n_traced outside), the
@@ -697,27 +773,56 @@ def control_variate_nodes(site, n_train, n_correct, n_surrogate, train_steps=120
     counts = np.zeros((pf.nrows, pf.ncols), np.int32)
     marks, st_train = trace_marks(site, dt, coef[:n_train], window, precision, counts=counts)
     t["trace_train"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
     tscale = 1.0 / (window.cell * window.cell)
     F = features(d, coef)
     std = Standardizer(F[:n_train])
     X = std(F)
     ncell = window.nrows * window.ncols
-    net = SurrogateMLP(X.shape[1], n_hidden, ncell, seed=seed + 1)
-    epochs = max(1, int(math.ceil(train_steps / max(1, n_train // batch))))
-    losses = net.train(X[:n_train], marks, tscale, epochs, batch, lr, seed=seed + 100)
-    t["train"] = time.perf_counter() - t0
-    del marks
-    t0 = time.perf_counter()
+    # the network on a context of its own: it trains (and sums the surrogate
+    # term) while this thread traces the correction set on the default one
+    ctx2 = _lib.Context(_lib.context().device)
+    net = SurrogateMLP(X.shape[1], n_hidden, ncell, seed=seed + 1, ctx=ctx2)
     npts, nrep = qmc_split(n_surrogate, qmc_replicates, qmc_pow2)
     surr_seed = int(rng.integers(1, 2 ** 63))
-    reps = _surrogate_sums_qmc(net, site, npts, [_replicate_seed(surr_seed, 0, k)
-                                                 for k in range(nrep)], std) / npts
-    t["surrogate"] = time.perf_counter() - t0
+    side = {}
+
+    def learn():
+        try:
+            t1 = time.perf_counter()
+            epochs = max(1, int(math.ceil(train_steps / max(1, n_train // batch))))
+            side["losses"] = net.train(X[:n_train], marks, tscale, epochs, batch, lr,
+                                       seed=seed + 100)
+            t2 = time.perf_counter()
+            seeds = [_replicate_seed(surr_seed, 0, k) for k in range(nrep)]
+            side["reps"] = _surrogate_sums_qmc(net, site, npts, seeds, std) / npts
+            side["times"] = (t2 - t1, time.perf_counter() - t2)
+        except BaseException as e:   # re-raised in the calling thread
+            side["error"] = e
+
     t0 = time.perf_counter()
-    sums, st_corr = node_sums(net, site, dc, coef[n_train:], X[n_train:], window, parts,
-                              precision, counts)
-    t["trace_correct"] = time.perf_counter() - t0
+    if overlap:
+        th = threading.Thread(target=learn)
+        th.start()
+    else:
+        learn()
+    t1 = time.perf_counter()
+    try:
+        nodeset, st_corr = node_trace(site, dc, coef[n_train:], window, precision, counts)
+    finally:
+        if overlap:
+            th.join()
+    t["trace_correct"] = time.perf_counter() - t1
+    if "error" in side:
+        nodeset.close()
+        raise side["error"]
+    t["train"], t["surrogate"] = side["times"]
+    t["train_and_correct"] = time.perf_counter() - t0
+    del marks
+    t0 = time.perf_counter()
+    sums = node_reduce(net, nodeset, X[n_train:], parts)
+    nodeset.close()
+    reps, losses = side["reps"], side["losses"]
+    t["reduce"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     bounds = [n_correct * p // parts for p in range(parts + 1)]
     n_rows = np.diff(bounds)

@@ -9,7 +9,7 @@ from onekapy_amd import _lib, sites
 from onekapy_amd.stochastic import draw_realizations, fit_and_draw
 from onekapy_amd.surrogate import (PAIRS, Standardizer, SurrogateMLP, _node_image, _window,
                                    choose_window, control_variate_nodes, features, neighbours,
-                                   node_sums, trace_counts, trace_marks)
+                                   node_reduce, node_sums, node_trace, trace_counts, trace_marks)
 from helpers import oracle_chronicle, oracle_fit_draw, small_site
 
 pytestmark = pytest.mark.gpu
@@ -96,6 +96,18 @@ def test_node_sums_equal_their_parts(gpu_ctx):
         np.testing.assert_allclose(sums["ys"][p], want, rtol=1e-12, atol=1e-12)
     assert sums["y"].sum() > 1000
     assert st.attempted > 0
+    # the same sums in two calls: the set traced and kept on the device, then
+    # reduced by the network on a context of its own
+    counts2 = np.zeros_like(pc)
+    ns, st2 = node_trace(s, d, coef, win, counts=counts2)
+    net2 = SurrogateMLP(X.shape[1], 128, 256, max_batch=256, seed=4,
+                        ctx=_lib.Context(_lib.context().device))
+    sums2 = node_reduce(net2, ns, X, parts=3)
+    ns.close()
+    assert np.array_equal(counts2, pc)
+    assert st2.attempted == st.attempted
+    for k in sums:
+        assert np.array_equal(sums2[k], sums[k]), k
 
 
 @pytest.fixture(scope="module")
@@ -125,6 +137,14 @@ def test_node_control_variate_is_unbiased_over_the_lattice(cfg0_plain_run_nodes)
     res = control_variate_nodes(s, n_train=8192, n_correct=16384, n_surrogate=1 << 17,
                                 train_steps=1500, batch=512, lr=1e-3, seed=21, n_hidden=256,
                                 qmc_replicates=16)
+    # the overlapped and the sequential pipeline give the same estimate, up
+    # to the order of the surrogate sums' FP64 atomic adds (amplified by the
+    # coefficients of nearly flat cells), far below the standard errors
+    seq = control_variate_nodes(s, n_train=8192, n_correct=16384, n_surrogate=1 << 17,
+                                train_steps=1500, batch=512, lr=1e-3, seed=21, n_hidden=256,
+                                qmc_replicates=16, overlap=False)
+    np.testing.assert_allclose(seq.p, res.p, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(seq.var, res.var, rtol=1e-3, atol=1e-12)
     p_ref = counts / R
     var_ref = p_ref * (1 - p_ref) / R
     assert res.p.shape == counts.shape
