@@ -567,8 +567,8 @@ def bench_cfg4(args, torch):
     st = m["stats"]
     flops = canonical_flops(st, len(site.wells))
     achieved = flops / (m["kernel_ms"] * 1e-3) / 1e12
-    # FP32 against FP64-exact on the first 1000 realizations
-    n = min(1000, R)
+    # FP32 against FP64-exact on the first 2000 realizations
+    n = min(2000, R)
     coef = run.coef.cpu().numpy()
     dd = {k: v[:n] for k, v in d.items()}
     log(f"cfg4: FP64-exact subset of {n} realizations (pool schedule)")

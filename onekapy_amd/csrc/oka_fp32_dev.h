@@ -155,17 +155,23 @@ struct FarField {
     const int *near;                // [nbx nby][near_stride] (a multiple of 4):
                                     // per box the count of near wells, their
                                     // indices, then -1
-    const int *wbox;                // [nw][2]: each well's box column and row
     float2 *beta;                   // [nbx nby][kFarTerms] of this slot
 };
 
 // beta of boxes lane, lane + stride, ... from the wells w[0, nw): each lane
 // walks the wells once per kFarChunk coefficients of one box, keeping only
-// those in registers (the power s_w / v_w^(k0+1) that starts a chunk is
-// rebuilt by repeated multiplication).  Coefficients from the order up to
-// kFarTerms are zero.
+// those in registers; the power s_w / v_w^(k0+1) that starts a chunk is
+// s_w / v_w times (1 / v_w^4)^(k0 / 4).  A confined slot's well records carry
+// their box in the fourth component (far_box), so the near test reads LDS
+// only.  Coefficients from the order up to kFarTerms are zero.
 constexpr int kFarChunk = 4;
 static_assert(kFarTerms % kFarChunk == 0 && kFarTerms % 2 == 0, "whole chunks and float4 pairs");
+
+// A well's box packed into a float's bits (column low, row high 16 bits).
+__device__ __forceinline__ float far_box(int bx, int by)
+{
+    return __int_as_float((int)((uint32_t)bx | ((uint32_t)by << 16)));
+}
 
 __device__ __forceinline__ void far_coefficients(const FarField &F, const float4 *w, int nw,
                                                  float2 *beta, int lane, int stride)
@@ -181,17 +187,22 @@ __device__ __forceinline__ void far_coefficients(const FarField &F, const float4
             for (int k = 0; k < kFarChunk; ++k)
                 ar[k] = ai[k] = 0.0f;
             for (int i = 0; k0 < F.order && i < nw; ++i) {
-                if (abs(F.wbox[2 * i] - bx) <= 1 && abs(F.wbox[2 * i + 1] - by) <= 1)
-                    continue;   // near: summed directly
                 const float4 wl = w[i];
+                const uint32_t wb = (uint32_t)__float_as_int(wl.w);
+                if (abs((int)(wb & 0xffffu) - bx) <= 1 && abs((int)(wb >> 16) - by) <= 1)
+                    continue;   // near: summed directly
                 const float vx = (wl.x - cx) * F.ilen, vy = (wl.y - cy) * F.ilen;
                 const float iv = 1.0f / fmaf(vx, vx, vy * vy);
                 const float qx = vx * iv, qy = -vy * iv;   // 1 / v_w
                 float tx = wl.z * qx, ty = wl.z * qy;      // s_w / v_w
-                for (int k = 0; k < k0; ++k) {
-                    const float nx = fmaf(tx, qx, -ty * qy);
-                    ty = fmaf(tx, qy, ty * qx);
-                    tx = nx;
+                if (k0 > 0) {
+                    const float q2x = fmaf(qx, qx, -qy * qy), q2y = 2.0f * qx * qy;
+                    const float q4x = fmaf(q2x, q2x, -q2y * q2y), q4y = 2.0f * q2x * q2y;
+                    for (int k = 0; k < k0; k += kFarChunk) {
+                        const float nx = fmaf(tx, q4x, -ty * q4y);
+                        ty = fmaf(tx, q4y, ty * q4x);
+                        tx = nx;
+                    }
                 }
 #pragma unroll
                 for (int k = 0; k < kFarChunk; ++k) {

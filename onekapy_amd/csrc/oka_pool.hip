@@ -18,7 +18,7 @@ using namespace fp32;
 using namespace pool;
 
 struct PoolFP32 {
-    using Well = float4;   // (xw - xo, yw - yo, q / (2 pi b n), q ln 2 / (4 pi h))
+    using Well = float4;   // (xw - xo, yw - yo, q / (2 pi b n), q ln 2 / (4 pi h) or the box)
     struct Consts {
         float a2, b2, c, d, e;              // regional velocity over -(b n)
         float pa, pb, pc, pd, pe, pf;       // regional potential over h (unconfined)
@@ -46,11 +46,15 @@ struct PoolFP32 {
         // h = K b^2 / 2, the potential at which the head reaches the top of
         // the aquifer (model.py:344)
         const double h = A.confined ? 1.0 : 0.5 * A.K[r] * (A.b[r] * A.b[r]);
+        // the fourth component: the unconfined potential's coefficient, or for
+        // a confined slot with far-well expansions the well's box (far_box)
+        const bool far = A.confined && A.far_order > 0;
         for (int i = threadIdx.x; i < A.nwells; i += kWave) {
             const double qw = A.q[r * A.nwells + i];
             w[i] = make_float4((float)(A.wells_xy[2 * i] - A.xo),
                                (float)(A.wells_xy[2 * i + 1] - A.yo), (float)(qw * kInv2Pi / bn),
-                               A.confined ? 0.0f : (float)(qw * (kInv4Pi * kLn2) / h));
+                               far ? far_box(A.far_wbox[2 * i], A.far_wbox[2 * i + 1])
+                                   : (A.confined ? 0.0f : (float)(qw * (kInv4Pi * kLn2) / h)));
         }
         return true;
     }
@@ -101,7 +105,6 @@ struct PoolFP32 {
         F.ilen = 1.0f / A.far_len;
         F.near_stride = A.far_near_stride;
         F.near = A.far_near;
-        F.wbox = A.far_wbox;
         F.beta = P.far_beta + ((int64_t)blockIdx.x * P.slots + s) * P.far_stride;   // 128-byte rows
     }
 
