@@ -88,6 +88,9 @@ def parse():
                     help="trace the unconfined velocity (compute_velocity) in the fp64/fp32 modes; "
                          "the roofline keeps the confined flop count")
     ap.add_argument("--only-cv", action="store_true", help="run only the control-variate leg")
+    ap.add_argument("--mlp-step", action="store_true",
+                    help="time the surrogate network alone: Adam steps of --cv-batch on "
+                         "synthetic inputs and inference of one batch (for kernel profiles)")
     ap.add_argument("--cv-train", type=int, default=100000,
                     help="traced realizations the larger estimator's network trains on")
     ap.add_argument("--cv-correct", type=int, default=200000,
@@ -192,6 +195,35 @@ def control_variate_leg(site, args, fp32_rate):
                                   "seconds": tj}})
     out["as_configured"] = summary(*configured)
     return out
+
+
+def mlp_step(args):
+    """The control variate's network alone (34 -> 512 -> 512 -> 4096): Adam
+    steps of --cv-batch samples (16 per epoch) on synthetic inputs and window
+    marks, one untimed epoch first."""
+    from onekapy_amd.surrogate import SurrogateMLP
+    rng = np.random.default_rng(0)
+    b = args.cv_batch
+    n = 16 * b
+    x = rng.normal(0, 1, (n, 34)).astype(np.float32)
+    marks = rng.integers(0, 17, (n, 4096)).astype(np.uint8)
+    net = SurrogateMLP(34, 512, 4096, max_batch=b, seed=1)
+    net.train(x, marks, 1.0 / 16, 1, b, args.cv_lr, seed=1)
+    wall = {2: [], 10: []}
+    for _ in range(3):
+        for epochs in wall:
+            t0 = time.perf_counter()
+            net.train(x, marks, 1.0 / 16, epochs, b, args.cv_lr, seed=epochs)
+            wall[epochs].append(time.perf_counter() - t0)
+    wall = {k: min(v) for k, v in wall.items()}
+    # every call stages the same samples once: the difference is 128 steps
+    step = (wall[10] - wall[2]) / 128
+    flops = 2.0 * b * (34 * 512 + 512 * 512 + 512 * 4096)
+    return {"network": "34->512->512->4096 GELU, BF16 MFMA", "batch": b,
+            "train_step_us": step * 1e6, "train_tflops": 3 * flops / step / 1e12,
+            "call_seconds": {"32 steps": wall[2], "160 steps": wall[10]},
+            "note": "per step: the difference of the fastest of 3 calls of 160 and of 32 "
+                    "steps, over 128"}
 
 
 def copy_calibration(torch):
@@ -660,6 +692,9 @@ def main():
         print(json.dumps(bench_cfg4(args, torch)))
         return
     site = sites.config1(args.nreal) if args.config == "cfg1" else sites.config0()
+    if args.mlp_step:
+        print(json.dumps({"mlp_step": mlp_step(args)}))
+        return
     if args.only_cv:
         cv = control_variate_leg(site, args, None)
         print(json.dumps({"control_variate": cv}))

@@ -35,8 +35,9 @@ struct GemmArgs {
     uint16_t *Cht;           // BF16 transposed output [N, M]
     const float *Zsaved;     // pre-activation [M, N] for kEpiGeluBwd
     float *dbias;            // kEpiStoreF32: row sums of A (the bias gradient of a dW product)
-    const uint8_t *marks;    // targets * (1 / tscale): kEpiLossGrad [M, N] (the batch's rows),
-                             // kEpiAccum marks[(row0 + m) * N + n]
+    const uint8_t *marks;    // targets * (1 / tscale): kEpiLossGrad the batch's rows in the
+                             // fragment order of launch_prep_input, kEpiAccum
+                             // marks[(row0 + m) * N + n]
     int64_t row0;
     float tscale, gscale;
     double *sums, *sums_sq, *diffs, *diffs_sq;
@@ -82,10 +83,14 @@ __host__ __device__ inline int gemm_tiles(const GemmArgs &g, int big = 0)
     return (g.N / t) * (g.M / t) * g.splits;
 }
 
+// Workgroups of a launch of g1 (and g2 when not null): one per block tile.
+int gemm2_workgroups(const GemmArgs &g1, const GemmArgs *g2);
 // One launch of up to two independent products (the second with e2 != kEpiNone).
 hipError_t launch_gemm(int epi, const GemmArgs &g, hipStream_t s);
 hipError_t launch_gemm2(int e1, const GemmArgs &g1, int e2, const GemmArgs &g2, hipStream_t s);
-// The batch's inputs (and, with marks set, its target rows into ym [M, nout]).
+// The batch's inputs (and, with marks set, its target rows into ym [M x nout
+// bytes] in MFMA fragment order: 16 x 16 blocks of 64 dwords, dword l of block
+// (bm, bn) holding rows 16 bm + 4 (l / 16) .. + 3 of column 16 bn + l % 16).
 hipError_t launch_prep_input(const float *x, int n_in, int kin, const int32_t *rowidx,
                              int64_t row0, int M, int mvalid, uint16_t *x0, uint16_t *x0t,
                              const uint8_t *marks, int nout, uint8_t *ym, hipStream_t s,

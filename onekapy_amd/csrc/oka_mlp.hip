@@ -78,8 +78,15 @@ struct Shape {
 
 template <int kBig>
 struct GemmSmem {
-    uint16_t a[2][Shape<kBig>::BM][kLd];  // double-buffered A and B slices
-    uint16_t b[2][Shape<kBig>::BN][kLd];
+    static_assert(Shape<kBig>::BM == Shape<kBig>::BN, "square block tiles");
+    union {
+        struct {
+            uint16_t a[2][Shape<kBig>::BM][kLd];  // double-buffered A and B slices
+            uint16_t b[2][Shape<kBig>::BN][kLd];
+        };
+        // after the main loop: the BF16 result tile, row-major [0] and transposed [1]
+        uint16_t out[2][Shape<kBig>::BM][Shape<kBig>::BN + 8];
+    };
     double red[4];
     float col[4][Shape<kBig>::BN];        // kEpiAccum: the workgroup's column sums
 };
@@ -131,6 +138,20 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
 #pragma unroll
         for (int j = 0; j < TN; ++j)
             acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // kEpiLossGrad: the lane's targets, loaded before the main loop so that
+    // their latency is hidden; the targets are in fragment order (one dword
+    // per lane and 16 x 16 block: rows mb .. mb + 3 of the lane's column)
+    uint32_t ytile[TM][TN];
+    if constexpr (kEpi == kEpiLossGrad) {
+        const uint32_t *y32 = reinterpret_cast<const uint32_t *>(g.marks);
+        const int64_t nbn = g.N >> 4;
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                ytile[i][j] = y32[(((m0 + wm) / 16 + i) * nbn + (n0 + wn) / 16 + j) * 64 + lane];
+    }
 
     auto load = [&](Stage<kBig> &r, int st) {
         const int64_t o = (int64_t)st * kBK;
@@ -283,6 +304,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
         }
         return;
     }
+    // BF16 results go through LDS and leave as 16-byte row segments
+    constexpr bool kStaged = kEpi == kEpiGeluFwd || kEpi == kEpiGeluBwd || kEpi == kEpiLossGrad;
     const bool want_loss = kEpi == kEpiLossGrad && g.loss != nullptr;
     float part_l = 0.f;   // the lane's 16 loss terms (monitoring only)
     float *cf = g.Cf;
@@ -306,24 +329,21 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
                     if (cf)   // the pre-activation is kept for the backward pass only
                         cf[(int64_t)m * g.ldc + n] = z;
                     hv[e] = f2bf(gelu(z));
-                    g.Ch[(int64_t)m * g.N + n] = hv[e];
                 } else if (kEpi == kEpiLogits) {
                     cf[(int64_t)m * g.ldc + n] = v + bias;
                 } else if (kEpi == kEpiGeluBwd) {
                     hv[e] = f2bf(v * gelu_grad(g.Zsaved[(int64_t)m * g.N + n]));
-                    g.Ch[(int64_t)m * g.N + n] = hv[e];
                 } else if (kEpi == kEpiStoreF32 || kEpi == kEpiPartialF32) {
                     cf[(int64_t)m * g.ldc + n] = v;
                 } else if (kEpi == kEpiLossGrad) {
                     const float z = v + bias;
-                    const float y = (float)g.marks[(int64_t)m * g.N + n] * g.tscale;
+                    const float y = (float)((ytile[i][j] >> (8 * e)) & 0xffu) * g.tscale;
                     // ez = e^-|z| in (0, 1] serves the logistic function and the
                     // loss (hardware exp2, log2 and reciprocal, about 1 ulp each)
                     const float ez = __builtin_amdgcn_exp2f(-1.4426950408889634f * fabsf(z));
                     const float r = __builtin_amdgcn_rcpf(1.0f + ez);
                     const float sg = z >= 0.f ? r : ez * r;
                     hv[e] = f2bf((sg - y) * g.gscale);
-                    g.Ch[(int64_t)m * g.N + n] = hv[e];
                     if (want_loss)   // BCE with logits: max(z, 0) - z y + log(1 + e^-|z|)
                         part_l += fmaxf(z, 0.f) - z * y +
                                   0.6931471805599453f * __builtin_amdgcn_logf(1.0f + ez);
@@ -332,12 +352,34 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, GemmSmem<
                         cf[(int64_t)m * g.ldc + n] = sigmoid(v + bias);
                 }
             }
-            if ((kEpi == kEpiGeluFwd || kEpi == kEpiGeluBwd || kEpi == kEpiLossGrad) && g.Cht) {
-                // transposed copy: 4 consecutive rows of column n are contiguous in C^T
-                uint2 packed;
-                packed.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
-                packed.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
-                *reinterpret_cast<uint2 *>(&g.Cht[(int64_t)n * g.M + mb]) = packed;
+            if constexpr (kStaged) {
+                const int ml = mb - m0, nl = n - n0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    sm.out[0][ml + e][nl] = hv[e];
+                if (g.Cht) {
+                    // transposed: 4 consecutive rows of column n are contiguous in C^T
+                    uint2 packed;
+                    packed.x = (uint32_t)hv[0] | ((uint32_t)hv[1] << 16);
+                    packed.y = (uint32_t)hv[2] | ((uint32_t)hv[3] << 16);
+                    *reinterpret_cast<uint2 *>(&sm.out[1][nl][ml]) = packed;
+                }
+            }
+        }
+    }
+    if constexpr (kStaged) {
+        // every wave has left the main loop (its last barrier) before writing
+        // LDS above; this barrier makes the whole tile visible
+        __syncthreads();
+        constexpr int RW = S::BN / 8;   // 16-byte words per tile row
+        for (int w = tid; w < S::BM * RW; w += 256) {
+            const int r = w / RW, c = (w - r * RW) * 8;
+            st16(g.Ch + (int64_t)(m0 + r) * g.N + n0 + c, ld16(&sm.out[0][r][c]));
+        }
+        if (g.Cht) {
+            for (int w = tid; w < S::BN * RW; w += 256) {
+                const int r = w / RW, c = (w - r * RW) * 8;
+                st16(g.Cht + (int64_t)(n0 + r) * g.M + m0 + c, ld16(&sm.out[1][r][c]));
             }
         }
     }
@@ -406,8 +448,11 @@ k_gemm_accum(GemmArgs g, int rt)
 
 // x (FP32, row stride n_in) of rows rowidx[row0 + m] (or row0 + m) -> X0
 // (BF16 [M, kin], zero padded) and X0^T ([kin, M]); rows >= mvalid are zero.
-// With marks set, the same rows of marks [*, nout] are gathered into ym
-// [M, nout] (16 bytes per lane).  In a captured training step (dstate set)
+// With marks set, the same rows of marks [*, nout] are gathered into ym in
+// fragment order: the 16 x 16 block (bm, bn) of the [M, nout] targets is the
+// 64 dwords at 64 (bm nout / 16 + bn), dword l holding rows 16 bm + 4 (l / 16)
+// + e (byte e) of column 16 bn + l % 16, the lane layout of an MFMA result
+// tile.  In a captured training step (dstate set)
 // row0 = dstate[0] * M, and this kernel, the step's first, advances the Adam
 // step counter dstate[1].
 __global__ void __launch_bounds__(256) k_prep_input(const float *x, int n_in, int kin,
@@ -433,15 +478,43 @@ __global__ void __launch_bounds__(256) k_prep_input(const float *x, int n_in, in
         x0[(int64_t)m * kin + k] = h;
         x0t[(int64_t)k * M + m] = h;
     }
+    // lane t: rows 4 mq .. 4 mq + 3, columns 16 q .. 16 q + 15 of the targets,
+    // transposed to 16 dwords (byte e of dword c: row 4 mq + e, column 16 q + c);
+    // four consecutive lanes fill one block's 256 bytes, consecutive blocks
+    // are consecutive in memory
     const int chunks = nout / 16;
-    if (marks && t < (int64_t)M * chunks) {
-        const int m = (int)(t / chunks), q = (int)(t % chunks);
-        uint4 w = make_uint4(0u, 0u, 0u, 0u);
-        if (m < mvalid) {
-            const int64_t row = rowidx ? rowidx[row0 + m] : row0 + m;
-            w = *reinterpret_cast<const uint4 *>(marks + row * nout + q * 16);
+    if (marks && t < (int64_t)(M / 4) * chunks) {
+        const int64_t blk = t >> 2;
+        const int q = (int)(blk % chunks);
+        const int mq = 4 * (int)(blk / chunks) + (int)(t & 3);
+        uint32_t w[4][4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int m = 4 * mq + e;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (m < mvalid) {
+                const int64_t row = rowidx ? rowidx[row0 + m] : row0 + m;
+                v = *reinterpret_cast<const uint4 *>(marks + row * nout + q * 16);
+            }
+            w[e][0] = v.x;
+            w[e][1] = v.y;
+            w[e][2] = v.z;
+            w[e][3] = v.w;
         }
-        *reinterpret_cast<uint4 *>(ym + (int64_t)m * nout + q * 16) = w;
+        uint32_t o[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const int sh = 8 * (c & 3);
+            o[c] = ((w[0][c >> 2] >> sh) & 0xffu) | (((w[1][c >> 2] >> sh) & 0xffu) << 8) |
+                   (((w[2][c >> 2] >> sh) & 0xffu) << 16) | (((w[3][c >> 2] >> sh) & 0xffu) << 24);
+        }
+        // block (mq / 4, q) starts at dword 64 (mq / 4 * chunks + q); its
+        // dwords 16 (mq % 4) .. + 15 are these rows
+        uint4 *dst = reinterpret_cast<uint4 *>(ym) + ((int64_t)(mq >> 2) * chunks + q) * 16 +
+                     4 * (mq & 3);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            dst[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
     }
 }
 
@@ -613,6 +686,12 @@ static bool use_big_tiles(const GemmArgs &g1, const GemmArgs *g2)
     return gemm_tiles(g1, 1) + (g2 ? gemm_tiles(*g2, 1) : 0) >= num_cu;
 }
 
+int gemm2_workgroups(const GemmArgs &g1, const GemmArgs *g2)
+{
+    const bool big = use_big_tiles(g1, g2);
+    return gemm_tiles(g1, big) + (g2 ? gemm_tiles(*g2, big) : 0);
+}
+
 hipError_t launch_gemm2(int e1, const GemmArgs &g1, int e2, const GemmArgs &g2, hipStream_t s)
 {
     if (g1.M % kTile || g1.N % kTile || g1.splits < 1 || g1.K % (g1.splits * kBK))
@@ -651,6 +730,8 @@ hipError_t launch_gemm2(int e1, const GemmArgs &g1, int e2, const GemmArgs &g2, 
     OKA_GEMM_CASE(kEpiPartialF32, kEpiNone)
     OKA_GEMM_CASE(kEpiStoreF32, kEpiGeluBwd)
     OKA_GEMM_CASE(kEpiStoreF32, kEpiPartialF32)
+    OKA_GEMM_CASE(kEpiPartialF32, kEpiGeluBwd)
+    OKA_GEMM_CASE(kEpiPartialF32, kEpiPartialF32)
 #undef OKA_GEMM_CASE
     return hipErrorInvalidValue;
 }
@@ -666,11 +747,11 @@ hipError_t launch_prep_input(const float *x, int n_in, int kin, const int32_t *r
                              const uint8_t *marks, int nout, uint8_t *ym, hipStream_t s,
                              int64_t *dstate)
 {
-    if (marks && (nout % 16 || !ym))
+    if (marks && (nout % 16 || M % 16 || !ym))
         return hipErrorInvalidValue;
     int64_t n = (int64_t)M * kin;
     if (marks)
-        n = n > (int64_t)M * (nout / 16) ? n : (int64_t)M * (nout / 16);
+        n = n > (int64_t)(M / 4) * (nout / 16) ? n : (int64_t)(M / 4) * (nout / 16);
     hipLaunchKernelGGL(k_prep_input, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n_in,
                        kin, rowidx, row0, M, mvalid, x0, x0t, marks, nout, ym, dstate);
     return hipGetLastError();

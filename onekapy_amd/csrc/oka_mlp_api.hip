@@ -27,8 +27,10 @@ struct oka_mlp {
     uint8_t *ym = nullptr;       // the batch's target rows [max_batch x n_out]
     float *acc = nullptr;        // split-K partial slices, acc_cap floats
     size_t acc_cap = 0;
-    float *w1s = nullptr;        // training: dW1 and db1 split-K slices (kMaxSplitSlices each)
-    float *b1s = nullptr;
+    // training: split-K slices of dW and db per layer (kMaxSplitSlices each),
+    // summed by Adam
+    float *ws[3] = {nullptr, nullptr, nullptr};
+    float *bs[3] = {nullptr, nullptr, nullptr};
     int64_t *dstate = nullptr;   // graph replay: [0] step within the call, [1] steps begun
     int64_t step = 0;
     std::vector<void *> allocs;
@@ -69,34 +71,48 @@ GemmArgs gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int Mr, in
     return g;
 }
 
-// The product G = (A B^T) * gelu'(Z) (BF16, + transposed) as one GEMM launch
-// argument block (*epi its epilogue).  A product whose output tiles would not
-// fill the GPU is split over K into FP32 partial slices of M->acc instead; the
-// return value is the number of slices, and when it exceeds 1 the caller
-// follows the product with launch_gelu_bwd.
-int gelu_bwd_plan(oka_mlp *M, const uint16_t *A, int lda, const uint16_t *B, int ldb, int Mr,
-                  int N, int K, const float *Z, uint16_t *G, uint16_t *Gt, GemmArgs *out, int *epi)
+// Split-K plan of one layer's backward launch: dW [n_out_l, n_in_l] over a K
+// of B (the batch) beside dX [B, n_in_l] over a K of n_out_l.  The two
+// products' K per workgroup are made equal first (the launch ends with its
+// slowest workgroup), then both are split further while the launch has fewer
+// than two workgroups per compute unit.  dW is split only when Adam takes its
+// slices (split_dw); dX split more than once leaves FP32 slices in M->acc for
+// launch_gelu_bwd, else the product's own epilogue applies gelu'.
+struct SplitPlan {
+    int dw = 1, dx = 1;
+};
+
+SplitPlan backward_plan(oka_mlp *M, int B, int nout_l, int nin_l, bool split_dw)
 {
-    const int tiles = (Mr / 64) * (N / 64);
-    int splits = 1;
-    while (splits < kMaxSplitSlices && tiles * splits < 2 * M->ctx->num_cu &&
-           K % (splits * 2 * 64) == 0 && K / (splits * 2) >= 128 &&
-           (size_t)Mr * N * splits * 2 <= M->acc_cap)
-        splits *= 2;
-    GemmArgs g = gemm(A, lda, B, ldb, Mr, N, K);
-    if (splits == 1) {
-        g.Zsaved = Z;
-        g.Ch = G;
-        g.Cht = Gt;
-        *epi = kEpiGeluBwd;
-    } else {
-        g.Cf = M->acc;
-        g.ldc = N;
-        g.splits = splits;
-        *epi = kEpiPartialF32;
+    SplitPlan p;
+    // K per LDS stage is 64; a slice keeps at least 128 of K
+    auto can = [](int K, int s, size_t slice, size_t cap) {
+        return 2 * s <= kMaxSplitSlices && K % (2 * s * 64) == 0 && K / (2 * s) >= 128 &&
+               slice * 2 * s <= cap;
+    };
+    const size_t wslice = (size_t)nout_l * nin_l, xslice = (size_t)B * nin_l;
+    const size_t wcap = (size_t)kMaxSplitSlices * wslice;
+    auto dw_ok = [&] { return split_dw && can(B, p.dw, wslice, wcap); };
+    auto dx_ok = [&] { return can(nout_l, p.dx, xslice, M->acc_cap); };
+    while (B / p.dw > nout_l / p.dx && dw_ok())
+        p.dw *= 2;
+    while (nout_l / p.dx > B / p.dw && dx_ok())
+        p.dx *= 2;
+    auto workgroups = [&] {
+        GemmArgs w{}, x{};
+        w.M = nout_l;
+        w.N = nin_l;
+        w.splits = p.dw;
+        x.M = B;
+        x.N = nin_l;
+        x.splits = p.dx;
+        return gemm2_workgroups(w, &x);
+    };
+    while (workgroups() < 2 * M->ctx->num_cu && dw_ok() && dx_ok()) {
+        p.dw *= 2;
+        p.dx *= 2;
     }
-    *out = g;
-    return splits;
+    return p;
 }
 
 // Forward through the two hidden layers for rows of x (rowidx/row0) into the
@@ -127,11 +143,9 @@ int hidden_forward(oka_mlp *M, int Bp, int valid, const float *x, const int32_t 
     return OKA_OK;
 }
 
-// Loss (when `loss` is set) and full gradient (into M->grad) for one batch of
-// B rows: 7 launches, each dW product sharing its launch with the layer's dX
-// product and forming its bias gradient.
-// dW1 (the first layer's weight gradient: 8 output tiles over a K of B) in
-// `w1_splits` split-K slices of M->w1s / M->b1s when above 1, for Adam to sum.
+// dW1 (the first layer's weight gradient: a few output tiles over a K of B)
+// in split-K slices of M->ws[0] / M->bs[0] when the result is above 1, for
+// Adam to sum.
 int w1_split_plan(int B)
 {
     int s = 1;
@@ -140,11 +154,30 @@ int w1_split_plan(int B)
     return s;
 }
 
-// Loss (when `loss` is set) and full gradient for one batch of B rows: into
-// M->grad, except dW1 and db1 when w1_splits > 1 (their slices, for Adam).
+// The split-K plans of one training step's backward launches (layer 3, 2
+// and 1's dW).
+struct StepPlan {
+    SplitPlan layer[2];
+    int w1 = 1;
+};
+
+StepPlan step_plan(oka_mlp *M, int B, bool split_dw)
+{
+    StepPlan p;
+    p.layer[0] = backward_plan(M, B, M->nout, M->nh, split_dw);
+    p.layer[1] = backward_plan(M, B, M->nh, M->nh, split_dw);
+    p.w1 = split_dw ? w1_split_plan(B) : 1;
+    return p;
+}
+
+// Loss (when `loss` is set) and full gradient for one batch of B rows, in 7
+// or more launches, each dW product sharing its launch with the layer's dX
+// product and forming its bias gradient.  Gradients go to M->grad, except the
+// dW and db that `plan` splits over K: those stay as slices in M->ws / M->bs
+// for Adam.
 int forward_backward(oka_mlp *M, int B, const float *x, const uint8_t *marks, float tscale,
-                     const int32_t *rowidx, int64_t row0, double *loss,
-                     int64_t *dstate = nullptr, int w1_splits = 1)
+                     const int32_t *rowidx, int64_t row0, double *loss, const StepPlan &plan,
+                     int64_t *dstate = nullptr)
 {
     oka_ctx *c = M->ctx;
     hipStream_t s = c->stream;
@@ -166,36 +199,50 @@ int forward_backward(oka_mlp *M, int B, const float *x, const uint8_t *marks, fl
         const uint16_t *g, *wt;      // G [B, n_out_l], W^T [n_in_l, n_out_l] (dX product)
         const float *z;              // pre-activation of the layer input
         uint16_t *gin, *gint;        // dX * gelu'(z): G of the layer below
+        float *ws, *bs;              // dW and db slices
     };
     const Layer layers[3] = {
-        {M->g3t, M->h2t, M->nout, M->nh, M->oW3, M->ob3, M->g3, M->w3t, M->z2, M->g2, M->g2t},
-        {M->g2t, M->h1t, M->nh, M->nh, M->oW2, M->ob2, M->g2, M->w2t, M->z1, M->g1, M->g1t},
+        {M->g3t, M->h2t, M->nout, M->nh, M->oW3, M->ob3, M->g3, M->w3t, M->z2, M->g2, M->g2t,
+         M->ws[2], M->bs[2]},
+        {M->g2t, M->h1t, M->nh, M->nh, M->oW2, M->ob2, M->g2, M->w2t, M->z1, M->g1, M->g1t,
+         M->ws[1], M->bs[1]},
         {M->g1t, M->x0t, M->nh, M->kin, M->oW1, M->ob1, nullptr, nullptr, nullptr, nullptr,
-         nullptr}};
-    for (const Layer &L : layers) {
+         nullptr, M->ws[0], M->bs[0]}};
+    for (int l = 0; l < 3; ++l) {
+        const Layer &L = layers[l];
+        const int dw_splits = l < 2 ? plan.layer[l].dw : plan.w1;
         // dW = G^T X and db = row sums of G^T
         GemmArgs dw = gemm(L.gt, B, L.xt, B, L.nout_l, L.nin_l, B);
         dw.Cf = M->grad + L.oW;
         dw.ldc = L.nin_l;
         dw.dbias = M->grad + L.ob;
+        int ew = kEpiStoreF32;
+        if (dw_splits > 1) {
+            dw.Cf = L.ws;
+            dw.dbias = L.bs;
+            dw.splits = dw_splits;
+            ew = kEpiPartialF32;
+        }
         if (!L.g) {
-            if (w1_splits > 1) {
-                dw.Cf = M->w1s;
-                dw.dbias = M->b1s;
-                dw.splits = w1_splits;
-                HIPCHK(c, launch_gemm(kEpiPartialF32, dw, s));
-            } else {
-                HIPCHK(c, launch_gemm(kEpiStoreF32, dw, s));
-            }
+            HIPCHK(c, launch_gemm(ew, dw, s));
             break;
         }
-        GemmArgs dx;
-        int epi;
-        const int splits = gelu_bwd_plan(M, L.g, L.nout_l, L.wt, L.nout_l, B, L.nin_l, L.nout_l,
-                                         L.z, L.gin, L.gint, &dx, &epi);
-        HIPCHK(c, launch_gemm2(kEpiStoreF32, dw, epi, dx, s));
-        if (splits > 1)
-            HIPCHK(c, launch_gelu_bwd(M->acc, splits, L.z, B, L.nin_l, L.gin, L.gint, s));
+        const int dx_splits = plan.layer[l].dx;
+        GemmArgs dx = gemm(L.g, L.nout_l, L.wt, L.nout_l, B, L.nin_l, L.nout_l);
+        int ex = kEpiGeluBwd;
+        if (dx_splits == 1) {
+            dx.Zsaved = L.z;
+            dx.Ch = L.gin;
+            dx.Cht = L.gint;
+        } else {
+            dx.Cf = M->acc;
+            dx.ldc = L.nin_l;
+            dx.splits = dx_splits;
+            ex = kEpiPartialF32;
+        }
+        HIPCHK(c, launch_gemm2(ew, dw, ex, dx, s));
+        if (dx_splits > 1)
+            HIPCHK(c, launch_gelu_bwd(M->acc, dx_splits, L.z, B, L.nin_l, L.gin, L.gint, s));
     }
     return OKA_OK;
 }
@@ -209,7 +256,7 @@ struct TrainStep {
     const int32_t *order;
     double *loss;
     const AdamArgs *adam;
-    int w1_splits;   // dW1 / db1 as split-K slices that Adam sums
+    StepPlan plan;   // the dW / db split-K slices that Adam sums
 };
 
 constexpr int kStepsPerGraph = 16;
@@ -224,8 +271,8 @@ int capture_steps(oka_mlp *M, const TrainStep &t, int steps, hipGraphExec_t *out
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = OKA_OK;
     for (int k = 0; k < steps && rc == OKA_OK; ++k) {
-        rc = forward_backward(M, t.batch, t.x, t.marks, t.tscale, t.order, 0, t.loss, M->dstate,
-                              t.w1_splits);
+        rc = forward_backward(M, t.batch, t.x, t.marks, t.tscale, t.order, 0, t.loss, t.plan,
+                              M->dstate);
         if (rc == OKA_OK && launch_adam(*t.adam, c->stream) != hipSuccess)
             rc = fail(c, OKA_ERR_DEVICE, "adam launch failed");
     }
@@ -340,8 +387,13 @@ int oka_mlp_create(oka_ctx *c, int32_t n_in, int32_t n_hidden, int32_t n_out, in
     rc = rc ? rc : alloc_t(M, B * M->nout, &M->ym);
     M->acc_cap = std::max(B * M->nh, (size_t)4 * c->num_cu * 64 * 64);
     rc = rc ? rc : alloc_t(M, M->acc_cap, &M->acc);
-    rc = rc ? rc : alloc_t(M, (size_t)kMaxSplitSlices * M->nh * M->kin, &M->w1s);
-    rc = rc ? rc : alloc_t(M, (size_t)kMaxSplitSlices * M->nh, &M->b1s);
+    {
+        const int rows[3] = {M->nh, M->nh, M->nout}, cols[3] = {M->kin, M->nh, M->nh};
+        for (int l = 0; l < 3; ++l) {
+            rc = rc ? rc : alloc_t(M, (size_t)kMaxSplitSlices * rows[l] * cols[l], &M->ws[l]);
+            rc = rc ? rc : alloc_t(M, (size_t)kMaxSplitSlices * rows[l], &M->bs[l]);
+        }
+    }
     rc = rc ? rc : alloc_t(M, 2, &M->dstate);
     if (rc == OKA_OK) {
         // zero state (padding columns of W1 must stay exactly zero)
@@ -417,8 +469,9 @@ int oka_mlp_grad(oka_mlp *M, int32_t batch, const float *x, const uint8_t *marks
         TRY(ensure(c, kSlotMlpLoss, sizeof(double), &dl));
         HIPCHK(c, launch_zero_f64((double *)dl, 1, c->stream));
     }
+    // every gradient into M->grad: no dW split
     TRY(forward_backward(M, batch, (const float *)dx, (const uint8_t *)dm, target_scale, nullptr, 0,
-                         (double *)dl));
+                         (double *)dl, step_plan(M, batch, false)));
     TRY(copy_params(M, grad, false, M->grad, kind_from_dev(flags)));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (loss) {
@@ -467,21 +520,21 @@ int oka_mlp_train(oka_mlp *M, int64_t nsamples, const float *x, const uint8_t *m
         a.eps = eps;
         a.tbase = M->step;
         a.dstate = M->dstate;
-        const int w1s = w1_split_plan(batch);
-        const float *sw = w1s > 1 ? M->w1s : nullptr, *sb = w1s > 1 ? M->b1s : nullptr;
-        const int ns = w1s > 1 ? w1s : 0;
-        const AdamSeg segs[6] = {
-            {M->oW1, M->nh, M->kin, M->w1h, M->w1t, sw, ns},
-            {M->ob1, M->nh / 32, 32, nullptr, nullptr, sb, ns},
-            {M->oW2, M->nh, M->nh, M->w2h, M->w2t, nullptr, 0},
-            {M->ob2, M->nh / 32, 32, nullptr, nullptr, nullptr, 0},
-            {M->oW3, M->nout, M->nh, M->w3h, M->w3t, nullptr, 0},
-            {M->ob3, M->nout / 32, 32, nullptr, nullptr, nullptr, 0}};
-        for (int i = 0; i < 6; ++i)
-            a.seg[i] = segs[i];
+        const StepPlan plan = step_plan(M, batch, true);
+        const int split[3] = {plan.w1, plan.layer[1].dw, plan.layer[0].dw};
+        const int64_t oW[3] = {M->oW1, M->oW2, M->oW3}, ob[3] = {M->ob1, M->ob2, M->ob3};
+        const int rows[3] = {M->nh, M->nh, M->nout}, cols[3] = {M->kin, M->nh, M->nh};
+        uint16_t *wh[3] = {M->w1h, M->w2h, M->w3h}, *wt[3] = {M->w1t, M->w2t, M->w3t};
+        for (int l = 0; l < 3; ++l) {
+            const bool sl = split[l] > 1;
+            a.seg[2 * l] = AdamSeg{oW[l], rows[l], cols[l], wh[l], wt[l], sl ? M->ws[l] : nullptr,
+                                   sl ? split[l] : 0};
+            a.seg[2 * l + 1] = AdamSeg{ob[l], rows[l] / 32, 32, nullptr, nullptr,
+                                       sl ? M->bs[l] : nullptr, sl ? split[l] : 0};
+        }
         a.nseg = 6;
         const TrainStep ts{batch, (const float *)dx, (const uint8_t *)dm, target_scale,
-                           (const int32_t *)dord, (double *)dl, &a, w1s};
+                           (const int32_t *)dord, (double *)dl, &a, plan};
         const int unroll = std::min<int32_t>(nsteps, kStepsPerGraph);
         const int full = nsteps / unroll, rest = nsteps % unroll;
         hipGraphExec_t ex_full = nullptr, ex_rest = nullptr;
