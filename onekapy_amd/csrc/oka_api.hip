@@ -270,15 +270,20 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
                                           pool_max_pool_tiles());
     sh.waves = c->num_cu * sh.wpc;
     sh.hbm_words = (int64_t)sh.waves * sh.slots * sh.max_tiles * 32;
-    // realization records: enough for every chunked realization in flight (a
-    // chunk waits while an older realization holds its record), at most 8 GiB;
-    // per record: owner (2 words), done and a pad word, the tiles, dirty bits
+    // realization records, at most 32 GiB; per record: owner (2 words), done
+    // and a pad word, the tiles, dirty bits.  Realization g uses record
+    // g % nrec, and a chunk waits while an older realization still holds it.
+    // A realization holds its record until its longest trace ends, which on
+    // sites with traces of very different lengths takes as long as the queue
+    // takes to advance by a few times the chunks in flight: with twice the
+    // slots' count of records, half the pool's iterations on a 200-well site
+    // found their intake waiting; with four times, a few in a thousand.
     sh.dirty_words = (sh.max_tiles + 31) / 32;
     const int64_t per_rec = (int64_t)sh.max_tiles * 32 + sh.dirty_words + 4;
     sh.nrec = 0;
     if (chunked > 0) {
-        sh.nrec = std::min<int64_t>(chunked, std::max<int64_t>(2LL * sh.waves * sh.slots, 64));
-        sh.nrec = std::max<int64_t>(1, std::min<int64_t>(sh.nrec, (2LL << 30) / per_rec));
+        sh.nrec = std::min<int64_t>(chunked, std::max<int64_t>(4LL * sh.waves * sh.slots, 64));
+        sh.nrec = std::max<int64_t>(1, std::min<int64_t>(sh.nrec, (8LL << 30) / per_rec));
     }
     sh.rec_words = sh.nrec * per_rec;
     return OKA_OK;
