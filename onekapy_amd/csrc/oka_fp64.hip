@@ -73,13 +73,15 @@ struct Chron<kModeVertices> {
 
 // One workgroup per realization (grid-stride), one trace per lane, lanes
 // refilled from the LDS counter s_next when their trace ends.
-// The confined tracer is held to 128 VGPRs (4 waves per SIMD; one 64-bit
-// value spills, stored before the realization loop and read after it), which
-// is 3 % faster than the 150 the compiler picks; the unconfined one would
-// spill 43 registers and keeps the compiler's choice.
+// Both tracers are held to 128 VGPRs (4 waves per SIMD).  The confined one
+// spills one 64-bit value, stored before the realization loop and read after
+// it, and is 3 % faster than with the 150 the compiler picks.  The unconfined
+// one spills 256 bytes per lane but runs 13 % faster than with the 219 VGPRs
+// (2 waves per SIMD) the compiler picks: its correctly rounded logarithms
+// wait on table loads, which more waves hide.
 template <bool kConfined, int kMode>
 __global__ void __launch_bounds__(kTraceBlock)
-__attribute__((amdgpu_waves_per_eu(kConfined ? 4 : 1))) k_trace_fp64(TraceArgs a)
+__attribute__((amdgpu_waves_per_eu(4))) k_trace_fp64(TraceArgs a)
 {
     constexpr bool kVerts = kMode == kModeVertices;
     extern __shared__ double smem[];
