@@ -82,8 +82,9 @@ def test_full_size_products_match_oracle(gpu_ctx):
 
 
 def test_training_step_gradient_matches_oracle(gpu_ctx):
-    """The captured training step computes dW1 and db1 in split-K slices that
-    the Adam kernel sums (the gradient call does not).  With eps far above
+    """The captured training step computes the weight and bias gradients in
+    split-K slices that the Adam kernel sums (at batch 1,024: dW3 in 2, dW2
+    in 4, dW1 in 8; the gradient call splits none).  With eps far above
     every gradient, one Adam step moves each parameter by -lr g / eps (m = g,
     v-hat = g^2 at step 1), so the step's own gradient can be read back and
     checked against the CPU oracle like the gradient call's."""
