@@ -708,6 +708,8 @@ def node_estimate(sums, n_rows, nb, mu_reps, ridge=1e-2, flat=1e-10, min_events=
     dq = np.zeros((nb.shape[0], len(PAIRS)))
     dv = dev.reshape(R, nr, nc)
     for h, (dr, dc) in enumerate(PAIRS):
+        if dr >= nr or abs(dc) >= nc:
+            continue                     # no pair of the window's cells this far apart
         c0, c1 = max(0, -dc), nc - max(0, dc)
         prod = dv[:, :nr - dr, c0:c1] * dv[:, dr:, c0 + dc:c1 + dc]
         dq.reshape(nr, nc, -1)[:nr - dr, c0:c1, h] = prod.sum(0)

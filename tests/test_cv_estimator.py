@@ -142,3 +142,47 @@ def test_rare_nodes_take_no_control():
     _, _, b30 = node_estimate(sums, n_rows, neighbours(WIN), reps, min_events=30)
     rare = np.minimum(y[:150].sum(0), 150 - y[:150].sum(0)) < 30   # events of part 0
     assert rare.any() and np.all(np.abs(b30[rare]).sum(-1) >= 0)
+
+
+def test_surrogate_mean_term_on_a_window_smaller_than_the_pair_offsets():
+    """The surrogate means' share of the variance, beta_bar' Cov(mu) beta_bar,
+    formed through the half-offset pair sums, equals numpy's covariance of the
+    replicate means over each cell's neighbours, on a 2 x 2-cell window whose
+    pair offsets reach beyond it (absent neighbours count as 0)."""
+    win = _Win(0, 0, 2, 2, 2)
+    ncell, nk = 4, 4
+    rng = np.random.default_rng(31)
+    n = 400
+    u = rng.random(n)
+    off = np.array([-0.1, 0.0, 0.05, 0.1])
+    s = np.clip(u[:, None] + off[None, :] + rng.normal(0, 0.05, (n, ncell)), 0.0, 1.0)
+    t = np.linspace(0.2, 0.8, ncell * nk).reshape(ncell, nk)
+    y = (u[:, None, None] < t[None, :, :]).astype(np.float64)
+    nb = neighbours(win)
+    pair = np.full((ncell, len(PAIRS)), -1)
+    for a in range(ncell):
+        r, c = divmod(a, 2)
+        for h, (dr, dc) in enumerate(PAIRS):
+            if r + dr < 2 and 0 <= c + dc < 2:
+                pair[a, h] = (r + dr) * 2 + c + dc
+    sp = np.concatenate([s, np.zeros((n, 1))], axis=1)
+    half = n // 2
+    sums = {k: [] for k in ("s", "ss", "y", "ys")}
+    for sl in (slice(0, half), slice(half, n)):
+        sums["s"].append(s[sl].sum(0))
+        sums["ss"].append((s[sl][:, :, None] * sp[sl][:, pair]).sum(0))
+        sums["y"].append(y[sl].sum(0))
+        sums["ys"].append((y[sl][:, :, :, None] * sp[sl][:, nb][:, :, None, :]).sum(0))
+    sums = {k: np.array(v) for k, v in sums.items()}
+    reps = rng.normal(0.5, 0.02, (12, ncell))
+    flat = np.repeat(reps.mean(0, keepdims=True), 12, axis=0)
+    p1, v1, b1 = node_estimate(sums, [half, n - half], nb, reps, min_events=0)
+    p0, v0, b0 = node_estimate(sums, [half, n - half], nb, flat, min_events=0)
+    np.testing.assert_array_equal(b1, b0)
+    mu_nb = np.where(nb[None] < 0, 0.0, reps[:, np.where(nb < 0, 0, nb)])   # [R, cells, 9]
+    want = np.zeros((ncell, nk))
+    for a in range(ncell):
+        cov = np.cov(mu_nb[:, a, :], rowvar=False) / reps.shape[0]
+        want[a] = np.einsum("km,ml,kl->k", b1[a], cov, b1[a])
+    np.testing.assert_allclose(v1 - v0, want, rtol=1e-9, atol=1e-15)
+    assert want.max() > 0
