@@ -158,6 +158,10 @@ template <bool kConfined, bool kPlain>
 __device__ __forceinline__ void well_sums(const Field64 &f, double x, double y, double &Qx,
                                           double &Qy, double &pw)
 {
+    // two wells per iteration in the confined field: one LDS address and
+    // loop step serve both (2 % on the 25-well site; by 4 gains nothing, and
+    // the unconfined loop, with its logarithms, is slower unrolled)
+#pragma unroll (kConfined ? 2 : 1)
     for (int w = 0; w < f.nw; ++w) {
         const Well64 wl = f.w[w];
         const double ddx = x - wl.x;
