@@ -81,9 +81,12 @@ def parse():
     ap.add_argument("--no-cv", action="store_true", help="skip the control-variate leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the configuration-2 and configuration-4 legs of the default run")
-    ap.add_argument("--extra-knobs", default="2,16,64",
-                    help="pool launch shape SLOTS,WAVES_PER_CU,CHUNK of the embedded "
-                         "configuration-2 and -4 legs")
+    ap.add_argument("--cfg2-knobs", default="2,16,128",
+                    help="pool shape (slots,waves per CU,particles per chunk) of the default run's "
+                         "cfg2 leg (the fastest of a sweep on the 64-site batch)")
+    ap.add_argument("--cfg4-knobs", default="2,16,64",
+                    help="pool shape of the default run's cfg4 leg (the fastest of a sweep on "
+                         "its 100,000-realization subset)")
     ap.add_argument("--unconfined", action="store_true",
                     help="trace the unconfined velocity (compute_velocity) in the fp64/fp32 modes; "
                          "the roofline keeps the confined flop count")
@@ -749,10 +752,11 @@ def main():
 
     extra = {}
     if not args.no_extra and args.config == "cfg1":
-        knobs = dict(zip(("pool_slots", "pool_waves_per_cu", "pool_chunk"),
-                         (int(v) for v in args.extra_knobs.split(","))))
-        extra["config2"] = cfg2_leg(knobs, args)
-        extra["config4"] = cfg4_leg(knobs, args, torch)
+        def knobs(text):
+            return dict(zip(("pool_slots", "pool_waves_per_cu", "pool_chunk"),
+                            (int(v) for v in text.split(","))))
+        extra["config2"] = cfg2_leg(knobs(args.cfg2_knobs), args)
+        extra["config4"] = cfg4_leg(knobs(args.cfg4_knobs), args, torch)
 
     def mode_summary(name):
         m = modes[name]
