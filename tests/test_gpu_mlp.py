@@ -81,14 +81,19 @@ def test_full_size_products_match_oracle(gpu_ctx):
         assert a @ b / (np.linalg.norm(a) * np.linalg.norm(b)) > 0.9995, name
 
 
-def test_training_step_gradient_matches_oracle(gpu_ctx):
+@pytest.mark.parametrize("nh,nout,B", [(512, 4096, 1024), (128, 256, 192)],
+                         ids=["cfg3_network_b1024", "small_b192"])
+def test_training_step_gradient_matches_oracle(gpu_ctx, nh, nout, B):
     """The captured training step computes the weight and bias gradients in
-    split-K slices that the Adam kernel sums (at batch 1,024: dW3 in 2, dW2
-    in 4, dW1 in 8; the gradient call splits none).  With eps far above
-    every gradient, one Adam step moves each parameter by -lr g / eps (m = g,
-    v-hat = g^2 at step 1), so the step's own gradient can be read back and
-    checked against the CPU oracle like the gradient call's."""
-    nin, nh, nout, B = 34, 512, 4096, 1024
+    split-K slices that the Adam kernel sums (at batch 1,024 on the 512-wide
+    network: dW3 in 2, dW2 in 4, dW1 in 8; at batch 192, a multiple of 64 but
+    not of 128, no dW splits and only layer 3's dX does, in 2, on 64 x 64
+    tiles; the gradient call splits no dW).
+    With eps far above every gradient, one Adam step moves each parameter by
+    -lr g / eps (m = g, v-hat = g^2 at step 1), so the step's own gradient
+    can be read back and checked against the CPU oracle like the gradient
+    call's."""
+    nin = 34
     net = SurrogateMLP(nin, nh, nout, max_batch=B, seed=21)
     rng = np.random.default_rng(22)
     x = rng.normal(0, 1, (B, nin)).astype(np.float32)
