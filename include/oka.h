@@ -157,13 +157,16 @@ typedef struct {
 
 /* How particles are scheduled on the GPU.  WORKGROUP: one 256-lane workgroup
  * per realization, its lanes refilled with the realization's particles.
- * POOL (FP32, confined, no window marks): a persistent grid of single waves
+ * POOL (no window marks): a persistent grid of single waves
  * working through a queue shared by all waves of chunks of pool_chunk
  * particles; each wave keeps pool_slots chunks in flight and refills a lane
  * with a particle of any of them, so uneven particle lifetimes and uneven
  * realizations (even of different sites) balance across the GPU.  The chunks
  * of one realization meet in an HBM record flushed by the last of them.
- * AUTO is WORKGROUP.  Both give the same counts. */
+ * AUTO is POOL where the far-well expansions apply (FP32, confined, a plan
+ * taken for far_order, no window marks, bitmap_mode and tile_slots left
+ * automatic, and a pool that fits), else WORKGROUP.  Both schedules give the
+ * same counts when every well is summed directly. */
 enum { OKA_SCHED_AUTO = 0, OKA_SCHED_WORKGROUP = 1, OKA_SCHED_POOL = 2 };
 
 /* Where a realization's visited-node bitmap lives.  AUTO picks LDS_FLAT when

@@ -1011,12 +1011,29 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
 
     const bool timed = stats != nullptr;
     const bool fp64 = o->precision == OKA_FP64_EXACT;
-    if (o->schedule == OKA_SCHED_POOL) {
+    // AUTO runs the pool where it has what the workgroup schedule lacks: the
+    // far-well expansions of an FP32 confined site of many wells (3.7x on a
+    // 200-well site).  Elsewhere the workgroup schedule is as fast or faster,
+    // with the same counts, and a forced bitmap placement or tile-slot count
+    // asks for it.
+    bool pool = o->schedule == OKA_SCHED_POOL;
+    if (o->schedule == OKA_SCHED_AUTO && !marks && o->bitmap_mode == OKA_BITMAP_AUTO &&
+        o->tile_slots == 0 && far_requested(a, o->precision, o->far_order) > 0) {
+        const int tx = (a.gncols + 31) / 32;
+        const int64_t nt = (int64_t)tx * ((a.gnrows + 31) / 32);
+        const size_t share = std::min(c->lds_per_cu / 4, c->max_lds);
+        if (nt <= pool_max_tiles() && pool_lds_bytes(o->precision, 2, a.nwells, 8) <= share) {
+            TRY(stage_far(c, site, o, flags, a));
+            pool = a.far_order > 0;   // the plan was taken
+        }
+    }
+    if (pool) {
         if (marks)
             return fail(c, OKA_ERR_ARG, "window marks need the workgroup schedule");
         a.tiles_x = (a.gncols + 31) / 32;
         a.ntiles = a.tiles_x * ((a.gnrows + 31) / 32);
-        TRY(stage_far(c, site, o, flags, a));
+        if (o->schedule == OKA_SCHED_POOL)
+            TRY(stage_far(c, site, o, flags, a));
         TRY(run_pool(c, &a, 1, o, timed));
         return chronicle_finish(c, a, rasterize, nnodes, ntr, counts, bbox, trace_status,
                                 trace_steps, stats, flags, win, marks, masks);

@@ -77,10 +77,11 @@ def chronicle_into_field(pfield, wells_xy, xo, yo, real, starts, duration, tol, 
     node (the caller registers them).  Nothing is registered here.
     bitmap_mode / tile_slots place the per-realization visited-node bitmap
     and schedule / pool_slots / pool_waves_per_cu / pool_chunk choose how particles are
-    scheduled (oka_trace_opts); every choice gives the same counts.  far_order
-    (FP32 pool, confined) sums distant wells as local expansions: 0 lets the
-    library choose (on from 64 wells), < 0 sums every well directly, as the
-    workgroup schedule does.
+    scheduled (oka_trace_opts); every choice gives the same counts when every
+    well is summed directly.  far_order (FP32 pool, confined) sums distant
+    wells as local expansions: 0 lets the library choose (on from 64 wells,
+    and then the automatic schedule is the pool), < 0 sums every well
+    directly, as the workgroup schedule does.
     """
     wells_xy = _lib.f64(wells_xy).reshape(-1, 2)
     starts = _lib.f64(starts).reshape(-1, 2)

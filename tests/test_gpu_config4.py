@@ -81,7 +81,7 @@ def test_config4_fp32_pool_agrees_with_fp64_exact(gpu_ctx, cfg4_oracle):
     s, d, coef, ocounts, ost = cfg4_oracle
     f64 = _run(s, d, coef, precision=_lib.FP64_EXACT)
     pool = _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL, far_order=-1)
-    wg = _run(s, d, coef, precision=_lib.FP32)
+    wg = _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_WORKGROUP)
     assert np.array_equal(pool.counts, wg.counts)
     assert np.array_equal(pool.trace_status, wg.trace_status)
     mism = int((pool.trace_status != f64.trace_status).sum())
@@ -122,6 +122,28 @@ def test_config4_far_expansions_agree_with_fp64_exact(gpu_ctx, cfg4_oracle, orde
     print(f"bbox deviation from the direct sum: median {np.median(dev):.3g} m, "
           f"95th percentile {np.percentile(dev, 95):.3g} m", flush=True)
     assert np.median(dev) < 1.0
+
+
+def test_config4_auto_schedule(gpu_ctx, cfg4_oracle):
+    """OKA_SCHED_AUTO on the 200-well site: FP32 takes the pool with the
+    far-well expansions (the same counts, statuses and steps as asking for
+    them); with far_order = -1, or a forced bitmap placement, and in
+    FP64-exact it takes the workgroup schedule (the same counts as asking for
+    it, and FP64-exact equal to the oracle)."""
+    s, d, coef, ocounts, ost = cfg4_oracle
+    auto32 = _run(s, d, coef, precision=_lib.FP32)
+    far = _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL)
+    assert np.array_equal(auto32.counts, far.counts)
+    assert np.array_equal(auto32.trace_status, far.trace_status)
+    assert auto32.stats.attempted == far.stats.attempted
+    wg32 = _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_WORKGROUP)
+    assert not np.array_equal(auto32.counts, wg32.counts)
+    direct = _run(s, d, coef, precision=_lib.FP32, far_order=-1)
+    assert np.array_equal(direct.counts, wg32.counts)
+    tiled = _run(s, d, coef, precision=_lib.FP32, bitmap_mode=_lib.BITMAP_TILED)
+    assert np.array_equal(tiled.counts, wg32.counts)
+    auto64 = _run(s, d, coef, precision=_lib.FP64_EXACT)
+    assert np.array_equal(auto64.counts, ocounts)
 
 
 def test_config4_far_order_above_limit_is_rejected(gpu_ctx, cfg4_oracle):

@@ -244,7 +244,7 @@ def test_config4_shape_properties(gpu_ctx):
                                     s.tol, s.maxstep, s.umbra, **kw)
 
     print("fp32 workgroup", flush=True)
-    wg = run(slice(None), precision=_lib.FP32)
+    wg = run(slice(None), precision=_lib.FP32, schedule=_lib.SCHED_WORKGROUP)
     print("fp32 pool", flush=True)
     pool = run(slice(None), precision=_lib.FP32, schedule=_lib.SCHED_POOL, far_order=-1)
     assert np.array_equal(pool.counts, wg.counts)
@@ -302,7 +302,7 @@ def _pool_vs_workgroup(s, d, coef, pf, umbra, **pool):
     real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
     starts = start_points(s.xt, s.yt, s.rt, s.npaths)
     args = (pf, s.wells_xy(), s.xt, s.yt, real, starts, s.duration, s.tol, s.maxstep, umbra)
-    wg = chronicle_into_field(*args, precision=_lib.FP32)
+    wg = chronicle_into_field(*args, precision=_lib.FP32, schedule=_lib.SCHED_WORKGROUP)
     pl = chronicle_into_field(*args, precision=_lib.FP32, schedule=_lib.SCHED_POOL, **pool)
     assert np.array_equal(pl.counts, wg.counts)
     assert np.array_equal(pl.trace_status, wg.trace_status)
