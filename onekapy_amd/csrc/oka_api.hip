@@ -1019,10 +1019,14 @@ int chronicle_impl(oka_ctx *c, const oka_site *site, int32_t nreal, const double
     bool pool = o->schedule == OKA_SCHED_POOL;
     if (o->schedule == OKA_SCHED_AUTO && !marks && o->bitmap_mode == OKA_BITMAP_AUTO &&
         o->tile_slots == 0 && far_requested(a, o->precision, o->far_order) > 0) {
+        // the pool must fit: its tile index, and the LDS share of the fewest
+        // waves per CU it would choose (or the caller's shape) for the slots
         const int tx = (a.gncols + 31) / 32;
         const int64_t nt = (int64_t)tx * ((a.gnrows + 31) / 32);
-        const size_t share = std::min(c->lds_per_cu / 4, c->max_lds);
-        if (nt <= pool_max_tiles() && pool_lds_bytes(o->precision, 2, a.nwells, 8) <= share) {
+        const int slots = o->pool_slots > 0 ? std::min(o->pool_slots, 16) : 2;
+        const int wpc = o->pool_waves_per_cu > 0 ? std::min(o->pool_waves_per_cu, 32) : 4;
+        const size_t share = std::min(c->lds_per_cu / (size_t)wpc, c->max_lds);
+        if (nt <= pool_max_tiles() && pool_lds_bytes(o->precision, slots, a.nwells, 8) <= share) {
             TRY(stage_far(c, site, o, flags, a));
             pool = a.far_order > 0;   // the plan was taken
         }
