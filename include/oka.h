@@ -165,8 +165,9 @@ typedef struct {
  * of one realization meet in an HBM record flushed by the last of them.
  * AUTO is POOL where the far-well expansions apply (FP32, confined, a plan
  * taken for far_order, no window marks, bitmap_mode and tile_slots left
- * automatic, and a pool that fits), else WORKGROUP.  Both schedules give the
- * same counts when every well is summed directly. */
+ * automatic, and a pool that fits: its LDS holds the site's wells and 8
+ * tiles per slot at pool_waves_per_cu, or 4 when that is 0), else WORKGROUP.
+ * Both schedules give the same counts when every well is summed directly. */
 enum { OKA_SCHED_AUTO = 0, OKA_SCHED_WORKGROUP = 1, OKA_SCHED_POOL = 2 };
 
 /* Where a realization's visited-node bitmap lives.  AUTO picks LDS_FLAT when
