@@ -1,12 +1,15 @@
 """GPU path against vectors written by the reference itself
-(tests/golden/make_reference_vectors.py)."""
+(tests/golden/make_reference_vectors.py, make_reference_sites.py)."""
+import hashlib
 import os
 
 import numpy as np
 import pytest
 
-from onekapy_amd import (Model, ProbabilityField, compute_backtrace, compute_capturezone,
-                         create_deterministic_capturezone)
+from onekapy_amd import (Model, ProbabilityField, Realizations, _lib, chronicle_into_field,
+                         compute_backtrace, compute_capturezone, create_deterministic_capturezone,
+                         start_points)
+from onekapy_amd.capturezone import trace_vertices
 from helpers import GOLDEN
 
 pytestmark = pytest.mark.gpu
@@ -66,3 +69,83 @@ def test_deterministic_capturezone_equals_reference(gpu_ctx, ref):
     assert (pf.xmin, pf.xmax, pf.ymin, pf.ymax) == tuple(g[:4])
     assert (pf.nrows, pf.ncols) == (int(g[4]), int(g[5]))
     assert np.array_equal(pf.pgrid, ref["det_pgrid"])
+
+
+# Vectors the reference wrote on the benchmark sites
+# (tests/golden/make_reference_sites.py).
+
+@pytest.fixture(scope="module")
+def sites_ref():
+    return np.load(os.path.join(GOLDEN, "reference_sites.npz"))
+
+
+def site_params(ref, cfg):
+    base, xt, yt, rt, npaths, duration, spacing, umbra, tol, maxstep = ref[cfg + "_params"]
+    return dict(base=base, xt=xt, yt=yt, rt=rt, npaths=int(npaths), duration=duration,
+                spacing=spacing, umbra=umbra, tol=tol, maxstep=maxstep)
+
+
+def vertex_hash(v):
+    return hashlib.sha256(np.ascontiguousarray(v, dtype=np.float64).tobytes()).hexdigest()
+
+
+def assert_site_field(pf, ref, cfg):
+    g = ref[cfg + "_geom"]
+    assert (pf.xmin, pf.xmax, pf.ymin, pf.ymax) == tuple(g[:4])
+    assert (pf.nrows, pf.ncols) == (int(g[4]), int(g[5])) and pf.total_weight == g[6]
+    pg = pf.pgrid.ravel()
+    nz = np.flatnonzero(pg)
+    assert np.array_equal(nz, ref[cfg + "_nodes"])
+    assert np.array_equal(pg[nz], ref[cfg + "_values"])
+
+
+def assert_site_traces(wells, aquifer, coef, p, starts, ref, cfg):
+    real = Realizations(aquifer[0], aquifer[1], aquifer[2], wells[:, 3], coef)
+    verts, nverts, status = trace_vertices(wells[:, :2], p["xt"], p["yt"], real, starts,
+                                           p["duration"], p["tol"], p["maxstep"])
+    assert np.all(status == 0)
+    assert np.array_equal(nverts, ref[cfg + "_trace_len"])
+    for k in range(len(starts)):
+        v = verts[k, :nverts[k]]
+        assert np.array_equal(v[-1], ref[cfg + "_trace_last"][k])
+        assert vertex_hash(v) == ref[cfg + "_trace_sha256"][k]
+
+
+def test_config1_two_realizations_equal_reference(gpu_ctx, sites_ref):
+    """2,000 traces of the 25-well site through the reference's own API
+    (compute_capturezone twice into one field, weights 1 and 0.5), and the
+    vertex hashes of every 25th trace of the first realization."""
+    ref, p = sites_ref, site_params(sites_ref, "cfg1")
+    pf = ProbabilityField(p["spacing"], p["spacing"], p["xt"], p["yt"])
+    for r, weight in enumerate(ref["cfg1_weights"]):
+        K, n, b = ref["cfg1_aquifer"][r]
+        m = Model(p["base"], K, n, b, [tuple(w) for w in ref["cfg1_wells"][r]], p["xt"], p["yt"],
+                  ref["cfg1_coef"][r])
+        compute_capturezone(p["xt"], p["yt"], p["rt"], p["npaths"], p["duration"], pf,
+                            p["umbra"], weight, p["tol"], p["maxstep"], m.velocity_function(True))
+    assert_site_field(pf, ref, "cfg1")
+    starts = start_points(p["xt"], p["yt"], p["rt"], p["npaths"])[ref["cfg1_trace_index"]]
+    assert_site_traces(ref["cfg1_wells"][0], ref["cfg1_aquifer"][0], ref["cfg1_coef"][0], p,
+                       starts, ref, "cfg1")
+
+
+@pytest.mark.parametrize("schedule", [_lib.SCHED_WORKGROUP, _lib.SCHED_POOL],
+                         ids=["workgroup", "pool"])
+def test_config4_traces_equal_reference(gpu_ctx, sites_ref, schedule):
+    """32 traces of the 200-well site (injection wells included): the FP64-exact
+    trace-and-rasterize kernel on either schedule registers the reference's
+    field, and the vertex sequences hash to the reference's."""
+    ref, p = sites_ref, site_params(sites_ref, "cfg4")
+    wells = ref["cfg4_wells"]
+    starts = start_points(p["xt"], p["yt"], p["rt"], p["npaths"])[ref["cfg4_trace_index"]]
+    K, n, b = ref["cfg4_aquifer"]
+    real = Realizations([K], [n], [b], wells[None, :, 3], ref["cfg4_coef"][None])
+    pf = ProbabilityField(p["spacing"], p["spacing"], p["xt"], p["yt"])
+    res = chronicle_into_field(pf, wells[:, :2], p["xt"], p["yt"], real, starts, p["duration"],
+                               p["tol"], p["maxstep"], p["umbra"], True, _lib.FP64_EXACT,
+                               schedule=schedule)
+    assert np.all(res.trace_status == 0)
+    pf.rgrid |= res.counts > 0
+    pf.register(1.0)
+    assert_site_field(pf, ref, "cfg4")
+    assert_site_traces(wells, ref["cfg4_aquifer"], ref["cfg4_coef"], p, starts, ref, "cfg4")
