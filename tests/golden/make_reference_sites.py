@@ -21,6 +21,10 @@ Writes tests/golden/reference_sites.npz (inputs and outputs only):
            trace the oracle finishes within 4,000 attempted steps (the
            reference is pure Python: a long trace on this site takes minutes).  Recorded: inputs, the particles, their vertex counts,
            last vertices and hashes, and the 32 traces' field.
+  cfg2s2_*, cfg2s50_*
+           the same for sites 2 (135 wells, 29.1 years) and 50 (18 wells,
+           22.8 years) of configuration 2 (onekapy_amd.sites.config2), with
+           the draws of seed 100 + site.
 
 np.float is aliased to float (numpy 2.x removed it; probabilityfield.py uses
 it).  The oracle only selects the cfg4 particles; every recorded output comes
@@ -118,10 +122,9 @@ def cfg1(out):
     out["cfg1_geom"], out["cfg1_nodes"], out["cfg1_values"] = field_record(pf)
 
 
-def cfg4(out):
+def subset(out, key, s, d):
+    """32 short traces of realization 0 of site s, and their field."""
     import oracle as O
-    s = sites.config4(1)
-    d = sites.draw(s, seed=0)
     m, w, coef = realization_model(s, d, 0)
     om = O.OracleModel(s.base, d["K"][0], d["n"][0], d["b"][0], w, s.xt, s.yt, coef)
     starts = start_points(s.xt, s.yt, s.rt, s.npaths)
@@ -135,28 +138,31 @@ def cfg4(out):
             break
     pick = np.array(pick, dtype=np.int64)
     t = time.perf_counter()
-    traces, (out["cfg4_trace_len"], out["cfg4_trace_last"],
-             out["cfg4_trace_sha256"]) = trace_records(m, s, starts[pick])
-    print(f"cfg4: {len(pick)} traces in {time.perf_counter() - t:.1f} s", flush=True)
+    traces, (out[key + "_trace_len"], out[key + "_trace_last"],
+             out[key + "_trace_sha256"]) = trace_records(m, s, starts[pick])
+    print(f"{key}: {len(pick)} traces in {time.perf_counter() - t:.1f} s", flush=True)
     # the same traces into a field as compute_capturezone registers its
     # traces (capturezone.py:117-123): rasterize each in turn, then register
     pf = ProbabilityField(s.spacing, s.spacing, s.xt, s.yt)
     for vertices in traces:
         pf.rasterize([v[0] for v in vertices], [v[1] for v in vertices], s.umbra)
     pf.register(1.0)
-    out["cfg4_params"] = np.array([s.base, s.xt, s.yt, s.rt, s.npaths, s.duration, s.spacing,
+    out[key + "_params"] = np.array([s.base, s.xt, s.yt, s.rt, s.npaths, s.duration, s.spacing,
                                    s.umbra, s.tol, s.maxstep])
-    out["cfg4_wells"] = w
-    out["cfg4_aquifer"] = np.array([d["K"][0], d["n"][0], d["b"][0]])
-    out["cfg4_coef"] = coef
-    out["cfg4_trace_index"] = pick
-    out["cfg4_geom"], out["cfg4_nodes"], out["cfg4_values"] = field_record(pf)
+    out[key + "_wells"] = w
+    out[key + "_aquifer"] = np.array([d["K"][0], d["n"][0], d["b"][0]])
+    out[key + "_coef"] = coef
+    out[key + "_trace_index"] = pick
+    out[key + "_geom"], out[key + "_nodes"], out[key + "_values"] = field_record(pf)
 
 
 def main():
     out = {}
     cfg1(out)
-    cfg4(out)
+    subset(out, "cfg4", sites.config4(1), sites.draw(sites.config4(1), seed=0))
+    c2 = sites.config2()
+    for i in (2, 50):
+        subset(out, f"cfg2s{i}", c2[i], sites.draw(c2[i], seed=100 + i))
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, {k: v.shape for k, v in out.items()})
 

@@ -131,15 +131,17 @@ def test_config1_two_realizations_equal_reference(gpu_ctx, sites_ref):
 
 @pytest.mark.parametrize("schedule", [_lib.SCHED_WORKGROUP, _lib.SCHED_POOL],
                          ids=["workgroup", "pool"])
-def test_config4_traces_equal_reference(gpu_ctx, sites_ref, schedule):
-    """32 traces of the 200-well site (injection wells included): the FP64-exact
+@pytest.mark.parametrize("cfg", ["cfg4", "cfg2s2", "cfg2s50"])
+def test_site_traces_equal_reference(gpu_ctx, sites_ref, cfg, schedule):
+    """32 traces on each of the 200-well site (injection wells included) and
+    two sites of the uneven 64-site workload (135 and 18 wells): the FP64-exact
     trace-and-rasterize kernel on either schedule registers the reference's
     field, and the vertex sequences hash to the reference's."""
-    ref, p = sites_ref, site_params(sites_ref, "cfg4")
-    wells = ref["cfg4_wells"]
-    starts = start_points(p["xt"], p["yt"], p["rt"], p["npaths"])[ref["cfg4_trace_index"]]
-    K, n, b = ref["cfg4_aquifer"]
-    real = Realizations([K], [n], [b], wells[None, :, 3], ref["cfg4_coef"][None])
+    ref, p = sites_ref, site_params(sites_ref, cfg)
+    wells = ref[cfg + "_wells"]
+    starts = start_points(p["xt"], p["yt"], p["rt"], p["npaths"])[ref[cfg + "_trace_index"]]
+    K, n, b = ref[cfg + "_aquifer"]
+    real = Realizations([K], [n], [b], wells[None, :, 3], ref[cfg + "_coef"][None])
     pf = ProbabilityField(p["spacing"], p["spacing"], p["xt"], p["yt"])
     res = chronicle_into_field(pf, wells[:, :2], p["xt"], p["yt"], real, starts, p["duration"],
                                p["tol"], p["maxstep"], p["umbra"], True, _lib.FP64_EXACT,
@@ -147,5 +149,5 @@ def test_config4_traces_equal_reference(gpu_ctx, sites_ref, schedule):
     assert np.all(res.trace_status == 0)
     pf.rgrid |= res.counts > 0
     pf.register(1.0)
-    assert_site_field(pf, ref, "cfg4")
-    assert_site_traces(wells, ref["cfg4_aquifer"], ref["cfg4_coef"], p, starts, ref, "cfg4")
+    assert_site_field(pf, ref, cfg)
+    assert_site_traces(wells, ref[cfg + "_aquifer"], ref[cfg + "_coef"], p, starts, ref, cfg)

@@ -1,8 +1,9 @@
 """The oracle against vectors the reference wrote on the benchmark sites
 (tests/golden/make_reference_sites.py): 2,000 traces of the 25-well site of
-configuration 1 registered into one field, and 32 traces of the 200-well site
-of configuration 4 (injection wells included), with vertex hashes of every
-25th configuration-1 trace and of every configuration-4 trace.  Both of the
+configuration 1 registered into one field, and fields of 32 traces each on
+the 200-well site of configuration 4 (injection wells included) and on two
+sites of configuration 2 (135 and 18 wells), with vertex hashes of every 25th
+configuration-1 trace and of each of the 32-trace subsets.  Both of the
 oracle's math modes must reproduce them bit for bit: glibc's (what the
 reference calls) and the correctly rounded one (the GPU's target)."""
 import hashlib
@@ -70,17 +71,19 @@ def test_config1_two_realizations_equal_reference(oracle_lib, ref, math):
     assert_traces(model(ref, "cfg1", 0), p, starts[ref["cfg1_trace_index"]], ref, "cfg1", math)
 
 
+@pytest.mark.parametrize("cfg", ["cfg4", "cfg2s2", "cfg2s50"])
 @pytest.mark.parametrize("math", [O.MATH_GLIBC, O.MATH_CR], ids=["glibc", "cr"])
-def test_config4_traces_equal_reference(oracle_lib, ref, math):
-    p = params(ref, "cfg4")
-    starts = start_points(p["xt"], p["yt"], p["rt"], p["npaths"])[ref["cfg4_trace_index"]]
-    m = model(ref, "cfg4")
-    assert (ref["cfg4_wells"][:, 3] < 0).sum() > 0   # injection wells on the site
+def test_site_traces_equal_reference(oracle_lib, ref, math, cfg):
+    p = params(ref, cfg)
+    starts = start_points(p["xt"], p["yt"], p["rt"], p["npaths"])[ref[cfg + "_trace_index"]]
+    m = model(ref, cfg)
+    if cfg == "cfg4":
+        assert (ref["cfg4_wells"][:, 3] < 0).sum() > 0   # injection wells on the site
     f = O.OracleField(p["spacing"], p["spacing"], p["xt"], p["yt"])
     f.capturezone(m, starts, p["umbra"], 1.0, O.trace_opts(p["duration"], p["tol"], p["maxstep"],
                                                            True, math))
-    assert_field(f, ref, "cfg4")
-    assert_traces(m, p, starts, ref, "cfg4", math)
+    assert_field(f, ref, cfg)
+    assert_traces(m, p, starts, ref, cfg, math)
 
 
 def test_vertex_hash_is_sensitive(oracle_lib, ref):
