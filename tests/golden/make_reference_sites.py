@@ -21,6 +21,9 @@ Writes tests/golden/reference_sites.npz (inputs and outputs only):
            trace the oracle finishes within 4,000 attempted steps (the
            reference is pure Python: a long trace on this site takes minutes).  Recorded: inputs, the particles, their vertex counts,
            last vertices and hashes, and the 32 traces' field.
+  cfg4long_*
+           four long traces (2,000 to 14,000 attempted steps) on the same site:
+           realization 0 of the draws of 64 realizations (seed 0).
   cfg2s2_*, cfg2s50_*
            the same for sites 2 (135 wells, 29.1 years) and 50 (18 wells,
            22.8 years) of configuration 2 (onekapy_amd.sites.config2), with
@@ -122,19 +125,22 @@ def cfg1(out):
     out["cfg1_geom"], out["cfg1_nodes"], out["cfg1_values"] = field_record(pf)
 
 
-def subset(out, key, s, d):
-    """32 short traces of realization 0 of site s, and their field."""
+def subset(out, key, s, d, count=32, steps=(0, 4000), stride=8):
+    """`count` traces of realization 0 of site s, and their field: the first
+    particles, in the order 0, stride, 2 stride, ..., 1, 1 + stride, ..., whose
+    oracle trace takes more than steps[0] and at most steps[1] attempted
+    steps."""
     import oracle as O
     m, w, coef = realization_model(s, d, 0)
     om = O.OracleModel(s.base, d["K"][0], d["n"][0], d["b"][0], w, s.xt, s.yt, coef)
     starts = start_points(s.xt, s.yt, s.rt, s.npaths)
     pick = []
-    for p in np.arange(s.npaths).reshape(-1, 8).T.ravel():
+    for p in np.arange(s.npaths).reshape(-1, stride).T.ravel():
         xs, ys = starts[p]
         _, res = om.backtrace(xs, ys, s.duration, s.tol, s.maxstep, math=O.MATH_GLIBC)
-        if res.attempted <= 4000:
+        if steps[0] < res.attempted <= steps[1]:
             pick.append(p)
-        if len(pick) == 32:
+        if len(pick) == count:
             break
     pick = np.array(pick, dtype=np.int64)
     t = time.perf_counter()
@@ -160,6 +166,8 @@ def main():
     out = {}
     cfg1(out)
     subset(out, "cfg4", sites.config4(1), sites.draw(sites.config4(1), seed=0))
+    c4 = sites.config4(64)
+    subset(out, "cfg4long", c4, sites.draw(c4, seed=0), count=4, steps=(2000, 14000), stride=16)
     c2 = sites.config2()
     for i in (2, 50):
         subset(out, f"cfg2s{i}", c2[i], sites.draw(c2[i], seed=100 + i))

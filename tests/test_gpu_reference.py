@@ -131,10 +131,11 @@ def test_config1_two_realizations_equal_reference(gpu_ctx, sites_ref):
 
 @pytest.mark.parametrize("schedule", [_lib.SCHED_WORKGROUP, _lib.SCHED_POOL],
                          ids=["workgroup", "pool"])
-@pytest.mark.parametrize("cfg", ["cfg4", "cfg2s2", "cfg2s50"])
+@pytest.mark.parametrize("cfg", ["cfg4", "cfg4long", "cfg2s2", "cfg2s50"])
 def test_site_traces_equal_reference(gpu_ctx, sites_ref, cfg, schedule):
     """32 traces on each of the 200-well site (injection wells included) and
-    two sites of the uneven 64-site workload (135 and 18 wells): the FP64-exact
+    two sites of the uneven 64-site workload (135 and 18 wells), and four long
+    traces (2,701 to 9,077 vertices) on the 200-well site: the FP64-exact
     trace-and-rasterize kernel on either schedule registers the reference's
     field, and the vertex sequences hash to the reference's."""
     ref, p = sites_ref, site_params(sites_ref, cfg)

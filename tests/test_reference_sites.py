@@ -2,8 +2,9 @@
 (tests/golden/make_reference_sites.py): 2,000 traces of the 25-well site of
 configuration 1 registered into one field, and fields of 32 traces each on
 the 200-well site of configuration 4 (injection wells included) and on two
-sites of configuration 2 (135 and 18 wells), with vertex hashes of every 25th
-configuration-1 trace and of each of the 32-trace subsets.  Both of the
+sites of configuration 2 (135 and 18 wells), a field of four long traces
+(2,701 to 9,077 vertices) on the 200-well site, and vertex hashes of every
+25th configuration-1 trace and of every trace of the other fields.  Both of the
 oracle's math modes must reproduce them bit for bit: glibc's (what the
 reference calls) and the correctly rounded one (the GPU's target)."""
 import hashlib
@@ -71,14 +72,14 @@ def test_config1_two_realizations_equal_reference(oracle_lib, ref, math):
     assert_traces(model(ref, "cfg1", 0), p, starts[ref["cfg1_trace_index"]], ref, "cfg1", math)
 
 
-@pytest.mark.parametrize("cfg", ["cfg4", "cfg2s2", "cfg2s50"])
+@pytest.mark.parametrize("cfg", ["cfg4", "cfg4long", "cfg2s2", "cfg2s50"])
 @pytest.mark.parametrize("math", [O.MATH_GLIBC, O.MATH_CR], ids=["glibc", "cr"])
 def test_site_traces_equal_reference(oracle_lib, ref, math, cfg):
     p = params(ref, cfg)
     starts = start_points(p["xt"], p["yt"], p["rt"], p["npaths"])[ref[cfg + "_trace_index"]]
     m = model(ref, cfg)
-    if cfg == "cfg4":
-        assert (ref["cfg4_wells"][:, 3] < 0).sum() > 0   # injection wells on the site
+    if cfg.startswith("cfg4"):
+        assert (ref[cfg + "_wells"][:, 3] < 0).sum() > 0   # injection wells on the site
     f = O.OracleField(p["spacing"], p["spacing"], p["xt"], p["yt"])
     f.capturezone(m, starts, p["umbra"], 1.0, O.trace_opts(p["duration"], p["tol"], p["maxstep"],
                                                            True, math))
