@@ -217,6 +217,7 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
     sh.first.assign(2 * ((size_t)nsites + 1), 0);
     int64_t total = 0, nreal = 0, chunked = 0;
     int nconf = 0;
+    bool far_share = false;   // a site whose realizations share far-well tables
     sh.max_nw = 1;
     sh.max_tiles = 1;
     for (int i = 0; i < nsites; ++i) {
@@ -231,9 +232,11 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
         if (sites[i].rasterize)
             sh.max_tiles = std::max(sh.max_tiles, (int)sites[i].ntiles);
         nconf += sites[i].confined ? 1 : 0;
-        if (sites[i].far_order > 0)
+        if (sites[i].far_order > 0) {
             sh.far_stride = std::max<int64_t>(
                 sh.far_stride, (int64_t)sites[i].far_nbx * sites[i].far_nby * kFarTerms);
+            far_share = far_share || nch >= kFarShareMinChunks;
+        }
     }
     sh.conf_mode = nconf == nsites ? 0 : (nconf == 0 ? 1 : 2);
     sh.first[nsites] = total;
@@ -271,7 +274,8 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
     sh.waves = c->num_cu * sh.wpc;
     sh.hbm_words = (int64_t)sh.waves * sh.slots * sh.max_tiles * 32;
     // realization records, at most 32 GiB; per record: owner (2 words), done
-    // and a pad word, the tiles, dirty bits.  Realization g uses record
+    // and a pad word, the shared far-well table's state (2 words), the tiles,
+    // dirty bits.  Realization g uses record
     // g % nrec, and a chunk waits while an older realization still holds it.
     // A realization holds its record until its longest trace ends, which on
     // sites with traces of very different lengths takes as long as the queue
@@ -279,13 +283,18 @@ int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_o
     // slots' count of records, half the pool's iterations on a 200-well site
     // found their intake waiting; with four times, a few in a thousand.
     sh.dirty_words = (sh.max_tiles + 31) / 32;
-    const int64_t per_rec = (int64_t)sh.max_tiles * 32 + sh.dirty_words + 4;
+    const int64_t per_rec = (int64_t)sh.max_tiles * 32 + sh.dirty_words + 6;
     sh.nrec = 0;
     if (chunked > 0) {
         sh.nrec = std::min<int64_t>(chunked, std::max<int64_t>(4LL * sh.waves * sh.slots, 64));
         sh.nrec = std::max<int64_t>(1, std::min<int64_t>(sh.nrec, (8LL << 30) / per_rec));
     }
     sh.rec_words = sh.nrec * per_rec;
+    // the chunks of a realization share one far-well table in its record
+    // (kFarShareMinChunks) when those tables take at most 4 GiB; else each
+    // chunk computes its own
+    const int64_t rec_table_bytes = sh.nrec * sh.far_stride * (int64_t)sizeof(float2);
+    sh.far_rec = far_share && rec_table_bytes <= (4LL << 30) ? sh.nrec : 0;
     return OKA_OK;
 }
 
@@ -405,6 +414,8 @@ PoolArgs pool_args(const PoolShape &sh, void *sites, void *first, void *queue, v
     PoolArgs p{};
     p.far_beta = (float2 *)far_beta;
     p.far_stride = sh.far_stride;
+    if (sh.far_rec > 0)
+        p.far_beta_rec = p.far_beta + (int64_t)sh.waves * sh.slots * sh.far_stride;
     p.sites = (const TraceArgs *)sites;
     p.nsites = sh.nsites;
     p.site_first = (const int64_t *)first;
@@ -422,7 +433,8 @@ PoolArgs pool_args(const PoolShape &sh, void *sites, void *first, void *queue, v
     if (sh.nrec > 0) {
         p.rec_owner = (int64_t *)rec;
         p.rec_done = (int32_t *)(p.rec_owner + sh.nrec);
-        p.rec_bits = (uint32_t *)(p.rec_done + 2 * sh.nrec);
+        p.rec_table = (unsigned long long *)(p.rec_done + 2 * sh.nrec);
+        p.rec_bits = (uint32_t *)(p.rec_table + sh.nrec);
         p.rec_dirty = p.rec_bits + sh.nrec * (int64_t)sh.max_tiles * 32;
     }
     return p;
@@ -872,7 +884,7 @@ int run_pool(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opt
     TRY(pool_shape(c, sites, nsites, o, o->precision, &sh));
     void *p_sites, *p_first, *p_queue, *p_hbm, *p_rec = nullptr, *p_beta = nullptr;
     if (sh.far_stride > 0)
-        TRY(ensure(c, kSlotFarBeta, sizeof(float2) * sh.waves * sh.slots * sh.far_stride,
+        TRY(ensure(c, kSlotFarBeta, sizeof(float2) * pool_beta_count(sh) * sh.far_stride,
                    &p_beta));
     TRY(ensure(c, kSlotPoolSites, sizeof(TraceArgs) * nsites, &p_sites));
     TRY(ensure(c, kSlotPoolFirst, sizeof(int64_t) * sh.first.size(), &p_first));

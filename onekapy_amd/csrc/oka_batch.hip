@@ -167,7 +167,7 @@ int capture(oka_batch *B)
         TRY(balloc(B, (size_t)sh.rec_words * 4, &B->d_rec));
         B->rec_cap = (size_t)sh.rec_words * 4;
     }
-    const size_t beta_bytes = sizeof(float2) * sh.waves * sh.slots * sh.far_stride;
+    const size_t beta_bytes = sizeof(float2) * pool_beta_count(sh) * sh.far_stride;
     if (beta_bytes > 0 && (!B->d_beta || B->beta_cap < beta_bytes)) {
         TRY(balloc(B, beta_bytes, &B->d_beta));
         B->beta_cap = beta_bytes;

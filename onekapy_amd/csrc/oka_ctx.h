@@ -85,13 +85,19 @@ struct PoolShape {
     int max_nw = 1, max_tiles = 1, dirty_words = 1;
     int64_t total = 0, nrec = 0, hbm_words = 0, rec_words = 0;
     int64_t far_stride = 0;       // float2 of far-well coefficients per slot
+    int64_t far_rec = 0;          // records holding a far-well table their chunks share
     std::vector<int64_t> first;   // items, then realizations, per site: 2 x (nsites + 1)
 };
+// Far-well tables of a launch: one per slot, then one per sharing record.
+inline int64_t pool_beta_count(const PoolShape &sh)
+{
+    return sh.far_stride > 0 ? (int64_t)sh.waves * sh.slots + sh.far_rec : 0;
+}
 int pool_shape(oka_ctx *c, const TraceArgs *sites, int nsites, const oka_trace_opts *o,
                int precision, PoolShape *out);
 // Device buffers: sites [nsites] TraceArgs, first [2 (nsites + 1)] int64
 // (uploaded by the caller), queue (8 bytes), hbm [hbm_words], rec [rec_words],
-// far_beta [waves * slots * far_stride] float2 (when far_stride > 0).
+// far_beta [pool_beta_count * far_stride] float2 (when far_stride > 0).
 PoolArgs pool_args(const PoolShape &sh, void *sites, void *first, void *queue, void *hbm,
                    void *rec, void *far_beta);
 // Clears the launch's queue, bitmaps and records and launches it: kernel

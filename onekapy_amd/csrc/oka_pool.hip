@@ -108,6 +108,11 @@ struct PoolFP32 {
         F.beta = P.far_beta + ((int64_t)blockIdx.x * P.slots + s) * P.far_stride;   // 128-byte rows
     }
 
+    // The far-well coefficients of the slot's realization: a table that the
+    // chunks of a realization may share.
+    __device__ static bool table_shared(const Consts &k) { return k.far.order > 0; }
+    __device__ static void table_to(Consts &k, float2 *beta) { k.far.beta = beta; }
+
     // The far-well coefficients of the slot's realization, one box per lane.
     __device__ static bool stage_slot(const Consts &k, const float4 *w, int nw)
     {

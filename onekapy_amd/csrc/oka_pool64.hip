@@ -89,6 +89,8 @@ struct PoolFP64 {
     }
 
     __device__ static bool stage_slot(const Consts &, const Well64 *, int) { return false; }
+    __device__ static bool table_shared(const Consts &) { return false; }
+    __device__ static void table_to(Consts &, float2 *) {}
 
     __device__ static Field64 field(const Consts &k, const Well64 *w, int nw)
     {

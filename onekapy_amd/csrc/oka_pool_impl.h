@@ -29,6 +29,9 @@
 //   stage_consts(...)   lane 0 fills the slot's constants;
 //   stage_slot(...)     the lanes' work on the staged slot (after a barrier),
 //                       such as per-realization tables that the lanes then read;
+//   table_shared(k)     whether that table may be shared by the chunks of a
+//                       realization (computed once, in its record);
+//   table_to(k, beta)   lane 0 points the slot's constants at the table beta;
 //   open(...)           a lane takes particle p of the slot's realization;
 //   advance<kConf>(...) one iteration of the lane's trace (one field
 //                       evaluation for a fresh trace, an attempted step
@@ -67,7 +70,8 @@ struct __align__(16) SlotHdr {   // a multiple of 16 bytes: the well records fol
     unsigned long long att, acc, esc, cand, vis, st[4];
     int64_t r;          // realization within its site
     int64_t g;          // realization id over all sites
-    int state;          // 0 free, 1 open, 2 waiting for its realization record
+    int state;          // 0 free, 1 open, 2 waiting for its realization record,
+                        // 3 waiting for its record's shared table
     int site;
     int next, np, rem;  // next particle to hand out, end of the chunk, not yet ended
     int nchunks, rec;   // chunks of the realization; its record when nchunks > 1
@@ -254,10 +258,77 @@ __device__ __forceinline__ bool claim_record(const PoolArgs &P, const SlotHdr<Pr
     return __shfl(ok, 0) != 0;
 }
 
+__device__ __forceinline__ bool shares_table(const PoolArgs &P, int nchunks)
+{
+    return nchunks >= kFarShareMinChunks && P.far_beta_rec != nullptr;
+}
+
+// A slot whose realization holds its record, when the realization's chunks
+// share one table (shares_table): the first chunk here computes it
+// into the record and publishes it; the others use it once it is published
+// (state 3 until then).  The record's table word names the realization
+// whose table the record holds, ((g + 1) << 1) | published.  While g holds
+// the record only g's chunks write that word (the previous holder's chunks
+// have all ended), so it goes from another realization's value to g's and
+// then to g's published value.  The builder computes the table without
+// waiting on anything, so a waiting chunk's wait ends.  Returns the slot's
+// state, 1 or 3.  Wave-uniform; ends with a barrier.
+template <class Prec>
+__device__ int open_table(const PoolArgs &P, const PoolLds<Prec> &L, int s)
+{
+    SlotHdr<Prec> &h = L.hdr[s];
+    if (!shares_table(P, h.nchunks) || !Prec::table_shared(h.k))
+        return 1;
+    const unsigned long long key = (unsigned long long)(h.g + 1) << 1;
+    int code = 0;   // 0: build it, 1: published, 3: being built by another chunk
+    if (threadIdx.x == 0) {
+        Prec::table_to(h.k, P.far_beta_rec + (int64_t)h.rec * P.far_stride);
+        unsigned long long cur = __hip_atomic_load(&P.rec_table[h.rec], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+        if (cur != key && cur != (key | 1ull)) {   // the previous holder's table
+            const unsigned long long prev = atomicCAS(&P.rec_table[h.rec], cur, key);
+            // another chunk of g got there first; any other value cannot occur,
+            // and computing the table again would write the same words
+            cur = prev == cur ? 0ull : prev;
+        }
+        code = cur == (key | 1ull) ? 1 : (cur == key ? 3 : 0);
+    }
+    code = __shfl(code, 0);
+    __syncthreads();   // the lanes see the slot's table pointer
+    if (code == 0) {
+        const typename Prec::Well *w = L.wells + (size_t)s * P.max_nwells;
+        Prec::stage_slot(h.k, w, h.nw);
+        __threadfence();   // each lane's coefficients, device-wide, before the flag
+        __syncthreads();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&P.rec_table[h.rec], key | 1ull, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        return 1;
+    }
+    if (code == 1)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the table after its flag
+    return code;
+}
+
+// Whether the shared table that slot s waits for (state 3) is published.
+// Wave-uniform.
+template <class Prec>
+__device__ bool table_published(const PoolArgs &P, const SlotHdr<Prec> &h)
+{
+    int ok = 0;
+    if (threadIdx.x == 0)
+        ok = __hip_atomic_load(&P.rec_table[h.rec], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+             (((unsigned long long)(h.g + 1) << 1) | 1ull);
+    ok = __shfl(ok, 0);
+    if (ok)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // every lane reads after the flag
+    return ok != 0;
+}
+
 // Opens slot s for queue item `item`: wells, constants, header.  A chunk of a
 // realization of several chunks also claims the realization's record (state
-// 2 while an older realization still holds it).  Wave-uniform; ends with a
-// barrier.
+// 2 while an older realization still holds it) and then its shared table
+// (open_table).  Wave-uniform; ends with a barrier.
 template <class Prec>
 __device__ void open_slot(const PoolArgs &P, const PoolLds<Prec> &L, int s, int64_t item)
 {
@@ -291,11 +362,13 @@ __device__ void open_slot(const PoolArgs &P, const PoolLds<Prec> &L, int s, int6
         h.confined = A.confined != 0;
     }
     __syncthreads();
-    if (Prec::stage_slot(L.hdr[s].k, w, A.nwells))
+    const bool chunked = L.hdr[s].nchunks > 1;
+    const bool shared = shares_table(P, L.hdr[s].nchunks) && Prec::table_shared(L.hdr[s].k);
+    if (!shared && Prec::stage_slot(L.hdr[s].k, w, A.nwells))
         __syncthreads();
-    const bool ready = L.hdr[s].nchunks <= 1 || claim_record(P, L.hdr[s]);
+    const int state = !chunked ? 1 : (claim_record(P, L.hdr[s]) ? open_table(P, L, s) : 2);
     if (threadIdx.x == 0)
-        L.hdr[s].state = ready ? 1 : 2;
+        L.hdr[s].state = state;
     __syncthreads();
 }
 
@@ -490,12 +563,17 @@ k_trace_pool(PoolArgs P)
         bool ended = false;
         unsigned long long need = __ballot(!live);
         while (need != 0ull) {
-            if (intake >= 0 && L.hdr[intake].state == 2) {   // its record is still busy
-                const bool ok = claim_record(P, L.hdr[intake]);
-                if (threadIdx.x == 0 && ok)
-                    L.hdr[intake].state = 1;
+            if (intake >= 0 && L.hdr[intake].state >= 2) {   // its record or table is busy
+                int st = L.hdr[intake].state;
+                if (st == 2)
+                    st = claim_record(P, L.hdr[intake]) ? open_table(P, L, intake) : 2;
+                else if (table_published(P, L.hdr[intake]))
+                    st = 1;
+                __syncthreads();   // every lane has read the state before lane 0 writes it
+                if (threadIdx.x == 0)
+                    L.hdr[intake].state = st;
                 __syncthreads();
-                if (!ok)
+                if (st != 1)
                     break;
             }
             if (intake >= 0 && L.hdr[intake].next < L.hdr[intake].np) {

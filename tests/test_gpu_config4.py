@@ -150,3 +150,25 @@ def test_config4_far_order_above_limit_is_rejected(gpu_ctx, cfg4_oracle):
     s, d, coef, _, _ = cfg4_oracle
     with pytest.raises(_lib.OkaError, match="far_order"):
         _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL, far_order=17)
+
+
+def test_config4_shared_far_tables_match_per_chunk_tables(gpu_ctx, cfg4_oracle):
+    """The chunks of a realization share one far-well table, computed by the
+    first of them into the realization's record (chunks of 64 and 32
+    particles: 4 and 8 chunks per realization); a realization of one chunk
+    (256 particles) computes its own.  The tables hold the same coefficients,
+    so every trace ends the same: equal counts, statuses, steps and bounding
+    boxes."""
+    s, d, coef, _, _ = cfg4_oracle
+    runs = {c: _run(s, d, coef, precision=_lib.FP32, schedule=_lib.SCHED_POOL, pool_chunk=c)
+            for c in (256, 64, 32)}
+    one = runs[256]
+    for c in (64, 32):
+        r = runs[c]
+        print(f"chunk {c}: attempted {r.stats.attempted} against {one.stats.attempted}",
+              flush=True)
+        assert np.array_equal(r.counts, one.counts)
+        assert np.array_equal(r.trace_status, one.trace_status)
+        assert np.array_equal(r.bbox, one.bbox)
+        assert (r.stats.attempted, r.stats.accepted, r.stats.visited) == (
+            one.stats.attempted, one.stats.accepted, one.stats.visited)
