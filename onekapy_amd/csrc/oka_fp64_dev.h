@@ -161,7 +161,8 @@ __device__ __forceinline__ void well_sums(const Field64 &f, double x, double y, 
     // two wells per iteration in the confined field: one LDS address and
     // loop step serve both (2 % on the 25-well site; by 4 gains nothing, and
     // the unconfined loop, with its logarithms, is slower unrolled)
-#pragma unroll (kConfined ? 2 : 1)
+    constexpr int kUnroll = kConfined ? 2 : 1;
+#pragma unroll kUnroll
     for (int w = 0; w < f.nw; ++w) {
         const Well64 wl = f.w[w];
         const double ddx = x - wl.x;
