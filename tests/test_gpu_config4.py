@@ -172,3 +172,32 @@ def test_config4_shared_far_tables_match_per_chunk_tables(gpu_ctx, cfg4_oracle):
         assert np.array_equal(r.bbox, one.bbox)
         assert (r.stats.attempted, r.stats.accepted, r.stats.visited) == (
             one.stats.attempted, one.stats.accepted, one.stats.visited)
+
+
+def test_config4_shared_far_tables_across_record_reuse(gpu_ctx):
+    """Shared far-well tables when records are reused: 8,000 realizations of
+    64 particles in chunks of 16 (4 chunks, one shared table each) on 1 wave
+    per compute unit, whose 2,048 records each serve about four realizations in
+    turn, against chunks of 64 (one chunk, a table per slot).  A table read
+    before its realization's coefficients reached the reader would change
+    traces; every count, status, step and bounding box is equal."""
+    s = sites.config4(8000)
+    d = sites.draw(s, seed=5)
+    _, _, coef = fit_and_draw(s.wells_xy(), s.xt, s.yt, s.base, s.observations, d["K"], d["b"],
+                              d["q"], d["z"], moments=False)
+    real = Realizations(d["K"], d["n"], d["b"], d["q"], coef)
+    starts = start_points(s.xt, s.yt, s.rt, 64)
+    out = {}
+    for chunk in (16, 64):
+        out[chunk] = chronicle_into_field(s.fixed_field(), s.wells_xy(), s.xt, s.yt, real, starts,
+                                          s.duration, s.tol, s.maxstep, s.umbra,
+                                          precision=_lib.FP32, max_steps=CAP,
+                                          schedule=_lib.SCHED_POOL, pool_slots=2,
+                                          pool_waves_per_cu=1, pool_chunk=chunk)
+        print(f"chunk {chunk}: attempted {out[chunk].stats.attempted}", flush=True)
+    a, b = out[16], out[64]
+    assert np.array_equal(a.counts, b.counts)
+    assert np.array_equal(a.trace_status, b.trace_status)
+    assert np.array_equal(a.bbox, b.bbox)
+    assert (a.stats.attempted, a.stats.accepted, a.stats.visited) == (
+        b.stats.attempted, b.stats.accepted, b.stats.visited)
