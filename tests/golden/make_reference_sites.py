@@ -24,6 +24,9 @@ Writes tests/golden/reference_sites.npz (inputs and outputs only):
   cfg4long_*
            four long traces (2,000 to 14,000 attempted steps) on the same site:
            realization 0 of the draws of 64 realizations (seed 0).
+  cfg1u_*  the same for the unconfined velocity (compute_velocity) on the
+           25-well site: realization 0 of the draws of 1 realization (seed 0),
+           32 particles in the order 0, 25, 50, ...
   cfg2s2_*, cfg2s50_*
            the same for sites 2 (135 wells, 29.1 years) and 50 (18 wells,
            22.8 years) of configuration 2 (onekapy_amd.sites.config2), with
@@ -71,9 +74,11 @@ def realization_model(site, d, r):
     return m, np.array(wells), coef
 
 
-def feval_of(m):
+def feval_of(m, confined=True):
+    velocity = m.compute_velocity_confined if confined else m.compute_velocity
+
     def fe(xy):
-        vx, vy = m.compute_velocity_confined(xy[0], xy[1])
+        vx, vy = velocity(xy[0], xy[1])
         return np.array([-vx, -vy])
     return fe
 
@@ -84,11 +89,12 @@ def field_record(pf):
     return geom, flat, pf.pgrid.ravel()[flat]
 
 
-def trace_records(m, site, starts):
+def trace_records(m, site, starts, confined=True):
     """The reference's vertex sequences from `starts`, and their records."""
     traces, lens, last, hashes = [], [], [], []
     for xs, ys in starts:
-        vertices = compute_backtrace(xs, ys, site.duration, site.tol, site.maxstep, feval_of(m))
+        vertices = compute_backtrace(xs, ys, site.duration, site.tol, site.maxstep,
+                                     feval_of(m, confined))
         v = np.array(vertices, dtype=np.float64)
         traces.append(vertices)
         lens.append(len(v))
@@ -125,11 +131,11 @@ def cfg1(out):
     out["cfg1_geom"], out["cfg1_nodes"], out["cfg1_values"] = field_record(pf)
 
 
-def subset(out, key, s, d, count=32, steps=(0, 4000), stride=8):
+def subset(out, key, s, d, count=32, steps=(0, 4000), stride=8, confined=True):
     """`count` traces of realization 0 of site s, and their field: the first
     particles, in the order 0, stride, 2 stride, ..., 1, 1 + stride, ..., whose
     oracle trace takes more than steps[0] and at most steps[1] attempted
-    steps."""
+    steps.  confined=False traces the unconfined velocity (compute_velocity)."""
     import oracle as O
     m, w, coef = realization_model(s, d, 0)
     om = O.OracleModel(s.base, d["K"][0], d["n"][0], d["b"][0], w, s.xt, s.yt, coef)
@@ -137,7 +143,8 @@ def subset(out, key, s, d, count=32, steps=(0, 4000), stride=8):
     pick = []
     for p in np.arange(s.npaths).reshape(-1, stride).T.ravel():
         xs, ys = starts[p]
-        _, res = om.backtrace(xs, ys, s.duration, s.tol, s.maxstep, math=O.MATH_GLIBC)
+        _, res = om.backtrace(xs, ys, s.duration, s.tol, s.maxstep, confined=confined,
+                              math=O.MATH_GLIBC)
         if steps[0] < res.attempted <= steps[1]:
             pick.append(p)
         if len(pick) == count:
@@ -145,7 +152,7 @@ def subset(out, key, s, d, count=32, steps=(0, 4000), stride=8):
     pick = np.array(pick, dtype=np.int64)
     t = time.perf_counter()
     traces, (out[key + "_trace_len"], out[key + "_trace_last"],
-             out[key + "_trace_sha256"]) = trace_records(m, s, starts[pick])
+             out[key + "_trace_sha256"]) = trace_records(m, s, starts[pick], confined)
     print(f"{key}: {len(pick)} traces in {time.perf_counter() - t:.1f} s", flush=True)
     # the same traces into a field as compute_capturezone registers its
     # traces (capturezone.py:117-123): rasterize each in turn, then register
@@ -159,6 +166,7 @@ def subset(out, key, s, d, count=32, steps=(0, 4000), stride=8):
     out[key + "_aquifer"] = np.array([d["K"][0], d["n"][0], d["b"][0]])
     out[key + "_coef"] = coef
     out[key + "_trace_index"] = pick
+    out[key + "_confined"] = np.array([int(confined)])
     out[key + "_geom"], out[key + "_nodes"], out[key + "_values"] = field_record(pf)
 
 
@@ -168,6 +176,8 @@ def main():
     subset(out, "cfg4", sites.config4(1), sites.draw(sites.config4(1), seed=0))
     c4 = sites.config4(64)
     subset(out, "cfg4long", c4, sites.draw(c4, seed=0), count=4, steps=(2000, 14000), stride=16)
+    c1 = sites.config1(1)
+    subset(out, "cfg1u", c1, sites.draw(c1, seed=0), stride=25, confined=False)
     c2 = sites.config2()
     for i in (2, 50):
         subset(out, f"cfg2s{i}", c2[i], sites.draw(c2[i], seed=100 + i))

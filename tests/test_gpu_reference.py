@@ -99,10 +99,10 @@ def assert_site_field(pf, ref, cfg):
     assert np.array_equal(pg[nz], ref[cfg + "_values"])
 
 
-def assert_site_traces(wells, aquifer, coef, p, starts, ref, cfg):
+def assert_site_traces(wells, aquifer, coef, p, starts, ref, cfg, confined=True):
     real = Realizations(aquifer[0], aquifer[1], aquifer[2], wells[:, 3], coef)
     verts, nverts, status = trace_vertices(wells[:, :2], p["xt"], p["yt"], real, starts,
-                                           p["duration"], p["tol"], p["maxstep"])
+                                           p["duration"], p["tol"], p["maxstep"], confined)
     assert np.all(status == 0)
     assert np.array_equal(nverts, ref[cfg + "_trace_len"])
     for k in range(len(starts)):
@@ -131,24 +131,27 @@ def test_config1_two_realizations_equal_reference(gpu_ctx, sites_ref):
 
 @pytest.mark.parametrize("schedule", [_lib.SCHED_WORKGROUP, _lib.SCHED_POOL],
                          ids=["workgroup", "pool"])
-@pytest.mark.parametrize("cfg", ["cfg4", "cfg4long", "cfg2s2", "cfg2s50"])
+@pytest.mark.parametrize("cfg", ["cfg4", "cfg4long", "cfg2s2", "cfg2s50", "cfg1u"])
 def test_site_traces_equal_reference(gpu_ctx, sites_ref, cfg, schedule):
     """32 traces on each of the 200-well site (injection wells included) and
-    two sites of the uneven 64-site workload (135 and 18 wells), and four long
-    traces (2,701 to 9,077 vertices) on the 200-well site: the FP64-exact
+    two sites of the uneven 64-site workload (135 and 18 wells), 32 traces of
+    the unconfined velocity on the 25-well site, and four long traces (2,701 to
+    9,077 vertices) on the 200-well site: the FP64-exact
     trace-and-rasterize kernel on either schedule registers the reference's
     field, and the vertex sequences hash to the reference's."""
     ref, p = sites_ref, site_params(sites_ref, cfg)
+    confined = bool(ref[cfg + "_confined"][0])
     wells = ref[cfg + "_wells"]
     starts = start_points(p["xt"], p["yt"], p["rt"], p["npaths"])[ref[cfg + "_trace_index"]]
     K, n, b = ref[cfg + "_aquifer"]
     real = Realizations([K], [n], [b], wells[None, :, 3], ref[cfg + "_coef"][None])
     pf = ProbabilityField(p["spacing"], p["spacing"], p["xt"], p["yt"])
     res = chronicle_into_field(pf, wells[:, :2], p["xt"], p["yt"], real, starts, p["duration"],
-                               p["tol"], p["maxstep"], p["umbra"], True, _lib.FP64_EXACT,
+                               p["tol"], p["maxstep"], p["umbra"], confined, _lib.FP64_EXACT,
                                schedule=schedule)
     assert np.all(res.trace_status == 0)
     pf.rgrid |= res.counts > 0
     pf.register(1.0)
     assert_site_field(pf, ref, cfg)
-    assert_site_traces(wells, ref[cfg + "_aquifer"], ref[cfg + "_coef"], p, starts, ref, cfg)
+    assert_site_traces(wells, ref[cfg + "_aquifer"], ref[cfg + "_coef"], p, starts, ref, cfg,
+                       confined)

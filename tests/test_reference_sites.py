@@ -2,9 +2,10 @@
 (tests/golden/make_reference_sites.py): 2,000 traces of the 25-well site of
 configuration 1 registered into one field, and fields of 32 traces each on
 the 200-well site of configuration 4 (injection wells included) and on two
-sites of configuration 2 (135 and 18 wells), a field of four long traces
-(2,701 to 9,077 vertices) on the 200-well site, and vertex hashes of every
-25th configuration-1 trace and of every trace of the other fields.  Both of the
+sites of configuration 2 (135 and 18 wells) and on the 25-well site with the
+unconfined velocity, a field of four long traces (2,701 to 9,077 vertices) on
+the 200-well site, and vertex hashes of every 25th configuration-1 trace and
+of every trace of the other fields.  Both of the
 oracle's math modes must reproduce them bit for bit: glibc's (what the
 reference calls) and the correctly rounded one (the GPU's target)."""
 import hashlib
@@ -51,9 +52,10 @@ def assert_field(f, ref, cfg):
     assert np.array_equal(pg[nz], ref[cfg + "_values"])
 
 
-def assert_traces(m, p, starts, ref, cfg, math):
+def assert_traces(m, p, starts, ref, cfg, math, confined=True):
     for k, (xs, ys) in enumerate(starts):
-        v, res = m.backtrace(xs, ys, p["duration"], p["tol"], p["maxstep"], math=math)
+        v, res = m.backtrace(xs, ys, p["duration"], p["tol"], p["maxstep"], confined=confined,
+                             math=math)
         assert res.status == 0
         assert len(v) == ref[cfg + "_trace_len"][k]
         assert np.array_equal(v[-1], ref[cfg + "_trace_last"][k])
@@ -72,19 +74,20 @@ def test_config1_two_realizations_equal_reference(oracle_lib, ref, math):
     assert_traces(model(ref, "cfg1", 0), p, starts[ref["cfg1_trace_index"]], ref, "cfg1", math)
 
 
-@pytest.mark.parametrize("cfg", ["cfg4", "cfg4long", "cfg2s2", "cfg2s50"])
+@pytest.mark.parametrize("cfg", ["cfg4", "cfg4long", "cfg2s2", "cfg2s50", "cfg1u"])
 @pytest.mark.parametrize("math", [O.MATH_GLIBC, O.MATH_CR], ids=["glibc", "cr"])
 def test_site_traces_equal_reference(oracle_lib, ref, math, cfg):
     p = params(ref, cfg)
+    confined = bool(ref[cfg + "_confined"][0])
     starts = start_points(p["xt"], p["yt"], p["rt"], p["npaths"])[ref[cfg + "_trace_index"]]
     m = model(ref, cfg)
     if cfg.startswith("cfg4"):
         assert (ref[cfg + "_wells"][:, 3] < 0).sum() > 0   # injection wells on the site
     f = O.OracleField(p["spacing"], p["spacing"], p["xt"], p["yt"])
     f.capturezone(m, starts, p["umbra"], 1.0, O.trace_opts(p["duration"], p["tol"], p["maxstep"],
-                                                           True, math))
+                                                           confined, math))
     assert_field(f, ref, cfg)
-    assert_traces(m, p, starts, ref, cfg, math)
+    assert_traces(m, p, starts, ref, cfg, math, confined)
 
 
 def test_vertex_hash_is_sensitive(oracle_lib, ref):
