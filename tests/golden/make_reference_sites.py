@@ -27,6 +27,10 @@ Writes tests/golden/reference_sites.npz (inputs and outputs only):
   cfg1u_*  the same for the unconfined velocity (compute_velocity) on the
            25-well site: realization 0 of the draws of 1 realization (seed 0),
            32 particles in the order 0, 25, 50, ...
+  cfg0dry_*
+           unconfined traces dry at their start point (AquiferError: the
+           trace is its start vertex) beside traces that end normally, on the
+           4-well site of configuration 0 with F lowered (dry_subset).
   cfg2s2_*, cfg2s50_*
            the same for sites 2 (135 wells, 29.1 years) and 50 (18 wells,
            22.8 years) of configuration 2 (onekapy_amd.sites.config2), with
@@ -170,6 +174,54 @@ def subset(out, key, s, d, count=32, steps=(0, 4000), stride=8, confined=True):
     out[key + "_geom"], out[key + "_nodes"], out[key + "_values"] = field_record(pf)
 
 
+def dry_subset(out, key="cfg0dry", shift=0.8, count=16):
+    """Unconfined traces that the reference ends at once (AquiferError at the
+    start point, caught by compute_backtrace, capturezone.py:218, 250-251),
+    beside traces that end normally: the 4-well site of configuration 0
+    (seed-11 draws of its 100 realizations) with F lowered by shift K b^2 / 2,
+    the first realization whose start ring holds at least count / 2 of each
+    (a backtrace climbs the potential, so a trace that starts wet stays wet);
+    half the picks are dry, half end normally, each set in particle order."""
+    import oracle as O
+    s = sites.config0()
+    d = sites.draw(s, seed=11)
+    starts = start_points(s.xt, s.yt, s.rt, s.npaths)
+    for r in range(len(d["K"])):
+        m, w, coef = realization_model(s, d, r)
+        coef[5] -= shift * 0.5 * d["K"][r] * d["b"][r] ** 2
+        m.coef = coef
+        om = O.OracleModel(s.base, d["K"][r], d["n"][r], d["b"][r], w, s.xt, s.yt, coef)
+        dry, ok = [], []
+        for p, (xs, ys) in enumerate(starts):
+            v, res = om.backtrace(xs, ys, s.duration, s.tol, s.maxstep, confined=False,
+                                  math=O.MATH_GLIBC)
+            if res.status == O.TRACE_TRUNCATED:
+                dry.append(p)
+            elif res.status == 0:
+                ok.append(p)
+        if len(dry) >= count // 2 and len(ok) >= count // 2:
+            break
+    else:
+        raise RuntimeError("no realization with dry and wet start points")
+    pick = np.array(sorted(dry[:count // 2] + ok[:count // 2]), dtype=np.int64)
+    traces, (out[key + "_trace_len"], out[key + "_trace_last"],
+             out[key + "_trace_sha256"]) = trace_records(m, s, starts[pick], False)
+    print(f"{key}: realization {r}, {len(pick)} traces", flush=True)
+    pf = ProbabilityField(s.spacing, s.spacing, s.xt, s.yt)
+    for vertices in traces:
+        pf.rasterize([v[0] for v in vertices], [v[1] for v in vertices], s.umbra)
+    pf.register(1.0)
+    out[key + "_params"] = np.array([s.base, s.xt, s.yt, s.rt, s.npaths, s.duration, s.spacing,
+                                     s.umbra, s.tol, s.maxstep])
+    out[key + "_wells"] = w
+    out[key + "_aquifer"] = np.array([d["K"][r], d["n"][r], d["b"][r]])
+    out[key + "_coef"] = coef
+    out[key + "_trace_index"] = pick
+    out[key + "_confined"] = np.array([0])
+    out[key + "_dry"] = np.array(sorted(dry[:count // 2]), dtype=np.int64)
+    out[key + "_geom"], out[key + "_nodes"], out[key + "_values"] = field_record(pf)
+
+
 def main():
     out = {}
     cfg1(out)
@@ -181,6 +233,7 @@ def main():
     c2 = sites.config2()
     for i in (2, 50):
         subset(out, f"cfg2s{i}", c2[i], sites.draw(c2[i], seed=100 + i))
+    dry_subset(out)
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, {k: v.shape for k, v in out.items()})
 

@@ -99,11 +99,18 @@ def assert_site_field(pf, ref, cfg):
     assert np.array_equal(pg[nz], ref[cfg + "_values"])
 
 
+def expected_status(ref, cfg):
+    """0 for each trace, 1 (truncated) for those the reference ends dry."""
+    pick = ref[cfg + "_trace_index"]
+    dry = ref[cfg + "_dry"] if cfg + "_dry" in ref.files else []
+    return np.isin(pick, dry).astype(np.int32)
+
+
 def assert_site_traces(wells, aquifer, coef, p, starts, ref, cfg, confined=True):
     real = Realizations(aquifer[0], aquifer[1], aquifer[2], wells[:, 3], coef)
     verts, nverts, status = trace_vertices(wells[:, :2], p["xt"], p["yt"], real, starts,
                                            p["duration"], p["tol"], p["maxstep"], confined)
-    assert np.all(status == 0)
+    assert np.array_equal(status, expected_status(ref, cfg))
     assert np.array_equal(nverts, ref[cfg + "_trace_len"])
     for k in range(len(starts)):
         v = verts[k, :nverts[k]]
@@ -131,12 +138,13 @@ def test_config1_two_realizations_equal_reference(gpu_ctx, sites_ref):
 
 @pytest.mark.parametrize("schedule", [_lib.SCHED_WORKGROUP, _lib.SCHED_POOL],
                          ids=["workgroup", "pool"])
-@pytest.mark.parametrize("cfg", ["cfg4", "cfg4long", "cfg2s2", "cfg2s50", "cfg1u"])
+@pytest.mark.parametrize("cfg", ["cfg4", "cfg4long", "cfg2s2", "cfg2s50", "cfg1u", "cfg0dry"])
 def test_site_traces_equal_reference(gpu_ctx, sites_ref, cfg, schedule):
     """32 traces on each of the 200-well site (injection wells included) and
     two sites of the uneven 64-site workload (135 and 18 wells), 32 traces of
-    the unconfined velocity on the 25-well site, and four long traces (2,701 to
-    9,077 vertices) on the 200-well site: the FP64-exact
+    the unconfined velocity on the 25-well site, four long traces (2,701 to
+    9,077 vertices) on the 200-well site, and unconfined traces dry at their
+    start point beside wet ones: the FP64-exact
     trace-and-rasterize kernel on either schedule registers the reference's
     field, and the vertex sequences hash to the reference's."""
     ref, p = sites_ref, site_params(sites_ref, cfg)
@@ -149,7 +157,7 @@ def test_site_traces_equal_reference(gpu_ctx, sites_ref, cfg, schedule):
     res = chronicle_into_field(pf, wells[:, :2], p["xt"], p["yt"], real, starts, p["duration"],
                                p["tol"], p["maxstep"], p["umbra"], confined, _lib.FP64_EXACT,
                                schedule=schedule)
-    assert np.all(res.trace_status == 0)
+    assert np.array_equal(res.trace_status, expected_status(ref, cfg))
     pf.rgrid |= res.counts > 0
     pf.register(1.0)
     assert_site_field(pf, ref, cfg)
