@@ -15,8 +15,9 @@ reference's own arithmetic, bit-exact against the oracle).  The FP32 tracer's
 rate is reported beside it (``fp32_gated``) with its capture-probability error
 against the FP64-exact tracer on the same realizations (max over nodes, gate
 0.01), as are the learned control variate on the same site, the 64-site uneven
-batch (configuration 2) and the 200-well stress site on a realization subset
-(configuration 4), each with its own error sample against FP64-exact.  The
+batch (configuration 2) and the 200-well stress site with all of its 10^6
+realizations (configuration 4), each with its own error sample against
+FP64-exact.  The
 dominant kernel's roofline fraction and the CPU baselines are in the line too.
 """
 import argparse
