@@ -33,9 +33,36 @@ __global__ void __launch_bounds__(256) k_fit_logs(FitArgs a, double *logr2)
     logr2[t] = log_rn(dx * dx + dy * dy);
 }
 
-// One weighted row (model.py:545-565).  Returns 0 or OKA_ERR_RANGE.
-__device__ __forceinline__ int fit_row(const FitArgs &a, int i, double K, double b,
-                                       const double *q, int64_t qs, double *row, double &wb)
+// Observations whose wells' potentials one pass over the realization's
+// discharges computes: each discharge is loaded once per block rather than
+// once per observation (the fit of 10^6 realizations of the 200-well,
+// 200-observation site: 402 ms unblocked, 106 ms in blocks of 4, 55 ms in
+// blocks of 8, the same bits).
+constexpr int kObsBlock = 8;
+
+// The wells' potential at observations i0 .. i0 + kObsBlock - 1 (those below
+// nobs), each summed over the wells in index order (model.py:258-264).
+__device__ __forceinline__ void well_potentials(const FitArgs &a, int i0, const double *q,
+                                                int64_t qs, double (&pw)[kObsBlock])
+{
+#pragma unroll
+    for (int j = 0; j < kObsBlock; ++j)
+        pw[j] = 0.0;
+    const double *L = a.logr2 + (int64_t)i0 * a.nwells;
+    const int nb = min(kObsBlock, a.nobs - i0);
+    for (int w = 0; w < a.nwells; ++w) {
+        const double qw = q[w * qs];
+#pragma unroll
+        for (int j = 0; j < kObsBlock; ++j)
+            if (j < nb)
+                pw[j] = pw[j] + (qw * L[(int64_t)j * a.nwells + w]) * kC4Pi;
+    }
+}
+
+// One weighted row (model.py:545-565), given the wells' potential pw at the
+// observation.  Returns 0 or OKA_ERR_RANGE.
+__device__ __forceinline__ int fit_row(const FitArgs &a, int i, double K, double b, double pw,
+                                       double *row, double &wb)
 {
     const double x = a.obs[4 * i], y = a.obs[4 * i + 1];
     const double z_ev = a.obs[4 * i + 2], z_std = a.obs[4 * i + 3];
@@ -57,10 +84,6 @@ __device__ __forceinline__ int fit_row(const FitArgs &a, int i, double K, double
     row[3] = dx / pot_std;
     row[4] = dy / pot_std;
     row[5] = 1.0 / pot_std;
-    const double *L = a.logr2 + (int64_t)i * a.nwells;
-    double pw = 0.0;
-    for (int w = 0; w < a.nwells; ++w)
-        pw = pw + (q[w * qs] * L[w]) * kC4Pi;
     wb = (pot_ev - pw) / pot_std;
     return 0;
 }
@@ -134,16 +157,23 @@ __device__ __forceinline__ void fit_one(const FitArgs &a, int64_t r)
     for (int j = 0; j < 6; ++j)
         rhs[j] = 0.0;
     int st = 0;
-    for (int i = 0; i < a.nobs && st == 0; ++i) {
-        st = fit_row(a, i, K, b, q, qs, row, wb);
-        if (st)
-            break;
+    for (int i0 = 0; i0 < a.nobs && st == 0; i0 += kObsBlock) {
+        double pw[kObsBlock];
+        well_potentials(a, i0, q, qs, pw);
 #pragma unroll
-        for (int j = 0; j < 6; ++j) {
+        for (int o = 0; o < kObsBlock; ++o) {
+            if (i0 + o >= a.nobs)
+                break;
+            st = fit_row(a, i0 + o, K, b, pw[o], row, wb);
+            if (st)
+                break;
 #pragma unroll
-            for (int k = 0; k <= j; ++k)
-                N[j * 6 + k] = N[j * 6 + k] + row[j] * row[k];
-            rhs[j] = rhs[j] + row[j] * wb;
+            for (int j = 0; j < 6; ++j) {
+#pragma unroll
+                for (int k = 0; k <= j; ++k)
+                    N[j * 6 + k] = N[j * 6 + k] + row[j] * row[k];
+                rhs[j] = rhs[j] + row[j] * wb;
+            }
         }
     }
     double ev[6], L[36], s[6];
@@ -163,15 +193,22 @@ __device__ __forceinline__ void fit_one(const FitArgs &a, int64_t r)
     if (st == 0) {
         chol6_solve(L, s, rhs, ev);
         double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, d[6];
-        for (int i = 0; i < a.nobs; ++i) {
-            fit_row(a, i, K, b, q, qs, row, wb);
-            double res = wb;
+        for (int i0 = 0; i0 < a.nobs; i0 += kObsBlock) {
+            double pw[kObsBlock];
+            well_potentials(a, i0, q, qs, pw);
 #pragma unroll
-            for (int j = 0; j < 6; ++j)
-                res = res - row[j] * ev[j];
+            for (int o = 0; o < kObsBlock; ++o) {
+                if (i0 + o >= a.nobs)
+                    break;
+                fit_row(a, i0 + o, K, b, pw[o], row, wb);
+                double res = wb;
 #pragma unroll
-            for (int j = 0; j < 6; ++j)
-                g[j] = g[j] + row[j] * res;
+                for (int j = 0; j < 6; ++j)
+                    res = res - row[j] * ev[j];
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+                    g[j] = g[j] + row[j] * res;
+            }
         }
         chol6_solve(L, s, g, d);
 #pragma unroll
